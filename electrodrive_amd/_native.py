@@ -1,0 +1,130 @@
+"""ctypes binding of ``libednative.so`` (the C ABI declared in include/ednative.h).
+
+The product path fails loudly when the library is missing: there is no CPU
+fallback for any compute entry point.
+"""
+import ctypes
+import os
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libednative.so")
+
+ED_OK = 0
+ED_ERR_ARG = 1
+ED_ERR_DEVICE = 2
+ED_ERR_ALLOC = 3
+ED_ERR_LAUNCH = 4
+ED_ERR_NONFINITE = 5
+ED_ERR_INTERNAL = 6
+
+c_i32 = ctypes.c_int32
+c_i64 = ctypes.c_int64
+c_dbl = ctypes.c_double
+c_vp = ctypes.c_void_p
+c_i64p = ctypes.POINTER(ctypes.c_int64)
+
+
+class PlanOpts(ctypes.Structure):
+    _fields_ = [("leaf_size", c_i32), ("order", c_i32), ("ws", c_i32), ("max_level", c_i32),
+                ("m2l_precision", c_i32), ("reserved", c_i32 * 3)]
+
+
+class GmresOpts(ctypes.Structure):
+    _fields_ = [("restart", c_i32), ("maxiter", c_i32), ("tol", c_dbl), ("precond", c_i32),
+                ("refine", c_i32), ("max_refine", c_i32), ("inner_tol", c_dbl),
+                ("use_graph", c_i32), ("check_every", c_i32)]
+
+
+class GmresInfo(ctypes.Structure):
+    _fields_ = [("code", c_i32), ("iters", c_i32), ("cycles", c_i32), ("refine_steps", c_i32),
+                ("first_iter_below_tol", c_i32), ("resid", c_dbl), ("resid0", c_dbl),
+                ("rel_resid", c_dbl), ("tol_abs", c_dbl)]
+
+
+_SIGS = {
+    "ed_last_error": (ctypes.c_char_p, []),
+    "ed_abi_version": (c_i32, []),
+    "ed_plan_opts_default": (None, [ctypes.POINTER(PlanOpts)]),
+    "ed_plan_create": (c_i32, [c_vp, c_vp, c_i64, ctypes.POINTER(PlanOpts), ctypes.POINTER(c_vp)]),
+    "ed_plan_destroy": (c_i32, [c_vp]),
+    "ed_plan_stats": (c_i32, [c_vp, c_vp, c_i32]),
+    "ed_plan_permutation": (c_i32, [c_vp, c_vp]),
+    "ed_fmm_matvec_f32": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "ed_fmm_matvec_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "ed_plan_attach_near": (c_i32, [c_vp, c_vp, c_vp, c_i64]),
+    "ed_dense_matvec_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
+    "ed_dense_matvec_f32": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
+    "ed_self_integrals_f64": (c_i32, [c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),
+    "ed_near_pairs_f64": (c_i32, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_i64p, c_vp]),
+    "ed_near_weights_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),
+    "ed_near_apply_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64,
+                                  c_i32, c_vp, c_vp]),
+    "ed_morton_keys_f64": (c_i32, [c_vp, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "ed_morton_keys_f32": (c_i32, [c_vp, c_i64, c_i32, c_vp, c_vp, c_vp, c_vp]),
+    "ed_gmres_opts_default": (None, [ctypes.POINTER(GmresOpts)]),
+    "ed_gmres_solve": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(GmresOpts),
+                               ctypes.POINTER(GmresInfo), c_vp]),
+    "ed_vec_dot_f64": (c_i32, [c_vp, c_vp, c_i64, c_vp, ctypes.POINTER(c_dbl), c_vp]),
+    "ed_vec_axpy_f64": (c_i32, [c_dbl, c_vp, c_vp, c_i64, c_vp]),
+    "ed_vec_scale_f64": (c_i32, [c_dbl, c_vp, c_vp, c_i64, c_vp]),
+    "ed_vec_mul_f64": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "ed_host_tree_build": (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
+                                   c_vp, c_vp, c_vp, c_vp]),
+    "ed_host_operator": (c_i32, [c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp]),
+    "ed_host_morton_keys_f64": (c_i32, [c_vp, c_i64, c_i32, c_vp, c_vp, c_vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load the native library (once).  Raises if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"native library {LIB_PATH} is missing; build it with "
+                "`python -m electrodrive_amd.build` (there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIGS.keys())
+
+
+def check(rc, what=""):
+    if rc == ED_OK:
+        return
+    msg = lib().ed_last_error().decode(errors="replace")
+    text = f"{what}: {msg}" if what else msg
+    if rc == ED_ERR_ARG:
+        raise ValueError(text)
+    raise RuntimeError(f"{text} (status {rc})")
+
+
+def ptr(t):
+    """Raw data pointer of a tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_ptr(device=None):
+    """Current HIP stream of torch on ``device`` (torch's stream is plumbing)."""
+    if not torch.cuda.is_available():
+        raise RuntimeError("no HIP device available")
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("electrodrive_amd compute paths require a HIP device "
+                           "(no CPU fallback)")

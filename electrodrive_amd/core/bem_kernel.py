@@ -1,0 +1,101 @@
+"""BEM matvec dispatcher and dense single-layer matvec (mirror of core/bem_kernel.py).
+
+``bem_matvec_gpu`` keeps the reference signature and dispatch rules
+(core/bem_kernel.py:558-707): backends {"auto", "torch_tiled", "keops",
+"external"}; unknown -> ValueError; "external" needs ``matvec_impl``;
+``self_correction`` is an alias of ``self_integrals``; ``use_near_quad`` is
+accepted and ignored.  "auto" / "torch_tiled" / "keops" all run the native
+dense kernel (``ed_dense_matvec_f64/f32``), which computes exactly what
+``_bem_matvec_core_torch`` does (core/bem_kernel.py:157-230):
+    V_i = sum_{j != i} K_E sigma_j A_j rsqrt(max(r_ij^2, 1e-24)) + K_ii sigma_i A_i,
+    K_ii = self_integrals[i] or 0, non-finite outputs clamped to 0.
+Inputs may be CPU tensors; they are moved to the current HIP device and the
+result is returned on sigma's device.
+"""
+from typing import Callable, Optional
+
+import torch
+
+from .. import _native as nat
+from .kernels import DEFAULT_SINGLE_LAYER_KERNEL, K_E, LaplaceSingleLayerKernel
+
+__all__ = ["bem_matvec_gpu", "bem_matvec_batched", "dense_matvec", "K_E",
+           "LaplaceSingleLayerKernel", "DEFAULT_SINGLE_LAYER_KERNEL"]
+
+
+def _ensure_tile_size(tile_size: Optional[int]) -> int:
+    """core/bem_kernel.py:31-52 (kept for signature parity; the native kernel
+    tiles internally)."""
+    import os
+    if tile_size is not None and tile_size > 0:
+        return int(tile_size)
+    env = os.getenv("EDE_BEM_TILE_SIZE")
+    if env is not None:
+        try:
+            v = int(env)
+            if v > 0:
+                return v
+        except ValueError:
+            pass
+    return 1024
+
+
+def dense_matvec(sigma, centroids, areas, self_integrals=None):
+    """Native dense matvec; sigma (N,) or (B, N).  Returns a new tensor on
+    sigma's device with sigma's dtype."""
+    nat.require_gpu()
+    dt = sigma.dtype
+    if dt not in (torch.float32, torch.float64):
+        raise ValueError("sigma must be float32 or float64")
+    dev = torch.device("cuda", torch.cuda.current_device()) if not sigma.is_cuda else sigma.device
+    s = sigma.to(dev).contiguous()
+    squeeze = s.dim() == 1
+    if squeeze:
+        s = s.unsqueeze(0)
+    if s.dim() != 2:
+        raise ValueError("sigma must have shape (N,) or (B, N).")
+    B, N = s.shape
+    C = centroids.to(dev, dt).contiguous()
+    A = areas.to(dev, dt).contiguous()
+    if C.shape != (N, 3) or A.shape != (N,):
+        raise ValueError("centroids/areas shape mismatch with sigma")
+    si = None if self_integrals is None else self_integrals.to(dev, dt).contiguous()
+    out = torch.empty_like(s)
+    fn = nat.lib().ed_dense_matvec_f64 if dt == torch.float64 else nat.lib().ed_dense_matvec_f32
+    with torch.cuda.device(dev):
+        nat.check(fn(nat.ptr(C), nat.ptr(A), nat.ptr(s), nat.ptr(si), nat.ptr(out), N, B,
+                     nat.stream_ptr(dev)), "ed_dense_matvec")
+    out = out[0] if squeeze else out
+    return out.to(sigma.device)
+
+
+def bem_matvec_gpu(sigma, src_centroids, areas, *, tile_size: int = 1024,
+                   self_integrals=None, logger=None, use_keops: bool = False, kernel=None,
+                   backend: str = "auto", matvec_impl: Optional[Callable] = None, **kwargs):
+    if "self_correction" in kwargs and self_integrals is None:
+        self_integrals = kwargs.get("self_correction")
+    if backend not in {"auto", "torch_tiled", "keops", "external"}:
+        raise ValueError(f"Unknown BEM backend: {backend!r}")
+    ker = kernel or DEFAULT_SINGLE_LAYER_KERNEL
+    tile_size = _ensure_tile_size(tile_size)
+    if backend == "external":
+        if matvec_impl is None:
+            raise ValueError("backend='external' requires matvec_impl.")
+        return matvec_impl(sigma=sigma, src_centroids=src_centroids, areas=areas,
+                           tile_size=tile_size, self_integrals=self_integrals, logger=logger,
+                           kernel=ker, **kwargs)
+    if not isinstance(ker, LaplaceSingleLayerKernel):
+        raise ValueError("only the Laplace single-layer kernel is implemented natively")
+    del use_keops
+    return dense_matvec(sigma, src_centroids, areas, self_integrals)
+
+
+def bem_matvec_batched(sigma, src_centroids, areas, *, tile_size: int = 1024,
+                       self_integrals=None, kernel=None, logger=None):
+    """core/bem_kernel.py:715-810: sigma (N,) or (B, N) -> V of the same shape."""
+    if sigma.ndim not in (1, 2):
+        raise ValueError("sigma must have shape (N,) or (B, N).")
+    ker = kernel or DEFAULT_SINGLE_LAYER_KERNEL
+    if not isinstance(ker, LaplaceSingleLayerKernel):
+        raise ValueError("only the Laplace single-layer kernel is implemented natively")
+    return dense_matvec(sigma, src_centroids, areas, self_integrals)

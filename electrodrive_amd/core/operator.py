@@ -1,0 +1,131 @@
+"""Single-layer collocation operator of one panel geometry, assembled on the GPU.
+
+``BemOperator`` performs the set-up of core/bem.py:1715-1960 natively:
+self integrals (refined rule, core/bem.py:1777-1816, or the equal-area disk
+when ``refined_self=False``), near pairs and near-correction weights
+(core/bem.py:1838-1957), the FMM plan, and the right-hand side
+b = bc - V_free (core/bem.py:2056).  Two operators share the same definition
+
+    (A sigma)_i = sum_{j != i} K_E A_j sigma_j / r_ij + self_corr_i A_i sigma_i
+                  + sum_j delta_ij sigma_j          (near correction, optional)
+
+``approx`` evaluates it with the FP32 FMM (the GMRES operator); ``accurate``
+with the FP64 dense kernel plus the FP64 near correction (the residual operator
+of the outer iterative refinement).
+"""
+import math
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from ..fmm3d.plan import FmmPlan
+from . import bem_quadrature as bq
+from .bem_kernel import dense_matvec
+from .kernels import K_E
+
+
+class FmmOperator:
+    """Callable A(sigma) backed by a native plan; exposes the native solver to
+    ``gmres_restart``."""
+
+    def __init__(self, plan: FmmPlan, self_corr):
+        self.plan = plan
+        self.self_corr = self_corr
+
+    def __call__(self, sigma):
+        s = sigma.to(self.self_corr.device, torch.float64).contiguous()
+        return self.plan.matvec(s, self.self_corr).to(sigma.device, sigma.dtype)
+
+    def _ed_native_solve(self, b, x0=None, diag=None, restart=50, tol=1e-8, maxiter=500):
+        dev = self.self_corr.device
+        bb = b.to(dev, torch.float64).contiguous()
+        x0d = None if x0 is None else x0.to(dev, torch.float64).contiguous()
+        dd = None if diag is None else diag.to(dev, torch.float64).contiguous()
+        x, info = self.plan.solve(bb, x0d, dd, self.self_corr, restart=restart, maxiter=maxiter,
+                                  tol=tol, precond=dd is not None, refine=False)
+        out = dict(iters=info["iters"], resid=info["resid"], success=info["success"],
+                   code=info["code"], resid_true0=info["resid0"], tol_abs=info["tol_abs"],
+                   first_iter_below_tol=info["first_iter_below_tol"], cycles=info["cycles"])
+        if out["success"]:
+            out["converged_on"] = "true"
+        return x, out
+
+
+class BemOperator:
+    def __init__(self, mesh, *, order=8, ws=1, leaf_size=64, max_level=16,
+                 m2l_precision="fp32", refined_self=True, self_depth=3, near=True,
+                 near_depth=3, near_order=2, distance_factor=1.5, device=None):
+        nat.require_gpu()
+        self.device = torch.device(device or "cuda")
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.mesh = mesh
+        self.n = mesh.n_panels
+        dev = self.device
+        with torch.cuda.device(dev):
+            self.C = torch.as_tensor(mesh.centroids, dtype=torch.float64).to(dev).contiguous()
+            self.A = torch.as_tensor(mesh.areas, dtype=torch.float64).to(dev).contiguous()
+            self.PV = torch.as_tensor(mesh.panel_vertices, dtype=torch.float64).to(dev).contiguous()
+            if refined_self:
+                self.I_self, self.self_corr = bq.self_integrals(self.C, self.PV, self.A,
+                                                                depth=self_depth, order=near_order)
+            else:
+                vals = 2.0 * math.pi * K_E * torch.sqrt(self.A.clamp_min(0.0) / math.pi)
+                self.I_self = vals
+                self.self_corr = vals / self.A.clamp_min(torch.finfo(torch.float64).tiny)
+            self.near_pairs = None
+            self.near_w = None
+            if near and self.n > 1:
+                self.near_pairs = bq.build_near_pairs(self.C, self.A, distance_factor)
+                self.near_w = bq.near_correction_weights(self.C, self.A, self.PV, self.near_pairs,
+                                                         quad_order=near_order, max_depth=near_depth)
+            self.plan = FmmPlan(mesh.centroids, mesh.areas, leaf_size=leaf_size, order=order,
+                                ws=ws, max_level=max_level, m2l_precision=m2l_precision)
+            if self.near_pairs is not None and self.near_pairs.shape[0] > 0:
+                self.plan.attach_near(self.near_pairs.cpu().numpy(), self.near_w.cpu().numpy())
+        self.fmm = FmmOperator(self.plan, self.self_corr)
+        # Jacobi diagonal of bem_solve: the raw self integral (core/bem.py:2059)
+        self.diag = self.I_self.clamp_min(1e-18)
+
+    # -- right-hand sides ------------------------------------------------------
+    def rhs_point_charges(self, charges, bc=0.0):
+        """b = bc - V_free with V_free = sum_k K_E q_k / max(|c - r_k|, 1e-12)
+        (core/bem.py:291-320, :2056).  charges: iterable of (q, (x, y, z))."""
+        C = self.C
+        V = torch.zeros(self.n, dtype=torch.float64, device=self.device)
+        for q, pos in charges:
+            p = torch.tensor(pos, dtype=torch.float64, device=self.device)
+            r = torch.linalg.norm(C - p[None, :], dim=1).clamp_min(1e-12)
+            V = V + (K_E * float(q)) / r
+        bcv = torch.as_tensor(bc, dtype=torch.float64, device=self.device)
+        return (bcv - V).contiguous()
+
+    # -- operators ---------------------------------------------------------------
+    def approx(self, sigma):
+        return self.plan.matvec(sigma.to(self.device, torch.float64).contiguous(), self.self_corr)
+
+    def accurate(self, sigma):
+        s = sigma.to(self.device, torch.float64).contiguous()
+        v = dense_matvec(s, self.C, self.A, self.self_corr)
+        if self.near_pairs is not None and self.near_pairs.shape[0] > 0:
+            v = bq.near_quadrature_matvec(v, s, self.C, self.A, None, self.near_pairs,
+                                          weights=self.near_w)
+        return v
+
+    def solve(self, b, x0=None, *, tol=1e-8, restart=50, maxiter=2000, refine=True,
+              inner_tol=1e-4, max_refine=8, precond=True, use_graph=True, check_every=4):
+        """FP64 GMRES on the FMM operator with FP64 iterative refinement against
+        the accurate operator (``refine=True``); returns (sigma, info)."""
+        if x0 is None:
+            x0 = (b / self.diag).contiguous()  # core/bem.py:2060
+        return self.plan.solve(b, x0, self.diag if precond else None, self.self_corr,
+                               restart=restart, maxiter=maxiter, tol=tol, precond=precond,
+                               refine=refine, max_refine=max_refine, inner_tol=inner_tol,
+                               use_graph=use_graph, check_every=check_every)
+
+    def total_charge(self, sigma):
+        return float((sigma.to(torch.float64) * self.A).sum())
+
+
+_ = np

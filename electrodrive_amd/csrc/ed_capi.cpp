@@ -1,0 +1,455 @@
+// C ABI: plan creation/destruction, FMM matvec entry points, host helpers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ed_plan.h"
+
+namespace ed {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+int m2l_shape(int p, int* ncp, int* kp, int* cols, int* enabled);
+void gmres_free(ed_plan* P);
+
+#define ED_HIP(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(e_ == hipErrorOutOfMemory ? ED_ERR_ALLOC : ED_ERR_DEVICE,            \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                  \
+  } while (0)
+
+template <typename T>
+static int upload(T** dst, const std::vector<T>& src) {
+  *dst = nullptr;
+  const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+  ED_HIP(hipMalloc((void**)dst, bytes));
+  if (!src.empty()) ED_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return ED_OK;
+}
+
+#define ED_TRY(x)              \
+  do {                         \
+    int rc_ = (x);             \
+    if (rc_ != ED_OK) return rc_; \
+  } while (0)
+
+void plan_free(ed_plan* P) {
+  void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->perm, P->node_ctr, P->node_rng,
+                  P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
+                  P->leaves, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
+                  P->m2l_pair_src, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
+                  P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair, P->t_m2l,
+                  P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
+                  P->near_w, P->cent64_t, P->area64_t, P->near_w64};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  gmres_free(P);
+}
+
+int plan_upload(ed_plan* P) {
+  const HostTree& H = P->host;
+  const int64_t n = H.n, M = (int64_t)H.level.size();
+  const int p = P->p, nc = n_real(p);
+  P->nc = nc;
+  P->n = n;
+  P->n_nodes = M;
+  // points (relative to the root centre), perm, areas handled by caller
+  std::vector<float4> pts(n);
+  (void)pts;
+  // nodes
+  std::vector<float4> ctr(M);
+  std::vector<int2> rng(M);
+  std::vector<int32_t> par(M), oct(M), fc(M), nch(M);
+  P->inv_side.assign(H.depth + 1, 0.0f);
+  for (int l = 0; l <= H.depth; ++l) P->inv_side[l] = (float)(1.0 / (2.0 * H.half / std::ldexp(1.0, l)));
+  for (int64_t i = 0; i < M; ++i) {
+    const int l = H.level[i];
+    const double s = 2.0 * H.half / std::ldexp(1.0, l);
+    const double cx = -H.half + (H.coord[3 * i] + 0.5) * s;
+    const double cy = -H.half + (H.coord[3 * i + 1] + 0.5) * s;
+    const double cz = -H.half + (H.coord[3 * i + 2] + 0.5) * s;
+    ctr[i] = make_float4((float)cx, (float)cy, (float)cz, (float)(1.0 / s));
+    rng[i] = make_int2((int)H.begin[i], (int)H.end[i]);
+    par[i] = (int32_t)std::max<int64_t>(H.parent[i], 0);
+    oct[i] = (H.coord[3 * i] & 1) | ((H.coord[3 * i + 1] & 1) << 1) | ((H.coord[3 * i + 2] & 1) << 2);
+    fc[i] = H.first_child[i] < 0 ? 0 : H.first_child[i];
+    nch[i] = H.n_child[i];
+  }
+  ED_TRY(upload(&P->node_ctr, ctr));
+  ED_TRY(upload(&P->node_rng, rng));
+  ED_TRY(upload(&P->node_parent, par));
+  ED_TRY(upload(&P->node_octant, oct));
+  ED_TRY(upload(&P->first_child, fc));
+  ED_TRY(upload(&P->n_child, nch));
+  ED_HIP(hipMalloc((void**)&P->M, sizeof(float) * (size_t)M * nc));
+  ED_HIP(hipMalloc((void**)&P->L, sizeof(float) * (size_t)M * nc));
+  ED_HIP(hipMemset(P->M, 0, sizeof(float) * (size_t)M * nc));
+  ED_HIP(hipMemset(P->L, 0, sizeof(float) * (size_t)M * nc));
+
+  // leaves and U lists
+  std::vector<int32_t> leaves, uptr{0}, usrc;
+  int64_t p2p_int = 0;
+  for (int64_t i = 0; i < M; ++i) {
+    if (!H.is_leaf[i]) continue;
+    leaves.push_back((int32_t)i);
+    for (int64_t q = H.u_ptr[i]; q < H.u_ptr[i + 1]; ++q) {
+      const int64_t s = H.u_src[q];
+      usrc.push_back((int32_t)s);
+      p2p_int += (H.end[i] - H.begin[i]) * (H.end[s] - H.begin[s]);
+    }
+    uptr.push_back((int32_t)usrc.size());
+  }
+  P->n_leaves = (int64_t)leaves.size();
+  P->n_u = (int64_t)usrc.size();
+  P->n_p2p_interactions = p2p_int;
+  ED_TRY(upload(&P->leaves, leaves));
+  ED_TRY(upload(&P->u_ptr, uptr));
+  ED_TRY(upload(&P->u_src, usrc));
+
+  // parents per level
+  std::vector<int32_t> plist;
+  P->parents_begin.assign(H.depth + 2, 0);
+  for (int l = 0; l <= H.depth; ++l) {
+    P->parents_begin[l] = (int64_t)plist.size();
+    for (int64_t i = H.level_begin[l]; i < H.level_begin[l + 1]; ++i)
+      if (H.n_child[i] > 0) plist.push_back((int32_t)i);
+  }
+  P->parents_begin[H.depth + 1] = (int64_t)plist.size();
+  ED_TRY(upload(&P->parents_list, plist));
+
+  // V lists and offset tables
+  std::map<int32_t, int32_t> slot_map;
+  for (int32_t s : H.v_off) slot_map.emplace(s, 0);
+  int32_t ntab = 0;
+  for (auto& kv : slot_map) kv.second = ntab++;
+  P->n_off = ntab;
+  std::vector<int32_t> vptr(M + 1), vsrc(H.v_src.size()), vtab(H.v_src.size());
+  for (int64_t i = 0; i <= M; ++i) vptr[i] = (int32_t)H.v_ptr[i];
+  for (size_t q = 0; q < H.v_src.size(); ++q) {
+    vsrc[q] = (int32_t)H.v_src[q];
+    vtab[q] = slot_map[H.v_off[q]];
+  }
+  P->n_v = (int64_t)vsrc.size();
+  ED_TRY(upload(&P->v_ptr, vptr));
+  ED_TRY(upload(&P->v_src, vsrc));
+  ED_TRY(upload(&P->v_tab, vtab));
+
+  const int R = offset_radius(P->ws), W = 2 * R + 1;
+  std::vector<double> Td((size_t)nc * nc);
+  std::vector<float> tv((size_t)std::max(ntab, 1) * nc * nc, 0.0f);
+  int ncp = 0, kp = 0, cols = 0, enabled = 0;
+  ED_TRY(m2l_shape(p, &ncp, &kp, &cols, &enabled));
+  P->ncp = ncp;
+  std::vector<float> tm((size_t)std::max(ntab, 1) * ncp * kp, 0.0f);
+  for (auto& kv : slot_map) {
+    const int slot = kv.first, tab = kv.second;
+    const int ox = slot % W - R, oy = (slot / W) % W - R, oz = slot / (W * W) - R;
+    m2l_matrix(p, ox, oy, oz, Td.data());
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < nc; ++j) {
+        const float v = (float)Td[(size_t)k * nc + j];
+        tv[(size_t)tab * nc * nc + (size_t)j * nc + k] = v;
+        tm[(size_t)tab * ncp * kp + (size_t)k * kp + j] = v;
+      }
+  }
+  ED_TRY(upload(&P->t_m2l, tv));
+  ED_TRY(upload(&P->t_m2l_mfma, tm));
+  std::vector<float> t8((size_t)8 * nc * nc), l8((size_t)8 * nc * nc);
+  for (int o = 0; o < 8; ++o) {
+    m2m_matrix(p, o, Td.data());
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < nc; ++j) t8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
+    l2l_matrix(p, o, Td.data());
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < nc; ++j) l8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
+  }
+  ED_TRY(upload(&P->t_m2m, t8));
+  ED_TRY(upload(&P->t_l2l, l8));
+
+  // GEMM-form M2L schedule: pairs grouped by table, tiles of `cols` pairs
+  P->n_m2l_tiles = 0;
+  if (enabled && P->n_v > 0) {
+    std::vector<std::vector<std::pair<int32_t, int32_t>>> by_tab(ntab);
+    for (int64_t t = 0; t < M; ++t)
+      for (int64_t q = H.v_ptr[t]; q < H.v_ptr[t + 1]; ++q)
+        by_tab[vtab[q]].emplace_back((int32_t)t, vsrc[q]);
+    std::vector<int32_t> psrc, ptgt, ttab, tbeg, tcnt;
+    for (int tab = 0; tab < ntab; ++tab) {
+      auto& v = by_tab[tab];
+      for (size_t b = 0; b < v.size(); b += (size_t)cols) {
+        ttab.push_back(tab);
+        tbeg.push_back((int32_t)psrc.size() + 0);
+        tcnt.push_back((int32_t)std::min<size_t>((size_t)cols, v.size() - b));
+        for (size_t c = b; c < std::min(v.size(), b + (size_t)cols); ++c) {
+          ptgt.push_back(v[c].first);
+          psrc.push_back(v[c].second);
+        }
+      }
+    }
+    // tiles were appended with tbeg = running pair count before each tile
+    {
+      int32_t run = 0;
+      for (size_t t = 0; t < ttab.size(); ++t) { tbeg[t] = run; run += tcnt[t]; }
+    }
+    std::vector<int32_t> tptr(M + 1, 0), trow(psrc.size());
+    for (int32_t t : ptgt) tptr[t + 1]++;
+    for (int64_t t = 0; t < M; ++t) tptr[t + 1] += tptr[t];
+    std::vector<int32_t> fillp(tptr.begin(), tptr.end() - 1);
+    for (size_t q = 0; q < ptgt.size(); ++q) trow[fillp[ptgt[q]]++] = (int32_t)q;
+    ED_TRY(upload(&P->m2l_pair_src, psrc));
+    ED_TRY(upload(&P->m2l_pair_tgt, ptgt));
+    ED_TRY(upload(&P->m2l_tile_tab, ttab));
+    ED_TRY(upload(&P->m2l_tile_beg, tbeg));
+    ED_TRY(upload(&P->m2l_tile_cnt, tcnt));
+    ED_TRY(upload(&P->m2l_tgt_ptr, tptr));
+    ED_TRY(upload(&P->m2l_tgt_row, trow));
+    ED_HIP(hipMalloc((void**)&P->ypair, sizeof(float) * std::max<size_t>(psrc.size(), 1) * ncp));
+    P->n_m2l_tiles = (int64_t)ttab.size();
+  }
+
+  // empty near CSR
+  std::vector<int32_t> nptr(n + 1, 0);
+  ED_TRY(upload(&P->near_ptr, nptr));
+  ED_HIP(hipMalloc((void**)&P->near_col, sizeof(int32_t)));
+  ED_HIP(hipMalloc((void**)&P->near_w, sizeof(float)));
+  ED_HIP(hipMalloc((void**)&P->near_w64, sizeof(double)));
+  P->n_near = 0;
+  return ED_OK;
+}
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" {
+
+const char* ed_last_error(void) { return g_last_error.c_str(); }
+int32_t ed_abi_version(void) { return 1; }
+
+void ed_plan_opts_default(ed_plan_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->leaf_size = 64;
+  o->order = 8;
+  o->ws = 1;
+  o->max_level = 16;
+  o->m2l_precision = 0;
+}
+
+int ed_plan_create(const double* centroids, const double* areas, int64_t n,
+                   const ed_plan_opts* opts, ed_plan** out) {
+  if (!out) return fail(ED_ERR_ARG, "ed_plan_create: out is NULL");
+  *out = nullptr;
+  if (!centroids || !areas) return fail(ED_ERR_ARG, "ed_plan_create: NULL geometry");
+  if (n <= 0 || n >= (1LL << 31)) return fail(ED_ERR_ARG, "ed_plan_create: n out of range");
+  ed_plan_opts o;
+  ed_plan_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.order != 4 && o.order != 6 && o.order != 8 && o.order != 10 && o.order != 12)
+    return fail(ED_ERR_ARG, "ed_plan_create: order must be one of 4, 6, 8, 10, 12");
+  if (o.m2l_precision < -1 || o.m2l_precision > 2)
+    return fail(ED_ERR_ARG, "ed_plan_create: m2l_precision must be -1, 0, 1 or 2");
+  for (int64_t i = 0; i < n; ++i)
+    if (!std::isfinite(areas[i])) return fail(ED_ERR_ARG, "ed_plan_create: non-finite area");
+  int dev = 0;
+  ED_HIP(hipGetDevice(&dev));
+  ed_plan* P = new ed_plan();
+  P->device = dev;
+  P->p = o.order;
+  P->ws = o.ws;
+  P->leaf_size = o.leaf_size;
+  P->max_level = o.max_level;
+  P->m2l_precision = o.m2l_precision;
+  int rc = build_host_tree(centroids, n, o.leaf_size, o.max_level, o.ws, P->host);
+  if (rc == ED_OK) rc = plan_upload(P);
+  if (rc == ED_OK) {
+    const HostTree& H = P->host;
+    std::vector<float4> pts(n);
+    std::vector<float> at(n);
+    std::vector<int32_t> perm(n);
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t o2 = H.perm[i];
+      pts[i] = make_float4((float)(centroids[3 * o2] - H.center[0]),
+                           (float)(centroids[3 * o2 + 1] - H.center[1]),
+                           (float)(centroids[3 * o2 + 2] - H.center[2]), 0.0f);
+      at[i] = (float)areas[o2];
+      perm[i] = (int32_t)o2;
+    }
+    std::vector<double> c64(3 * n), a64(n);
+    for (int64_t i = 0; i < n; ++i) {
+      const int64_t o2 = H.perm[i];
+      for (int a = 0; a < 3; ++a) c64[3 * i + a] = centroids[3 * o2 + a];
+      a64[i] = areas[o2];
+    }
+    rc = upload(&P->pts, pts);
+    if (rc == ED_OK) rc = upload(&P->cent64_t, c64);
+    if (rc == ED_OK) rc = upload(&P->area64_t, a64);
+    if (rc == ED_OK) rc = upload(&P->area_t, at);
+    if (rc == ED_OK) rc = upload(&P->perm, perm);
+    if (rc == ED_OK) {
+      hipError_t e1 = hipMalloc((void**)&P->sig_t, sizeof(float) * n);
+      hipError_t e2 = hipMalloc((void**)&P->selfc_t, sizeof(float) * n);
+      if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(ED_ERR_ALLOC, "ed_plan_create: alloc");
+    }
+  }
+  if (rc != ED_OK) {
+    plan_free(P);
+    delete P;
+    return rc;
+  }
+  *out = P;
+  return ED_OK;
+}
+
+int ed_plan_destroy(ed_plan* P) {
+  if (!P) return ED_OK;
+  plan_free(P);
+  delete P;
+  return ED_OK;
+}
+
+int ed_plan_stats(const ed_plan* P, int64_t* out, int32_t n_out) {
+  if (!P || !out) return fail(ED_ERR_ARG, "ed_plan_stats: NULL argument");
+  int levels_with_m2l = 0;
+  for (int l = 0; l <= P->host.depth; ++l) {
+    bool any = false;
+    for (int64_t t = P->host.level_begin[l]; t < P->host.level_begin[l + 1] && !any; ++t)
+      any = P->host.v_ptr[t + 1] > P->host.v_ptr[t];
+    levels_with_m2l += any;
+  }
+  const int64_t vals[12] = {P->n, P->n_nodes, P->n_leaves, P->host.depth, P->n_u,
+                            P->n_p2p_interactions, P->n_v, P->n_off, P->p, P->ws,
+                            P->n_near, levels_with_m2l};
+  for (int i = 0; i < n_out && i < 12; ++i) out[i] = vals[i];
+  return ED_OK;
+}
+
+int ed_plan_permutation(const ed_plan* P, int64_t* perm_host) {
+  if (!P || !perm_host) return fail(ED_ERR_ARG, "ed_plan_permutation: NULL argument");
+  std::memcpy(perm_host, P->host.perm.data(), sizeof(int64_t) * P->n);
+  return ED_OK;
+}
+
+int ed_fmm_matvec_f32(ed_plan* P, const float* sigma, const float* selfi, float* out,
+                      ed_stream stream) {
+  if (!P || !sigma || !out) return fail(ED_ERR_ARG, "ed_fmm_matvec_f32: NULL argument");
+  return fmm_launch(P, sigma, 0, 0, selfi, out, 0, 0, (hipStream_t)stream);
+}
+
+int ed_fmm_matvec_f64(ed_plan* P, const double* sigma, const double* selfi, double* out,
+                      ed_stream stream) {
+  if (!P || !sigma || !out) return fail(ED_ERR_ARG, "ed_fmm_matvec_f64: NULL argument");
+  return fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, (hipStream_t)stream);
+}
+
+int ed_plan_attach_near(ed_plan* P, const int64_t* pairs, const double* w, int64_t np) {
+  if (!P) return fail(ED_ERR_ARG, "ed_plan_attach_near: NULL plan");
+  if (np < 0 || (np > 0 && (!pairs || !w))) return fail(ED_ERR_ARG, "ed_plan_attach_near: bad pairs");
+  const int64_t n = P->n;
+  std::vector<int64_t> inv(n);
+  for (int64_t i = 0; i < n; ++i) inv[P->host.perm[i]] = i;
+  struct E { int64_t row, col_orig; int32_t col; double w; };
+  std::vector<E> es;
+  es.reserve(np);
+  for (int64_t q = 0; q < np; ++q) {
+    const int64_t i = pairs[2 * q], j = pairs[2 * q + 1];
+    if (i < 0 || i >= n || j < 0 || j >= n) continue;  // core/bem.py:709-715 drops invalid
+    es.push_back({inv[i], j, (int32_t)inv[j], w[q]});
+  }
+  std::stable_sort(es.begin(), es.end(), [](const E& a, const E& b) { return a.row < b.row; });
+  gmres_free(P);  // a captured Arnoldi graph holds the old near arrays
+  std::vector<int32_t> ptr(n + 1, 0), col(es.size());
+  std::vector<float> ww(es.size());
+  std::vector<double> ww64(es.size());
+  for (size_t q = 0; q < es.size(); ++q) {
+    ptr[es[q].row + 1]++;
+    col[q] = es[q].col;
+    ww[q] = (float)es[q].w;
+    ww64[q] = es[q].w;
+  }
+  for (int64_t i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+  (void)hipFree(P->near_ptr);
+  (void)hipFree(P->near_col);
+  (void)hipFree(P->near_w);
+  (void)hipFree(P->near_w64);
+  P->near_ptr = nullptr; P->near_col = nullptr; P->near_w = nullptr; P->near_w64 = nullptr;
+  ED_TRY(upload(&P->near_ptr, ptr));
+  ED_TRY(upload(&P->near_col, col));
+  ED_TRY(upload(&P->near_w, ww));
+  ED_TRY(upload(&P->near_w64, ww64));
+  P->n_near = (int64_t)es.size();
+  return ED_OK;
+}
+
+int ed_host_tree_build(const double* C, int64_t n, int32_t leaf_size, int32_t max_level,
+                       int32_t ws, int64_t* counts, int64_t* perm, int32_t* node_level,
+                       int64_t* node_begin, int64_t* node_end, int64_t* node_parent,
+                       int64_t* v_pairs, int64_t* u_pairs) {
+  if (!C || !counts) return fail(ED_ERR_ARG, "ed_host_tree_build: NULL argument");
+  HostTree t;
+  ED_TRY(build_host_tree(C, n, leaf_size, max_level, ws, t));
+  const int64_t M = (int64_t)t.level.size();
+  counts[0] = M;
+  counts[1] = t.n_leaves();
+  counts[2] = (int64_t)t.v_src.size();
+  counts[3] = (int64_t)t.u_src.size();
+  if (perm) std::memcpy(perm, t.perm.data(), sizeof(int64_t) * n);
+  if (node_level) for (int64_t i = 0; i < M; ++i) node_level[i] = t.level[i];
+  if (node_begin) std::memcpy(node_begin, t.begin.data(), sizeof(int64_t) * M);
+  if (node_end) std::memcpy(node_end, t.end.data(), sizeof(int64_t) * M);
+  if (node_parent) std::memcpy(node_parent, t.parent.data(), sizeof(int64_t) * M);
+  if (v_pairs)
+    for (int64_t tt = 0; tt < M; ++tt)
+      for (int64_t q = t.v_ptr[tt]; q < t.v_ptr[tt + 1]; ++q) {
+        v_pairs[2 * q] = tt;
+        v_pairs[2 * q + 1] = t.v_src[q];
+      }
+  if (u_pairs)
+    for (int64_t tt = 0; tt < M; ++tt)
+      for (int64_t q = t.u_ptr[tt]; q < t.u_ptr[tt + 1]; ++q) {
+        u_pairs[2 * q] = tt;
+        u_pairs[2 * q + 1] = t.u_src[q];
+      }
+  return ED_OK;
+}
+
+int ed_host_operator(int32_t order, int32_t kind, int32_t ox, int32_t oy, int32_t oz,
+                     int32_t octant, double* out) {
+  if (!out || order < 1 || order > 16) return fail(ED_ERR_ARG, "ed_host_operator: bad argument");
+  if (kind == 0) {
+    if (std::max(std::abs(ox), std::max(std::abs(oy), std::abs(oz))) < 2)
+      return fail(ED_ERR_ARG, "ed_host_operator: M2L offset must be well separated");
+    m2l_matrix(order, ox, oy, oz, out);
+  } else if (kind == 1 || kind == 2) {
+    if (octant < 0 || octant > 7) return fail(ED_ERR_ARG, "ed_host_operator: octant");
+    if (kind == 1) m2m_matrix(order, octant, out);
+    else l2l_matrix(order, octant, out);
+  } else {
+    return fail(ED_ERR_ARG, "ed_host_operator: kind must be 0, 1 or 2");
+  }
+  return ED_OK;
+}
+
+int ed_host_morton_keys_f64(const double* pts, int64_t n, int32_t level, const double* bc,
+                            const double* bh, int64_t* keys) {
+  if ((!pts && n > 0) || !keys) return fail(ED_ERR_ARG, "ed_host_morton_keys_f64: NULL");
+  if (level < 1 || level > 21) return fail(ED_ERR_ARG, "level must be in [1, 21]");
+  if ((bc == nullptr) != (bh == nullptr)) return fail(ED_ERR_ARG, "bbox center/half mismatch");
+  morton_keys_host(pts, n, level, bc, bh, keys);
+  return ED_OK;
+}
+
+}  // extern "C"

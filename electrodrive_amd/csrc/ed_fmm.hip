@@ -1,0 +1,512 @@
+// FMM matvec kernels for gfx950 (CDNA4, wave64).
+//
+// Data layout (tree order, Morton-sorted):
+//   pts[i]       float4 (x, y, z, q) relative to the root centre, q = sigma A
+//   M, L         n_nodes x nc (real-packed solid-harmonic coefficients)
+// Stages of one matvec (one launch each, level loops for M2M / L2L):
+//   gather      q, sigma, self term into tree order
+//   p2m         one wave per leaf: lanes stream points, LDS column reduction
+//   m2m         one block per parent: sum over children of T_oct * M_child
+//   m2l         MFMA GEMM per offset table: Y[pair] = T_off * M[src] (fp32
+//               v_mfma_f32_32x32x2f32), then a deterministic per-target sum
+//   l2l         one block per node: L += T_oct * L_parent (level by level)
+//   leaf        one wave per leaf: L2P + P2P over the U list + near CSR +
+//               self term, K_E scaling, scatter to the output order
+// No float atomics anywhere: every sum has a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "ed_plan.h"
+
+namespace ed {
+
+constexpr int kLeafBlock = 64;
+
+// ---------------------------------------------------------------------------
+// Solid-harmonic streaming helpers (m-major recurrences; see ed_operators.cpp)
+// ---------------------------------------------------------------------------
+template <int P>
+__device__ __forceinline__ void p2m_accum(float x, float y, float z, float q, float* Mr) {
+  const float r2 = x * x + y * y + z * z;
+  float pmr = 1.0f, pmi = 0.0f;
+#pragma unroll
+  for (int m = 0; m <= P; ++m) {
+    if (m > 0) {
+      const float f = -0.5f / (float)m;
+      const float nr = f * (x * pmr - y * pmi);
+      const float ni = f * (x * pmi + y * pmr);
+      pmr = nr;
+      pmi = ni;
+    }
+    float ar = 0.0f, ai = 0.0f, br = pmr, bi = pmi;  // R_{n-1}^m, R_n^m
+#pragma unroll
+    for (int n = m; n <= P; ++n) {
+      if (m == 0) {
+        Mr[n * n] += q * br;
+      } else {
+        Mr[n * n + 2 * m - 1] += q * br;
+        Mr[n * n + 2 * m] -= q * bi;
+      }
+      if (n < P) {
+        const float inv = 1.0f / (float)((n + 1) * (n + 1) - m * m);
+        const float c = (float)(2 * n + 1) * z;
+        const float nr = (c * br - r2 * ar) * inv;
+        const float ni = (c * bi - r2 * ai) * inv;
+        ar = br; ai = bi; br = nr; bi = ni;
+      }
+    }
+  }
+}
+
+template <int P>
+__device__ __forceinline__ float l2p_eval(float x, float y, float z, const float* __restrict__ Lc) {
+  const float r2 = x * x + y * y + z * z;
+  float pmr = 1.0f, pmi = 0.0f;
+  float acc0 = 0.0f, acc1 = 0.0f;
+#pragma unroll
+  for (int m = 0; m <= P; ++m) {
+    if (m > 0) {
+      const float f = -0.5f / (float)m;
+      const float nr = f * (x * pmr - y * pmi);
+      const float ni = f * (x * pmi + y * pmr);
+      pmr = nr;
+      pmi = ni;
+    }
+    float ar = 0.0f, ai = 0.0f, br = pmr, bi = pmi;
+#pragma unroll
+    for (int n = m; n <= P; ++n) {
+      if (m == 0) {
+        acc0 += Lc[n * n] * br;
+      } else {
+        acc1 += Lc[n * n + 2 * m - 1] * br - Lc[n * n + 2 * m] * bi;
+      }
+      if (n < P) {
+        const float inv = 1.0f / (float)((n + 1) * (n + 1) - m * m);
+        const float c = (float)(2 * n + 1) * z;
+        const float nr = (c * br - r2 * ar) * inv;
+        const float ni = (c * bi - r2 * ai) * inv;
+        ar = br; ai = bi; br = nr; bi = ni;
+      }
+    }
+  }
+  return acc0 + 2.0f * acc1;
+}
+
+// ---------------------------------------------------------------------------
+// gather: tree-order q, sigma, self term
+// ---------------------------------------------------------------------------
+template <typename TS>
+__global__ void k_gather(int n, const int32_t* __restrict__ perm, const TS* __restrict__ sigma,
+                         int sigma_tree, const TS* __restrict__ selfi, float4* __restrict__ pts,
+                         const float* __restrict__ area_t, float* __restrict__ sig_t,
+                         float* __restrict__ selfc_t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int o = perm[i];
+  const float s = (float)sigma[sigma_tree ? i : o];
+  sig_t[i] = s;
+  pts[i].w = s * area_t[i];
+  selfc_t[i] = selfi ? (float)selfi[o] : 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// P2M: one 64-lane block per leaf
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ leaves,
+                                                    const int2* __restrict__ rng,
+                                                    const float4* __restrict__ ctr,
+                                                    const float4* __restrict__ pts,
+                                                    float* __restrict__ M) {
+  constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float red[kLeafBlock][NC + 1];
+  const int node = leaves[blockIdx.x];
+  const int2 r = rng[node];
+  const float4 c = ctr[node];
+  float acc[NC];
+#pragma unroll
+  for (int k = 0; k < NC; ++k) acc[k] = 0.0f;
+  for (int i = r.x + threadIdx.x; i < r.y; i += kLeafBlock) {
+    const float4 pq = pts[i];
+    p2m_accum<P>((pq.x - c.x) * c.w, (pq.y - c.y) * c.w, (pq.z - c.z) * c.w, pq.w, acc);
+  }
+#pragma unroll
+  for (int k = 0; k < NC; ++k) red[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  for (int k = threadIdx.x; k < NC; k += kLeafBlock) {
+    float s = 0.0f;
+    for (int t = 0; t < kLeafBlock; ++t) s += red[t][k];
+    M[(size_t)node * NC + k] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// M2M: one block per parent; M_parent = sum_c T_m2m[oct(c)] M_c
+// T stored column-major: T[oct][j][k] (input j, output k)
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(128) void k_m2m(const int32_t* __restrict__ parents,
+                                             const int32_t* __restrict__ first_child,
+                                             const int32_t* __restrict__ n_child,
+                                             const int32_t* __restrict__ octant,
+                                             const float* __restrict__ T,
+                                             float* __restrict__ M) {
+  constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float mc[8][NC];
+  __shared__ int oc[8];
+  const int par = parents[blockIdx.x];
+  const int c0 = first_child[par], nch = n_child[par];
+  for (int t = threadIdx.x; t < nch * NC; t += blockDim.x) {
+    const int c = t / NC, j = t - c * NC;
+    mc[c][j] = M[(size_t)(c0 + c) * NC + j];
+  }
+  if (threadIdx.x < nch) oc[threadIdx.x] = octant[c0 + threadIdx.x];
+  __syncthreads();
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
+    float s = 0.0f;
+    for (int c = 0; c < nch; ++c) {
+      const float* Tc = T + (size_t)oc[c] * NC * NC + k;
+      for (int j = 0; j < NC; ++j) s += Tc[(size_t)j * NC] * mc[c][j];
+    }
+    M[(size_t)par * NC + k] = s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// M2L (VALU, CSR by target): L[t] = inv_s * sum_{s in V(t)} T_off M[s].
+// Kept as the plain reference path of the GEMM form below.
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(128) void k_m2l_valu(int node0, const int32_t* __restrict__ v_ptr,
+                                                  const int32_t* __restrict__ v_src,
+                                                  const int32_t* __restrict__ v_tab,
+                                                  const float4* __restrict__ ctr,
+                                                  const float* __restrict__ T,
+                                                  const float* __restrict__ M,
+                                                  float* __restrict__ L) {
+  constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float ms[NC];
+  const int t = node0 + blockIdx.x;
+  const int b = v_ptr[t], e = v_ptr[t + 1];
+  float acc0 = 0.0f, acc1 = 0.0f;
+  const int k0 = threadIdx.x, k1 = threadIdx.x + 128;
+  for (int q = b; q < e; ++q) {
+    const int s = v_src[q];
+    __syncthreads();
+    for (int j = threadIdx.x; j < NC; j += blockDim.x) ms[j] = M[(size_t)s * NC + j];
+    __syncthreads();
+    const float* Tq = T + (size_t)v_tab[q] * NC * NC;
+    if (k0 < NC)
+      for (int j = 0; j < NC; ++j) acc0 += Tq[(size_t)j * NC + k0] * ms[j];
+    if (k1 < NC)
+      for (int j = 0; j < NC; ++j) acc1 += Tq[(size_t)j * NC + k1] * ms[j];
+  }
+  const float is = ctr[t].w;
+  if (k0 < NC) L[(size_t)t * NC + k0] = acc0 * is;
+  if (k1 < NC) L[(size_t)t * NC + k1] = acc1 * is;
+}
+
+// ---------------------------------------------------------------------------
+// M2L (MFMA): one block of 4 waves per tile of <= 128 pairs sharing one offset
+// table.  Y[pair][0..NCP) = T_tab (NCP x KP, row-major, zero padded) * M[src].
+// v_mfma_f32_32x32x2f32 lane layout: A[i][k] from lane (i + 32 k),
+// B[k][j] from lane (j + 32 k), D[i][j] in lane j + 32*((i/4)%2), register
+// 4*(i/8) + i%4.
+// ---------------------------------------------------------------------------
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+template <int P>
+struct M2LShape {
+  static constexpr int NC = (P + 1) * (P + 1);
+  static constexpr int NCP = (NC + 31) / 32 * 32;  // output rows, padded
+  static constexpr int KP = (NC + 1) / 2 * 2;      // inner dim, padded to K=2
+  static constexpr int KS = KP + 1;                // LDS row stride
+  static constexpr int COLS = (P <= 8) ? 128 : 64;  // pairs per tile
+  static constexpr int THREADS = COLS * 2;          // one wave per 32 columns
+  static constexpr bool ENABLED = P <= 10;          // LDS budget (160 KiB)
+};
+
+template <int P>
+__global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t* __restrict__ tile_tab,
+                                                  const int32_t* __restrict__ tile_beg,
+                                                  const int32_t* __restrict__ tile_cnt,
+                                                  const int32_t* __restrict__ pair_src,
+                                                  const float* __restrict__ Tm,
+                                                  const float* __restrict__ M,
+                                                  float* __restrict__ Y) {
+  using S = M2LShape<P>;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
+  constexpr int NT = S::THREADS;
+  __shared__ float ta[NCP * KS];
+  __shared__ float bm[COLS * KS];
+  const int tile = blockIdx.x;
+  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
+  const float* Tt = Tm + (size_t)tab * NCP * KP;
+  for (int t = threadIdx.x; t < NCP * KP; t += NT) {
+    const int i = t / KP, k = t - i * KP;
+    ta[i * KS + k] = Tt[t];
+  }
+  for (int t = threadIdx.x; t < COLS * KP; t += NT) {
+    const int c = t / KP, k = t - c * KP;
+    float v = 0.0f;
+    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k];
+    bm[c * KS + k] = v;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col0 = wave * 32;
+  if (col0 >= cnt) return;
+  const int li = lane & 31, lk = lane >> 5;
+  constexpr int NRB = NCP / 32;
+  f32x16 acc[NRB];
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
+  const float* brow = bm + (col0 + li) * KS + lk;
+#pragma unroll 4
+  for (int s = 0; s < KP / 2; ++s) {
+    const float b = brow[2 * s];
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+      const float a = ta[(rb * 32 + li) * KS + 2 * s + lk];
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[rb], 0, 0, 0);
+    }
+  }
+  const int col = col0 + li;
+  if (col < cnt) {
+    float* yrow = Y + (size_t)(beg + col) * NCP;
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = rb * 32 + q * 8 + lk * 4;
+        float4 v = make_float4(acc[rb][4 * q + 0], acc[rb][4 * q + 1], acc[rb][4 * q + 2],
+                               acc[rb][4 * q + 3]);
+        *reinterpret_cast<float4*>(yrow + row) = v;
+      }
+  }
+}
+
+// Deterministic per-target sum of the GEMM rows: L[t] = inv_s * sum Y[row].
+template <int P>
+__global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __restrict__ tptr,
+                                                    const int32_t* __restrict__ trow,
+                                                    const float4* __restrict__ ctr,
+                                                    const float* __restrict__ Y,
+                                                    float* __restrict__ L) {
+  using S = M2LShape<P>;
+  constexpr int NC = S::NC, NCP = S::NCP;
+  const int t = node0 + blockIdx.x;
+  const int b = tptr[t], e = tptr[t + 1];
+  const float is = ctr[t].w;
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
+    float s = 0.0f;
+    for (int q = b; q < e; ++q) s += Y[(size_t)trow[q] * NCP + k];
+    L[(size_t)t * NC + k] = s * is;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// L2L: one block per node of a level: L[c] += T_l2l[oct(c)] L[parent(c)]
+// ---------------------------------------------------------------------------
+template <int P>
+__global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restrict__ parent,
+                                             const int32_t* __restrict__ octant,
+                                             const float* __restrict__ T,
+                                             float* __restrict__ L) {
+  constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float lp[NC];
+  const int c = node0 + blockIdx.x;
+  const int par = parent[c];
+  for (int j = threadIdx.x; j < NC; j += blockDim.x) lp[j] = L[(size_t)par * NC + j];
+  __syncthreads();
+  const float* Tc = T + (size_t)octant[c] * NC * NC;
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
+    float s = 0.0f;
+    for (int j = 0; j < NC; ++j) s += Tc[(size_t)j * NC + k] * lp[j];
+    L[(size_t)c * NC + k] += s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Leaf evaluation: L2P + P2P (U list) + near CSR + self, scatter.
+// ---------------------------------------------------------------------------
+template <int P, typename TO>
+__global__ __launch_bounds__(kLeafBlock) void k_leaf(
+    const int32_t* __restrict__ leaves, const int2* __restrict__ rng,
+    const float4* __restrict__ ctr, const float4* __restrict__ pts,
+    const float* __restrict__ L, const int32_t* __restrict__ u_ptr,
+    const int32_t* __restrict__ u_src, const int32_t* __restrict__ near_ptr,
+    const int32_t* __restrict__ near_col, const float* __restrict__ near_w,
+    const float* __restrict__ sig_t, const float* __restrict__ selfc_t,
+    const int32_t* __restrict__ perm, int out_tree, float ke, TO* __restrict__ out) {
+  constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float lc[NC];
+  __shared__ float4 src[kLeafBlock];
+  const int li = blockIdx.x;
+  const int node = leaves[li];
+  const int2 r = rng[node];
+  const float4 c = ctr[node];
+  for (int j = threadIdx.x; j < NC; j += kLeafBlock) lc[j] = L[(size_t)node * NC + j];
+  const int ub = u_ptr[li], ue = u_ptr[li + 1];
+  for (int t0 = r.x; t0 < r.y; t0 += kLeafBlock) {
+    const int i = t0 + threadIdx.x;
+    const bool active = i < r.y;
+    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active) xi = pts[i];
+    __syncthreads();
+    float far = 0.0f;
+    if (active) far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w, lc);
+    float nearf = 0.0f;
+    for (int u = ub; u < ue; ++u) {
+      const int2 sr = rng[u_src[u]];
+      for (int s0 = sr.x; s0 < sr.y; s0 += kLeafBlock) {
+        const int ns = min(kLeafBlock, sr.y - s0);
+        __syncthreads();
+        if (threadIdx.x < ns) src[threadIdx.x] = pts[s0 + threadIdx.x];
+        __syncthreads();
+        const bool selfblk = (s0 <= i) && (i < s0 + ns);
+        if (!selfblk) {
+          for (int j = 0; j < ns; ++j) {
+            const float4 sj = src[j];
+            const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
+            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
+            nearf += sj.w * __builtin_amdgcn_rsqf(r2);
+          }
+        } else {
+          const int me = i - s0;
+          for (int j = 0; j < ns; ++j) {
+            const float4 sj = src[j];
+            const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
+            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
+            const float v = sj.w * __builtin_amdgcn_rsqf(r2);
+            nearf += (j == me) ? 0.0f : v;
+          }
+        }
+      }
+    }
+    if (active) {
+      float v = ke * (far + nearf) + selfc_t[i] * xi.w;
+      const int nb = near_ptr[i], ne = near_ptr[i + 1];
+      float corr = 0.0f;
+      for (int q = nb; q < ne; ++q) corr += near_w[q] * sig_t[near_col[q]];
+      v += corr;
+      if (!isfinite(v)) v = 0.0f;
+      out[out_tree ? i : perm[i]] = (TO)v;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launcher
+// ---------------------------------------------------------------------------
+#define ED_LAUNCH_CHECK(msg)                                                   \
+  do {                                                                         \
+    hipError_t e_ = hipGetLastError();                                         \
+    if (e_ != hipSuccess) return fail(ED_ERR_LAUNCH, std::string(msg) + ": " + \
+                                                         hipGetErrorString(e_)); \
+  } while (0)
+
+template <int P>
+static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
+                   void* out, int out64, int out_tree, hipStream_t st) {
+  constexpr int NC = (P + 1) * (P + 1);
+  const int n = (int)Pn->n;
+  const int gb = (n + 255) / 256;
+  if (sig64)
+    hipLaunchKernelGGL(k_gather<double>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
+                       (const double*)sigma, sig_tree, (const double*)selfi, Pn->pts, Pn->area_t,
+                       Pn->sig_t, Pn->selfc_t);
+  else
+    hipLaunchKernelGGL(k_gather<float>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
+                       (const float*)sigma, sig_tree, (const float*)selfi, Pn->pts, Pn->area_t,
+                       Pn->sig_t, Pn->selfc_t);
+  ED_LAUNCH_CHECK("gather");
+  const int nleaf = (int)Pn->n_leaves;
+  const auto& H = Pn->host;
+  const int depth = H.depth;
+  if (depth >= 2) {
+    hipLaunchKernelGGL(k_p2m<P>, dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
+                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->M);
+    ED_LAUNCH_CHECK("p2m");
+    // upward pass: parents at level l (children at l+1), deepest first
+    for (int l = depth - 1; l >= 2; --l) {
+      const int64_t pb = Pn->parents_begin[l], pe = Pn->parents_begin[l + 1];
+      if (pe > pb) {
+        hipLaunchKernelGGL(k_m2m<P>, dim3((unsigned)(pe - pb)), dim3(128), 0, st,
+                           Pn->parents_list + pb, Pn->first_child, Pn->n_child,
+                           Pn->node_octant, Pn->t_m2m, Pn->M);
+        ED_LAUNCH_CHECK("m2m");
+      }
+    }
+    // M2L for all nodes at levels >= 2
+    const int node0 = (int)H.level_begin[2];
+    const int nm = (int)(Pn->n_nodes - node0);
+    bool use_mfma = Pn->m2l_precision >= 0 && Pn->n_m2l_tiles > 0;
+    if constexpr (!M2LShape<P>::ENABLED) use_mfma = false;
+    if (!use_mfma) {
+      hipLaunchKernelGGL(k_m2l_valu<P>, dim3(nm), dim3(128), 0, st, node0, Pn->v_ptr,
+                         Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Pn->M, Pn->L);
+      ED_LAUNCH_CHECK("m2l_valu");
+    } else if constexpr (M2LShape<P>::ENABLED) {
+      hipLaunchKernelGGL(k_m2l_mfma<P>, dim3((unsigned)Pn->n_m2l_tiles),
+                         dim3(M2LShape<P>::THREADS), 0, st,
+                         Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
+                         Pn->m2l_pair_src, Pn->t_m2l_mfma, Pn->M, Pn->ypair);
+      ED_LAUNCH_CHECK("m2l_mfma");
+      hipLaunchKernelGGL(k_m2l_reduce<P>, dim3(nm), dim3(128), 0, st, node0, Pn->m2l_tgt_ptr,
+                         Pn->m2l_tgt_row, Pn->node_ctr, Pn->ypair, Pn->L);
+      ED_LAUNCH_CHECK("m2l_reduce");
+    }
+    // downward pass: levels 3..depth (levels 0..2 have no inherited local)
+    for (int l = 3; l <= depth; ++l) {
+      const int64_t b = H.level_begin[l], e = H.level_begin[l + 1];
+      hipLaunchKernelGGL(k_l2l<P>, dim3((unsigned)(e - b)), dim3(128), 0, st, (int)b,
+                         Pn->node_parent, Pn->node_octant, Pn->t_l2l, Pn->L);
+      ED_LAUNCH_CHECK("l2l");
+    }
+  }
+  const float ke = (float)kKE;
+  if (out64)
+    hipLaunchKernelGGL((k_leaf<P, double>), dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
+                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->L, Pn->u_ptr, Pn->u_src,
+                       Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm,
+                       out_tree, ke, (double*)out);
+  else
+    hipLaunchKernelGGL((k_leaf<P, float>), dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
+                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->L, Pn->u_ptr, Pn->u_src,
+                       Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm,
+                       out_tree, ke, (float*)out);
+  ED_LAUNCH_CHECK("leaf");
+  (void)NC;
+  return ED_OK;
+}
+
+int fmm_launch(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
+               void* out, int out64, int out_tree, hipStream_t st) {
+  switch (Pn->p) {
+    case 4: return fmm_run<4>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
+    case 6: return fmm_run<6>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
+    case 8: return fmm_run<8>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
+    case 10: return fmm_run<10>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
+    case 12: return fmm_run<12>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
+    default: return fail(ED_ERR_ARG, "expansion order must be one of 4, 6, 8, 10, 12");
+  }
+}
+
+int m2l_shape(int p, int* ncp, int* kp, int* cols, int* enabled) {
+  switch (p) {
+#define ED_SHAPE(PP)                                                          \
+  case PP:                                                                    \
+    *ncp = M2LShape<PP>::NCP; *kp = M2LShape<PP>::KP;                         \
+    *cols = M2LShape<PP>::COLS; *enabled = M2LShape<PP>::ENABLED ? 1 : 0;     \
+    return ED_OK;
+    ED_SHAPE(4) ED_SHAPE(6) ED_SHAPE(8) ED_SHAPE(10) ED_SHAPE(12)
+#undef ED_SHAPE
+    default: return fail(ED_ERR_ARG, "unsupported order");
+  }
+}
+
+}  // namespace ed

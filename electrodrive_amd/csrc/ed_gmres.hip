@@ -1,0 +1,607 @@
+// Restarted, left-(Jacobi-)preconditioned GMRES in FP64 on the device, with the
+// plan's FMM operator (FP32 arithmetic) as A and, optionally, an outer FP64
+// iterative refinement whose residual uses the dense FP64 operator plus the
+// FP64 near correction (the accurate operator).
+//
+// Semantics follow core/bem_solver.py:125-920 (tol_abs = tol * ||b - A x0||,
+// convergence on the TRUE residual at cycle end, codes 0/1/2/3/5/6), with two
+// deliberate differences: orthogonalisation is classical Gram-Schmidt applied
+// twice (CGS2, fused multi-dot kernels with fixed-order reductions) instead of
+// MGS, and a cycle ends as soon as the preconditioned residual estimate meets
+// its tolerance instead of always running `restart` steps (bem_solver.py:703-707).
+// All vectors live in tree order; b / x / diag are permuted on entry and exit.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "ed_plan.h"
+
+namespace ed {
+
+constexpr int kRedBlocks = 240;  // partial-sum blocks for multi-dots
+constexpr int kRedThreads = 256;
+
+struct GmresWS {
+  int64_t n = 0;
+  int m = 0;
+  double *V = nullptr, *w = nullptr, *vin = nullptr, *x = nullptr, *b = nullptr, *r = nullptr,
+         *t = nullptr, *dinv = nullptr, *selfi_t = nullptr, *d = nullptr, *rhs = nullptr;
+  double *H = nullptr, *g = nullptr, *cs = nullptr, *sn = nullptr, *h = nullptr, *h2 = nullptr;
+  double* partial = nullptr;  // kRedBlocks x (m + 2)
+  double* scal = nullptr;     // device scalars
+  double* hscal = nullptr;    // pinned host mirror
+  int* dj = nullptr;          // device Arnoldi index
+  hipStream_t stream = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  const double* graph_selfi = nullptr;
+};
+
+// device scalar slots
+enum { S_EST = 0, S_BETA = 1, S_HNEXT = 2, S_NRM_R = 3, S_NRM_Z = 4, S_BREAK = 5, S_NSCAL = 8 };
+
+void gmres_free(ed_plan* P) {
+  GmresWS* W = (GmresWS*)P->gmres_ws;
+  if (!W) return;
+  if (W->gexec) (void)hipGraphExecDestroy(W->gexec);
+  if (W->graph) (void)hipGraphDestroy(W->graph);
+  void* ptrs[] = {W->V, W->w, W->vin, W->x, W->b, W->r, W->t, W->dinv, W->selfi_t, W->d, W->rhs,
+                  W->H, W->g, W->cs, W->sn, W->h, W->h2, W->partial, W->scal, W->dj};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  if (W->hscal) (void)hipHostFree(W->hscal);
+  if (W->stream) (void)hipStreamDestroy(W->stream);
+  delete W;
+  P->gmres_ws = nullptr;
+}
+
+#define ED_G(call)                                                                       \
+  do {                                                                                   \
+    hipError_t e_ = (call);                                                              \
+    if (e_ != hipSuccess)                                                                \
+      return fail(e_ == hipErrorOutOfMemory ? ED_ERR_ALLOC : ED_ERR_DEVICE,              \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                    \
+  } while (0)
+#define ED_T(x)                     \
+  do {                              \
+    int rc_ = (x);                  \
+    if (rc_ != ED_OK) return rc_;   \
+  } while (0)
+#define ED_KCHECK(msg)                                                                  \
+  do {                                                                                  \
+    hipError_t e_ = hipGetLastError();                                                  \
+    if (e_ != hipSuccess)                                                               \
+      return fail(ED_ERR_LAUNCH, std::string(msg) + ": " + hipGetErrorString(e_));      \
+  } while (0)
+
+static int ws_get(ed_plan* P, int m, GmresWS** out) {
+  GmresWS* W = (GmresWS*)P->gmres_ws;
+  if (W && W->m == m && W->n == P->n) { *out = W; return ED_OK; }
+  gmres_free(P);
+  W = new GmresWS();
+  P->gmres_ws = W;
+  W->n = P->n;
+  W->m = m;
+  const size_t n = (size_t)P->n;
+  ED_G(hipMalloc((void**)&W->V, sizeof(double) * n * (m + 1)));
+  double** vecs[] = {&W->w, &W->vin, &W->x, &W->b, &W->r, &W->t, &W->dinv, &W->selfi_t, &W->d,
+                     &W->rhs};
+  for (double** v : vecs) ED_G(hipMalloc((void**)v, sizeof(double) * n));
+  ED_G(hipMalloc((void**)&W->H, sizeof(double) * (m + 1) * m));
+  ED_G(hipMalloc((void**)&W->g, sizeof(double) * (m + 1)));
+  ED_G(hipMalloc((void**)&W->cs, sizeof(double) * m));
+  ED_G(hipMalloc((void**)&W->sn, sizeof(double) * m));
+  ED_G(hipMalloc((void**)&W->h, sizeof(double) * (m + 2)));
+  ED_G(hipMalloc((void**)&W->h2, sizeof(double) * (m + 2)));
+  ED_G(hipMalloc((void**)&W->partial, sizeof(double) * kRedBlocks * (m + 2)));
+  ED_G(hipMalloc((void**)&W->scal, sizeof(double) * S_NSCAL));
+  ED_G(hipMalloc((void**)&W->dj, sizeof(int)));
+  ED_G(hipHostMalloc((void**)&W->hscal, sizeof(double) * S_NSCAL, hipHostMallocDefault));
+  ED_G(hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
+  *out = W;
+  return ED_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void k_perm_in(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                          double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+__global__ void k_perm_out(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                           double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+// Jacobi inverse: 1/d with non-finite or |d| < 1e-18 entries -> 1
+// (core/bem_solver.py:67-78); identity when diag is NULL.
+__global__ void k_dinv(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ diag,
+                       double* __restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = diag ? diag[perm[i]] : 1.0;
+  if (!isfinite(d) || fabs(d) < 1e-18) d = 1.0;
+  dinv[i] = 1.0 / d;
+}
+
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+  return s;  // valid in thread 0
+}
+
+// r = b - t, partial norms of r and of dinv .* r (slots 0, 1)
+__global__ __launch_bounds__(kRedThreads) void k_resid(int64_t n, const double* __restrict__ b,
+                                                       const double* __restrict__ t,
+                                                       const double* __restrict__ dinv,
+                                                       double* __restrict__ r,
+                                                       double* __restrict__ z,
+                                                       double* __restrict__ partial) {
+  __shared__ double sh[4];
+  double s0 = 0.0, s1 = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double ri = b[i] - (t ? t[i] : 0.0);
+    if (r) r[i] = ri;
+    const double zi = dinv[i] * ri;
+    if (z) z[i] = zi;
+    s0 += ri * ri;
+    s1 += zi * zi;
+  }
+  double a = block_sum(s0, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+  a = block_sum(s1, sh);
+  if (threadIdx.x == 0) partial[kRedBlocks + blockIdx.x] = a;
+}
+
+// Fixed-order reduction of `cnt` partial vectors; writes sqrt into scal slots.
+__global__ void k_finish_norms(const double* __restrict__ partial, int cnt, double* __restrict__ scal,
+                               int slot0) {
+  __shared__ double sh[4];
+  for (int k = 0; k < cnt; ++k) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[k * kRedBlocks + b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) scal[slot0 + k] = sqrt(v);
+  }
+}
+
+// V0 = z / beta; g = [beta, 0...]; j = 0
+__global__ void k_start(int64_t n, int m, const double* __restrict__ z, double* __restrict__ V,
+                        double* __restrict__ g, const double* __restrict__ scal, int* dj) {
+  const double beta = scal[S_NRM_Z];
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    V[i] = z[i] / beta;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *dj = 0;
+  }
+  if (blockIdx.x == 0)
+    for (int k = threadIdx.x; k <= m; k += blockDim.x) g[k] = (k == 0) ? beta : 0.0;
+}
+
+__global__ void k_copy_vj(int64_t n, const double* __restrict__ V, const int* __restrict__ dj,
+                          double* __restrict__ vin) {
+  const int j = *dj;
+  const double* vj = V + (size_t)j * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    vin[i] = vj[i];
+}
+
+// w = dinv .* w; partial dots V_k . w for k <= j
+__global__ __launch_bounds__(kRedThreads) void k_dots1(int64_t n, const double* __restrict__ dinv,
+                                                       double* __restrict__ w,
+                                                       const double* __restrict__ V,
+                                                       const int* __restrict__ dj,
+                                                       double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int j = *dj;
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) w[i] *= dinv[i];
+  __syncthreads();
+  for (int k = 0; k <= j; ++k) {
+    const double* vk = V + (size_t)k * n;
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += vk[i] * w[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[(size_t)k * kRedBlocks + blockIdx.x] = s;
+  }
+}
+
+// h[k] (=, or += when accumulate) = sum_b partial[k][b], k <= j (+1 extra row
+// when `extra` is set: the norm slot at index j+1)
+__global__ void k_finish_dots(const double* __restrict__ partial, const int* __restrict__ dj,
+                              double* __restrict__ h, int accumulate) {
+  __shared__ double sh[4];
+  const int j = *dj;
+  for (int k = 0; k <= j; ++k) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[(size_t)k * kRedBlocks + b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) h[k] = accumulate ? h[k] + v : v;
+  }
+}
+
+// w -= sum_k c[k] V_k  (c = h, or c = h2 = h - h_prev stored by caller); then
+// partial dots of V_k . w (second CGS pass) or of w . w (norm pass).
+__global__ __launch_bounds__(kRedThreads) void k_update(int64_t n, double* __restrict__ w,
+                                                        const double* __restrict__ V,
+                                                        const int* __restrict__ dj,
+                                                        const double* __restrict__ c,
+                                                        int norm_pass,
+                                                        double* __restrict__ partial) {
+  __shared__ double sh[4];
+  __shared__ double cc[256];
+  const int j = *dj;
+  for (int k = threadIdx.x; k <= j; k += blockDim.x) cc[k] = c[k];
+  __syncthreads();
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  double nrm = 0.0;
+  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+    double wi = w[i];
+    for (int k = 0; k <= j; ++k) wi -= cc[k] * V[(size_t)k * n + i];
+    w[i] = wi;
+    nrm += wi * wi;
+  }
+  __syncthreads();
+  if (norm_pass) {
+    nrm = block_sum(nrm, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = nrm;
+    return;
+  }
+  for (int k = 0; k <= j; ++k) {
+    const double* vk = V + (size_t)k * n;
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += vk[i] * w[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[(size_t)k * kRedBlocks + blockIdx.x] = s;
+  }
+}
+
+// After CGS2: H[:, j] = h1 + h2, H[j+1, j] = ||w||; apply Givens rotations
+// (core/bem_solver.py:590-616); est = |g[j+1]|; j++.
+__global__ void k_givens(int m, const double* __restrict__ partial, double* __restrict__ h1,
+                         const double* __restrict__ h2, double* __restrict__ H,
+                         double* __restrict__ g, double* __restrict__ cs, double* __restrict__ sn,
+                         double* __restrict__ scal, int* dj) {
+  __shared__ double sh[4];
+  double v = 0.0;
+  for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[b];
+  v = block_sum(v, sh);
+  if (threadIdx.x != 0) return;
+  const int j = *dj;
+  const int ld = m + 1;
+  double* col = H + (size_t)j * ld;
+  for (int k = 0; k <= j; ++k) col[k] = h1[k] + h2[k];
+  const double hn = sqrt(v);
+  col[j + 1] = hn;
+  scal[S_HNEXT] = hn;
+  if (!isfinite(hn) || hn <= 0.0) scal[S_BREAK] = 1.0;
+  for (int k = 0; k < j; ++k) {
+    const double t = cs[k] * col[k] + sn[k] * col[k + 1];
+    col[k + 1] = -sn[k] * col[k] + cs[k] * col[k + 1];
+    col[k] = t;
+  }
+  const double den = sqrt(col[j] * col[j] + col[j + 1] * col[j + 1]);
+  double c = 1.0, s = 0.0;
+  if (den >= 1e-14 && isfinite(den)) { c = col[j] / den; s = col[j + 1] / den; }
+  cs[j] = c;
+  sn[j] = s;
+  col[j] = c * col[j] + s * col[j + 1];
+  col[j + 1] = 0.0;
+  const double gj = g[j];
+  g[j] = c * gj;
+  g[j + 1] = -s * gj;
+  scal[S_EST] = fabs(g[j + 1]);
+  *dj = j + 1;
+}
+
+// V_{j} = w / hnext  (j already incremented)
+__global__ void k_scale_next(int64_t n, const double* __restrict__ w, double* __restrict__ V,
+                             const int* __restrict__ dj, const double* __restrict__ scal) {
+  const int j = *dj;
+  const double hn = scal[S_HNEXT];
+  const double inv = (hn > 0.0 && isfinite(hn)) ? 1.0 / hn : 0.0;
+  double* vj = V + (size_t)j * n;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    vj[i] = w[i] * inv;
+}
+
+// y = R^{-1} g (back substitution, m_use = *dj), stored in h
+__global__ void k_backsolve(int m, const double* __restrict__ H, const double* __restrict__ g,
+                            const int* __restrict__ dj, double* __restrict__ y) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const int mu = *dj;
+  const int ld = m + 1;
+  for (int i = mu - 1; i >= 0; --i) {
+    double s = g[i];
+    for (int k = i + 1; k < mu; ++k) s -= H[(size_t)k * ld + i] * y[k];
+    const double d = H[(size_t)i * ld + i];
+    y[i] = d != 0.0 ? s / d : 0.0;
+  }
+}
+
+// x += sum_{k < m_use} y_k V_k
+__global__ void k_xupdate(int64_t n, double* __restrict__ x, const double* __restrict__ V,
+                          const double* __restrict__ y, const int* __restrict__ dj) {
+  __shared__ double yy[256];
+  const int mu = *dj;
+  for (int k = threadIdx.x; k < mu; k += blockDim.x) yy[k] = y[k];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double s = x[i];
+    for (int k = 0; k < mu; ++k) s += yy[k] * V[(size_t)k * n + i];
+    x[i] = s;
+  }
+}
+
+__global__ void k_axpy(int64_t n, double a, const double* __restrict__ xx, double* __restrict__ yy) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) yy[i] += a * xx[i];
+}
+
+// out += sum_j w64 x_j over the tree-order near CSR (FP64)
+__global__ void k_near_csr(int64_t n, const int32_t* __restrict__ ptr, const int32_t* __restrict__ col,
+                           const double* __restrict__ w, const double* __restrict__ x,
+                           double* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int q = ptr[i]; q < ptr[i + 1]; ++q) s += w[q] * x[col[q]];
+  out[i] += s;
+}
+
+// ---------------------------------------------------------------------------
+// host driver
+// ---------------------------------------------------------------------------
+int dense_launch_f64(const double* C, const double* A, const double* sigma, const double* selfi,
+                     double* out, int64_t n, int64_t nrhs, hipStream_t st);
+
+struct Solver {
+  ed_plan* P;
+  GmresWS* W;
+  const ed_gmres_opts* o;
+  const double* selfi_orig;
+  int64_t n;
+  unsigned gb;
+  int total_iters = 0, cycles = 0, first_below = -1;
+
+  int approx_op(const double* in_tree, double* out_tree) {
+    return fmm_launch(P, in_tree, 1, 1, selfi_orig, out_tree, 1, 1, W->stream);
+  }
+  int accurate_op(const double* in_tree, double* out_tree) {
+    ED_T(dense_launch_f64(P->cent64_t, P->area64_t, in_tree, W->selfi_t, out_tree, n, 1, W->stream));
+    if (P->n_near > 0) {
+      hipLaunchKernelGGL(k_near_csr, dim3(gb), dim3(256), 0, W->stream, n, P->near_ptr,
+                         P->near_col, P->near_w64, in_tree, out_tree);
+      ED_KCHECK("near csr");
+    }
+    return ED_OK;
+  }
+  // norms of (b - t) and dinv (b - t); r, z optional outputs
+  int resid_norms(const double* b, const double* t, double* r, double* z, double* nr, double* nz) {
+    hipLaunchKernelGGL(k_resid, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, b, t,
+                       W->dinv, r, z, W->partial);
+    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, W->stream, W->partial, 2, W->scal,
+                       S_NRM_R);
+    ED_KCHECK("resid");
+    ED_G(hipMemcpyAsync(W->hscal, W->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, W->stream));
+    ED_G(hipStreamSynchronize(W->stream));
+    *nr = W->hscal[S_NRM_R];
+    *nz = W->hscal[S_NRM_Z];
+    return ED_OK;
+  }
+  int arnoldi_step() {
+    const int m = W->m;
+    hipLaunchKernelGGL(k_copy_vj, dim3(gb), dim3(256), 0, W->stream, n, W->V, W->dj, W->vin);
+    ED_T(approx_op(W->vin, W->w));
+    hipLaunchKernelGGL(k_dots1, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->dinv,
+                       W->w, W->V, W->dj, W->partial);
+    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(256), 0, W->stream, W->partial, W->dj, W->h, 0);
+    hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->w, W->V,
+                       W->dj, W->h, 0, W->partial);
+    // second Gram-Schmidt pass: coefficients h2
+    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(256), 0, W->stream, W->partial, W->dj, W->h2, 0);
+    hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->w, W->V,
+                       W->dj, W->h2, 1, W->partial);
+    hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, W->stream, m, W->partial, W->h, W->h2, W->H,
+                       W->g, W->cs, W->sn, W->scal, W->dj);
+    hipLaunchKernelGGL(k_scale_next, dim3(gb), dim3(256), 0, W->stream, n, W->w, W->V, W->dj,
+                       W->scal);
+    ED_KCHECK("arnoldi");
+    return ED_OK;
+  }
+
+  // Restarted GMRES on approx_op for A d = rhs (tree order), x updated in place.
+  // tol_abs applies to the true residual of approx_op.  Returns code.
+  int gmres(const double* rhs, double* x, double tol_rel, int maxiter, int* code,
+            double* r0_out, double* r_out) {
+    const int m = W->m;
+    double nr = 0.0, nz = 0.0;
+    ED_T(approx_op(x, W->t));
+    ED_T(resid_norms(rhs, W->t, W->r, W->w, &nr, &nz));
+    if (r0_out) *r0_out = nr;
+    if (!std::isfinite(nr)) { *code = 3; if (r_out) *r_out = nr; return ED_OK; }
+    const double init_true = nr > 1e-300 ? nr : 1.0;
+    const double tol_abs = std::max(tol_rel * init_true, 1e-300);
+    if (nr <= tol_abs) { *code = 0; if (r_out) *r_out = nr; return ED_OK; }
+    int local = 0;
+    bool use_graph = o->use_graph != 0;
+    // target of the preconditioned estimate |g_{j+1}|: tol * ||D^-1 r0||
+    const double est_tol = tol_rel * nz;
+    while (local < maxiter) {
+      if (!std::isfinite(nz) || nz == 0.0) { *code = nz == 0.0 ? 6 : 7; if (r_out) *r_out = nr; return ED_OK; }
+      hipLaunchKernelGGL(k_start, dim3(gb), dim3(256), 0, W->stream, n, m, W->w, W->V, W->g, W->scal,
+                         W->dj);
+      ED_G(hipMemsetAsync(W->scal + S_BREAK, 0, sizeof(double), W->stream));
+      ED_KCHECK("start");
+      int j = 0;
+      bool stop = false;
+      const int ce = std::max(1, o->check_every);
+      while (j < m && local < maxiter && !stop) {
+        const int steps = std::min(std::min(ce, m - j), maxiter - local);
+        for (int s = 0; s < steps; ++s) {
+          if (use_graph) {
+            if (!W->gexec || W->graph_selfi != selfi_orig) {
+              if (W->gexec) { (void)hipGraphExecDestroy(W->gexec); W->gexec = nullptr; }
+              if (W->graph) { (void)hipGraphDestroy(W->graph); W->graph = nullptr; }
+              ED_G(hipStreamBeginCapture(W->stream, hipStreamCaptureModeThreadLocal));
+              int rc = arnoldi_step();
+              hipGraph_t gr = nullptr;
+              hipError_t ec = hipStreamEndCapture(W->stream, &gr);
+              if (rc != ED_OK) return rc;
+              if (ec != hipSuccess) return fail(ED_ERR_DEVICE, std::string("graph capture: ") + hipGetErrorString(ec));
+              W->graph = gr;
+              ED_G(hipGraphInstantiate(&W->gexec, gr, nullptr, nullptr, 0));
+              W->graph_selfi = selfi_orig;
+            }
+            ED_G(hipGraphLaunch(W->gexec, W->stream));
+          } else {
+            ED_T(arnoldi_step());
+          }
+        }
+        j += steps;
+        local += steps;
+        total_iters += steps;
+        ED_G(hipMemcpyAsync(W->hscal, W->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, W->stream));
+        ED_G(hipStreamSynchronize(W->stream));
+        const double est = W->hscal[S_EST];
+        if (W->hscal[S_BREAK] != 0.0) stop = true;
+        if (!std::isfinite(est)) { *code = 5; if (r_out) *r_out = est; return ED_OK; }
+        if (est <= est_tol) {
+          if (first_below < 0) first_below = total_iters;
+          stop = true;
+        }
+      }
+      // with a breakdown at step j the last column is still valid (hn = 0 only)
+      hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, W->stream, m, W->H, W->g, W->dj, W->h);
+      hipLaunchKernelGGL(k_xupdate, dim3(gb), dim3(256), 0, W->stream, n, x, W->V, W->h, W->dj);
+      ED_KCHECK("update");
+      ++cycles;
+      ED_T(approx_op(x, W->t));
+      ED_T(resid_norms(rhs, W->t, W->r, W->w, &nr, &nz));
+      if (!std::isfinite(nr)) { *code = 6; if (r_out) *r_out = nr; return ED_OK; }
+      if (nr <= tol_abs) { *code = 0; if (r_out) *r_out = nr; return ED_OK; }
+      if (W->hscal[S_BREAK] != 0.0 && j <= 1) { *code = 2; if (r_out) *r_out = nr; return ED_OK; }
+    }
+    *code = 1;
+    if (r_out) *r_out = nr;
+    return ED_OK;
+  }
+};
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" void ed_gmres_opts_default(ed_gmres_opts* o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof(*o));
+  o->restart = 50;
+  o->maxiter = 2000;
+  o->tol = 1e-8;
+  o->precond = 1;
+  o->refine = 1;
+  o->max_refine = 8;
+  o->inner_tol = 1e-4;
+  o->use_graph = 1;
+  o->check_every = 4;
+}
+
+extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const double* diag,
+                              const double* selfi, const ed_gmres_opts* opts,
+                              ed_gmres_info* info, ed_stream stream) {
+  if (!P || !b || !x || !info) return fail(ED_ERR_ARG, "ed_gmres_solve: NULL argument");
+  ed_gmres_opts o;
+  ed_gmres_opts_default(&o);
+  if (opts) o = *opts;
+  if (o.restart <= 0 || o.restart > 255) return fail(ED_ERR_ARG, "restart must be in [1, 255]");
+  if (o.maxiter <= 0) return fail(ED_ERR_ARG, "maxiter must be positive");
+  std::memset(info, 0, sizeof(*info));
+  info->first_iter_below_tol = -1;
+  GmresWS* W = nullptr;
+  ED_T(ws_get(P, o.restart, &W));
+  const int64_t n = P->n;
+  const unsigned gb = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  const unsigned gn = (unsigned)((n + 255) / 256);
+  hipStream_t caller = (hipStream_t)stream;
+  // order the solver's stream after the caller's work
+  hipEvent_t ev;
+  ED_G(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+  ED_G(hipEventRecord(ev, caller));
+  ED_G(hipStreamWaitEvent(W->stream, ev, 0));
+  hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, W->stream, n, P->perm, b, W->b);
+  hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, W->stream, n, P->perm, x, W->x);
+  hipLaunchKernelGGL(k_dinv, dim3(gn), dim3(256), 0, W->stream, n, P->perm,
+                     o.precond ? diag : nullptr, W->dinv);
+  if (selfi)
+    hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, W->stream, n, P->perm, selfi, W->selfi_t);
+  else
+    ED_G(hipMemsetAsync(W->selfi_t, 0, sizeof(double) * n, W->stream));
+  ED_KCHECK("solver setup");
+  Solver S{P, W, &o, selfi, n, gb};
+  int code = 1;
+  double bn = 0.0, dummy = 0.0;
+  ED_T(S.resid_norms(W->b, nullptr, nullptr, nullptr, &bn, &dummy));
+  if (!o.refine) {
+    double r0 = 0, r1 = 0;
+    ED_T(S.gmres(W->b, W->x, o.tol, o.maxiter, &code, &r0, &r1));
+    info->resid0 = r0;
+    info->resid = r1;
+    info->tol_abs = o.tol * (r0 > 1e-300 ? r0 : 1.0);
+  } else {
+    // outer FP64 iterative refinement with the accurate operator
+    double rn = 0.0, zn = 0.0;
+    code = 1;
+    for (int it = 0; it <= o.max_refine; ++it) {
+      ED_T(S.accurate_op(W->x, W->t));
+      ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, &rn, &zn));
+      if (it == 0) {
+        info->resid0 = rn;
+        info->tol_abs = o.tol * (bn > 1e-300 ? bn : 1.0);
+      }
+      info->refine_steps = it;
+      if (!std::isfinite(rn)) { code = 6; break; }
+      if (rn <= info->tol_abs) { code = 0; break; }
+      if (it == o.max_refine || S.total_iters >= o.maxiter) { code = 1; break; }
+      // inner solve  A_fmm d = r with d = 0; the inner solver reuses r as its
+      // own scratch, so the outer residual is kept in `rhs`
+      ED_G(hipMemsetAsync(W->d, 0, sizeof(double) * n, W->stream));
+      ED_G(hipMemcpyAsync(W->rhs, W->r, sizeof(double) * n, hipMemcpyDeviceToDevice, W->stream));
+      int icode = 1;
+      double ir0 = 0, ir1 = 0;
+      ED_T(S.gmres(W->rhs, W->d, o.inner_tol, o.maxiter - S.total_iters, &icode, &ir0, &ir1));
+      if (icode == 3 || icode == 5 || icode == 6 || icode == 7) { code = icode; break; }
+      hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, W->stream, n, 1.0, W->d, W->x);
+      ED_KCHECK("refine update");
+    }
+    info->resid = rn;
+  }
+  info->code = code;
+  info->iters = S.total_iters;
+  info->cycles = S.cycles;
+  info->first_iter_below_tol = S.first_below;
+  info->rel_resid = info->resid / (bn > 1e-300 ? bn : 1.0);
+  hipLaunchKernelGGL(k_perm_out, dim3(gn), dim3(256), 0, W->stream, n, P->perm, W->x, x);
+  ED_KCHECK("solver output");
+  ED_G(hipEventRecord(ev, W->stream));
+  ED_G(hipStreamWaitEvent(caller, ev, 0));
+  ED_G(hipStreamSynchronize(W->stream));
+  (void)hipEventDestroy(ev);
+  return ED_OK;
+}

@@ -1,0 +1,96 @@
+// Device-resident FMM plan (one per geometry).  All buffers are allocated at
+// ed_plan_create; ed_fmm_matvec_* allocates nothing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "ed_internal.h"
+
+struct ed_plan {
+  int device = 0;
+  int p = 8, nc = 81, ws = 1, leaf_size = 64, max_level = 16;
+  int m2l_precision = 0;
+  int64_t n = 0, n_nodes = 0, n_leaves = 0;
+  ed::HostTree host;
+
+  // points in tree order: x, y, z relative to the root centre (FP32), w = q
+  float4* pts = nullptr;
+  float* area_t = nullptr;   // areas, tree order
+  float* sig_t = nullptr;    // sigma, tree order (written per matvec)
+  float* selfc_t = nullptr;  // per-area self term, tree order (per matvec)
+  int32_t* perm = nullptr;   // tree position -> original index
+
+  // nodes
+  float4* node_ctr = nullptr;   // centre (relative), w = 1/s (s = box side)
+  int2* node_rng = nullptr;     // [begin, end)
+  int32_t* node_parent = nullptr;
+  int32_t* node_octant = nullptr;  // octant within parent
+  int32_t* first_child = nullptr;
+  int32_t* n_child = nullptr;
+  float* M = nullptr;  // n_nodes * nc
+  float* L = nullptr;  // n_nodes * nc
+
+  // leaves (node ids) and their U lists (CSR over leaf ordinal)
+  int32_t* leaves = nullptr;
+  int32_t* u_ptr = nullptr;
+  int32_t* u_src = nullptr;  // node ids
+  int64_t n_u = 0, n_p2p_interactions = 0;
+
+  // upward/downward passes: per level l, nodes of level l that have children
+  std::vector<int64_t> parents_begin;  // host offsets into parents_list per level
+  int32_t* parents_list = nullptr;
+
+  // M2L: CSR over target nodes (v_ptr over all nodes), source ids and table ids
+  int32_t* v_ptr = nullptr;
+  int32_t* v_src = nullptr;
+  int32_t* v_tab = nullptr;
+  int64_t n_v = 0;
+  int n_off = 0;                 // number of distinct offsets (tables)
+  std::vector<int32_t> slot_to_tab;
+
+  // M2L in GEMM form: pairs grouped by offset table, tiles of kM2LTile pairs
+  int32_t* m2l_pair_src = nullptr;   // n_v, grouped by table
+  int32_t* m2l_pair_tgt = nullptr;
+  int32_t* m2l_tile_tab = nullptr;   // per tile: table id
+  int32_t* m2l_tile_beg = nullptr;   // per tile: first pair
+  int32_t* m2l_tile_cnt = nullptr;   // per tile: pair count
+  int64_t n_m2l_tiles = 0;
+  int32_t* m2l_tgt_ptr = nullptr;    // CSR target -> rows of Ypair (n_nodes+1)
+  int32_t* m2l_tgt_row = nullptr;    // Ypair row of each (target, pair)
+  float* ypair = nullptr;            // n_v * ncp (ncp = nc padded)
+  int ncp = 96;
+
+  // operator tables (column-major: T[tab][j][k] = out k <- in j)
+  float* t_m2l = nullptr;       // n_off * nc * nc, fp32 (VALU path)
+  float* t_m2l_mfma = nullptr;  // n_off * ncp * ncp, row-major padded (MFMA A operand)
+  uint16_t* t_m2l_h = nullptr;  // n_off * ncp * ncp, bf16/fp16
+  float* t_m2m = nullptr;       // 8 * nc * nc
+  float* t_l2l = nullptr;       // 8 * nc * nc
+
+  // near-field correction CSR in tree order
+  int32_t* near_ptr = nullptr;  // n + 1
+  int32_t* near_col = nullptr;
+  float* near_w = nullptr;
+  int64_t n_near = 0;
+
+  // per-level inverse box side
+  std::vector<float> inv_side;
+
+  // FP64 copies in tree order for the accurate (dense) residual operator
+  double* cent64_t = nullptr;  // n x 3
+  double* area64_t = nullptr;  // n
+  double* near_w64 = nullptr;  // near CSR weights in FP64
+
+  // GMRES workspace (allocated lazily by the solver, owned by the plan)
+  void* gmres_ws = nullptr;
+};
+
+namespace ed {
+int plan_upload(ed_plan* P);
+void plan_free(ed_plan* P);
+// FMM matvec with tree-order q/sigma/self already in place; writes the result
+// either in original order (out_orig) or tree order (out_tree), FP32 or FP64.
+int fmm_launch(ed_plan* P, const void* sigma, int sigma_is_f64, int sigma_tree_order,
+               const void* self_integrals, void* out, int out_is_f64,
+               int out_tree_order, hipStream_t stream);
+}  // namespace ed

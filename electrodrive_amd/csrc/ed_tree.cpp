@@ -1,0 +1,260 @@
+// Host lattice octree, U/V interaction lists and Morton keys.
+//
+// The tree is a cubic lattice octree over the panel centroids, ordered by the
+// reference's Morton key (fmm3d/geometry.py:736-799) evaluated in FP64 at level
+// 21 inside the root cube.  Because boxes sit on a lattice, every M2L
+// translation depends only on an integer offset between same-level boxes, so
+// translation operators are shared tables and M2L becomes a batched GEMM.
+//
+// Lists (ws = well-separation radius in boxes):
+//   adjacent(A,B): same level and max|coord(A)-coord(B)| <= ws
+//   V(T): children of non-leaf nodes adjacent to parent(T) that are not
+//         adjacent to T (M2L sources of T)
+//   U(T): for a leaf T, the leaves S with adjacent(anc_m(T), anc_m(S)) at
+//         m = min(level T, level S) (P2P sources of T)
+// These cover every ordered pair of points exactly once.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <unordered_map>
+
+#include "ed_internal.h"
+
+namespace ed {
+
+static inline uint64_t interleave3(uint64_t x, uint64_t y, uint64_t z, int bits) {
+  uint64_t k = 0;
+  for (int b = 0; b < bits; ++b) {
+    k |= ((x >> b) & 1ULL) << (3 * b);
+    k |= ((y >> b) & 1ULL) << (3 * b + 1);
+    k |= ((z >> b) & 1ULL) << (3 * b + 2);
+  }
+  return k;
+}
+
+// fmm3d/geometry.py:696-703 and :785-799 in FP64: normalised = (p - min)/extent
+// clamped to [0, 1-1e-9], extent = max(max-min, eps); floor(x 2^level) clamped.
+static inline void quantize(const double* p, const double* mins, const double* ext,
+                            int level, int64_t* ijk) {
+  const double hi = 1.0 - 1e-9;
+  const double scale = (double)(1LL << level);
+  const int64_t maxi = (1LL << level) - 1;
+  for (int a = 0; a < 3; ++a) {
+    double v = (p[a] - mins[a]) / ext[a];
+    v = v < 0.0 ? 0.0 : v;
+    v = v > hi ? hi : v;
+    int64_t q = (int64_t)std::floor(v * scale);
+    ijk[a] = q < 0 ? 0 : (q > maxi ? maxi : q);
+  }
+}
+
+void morton_keys_host(const double* pts, int64_t n, int level,
+                      const double* bbox_center, const double* bbox_half,
+                      int64_t* keys) {
+  if (n <= 0) return;
+  double mins[3], maxs[3], ext[3];
+  if (bbox_center) {
+    for (int a = 0; a < 3; ++a) {
+      mins[a] = bbox_center[a] - bbox_half[a];
+      maxs[a] = bbox_center[a] + bbox_half[a];
+    }
+  } else {
+    // geometry.py:678-693: tight bbox, degenerate axes padded by
+    // 1e-12 * (1 + max_extent).
+    for (int a = 0; a < 3; ++a) { mins[a] = pts[a]; maxs[a] = pts[a]; }
+    for (int64_t i = 1; i < n; ++i)
+      for (int a = 0; a < 3; ++a) {
+        mins[a] = std::min(mins[a], pts[3 * i + a]);
+        maxs[a] = std::max(maxs[a], pts[3 * i + a]);
+      }
+    double e[3], me = 0.0;
+    for (int a = 0; a < 3; ++a) { e[a] = maxs[a] - mins[a]; me = std::max(me, std::fabs(e[a])); }
+    const double padded = 1e-12 * (1.0 + me);
+    for (int a = 0; a < 3; ++a) maxs[a] = mins[a] + (e[a] > 0 ? e[a] : padded);
+  }
+  const double eps = 2.220446049250313e-16;
+  for (int a = 0; a < 3; ++a) ext[a] = std::max(maxs[a] - mins[a], eps);
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t ijk[3];
+    quantize(pts + 3 * i, mins, ext, level, ijk);
+    keys[i] = (int64_t)interleave3(ijk[0], ijk[1], ijk[2], level);
+  }
+}
+
+int64_t HostTree::n_leaves() const {
+  int64_t c = 0;
+  for (auto v : is_leaf) c += v;
+  return c;
+}
+
+static constexpr int kKeyLevel = 21;
+
+int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
+                    int ws, HostTree& t) {
+  if (n <= 0) return fail(ED_ERR_ARG, "tree build: n must be positive");
+  if (leaf_size < 1) return fail(ED_ERR_ARG, "tree build: leaf_size must be >= 1");
+  if (max_level < 1 || max_level > 20)
+    return fail(ED_ERR_ARG, "tree build: max_level must be in [1, 20]");
+  if (ws < 1 || ws > 2) return fail(ED_ERR_ARG, "tree build: ws must be 1 or 2");
+  t = HostTree();
+  t.n = n;
+  t.leaf_size = leaf_size;
+  t.max_level = max_level;
+  t.ws = ws;
+  double mins[3], maxs[3];
+  for (int a = 0; a < 3; ++a) { mins[a] = C[a]; maxs[a] = C[a]; }
+  for (int64_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const double v = C[3 * i + a];
+      if (!std::isfinite(v)) return fail(ED_ERR_ARG, "tree build: non-finite centroid");
+      mins[a] = std::min(mins[a], v);
+      maxs[a] = std::max(maxs[a], v);
+    }
+  double half = 0.0;
+  for (int a = 0; a < 3; ++a) {
+    t.center[a] = 0.5 * (mins[a] + maxs[a]);
+    half = std::max(half, 0.5 * (maxs[a] - mins[a]));
+  }
+  half = half * (1.0 + std::ldexp(1.0, -20)) + 1e-300;
+  t.half = half;
+  const double bh[3] = {half, half, half};
+  std::vector<int64_t> keys(n);
+  morton_keys_host(C, n, kKeyLevel, t.center, bh, keys.data());
+  t.perm.resize(n);
+  std::iota(t.perm.begin(), t.perm.end(), 0);
+  std::sort(t.perm.begin(), t.perm.end(), [&](int64_t a, int64_t b) {
+    return keys[a] != keys[b] ? keys[a] < keys[b] : a < b;
+  });
+  t.keys.resize(n);
+  for (int64_t i = 0; i < n; ++i) t.keys[i] = (uint64_t)keys[t.perm[i]];
+
+  // Level-by-level node construction.
+  struct Pending { int64_t b, e, parent; };
+  std::vector<Pending> frontier{{0, n, -1}};
+  int lvl = 0;
+  while (!frontier.empty()) {
+    const int64_t base = (int64_t)t.level.size();
+    t.level_begin.push_back(base);
+    for (auto& f : frontier) {
+      t.level.push_back(lvl);
+      t.begin.push_back(f.b);
+      t.end.push_back(f.e);
+      t.parent.push_back(f.parent);
+      const uint64_t pref = t.keys[f.b] >> (3 * (kKeyLevel - lvl));
+      // de-interleave the level-lvl prefix into integer coords
+      int32_t xyz[3] = {0, 0, 0};
+      for (int b = 0; b < lvl; ++b)
+        for (int a = 0; a < 3; ++a) xyz[a] |= (int32_t)((pref >> (3 * b + a)) & 1ULL) << b;
+      t.coord.insert(t.coord.end(), xyz, xyz + 3);
+      const bool leaf = (f.e - f.b) <= leaf_size || lvl >= max_level;
+      t.is_leaf.push_back(leaf ? 1 : 0);
+      t.first_child.push_back(-1);
+      t.n_child.push_back(0);
+    }
+    std::vector<Pending> next;
+    for (size_t k = 0; k < frontier.size(); ++k) {
+      const int64_t idx = base + (int64_t)k;
+      if (t.is_leaf[idx]) continue;
+      const int shift = 3 * (kKeyLevel - lvl - 1);
+      t.first_child[idx] = (int32_t)(base + (int64_t)frontier.size() + (int64_t)next.size());
+      int64_t s0 = frontier[k].b;
+      int nc = 0;
+      for (int64_t i = frontier[k].b + 1; i <= frontier[k].e; ++i) {
+        if (i == frontier[k].e || (t.keys[i] >> shift) != (t.keys[i - 1] >> shift)) {
+          next.push_back({s0, i, idx});
+          s0 = i;
+          ++nc;
+        }
+      }
+      t.n_child[idx] = nc;
+    }
+    frontier.swap(next);
+    ++lvl;
+  }
+  t.depth = lvl - 1;
+  t.level_begin.push_back((int64_t)t.level.size());
+  const int64_t M = (int64_t)t.level.size();
+
+  // Locational-code hash: (1 << 3l) | prefix -> node.
+  std::unordered_map<uint64_t, int64_t> where;
+  where.reserve((size_t)M * 2);
+  auto code_of = [&](int l, int64_t x, int64_t y, int64_t z) -> uint64_t {
+    return (1ULL << (3 * l)) | interleave3((uint64_t)x, (uint64_t)y, (uint64_t)z, l);
+  };
+  for (int64_t i = 0; i < M; ++i)
+    where[code_of(t.level[i], t.coord[3 * i], t.coord[3 * i + 1], t.coord[3 * i + 2])] = i;
+
+  auto colleagues = [&](int64_t a, std::vector<int64_t>& out) {
+    out.clear();
+    const int l = t.level[a];
+    const int64_t lim = 1LL << l;
+    for (int dz = -ws; dz <= ws; ++dz)
+      for (int dy = -ws; dy <= ws; ++dy)
+        for (int dx = -ws; dx <= ws; ++dx) {
+          const int64_t x = t.coord[3 * a] + dx, y = t.coord[3 * a + 1] + dy,
+                        z = t.coord[3 * a + 2] + dz;
+          if (x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) continue;
+          auto it = where.find(code_of(l, x, y, z));
+          if (it != where.end()) out.push_back(it->second);
+        }
+  };
+  auto adjacent = [&](int64_t a, int64_t b) {
+    for (int k = 0; k < 3; ++k)
+      if (std::abs(t.coord[3 * a + k] - t.coord[3 * b + k]) > ws) return false;
+    return true;
+  };
+
+  t.v_ptr.assign(M + 1, 0);
+  t.u_ptr.assign(M + 1, 0);
+  std::vector<int64_t> col, col2, stack, us;
+  std::vector<std::pair<int64_t, int32_t>> vs;
+  for (int64_t nidx = 0; nidx < M; ++nidx) {
+    vs.clear();
+    const int64_t P = t.parent[nidx];
+    if (P >= 0) {
+      colleagues(P, col);
+      for (int64_t c : col) {
+        if (t.is_leaf[c]) continue;
+        for (int64_t s = t.first_child[c]; s < t.first_child[c] + t.n_child[c]; ++s) {
+          if (adjacent(s, nidx)) continue;
+          const int ox = t.coord[3 * nidx] - t.coord[3 * s];
+          const int oy = t.coord[3 * nidx + 1] - t.coord[3 * s + 1];
+          const int oz = t.coord[3 * nidx + 2] - t.coord[3 * s + 2];
+          vs.emplace_back(s, offset_slot(ws, ox, oy, oz));
+        }
+      }
+      std::sort(vs.begin(), vs.end());
+    }
+    for (auto& e : vs) { t.v_src.push_back(e.first); t.v_off.push_back(e.second); }
+    t.v_ptr[nidx + 1] = (int64_t)t.v_src.size();
+
+    us.clear();
+    if (t.is_leaf[nidx]) {
+      for (int64_t a = nidx; a >= 0; a = t.parent[a]) {
+        colleagues(a, col2);
+        for (int64_t c : col2) {
+          if (t.is_leaf[c]) {
+            us.push_back(c);
+          } else if (a == nidx) {
+            stack.assign(1, c);
+            while (!stack.empty()) {
+              const int64_t d = stack.back();
+              stack.pop_back();
+              if (t.is_leaf[d]) { us.push_back(d); continue; }
+              for (int64_t s = t.first_child[d]; s < t.first_child[d] + t.n_child[d]; ++s)
+                stack.push_back(s);
+            }
+          }
+        }
+      }
+      std::sort(us.begin(), us.end());
+      us.erase(std::unique(us.begin(), us.end()), us.end());
+    }
+    t.u_src.insert(t.u_src.end(), us.begin(), us.end());
+    t.u_ptr[nidx + 1] = (int64_t)t.u_src.size();
+  }
+  return ED_OK;
+}
+
+}  // namespace ed

@@ -1,0 +1,131 @@
+"""Owning wrapper of the native ``ed_plan`` (tree, lists, operators on HBM)."""
+import ctypes
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from .config import M2L_PRECISIONS
+
+STAT_NAMES = ("n_points", "n_nodes", "n_leaves", "depth", "n_p2p_leaf_pairs",
+              "n_p2p_interactions", "n_m2l_pairs", "n_m2l_offsets", "order", "ws",
+              "n_near_pairs", "n_levels_with_m2l")
+
+
+class FmmPlan:
+    """Native FMM plan for one panel geometry (centroids, areas on the host).
+
+    ``matvec`` evaluates K_E * sum_{j != i} sigma_j A_j / r_ij (FMM), plus the
+    per-area self term and the attached near correction, on device tensors in
+    original panel order.  No allocation happens inside ``matvec`` when ``out``
+    is given.
+    """
+
+    def __init__(self, centroids, areas, *, leaf_size=64, order=8, ws=1, max_level=16,
+                 m2l_precision="fp32"):
+        nat.require_gpu()
+        L = nat.lib()
+        C = np.ascontiguousarray(np.asarray(centroids, dtype=np.float64).reshape(-1, 3))
+        A = np.ascontiguousarray(np.asarray(areas, dtype=np.float64).reshape(-1))
+        if C.shape[0] != A.shape[0]:
+            raise ValueError("centroids and areas must have the same length")
+        self.n = int(C.shape[0])
+        opts = nat.PlanOpts()
+        L.ed_plan_opts_default(ctypes.byref(opts))
+        opts.leaf_size = int(leaf_size)
+        opts.order = int(order)
+        opts.ws = int(ws)
+        opts.max_level = int(max_level)
+        opts.m2l_precision = int(M2L_PRECISIONS[m2l_precision]
+                                 if isinstance(m2l_precision, str) else m2l_precision)
+        h = ctypes.c_void_p()
+        nat.check(L.ed_plan_create(C.ctypes.data_as(ctypes.c_void_p),
+                                   A.ctypes.data_as(ctypes.c_void_p), self.n,
+                                   ctypes.byref(opts), ctypes.byref(h)), "ed_plan_create")
+        self._h = h
+        self.order, self.ws, self.leaf_size = int(order), int(ws), int(leaf_size)
+        self.m2l_precision = m2l_precision
+
+    @property
+    def handle(self):
+        if self._h is None:
+            raise RuntimeError("plan has been destroyed")
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            nat.lib().ed_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stats(self):
+        out = (ctypes.c_int64 * len(STAT_NAMES))()
+        nat.check(nat.lib().ed_plan_stats(self.handle, out, len(STAT_NAMES)), "ed_plan_stats")
+        return {k: int(v) for k, v in zip(STAT_NAMES, out)}
+
+    def permutation(self):
+        perm = np.empty(self.n, dtype=np.int64)
+        nat.check(nat.lib().ed_plan_permutation(self.handle, perm.ctypes.data_as(ctypes.c_void_p)),
+                  "ed_plan_permutation")
+        return perm
+
+    def attach_near(self, pairs, weights):
+        pairs = np.ascontiguousarray(np.asarray(pairs, dtype=np.int64).reshape(-1, 2))
+        w = np.ascontiguousarray(np.asarray(weights, dtype=np.float64).reshape(-1))
+        if pairs.shape[0] != w.shape[0]:
+            raise ValueError("pairs and weights length mismatch")
+        nat.check(nat.lib().ed_plan_attach_near(self.handle, pairs.ctypes.data_as(ctypes.c_void_p),
+                                                w.ctypes.data_as(ctypes.c_void_p), pairs.shape[0]),
+                  "ed_plan_attach_near")
+
+    def matvec(self, sigma, self_integrals=None, out=None, stream=None):
+        if not sigma.is_cuda or sigma.dim() != 1 or sigma.shape[0] != self.n:
+            raise ValueError("sigma must be a 1-D device tensor of length n")
+        if sigma.dtype not in (torch.float32, torch.float64):
+            raise ValueError("sigma must be float32 or float64")
+        sigma = sigma.contiguous()
+        if self_integrals is not None:
+            if (not self_integrals.is_cuda or self_integrals.dtype != sigma.dtype
+                    or self_integrals.shape != sigma.shape):
+                raise ValueError("self_integrals must match sigma (device, dtype, shape)")
+            self_integrals = self_integrals.contiguous()
+        if out is None:
+            out = torch.empty_like(sigma)
+        st = stream if stream is not None else nat.stream_ptr(sigma.device)
+        fn = (nat.lib().ed_fmm_matvec_f64 if sigma.dtype == torch.float64
+              else nat.lib().ed_fmm_matvec_f32)
+        nat.check(fn(self.handle, nat.ptr(sigma), nat.ptr(self_integrals), nat.ptr(out), st),
+                  "ed_fmm_matvec")
+        return out
+
+    def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
+              tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,
+              use_graph=True, check_every=4):
+        """Native FP64 GMRES (+ iterative refinement); returns (x, info dict)."""
+        def f64(t):
+            if t is None:
+                return None
+            if not t.is_cuda or t.dtype != torch.float64 or t.shape != (self.n,):
+                raise ValueError("solver vectors must be float64 device tensors of length n")
+            return t.contiguous()
+        b = f64(b)
+        x = torch.zeros_like(b) if x0 is None else f64(x0).clone()
+        diag, self_integrals = f64(diag), f64(self_integrals)
+        L = nat.lib()
+        o = nat.GmresOpts()
+        L.ed_gmres_opts_default(ctypes.byref(o))
+        o.restart, o.maxiter, o.tol = int(restart), int(maxiter), float(tol)
+        o.precond, o.refine, o.max_refine = int(bool(precond)), int(bool(refine)), int(max_refine)
+        o.inner_tol, o.use_graph, o.check_every = float(inner_tol), int(bool(use_graph)), int(check_every)
+        info = nat.GmresInfo()
+        nat.check(L.ed_gmres_solve(self.handle, nat.ptr(b), nat.ptr(x), nat.ptr(diag),
+                                   nat.ptr(self_integrals), ctypes.byref(o), ctypes.byref(info),
+                                   nat.stream_ptr(b.device)), "ed_gmres_solve")
+        d = {k: getattr(info, k) for k, _ in nat.GmresInfo._fields_}
+        d["success"] = d["code"] == 0
+        return x, d

@@ -1,0 +1,231 @@
+/*
+ * ednative.h — C ABI of the MI355X (gfx950) BEM/FMM solve hot path.
+ *
+ * Plain pointers and sizes only.  Device pointers are HIP device memory on the
+ * current device; `ed_stream` is a hipStream_t (NULL = the default stream).
+ * Every function returns ED_OK (0) or an ED_ERR_* status; ed_last_error()
+ * gives a thread-local message for the last failure.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the Electrodrive repository root).
+ */
+#ifndef EDNATIVE_H
+#define EDNATIVE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ED_OK 0
+#define ED_ERR_ARG 1        /* shape / value misuse  -> Python ValueError   */
+#define ED_ERR_DEVICE 2     /* no device / HIP error -> Python RuntimeError */
+#define ED_ERR_ALLOC 3      /* device allocation failed                      */
+#define ED_ERR_LAUNCH 4     /* kernel launch failed                          */
+#define ED_ERR_NONFINITE 5  /* non-finite values were produced / clamped     */
+#define ED_ERR_INTERNAL 6
+
+typedef void* ed_stream;
+typedef struct ed_plan ed_plan;
+
+const char* ed_last_error(void);
+int32_t ed_abi_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* FMM plan: replaces LaplaceFmm3D.__init__ / make_laplace_fmm_backend       */
+/* (fmm3d/bem_fmm.py:127-320, :888-940): builds the octree, interaction      */
+/* lists (fmm3d/tree.py:968-1050, fmm3d/interaction_lists.py:531-577) and    */
+/* translation operators once per geometry.  Geometry is host FP64, as the   */
+/* reference requires CPU geometry (bem_fmm.py:169-181).                     */
+/* ------------------------------------------------------------------------ */
+typedef struct ed_plan_opts {
+  int32_t leaf_size;     /* max panels per leaf (reference default 64)        */
+  int32_t order;         /* expansion order p, 2..16 (reference default 8)   */
+  int32_t ws;            /* well-separation radius in boxes: 1 or 2          */
+  int32_t max_level;     /* deepest octree level, <= 20                      */
+  int32_t m2l_precision; /* 0 = fp32 MFMA, 1 = bf16 MFMA, 2 = fp16 MFMA      */
+  int32_t reserved[3];
+} ed_plan_opts;
+
+void ed_plan_opts_default(ed_plan_opts* opts);
+
+int ed_plan_create(const double* centroids_host, const double* areas_host,
+                   int64_t n, const ed_plan_opts* opts, ed_plan** out);
+int ed_plan_destroy(ed_plan* plan);
+
+/* Counters (TreeStatistics / InteractionStats analogue, fmm3d/tree.py:120-162,
+ * fmm3d/interaction_lists.py:59-88).  out[i] for i < n_out:
+ * 0 n_points 1 n_nodes 2 n_leaves 3 depth 4 n_p2p_leaf_pairs
+ * 5 n_p2p_interactions 6 n_m2l_pairs 7 n_m2l_offsets 8 order 9 ws
+ * 10 n_near_pairs 11 n_levels_with_m2l */
+int ed_plan_stats(const ed_plan* plan, int64_t* out, int32_t n_out);
+
+/* Tree-order permutation (index_map, fmm3d/tree.py:428-485): perm[i] is the
+ * original index of tree position i.  Host output of length n. */
+int ed_plan_permutation(const ed_plan* plan, int64_t* perm_host);
+
+/* FMM matvec: replaces LaplaceFmm3D.matvec (fmm3d/bem_fmm.py:760-885):
+ * out_i = K_E * sum_{j != i} sigma_j A_j / |c_i - c_j|  (FMM-approximated)
+ *         + self_integrals_i * sigma_i * A_i            (if non-NULL)
+ *         + sum_j delta_ij sigma_j                      (if near attached).
+ * sigma / self_integrals / out are device arrays in original panel order;
+ * non-finite outputs are clamped to 0 (bem_fmm.py:741-749).  The f64 variant
+ * takes and returns FP64 with FP32 arithmetic inside.  No allocation. */
+int ed_fmm_matvec_f32(ed_plan* plan, const float* sigma, const float* self_integrals,
+                      float* out, ed_stream stream);
+int ed_fmm_matvec_f64(ed_plan* plan, const double* sigma, const double* self_integrals,
+                      double* out, ed_stream stream);
+
+/* Attach the near-singular correction (core/bem.py:1838-1957): pairs (P,2)
+ * int64 (i, j) and FP64 weights delta_ij, host arrays in original order.  They
+ * are converted to a tree-ordered CSR applied inside ed_fmm_matvec_*.  P = 0
+ * detaches. */
+int ed_plan_attach_near(ed_plan* plan, const int64_t* pairs_host,
+                        const double* weights_host, int64_t n_pairs);
+
+/* ------------------------------------------------------------------------ */
+/* Dense single-layer matvec: replaces _bem_matvec_core_torch and            */
+/* bem_matvec_batched (core/bem_kernel.py:157-230, :715-810).                */
+/* centroids (n,3), areas (n), sigma (nrhs,n), out (nrhs,n) device arrays;   */
+/* self_integrals may be NULL (diagonal 0).                                  */
+/* ------------------------------------------------------------------------ */
+int ed_dense_matvec_f64(const double* centroids, const double* areas,
+                        const double* sigma, const double* self_integrals,
+                        double* out, int64_t n, int64_t nrhs, ed_stream stream);
+int ed_dense_matvec_f32(const float* centroids, const float* areas,
+                        const float* sigma, const float* self_integrals,
+                        float* out, int64_t n, int64_t nrhs, ed_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Quadrature                                                                */
+/* ------------------------------------------------------------------------ */
+/* Self integrals I_i = sum_k K_E w_k / max(|c_i - y_k|, 1e-12) over the panel
+ * refined `depth` times toward its centroid with the order-`order` rule
+ * (core/bem.py:1777-1816, core/bem_quadrature.py:217-308).  panel_vertices
+ * (n,3,3), centroids (n,3), out (n) device FP64. */
+int ed_self_integrals_f64(const double* panel_vertices, const double* centroids,
+                          int64_t n, int32_t depth, int32_t order, double* out,
+                          ed_stream stream);
+
+/* Near pairs (core/bem.py:475-555): (i, j), i != j,
+ * 0 < |c_i - c_j| <= factor (R_i + R_j), R = sqrt(A/pi), i-major, j ascending.
+ * Two calls: counts (pairs_out NULL) then fill.  Device inputs, device
+ * outputs; *n_pairs_out is a host int64. */
+int ed_near_pairs_f64(const double* centroids, const double* areas, int64_t n,
+                      double factor, int64_t* pairs_out, int64_t capacity,
+                      int64_t* n_pairs_out, ed_stream stream);
+
+/* Near weights (core/bem.py:558-667): delta_ij = sum_k K_E w_k/max(|c_i-y_k|,
+ * 1e-12) - K_E A_j / max(|c_i - c_j|, 1e-12), panel j uniformly refined
+ * `depth` times.  Device arrays. */
+int ed_near_weights_f64(const double* centroids, const double* areas,
+                        const double* panel_vertices, const int64_t* pairs,
+                        int64_t n_pairs, int32_t order, int32_t depth,
+                        double* weights_out, ed_stream stream);
+
+/* Near correction apply: replaces the CUDA extension's near_quadrature_matvec
+ * (core/bem_near_cuda.cpp:61-233, core/bem_near_cuda_kernel.cu:110-232) and
+ * _apply_near_quadrature_matvec_torch (core/bem.py:675-734):
+ * out = v_far + sum_j sigma_j * delta_ij.  When weights is NULL, delta_ij is
+ * computed on the fly with the unrefined order-`quad_order` rule, as the CUDA
+ * kernel does.  Deterministic (pairs must be i-major; no atomics). */
+int ed_near_apply_f64(const double* v_far, const double* sigma,
+                      const double* centroids, const double* areas,
+                      const double* panel_vertices, const int64_t* pairs,
+                      const double* weights, int64_t n_pairs, int64_t n,
+                      int32_t quad_order, double* out, ed_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Morton keys: replaces morton_keys_from_points (fmm3d/geometry.py:736-799).*/
+/* points (n,3) device; bbox_center/bbox_half (3) host or NULL (tight bbox). */
+/* keys (n) device int64.  Bit-exact with the reference.                      */
+/* ------------------------------------------------------------------------ */
+int ed_morton_keys_f64(const double* points, int64_t n, int32_t level,
+                       const double* bbox_center, const double* bbox_half,
+                       int64_t* keys, ed_stream stream);
+int ed_morton_keys_f32(const float* points, int64_t n, int32_t level,
+                       const float* bbox_center, const float* bbox_half,
+                       int64_t* keys, ed_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* GMRES in FP64 on the device: replaces gmres_restart                       */
+/* (core/bem_solver.py:125-920) when A is the plan's operator.               */
+/* ------------------------------------------------------------------------ */
+typedef struct ed_gmres_opts {
+  int32_t restart;        /* Krylov dimension per cycle                      */
+  int32_t maxiter;        /* total Arnoldi steps                             */
+  double tol;             /* relative tolerance                              */
+  int32_t precond;        /* 0 none, 1 Jacobi on diag (1/diag, bad -> 1)     */
+  int32_t refine;         /* 1: outer FP64 iterative refinement with the     */
+                          /*    dense FP64 operator as the residual operator */
+  int32_t max_refine;     /* outer refinement steps                          */
+  double inner_tol;       /* relative tolerance of each inner solve          */
+  int32_t use_graph;      /* capture one Arnoldi step in a hipGraph          */
+  int32_t check_every;    /* host poll interval in Arnoldi steps             */
+} ed_gmres_opts;
+
+typedef struct ed_gmres_info {
+  int32_t code;           /* 0 converged, 1 maxiter, 2 stagnation, 3/5/6 non-finite */
+  int32_t iters;          /* Arnoldi steps (matvecs) in total                */
+  int32_t cycles;
+  int32_t refine_steps;
+  int32_t first_iter_below_tol; /* first step whose estimate met tol, or -1 */
+  double resid;           /* final true residual norm ||b - A x||_2          */
+  double resid0;          /* initial true residual norm                      */
+  double rel_resid;       /* resid / ||b||_2                                 */
+  double tol_abs;
+} ed_gmres_info;
+
+void ed_gmres_opts_default(ed_gmres_opts* opts);
+
+/* b (n), x (n, in: initial guess, out: solution), diag (n, Jacobi diagonal,
+ * may be NULL), self_integrals (n, per-area diagonal of the operator) — all
+ * device FP64 in original order. */
+int ed_gmres_solve(ed_plan* plan, const double* b, double* x, const double* diag,
+                   const double* self_integrals, const ed_gmres_opts* opts,
+                   ed_gmres_info* info, ed_stream stream);
+
+/* FP64 level-1 kernels for GMRES over an arbitrary operator callable: they
+ * replace the torch.dot / norm / axpy steps of gmres_restart's Arnoldi loop
+ * (core/bem_solver.py:536-541, :771-773).  Deterministic reductions.
+ * ed_vec_dot_f64 needs >= 241 doubles of device scratch and synchronises. */
+int ed_vec_dot_f64(const double* x, const double* y, int64_t n, double* scratch,
+                   double* result_host, ed_stream stream);
+int ed_vec_axpy_f64(double a, const double* x, double* y, int64_t n, ed_stream stream);
+int ed_vec_scale_f64(double a, const double* x, double* y, int64_t n, ed_stream stream);
+/* y = d .* x (Jacobi apply, core/bem_solver.py:80-83) */
+int ed_vec_mul_f64(const double* d, const double* x, double* y, int64_t n, ed_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Host-only helpers (no device needed): expose the plan builder's octree,   */
+/* lists and operator tables so they can be checked on a CPU-only machine.   */
+/* ------------------------------------------------------------------------ */
+/* Builds the lattice tree on the host.  Returns counts in out_counts[0..3] =
+ * n_nodes, n_leaves, n_v_pairs, n_u_pairs; when the array pointers are
+ * non-NULL also fills: perm (n), node_level (n_nodes), node_begin, node_end,
+ * node_parent (n_nodes), v_pairs (n_v_pairs,2) as (target, source),
+ * u_pairs (n_u_pairs,2) as (target leaf, source leaf). */
+int ed_host_tree_build(const double* centroids, int64_t n, int32_t leaf_size,
+                       int32_t max_level, int32_t ws, int64_t* out_counts,
+                       int64_t* perm, int32_t* node_level, int64_t* node_begin,
+                       int64_t* node_end, int64_t* node_parent,
+                       int64_t* v_pairs, int64_t* u_pairs);
+
+/* Operator tables in FP64: kind 0 = M2L for integer offset (ox,oy,oz)
+ * (unscaled, caller multiplies by 1/s), kind 1 = M2M child octant, kind 2 =
+ * L2L child octant (octant = x | y<<1 | z<<2).  out is (p+1)^2 x (p+1)^2
+ * row-major. */
+int ed_host_operator(int32_t order, int32_t kind, int32_t ox, int32_t oy,
+                     int32_t oz, int32_t octant, double* out);
+
+/* Morton keys on the host (same arithmetic as the device kernel). */
+int ed_host_morton_keys_f64(const double* points, int64_t n, int32_t level,
+                            const double* bbox_center, const double* bbox_half,
+                            int64_t* keys);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EDNATIVE_H */

@@ -1,0 +1,30 @@
+"""CPU oracle for the Electrodrive BEM/FMM solve hot path.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the ``electrodrive_amd`` package imports,
+links or executes anything under ``oracle/``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` use it,
+and only as the checker (or as the timed CPU port of the reference), never as
+the product path.
+
+Each function restates the reference algorithm it names (``file:line`` relative
+to the Electrodrive repository root) in plain numpy (FP64) or C.  Modules:
+
+- ``constants``   K_E (electrodrive/utils/config.py:34-35)
+- ``mesh``        panel centroids/areas/normals (core/bem_mesh.py:103-126),
+                  torus (core/bem_mesh.py:180-212), plane patch (:139-151)
+- ``morton``      Morton keys (fmm3d/geometry.py:671-799)
+- ``dense``       dense single-layer matvec (core/bem_kernel.py:157-230, :715-810)
+- ``quadrature``  triangle rules, near-singular refinement, self integrals,
+                  near pairs, near weights, near apply (core/bem_quadrature.py,
+                  core/bem.py:475-734, :1764-1816)
+- ``gmres``       restarted left-preconditioned GMRES (core/bem_solver.py:36-920)
+- ``analytic``    grounded-sphere Kelvin image solution (core/images.py:165-203)
+- ``fmm_math``    Laplace addition-theorem expansions and translation operators
+                  in the convention the native FMM uses (pinned against direct
+                  summation, not against a reference file)
+- ``lattice_tree`` restatement of the native lattice octree and its U/V lists,
+                  with a brute-force partition check
+- ``ref_tree``    the reference adaptive tight-bbox octree (fmm3d/tree.py:801-960)
+
+Parity status per function is recorded in each module header.
+"""
