@@ -9,9 +9,10 @@
 
 namespace ed {
 
-// Coulomb constant 1/(4 pi eps0), eps0 = 8.854187817e-12
-// (electrodrive/utils/config.py:34-35).
-constexpr double kKE = 8.9875517873681764e9;
+// Coulomb constant 1/(4 pi eps0), eps0 = 8.854187817e-12, exactly as the
+// double the reference computes (electrodrive/utils/config.py:34-35):
+// 8987551787.997911.
+constexpr double kKE = 0x1.0bd99415ffbb9p+33;
 
 void set_error(const std::string& msg);
 int fail(int code, const std::string& msg);
