@@ -49,11 +49,15 @@ _SIGS = {
     "ed_plan_opts_default": (None, [ctypes.POINTER(PlanOpts)]),
     "ed_plan_create": (c_i32, [c_vp, c_vp, c_i64, ctypes.POINTER(PlanOpts), ctypes.POINTER(c_vp)]),
     "ed_plan_destroy": (c_i32, [c_vp]),
+    "ed_plan_create_batched": (c_i32, [c_vp, c_vp, c_i64, c_vp, c_i64, ctypes.POINTER(PlanOpts),
+                                       ctypes.POINTER(c_vp)]),
+    "ed_plan_geometries": (c_i32, [c_vp, c_i64p, c_vp, c_i64]),
     "ed_plan_stats": (c_i32, [c_vp, c_vp, c_i32]),
     "ed_plan_permutation": (c_i32, [c_vp, c_vp]),
     "ed_fmm_matvec_f32": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ed_fmm_matvec_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ed_plan_attach_near": (c_i32, [c_vp, c_vp, c_vp, c_i64]),
+    "ed_fmm_profile": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
     "ed_dense_matvec_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     "ed_dense_matvec_f32": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     "ed_self_integrals_f64": (c_i32, [c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),

@@ -53,6 +53,9 @@ class FmmOperator:
 
 
 class BemOperator:
+    """``mesh`` is one TriMesh or a list of TriMesh (a batch of independent
+    geometries: one plan, block-diagonal operator, geometries never interact)."""
+
     def __init__(self, mesh, *, order=8, ws=1, leaf_size=64, max_level=16,
                  m2l_precision="fp32", refined_self=True, self_depth=3, near=True,
                  near_depth=3, near_order=2, distance_factor=1.5, device=None):
@@ -60,13 +63,20 @@ class BemOperator:
         self.device = torch.device(device or "cuda")
         if self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
-        self.mesh = mesh
-        self.n = mesh.n_panels
+        meshes = list(mesh) if isinstance(mesh, (list, tuple)) else [mesh]
+        self.meshes = meshes
+        self.mesh = meshes[0] if len(meshes) == 1 else None
+        sizes = [m.n_panels for m in meshes]
+        self.geom_offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        self.n = int(self.geom_offsets[-1])
+        cents = np.concatenate([m.centroids for m in meshes])
+        areas = np.concatenate([m.areas for m in meshes])
+        pverts = np.concatenate([m.panel_vertices for m in meshes])
         dev = self.device
         with torch.cuda.device(dev):
-            self.C = torch.as_tensor(mesh.centroids, dtype=torch.float64).to(dev).contiguous()
-            self.A = torch.as_tensor(mesh.areas, dtype=torch.float64).to(dev).contiguous()
-            self.PV = torch.as_tensor(mesh.panel_vertices, dtype=torch.float64).to(dev).contiguous()
+            self.C = torch.as_tensor(cents, dtype=torch.float64).to(dev).contiguous()
+            self.A = torch.as_tensor(areas, dtype=torch.float64).to(dev).contiguous()
+            self.PV = torch.as_tensor(pverts, dtype=torch.float64).to(dev).contiguous()
             if refined_self:
                 self.I_self, self.self_corr = bq.self_integrals(self.C, self.PV, self.A,
                                                                 depth=self_depth, order=near_order)
@@ -77,11 +87,19 @@ class BemOperator:
             self.near_pairs = None
             self.near_w = None
             if near and self.n > 1:
-                self.near_pairs = bq.build_near_pairs(self.C, self.A, distance_factor)
+                parts = []
+                for g in range(len(meshes)):
+                    b, e = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
+                    if e - b > 1:
+                        parts.append(bq.build_near_pairs(self.C[b:e], self.A[b:e],
+                                                         distance_factor) + b)
+                self.near_pairs = (torch.cat(parts) if parts
+                                   else torch.zeros((0, 2), dtype=torch.int64, device=dev))
                 self.near_w = bq.near_correction_weights(self.C, self.A, self.PV, self.near_pairs,
                                                          quad_order=near_order, max_depth=near_depth)
-            self.plan = FmmPlan(mesh.centroids, mesh.areas, leaf_size=leaf_size, order=order,
-                                ws=ws, max_level=max_level, m2l_precision=m2l_precision)
+            self.plan = FmmPlan(cents, areas, leaf_size=leaf_size, order=order, ws=ws,
+                                max_level=max_level, m2l_precision=m2l_precision,
+                                geom_offsets=None if len(meshes) == 1 else self.geom_offsets)
             if self.near_pairs is not None and self.near_pairs.shape[0] > 0:
                 self.plan.attach_near(self.near_pairs.cpu().numpy(), self.near_w.cpu().numpy())
         self.fmm = FmmOperator(self.plan, self.self_corr)
@@ -89,15 +107,22 @@ class BemOperator:
         self.diag = self.I_self.clamp_min(1e-18)
 
     # -- right-hand sides ------------------------------------------------------
-    def rhs_point_charges(self, charges, bc=0.0):
+    def rhs_point_charges(self, charges, bc=0.0, per_geometry=False):
         """b = bc - V_free with V_free = sum_k K_E q_k / max(|c - r_k|, 1e-12)
-        (core/bem.py:291-320, :2056).  charges: iterable of (q, (x, y, z))."""
-        C = self.C
+        (core/bem.py:291-320, :2056).  charges: iterable of (q, (x, y, z)).
+        With ``per_geometry`` (batches), ``charges`` holds one charge list per
+        geometry and each list acts on its own geometry only."""
         V = torch.zeros(self.n, dtype=torch.float64, device=self.device)
-        for q, pos in charges:
-            p = torch.tensor(pos, dtype=torch.float64, device=self.device)
-            r = torch.linalg.norm(C - p[None, :], dim=1).clamp_min(1e-12)
-            V = V + (K_E * float(q)) / r
+        if per_geometry and len(charges) != len(self.meshes):
+            raise ValueError("per_geometry charges need one list per geometry")
+        groups = charges if per_geometry else [charges] * len(self.meshes)
+        for g, chs in enumerate(groups):
+            b, e = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
+            C = self.C[b:e]
+            for q, pos in chs:
+                p = torch.tensor(pos, dtype=torch.float64, device=self.device)
+                r = torch.linalg.norm(C - p[None, :], dim=1).clamp_min(1e-12)
+                V[b:e] = V[b:e] + (K_E * float(q)) / r
         bcv = torch.as_tensor(bc, dtype=torch.float64, device=self.device)
         return (bcv - V).contiguous()
 
@@ -107,7 +132,10 @@ class BemOperator:
 
     def accurate(self, sigma):
         s = sigma.to(self.device, torch.float64).contiguous()
-        v = dense_matvec(s, self.C, self.A, self.self_corr)
+        v = torch.empty_like(s)
+        for g in range(len(self.meshes)):
+            b, e = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
+            v[b:e] = dense_matvec(s[b:e], self.C[b:e], self.A[b:e], self.self_corr[b:e])
         if self.near_pairs is not None and self.near_pairs.shape[0] > 0:
             v = bq.near_quadrature_matvec(v, s, self.C, self.A, None, self.near_pairs,
                                           weights=self.near_w)

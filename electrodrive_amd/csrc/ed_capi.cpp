@@ -20,7 +20,7 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-int m2l_shape(int p, int* ncp, int* kp, int* cols, int* enabled);
+int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled);
 void gmres_free(ed_plan* P);
 
 #define ED_HIP(call)                                                                   \
@@ -30,6 +30,20 @@ void gmres_free(ed_plan* P);
       return fail(e_ == hipErrorOutOfMemory ? ED_ERR_ALLOC : ED_ERR_DEVICE,            \
                   std::string(#call) + ": " + hipGetErrorString(e_));                  \
   } while (0)
+
+// Round-to-nearest-even conversions of the M2L tables to 16-bit formats.
+static uint16_t f32_to_bf16(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+static uint16_t f32_to_f16(float f) {
+  _Float16 h = (_Float16)f;
+  uint16_t u;
+  std::memcpy(&u, &h, 2);
+  return u;
+}
 
 template <typename T>
 static int upload(T** dst, const std::vector<T>& src) {
@@ -53,7 +67,8 @@ void plan_free(ed_plan* P) {
                   P->m2l_pair_src, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair, P->t_m2l,
                   P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
-                  P->near_w, P->cent64_t, P->area64_t, P->near_w64};
+                  P->near_w, P->cent64_t, P->area64_t, P->near_w64, P->m2l_ascale,
+                  P->m2l_binv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   gmres_free(P);
@@ -73,15 +88,10 @@ int plan_upload(ed_plan* P) {
   std::vector<float4> ctr(M);
   std::vector<int2> rng(M);
   std::vector<int32_t> par(M), oct(M), fc(M), nch(M);
-  P->inv_side.assign(H.depth + 1, 0.0f);
-  for (int l = 0; l <= H.depth; ++l) P->inv_side[l] = (float)(1.0 / (2.0 * H.half / std::ldexp(1.0, l)));
   for (int64_t i = 0; i < M; ++i) {
-    const int l = H.level[i];
-    const double s = 2.0 * H.half / std::ldexp(1.0, l);
-    const double cx = -H.half + (H.coord[3 * i] + 0.5) * s;
-    const double cy = -H.half + (H.coord[3 * i + 1] + 0.5) * s;
-    const double cz = -H.half + (H.coord[3 * i + 2] + 0.5) * s;
-    ctr[i] = make_float4((float)cx, (float)cy, (float)cz, (float)(1.0 / s));
+    const double s = H.node_s[i];
+    ctr[i] = make_float4((float)H.node_c[3 * i], (float)H.node_c[3 * i + 1],
+                         (float)H.node_c[3 * i + 2], (float)(1.0 / s));
     rng[i] = make_int2((int)H.begin[i], (int)H.end[i]);
     par[i] = (int32_t)std::max<int64_t>(H.parent[i], 0);
     oct[i] = (H.coord[3 * i] & 1) | ((H.coord[3 * i + 1] & 1) << 1) | ((H.coord[3 * i + 2] & 1) << 2);
@@ -101,7 +111,7 @@ int plan_upload(ed_plan* P) {
 
   // leaves and U lists
   std::vector<int32_t> leaves, uptr{0}, usrc;
-  int64_t p2p_int = 0;
+  int64_t p2p_int = 0, p2p_src = 0;
   for (int64_t i = 0; i < M; ++i) {
     if (!H.is_leaf[i]) continue;
     leaves.push_back((int32_t)i);
@@ -109,12 +119,14 @@ int plan_upload(ed_plan* P) {
       const int64_t s = H.u_src[q];
       usrc.push_back((int32_t)s);
       p2p_int += (H.end[i] - H.begin[i]) * (H.end[s] - H.begin[s]);
+      p2p_src += H.end[s] - H.begin[s];
     }
     uptr.push_back((int32_t)usrc.size());
   }
   P->n_leaves = (int64_t)leaves.size();
   P->n_u = (int64_t)usrc.size();
   P->n_p2p_interactions = p2p_int;
+  P->n_p2p_src = p2p_src;
   ED_TRY(upload(&P->leaves, leaves));
   ED_TRY(upload(&P->u_ptr, uptr));
   ED_TRY(upload(&P->u_src, usrc));
@@ -150,23 +162,49 @@ int plan_upload(ed_plan* P) {
   const int R = offset_radius(P->ws), W = 2 * R + 1;
   std::vector<double> Td((size_t)nc * nc);
   std::vector<float> tv((size_t)std::max(ntab, 1) * nc * nc, 0.0f);
+  const int prec = P->m2l_precision;
   int ncp = 0, kp = 0, cols = 0, enabled = 0;
-  ED_TRY(m2l_shape(p, &ncp, &kp, &cols, &enabled));
+  ED_TRY(m2l_shape(p, prec < 0 ? 0 : prec, &ncp, &kp, &cols, &enabled));
+  if (prec < 0) enabled = 0;
   P->ncp = ncp;
-  std::vector<float> tm((size_t)std::max(ntab, 1) * ncp * kp, 0.0f);
+  // Balanced scaling for the GEMM form: with a_n^m = sqrt((n+m)!(n-m)!) the
+  // table diag(1/a) T diag(1/a) has entries below 1 in magnitude (raw
+  // entries reach 1e17 at p = 10), so it survives fp16 / bf16 rounding.
+  std::vector<double> asc(nc);
+  for (int n2 = 0; n2 <= p; ++n2)
+    for (int m2 = 0; m2 <= n2; ++m2) {
+      const double f = std::sqrt(std::tgamma(n2 + m2 + 1.0) * std::tgamma(n2 - m2 + 1.0));
+      asc[n2 * n2 + (m2 == 0 ? 0 : 2 * m2 - 1)] = f;
+      if (m2 > 0) asc[n2 * n2 + 2 * m2] = f;
+    }
+  const size_t tsz = (size_t)std::max(ntab, 1) * ncp * kp;
+  std::vector<float> tm(enabled && prec == 0 ? tsz : 1, 0.0f);
+  std::vector<uint16_t> th(enabled && prec > 0 ? tsz : 1, 0);
   for (auto& kv : slot_map) {
     const int slot = kv.first, tab = kv.second;
     const int ox = slot % W - R, oy = (slot / W) % W - R, oz = slot / (W * W) - R;
     m2l_matrix(p, ox, oy, oz, Td.data());
     for (int k = 0; k < nc; ++k)
       for (int j = 0; j < nc; ++j) {
-        const float v = (float)Td[(size_t)k * nc + j];
-        tv[(size_t)tab * nc * nc + (size_t)j * nc + k] = v;
-        tm[(size_t)tab * ncp * kp + (size_t)k * kp + j] = v;
+        const double t = Td[(size_t)k * nc + j];
+        tv[(size_t)tab * nc * nc + (size_t)j * nc + k] = (float)t;
+        if (!enabled) continue;
+        const double sc = t / (asc[k] * asc[j]);
+        const size_t at = (size_t)tab * ncp * kp + (size_t)k * kp + j;
+        if (prec == 0) tm[at] = (float)sc;
+        else th[at] = prec == 1 ? f32_to_bf16((float)sc) : f32_to_f16((float)sc);
       }
   }
   ED_TRY(upload(&P->t_m2l, tv));
   ED_TRY(upload(&P->t_m2l_mfma, tm));
+  ED_TRY(upload(&P->t_m2l_h, th));
+  // T = diag(a) T_s diag(a): the GEMM kernels multiply each gathered
+  // multipole coefficient by a (ascale) and the reduce multiplies each summed
+  // row by a (binv), so L = T M exactly up to rounding.
+  std::vector<float> af(nc), bf(nc);
+  for (int k = 0; k < nc; ++k) { af[k] = (float)asc[k]; bf[k] = (float)asc[k]; }
+  ED_TRY(upload(&P->m2l_ascale, af));
+  ED_TRY(upload(&P->m2l_binv, bf));
   std::vector<float> t8((size_t)8 * nc * nc), l8((size_t)8 * nc * nc);
   for (int o = 0; o < 8; ++o) {
     m2m_matrix(p, o, Td.data());
@@ -249,8 +287,9 @@ void ed_plan_opts_default(ed_plan_opts* o) {
   o->m2l_precision = 0;
 }
 
-int ed_plan_create(const double* centroids, const double* areas, int64_t n,
-                   const ed_plan_opts* opts, ed_plan** out) {
+static int plan_create_impl(const double* centroids, const double* areas, int64_t n,
+                            const int64_t* geom_off, int64_t n_geoms, const ed_plan_opts* opts,
+                            ed_plan** out) {
   if (!out) return fail(ED_ERR_ARG, "ed_plan_create: out is NULL");
   *out = nullptr;
   if (!centroids || !areas) return fail(ED_ERR_ARG, "ed_plan_create: NULL geometry");
@@ -262,6 +301,14 @@ int ed_plan_create(const double* centroids, const double* areas, int64_t n,
     return fail(ED_ERR_ARG, "ed_plan_create: order must be one of 4, 6, 8, 10, 12");
   if (o.m2l_precision < -1 || o.m2l_precision > 2)
     return fail(ED_ERR_ARG, "ed_plan_create: m2l_precision must be -1, 0, 1 or 2");
+  if (n_geoms < 1) return fail(ED_ERR_ARG, "ed_plan_create: n_geoms must be >= 1");
+  if (geom_off) {
+    if (geom_off[0] != 0 || geom_off[n_geoms] != n)
+      return fail(ED_ERR_ARG, "ed_plan_create_batched: offsets must run from 0 to n");
+    for (int64_t g = 0; g < n_geoms; ++g)
+      if (geom_off[g + 1] <= geom_off[g])
+        return fail(ED_ERR_ARG, "ed_plan_create_batched: every geometry needs >= 1 panel");
+  }
   for (int64_t i = 0; i < n; ++i)
     if (!std::isfinite(areas[i])) return fail(ED_ERR_ARG, "ed_plan_create: non-finite area");
   int dev = 0;
@@ -273,26 +320,36 @@ int ed_plan_create(const double* centroids, const double* areas, int64_t n,
   P->leaf_size = o.leaf_size;
   P->max_level = o.max_level;
   P->m2l_precision = o.m2l_precision;
-  int rc = build_host_tree(centroids, n, o.leaf_size, o.max_level, o.ws, P->host);
+  int rc = ED_OK;
+  if (!geom_off || n_geoms == 1) {
+    rc = build_host_tree(centroids, n, o.leaf_size, o.max_level, o.ws, P->host);
+  } else {
+    std::vector<HostTree> parts((size_t)n_geoms);
+    for (int64_t g = 0; g < n_geoms && rc == ED_OK; ++g)
+      rc = build_host_tree(centroids + 3 * geom_off[g], geom_off[g + 1] - geom_off[g],
+                           o.leaf_size, o.max_level, o.ws, parts[(size_t)g]);
+    if (rc == ED_OK) rc = merge_host_trees(parts, P->host);
+  }
   if (rc == ED_OK) rc = plan_upload(P);
   if (rc == ED_OK) {
     const HostTree& H = P->host;
     std::vector<float4> pts(n);
     std::vector<float> at(n);
     std::vector<int32_t> perm(n);
-    for (int64_t i = 0; i < n; ++i) {
-      const int64_t o2 = H.perm[i];
-      pts[i] = make_float4((float)(centroids[3 * o2] - H.center[0]),
-                           (float)(centroids[3 * o2 + 1] - H.center[1]),
-                           (float)(centroids[3 * o2 + 2] - H.center[2]), 0.0f);
-      at[i] = (float)areas[o2];
-      perm[i] = (int32_t)o2;
-    }
     std::vector<double> c64(3 * n), a64(n);
-    for (int64_t i = 0; i < n; ++i) {
-      const int64_t o2 = H.perm[i];
-      for (int a = 0; a < 3; ++a) c64[3 * i + a] = centroids[3 * o2 + a];
-      a64[i] = areas[o2];
+    const int64_t G = (int64_t)H.geom_pt_off.size() - 1;
+    for (int64_t g = 0; g < G; ++g) {
+      const double* gc = H.geom_center.data() + 3 * g;
+      for (int64_t i = H.geom_pt_off[g]; i < H.geom_pt_off[g + 1]; ++i) {
+        const int64_t o2 = H.perm[i];
+        pts[i] = make_float4((float)(centroids[3 * o2] - gc[0]),
+                             (float)(centroids[3 * o2 + 1] - gc[1]),
+                             (float)(centroids[3 * o2 + 2] - gc[2]), 0.0f);
+        at[i] = (float)areas[o2];
+        perm[i] = (int32_t)o2;
+        for (int a = 0; a < 3; ++a) c64[3 * i + a] = centroids[3 * o2 + a];
+        a64[i] = areas[o2];
+      }
     }
     rc = upload(&P->pts, pts);
     if (rc == ED_OK) rc = upload(&P->cent64_t, c64);
@@ -314,6 +371,30 @@ int ed_plan_create(const double* centroids, const double* areas, int64_t n,
   return ED_OK;
 }
 
+int ed_plan_create(const double* centroids, const double* areas, int64_t n,
+                   const ed_plan_opts* opts, ed_plan** out) {
+  return plan_create_impl(centroids, areas, n, nullptr, 1, opts, out);
+}
+
+int ed_plan_create_batched(const double* centroids, const double* areas, int64_t n,
+                           const int64_t* geom_offsets, int64_t n_geoms,
+                           const ed_plan_opts* opts, ed_plan** out) {
+  if (!geom_offsets) return fail(ED_ERR_ARG, "ed_plan_create_batched: NULL offsets");
+  return plan_create_impl(centroids, areas, n, geom_offsets, n_geoms, opts, out);
+}
+
+int ed_plan_geometries(const ed_plan* P, int64_t* n_geoms_out, int64_t* offsets_host,
+                       int64_t capacity) {
+  if (!P || !n_geoms_out) return fail(ED_ERR_ARG, "ed_plan_geometries: NULL argument");
+  const int64_t G = (int64_t)P->host.geom_pt_off.size() - 1;
+  *n_geoms_out = G;
+  if (offsets_host) {
+    if (capacity < G + 1) return fail(ED_ERR_ARG, "ed_plan_geometries: capacity");
+    for (int64_t g = 0; g <= G; ++g) offsets_host[g] = P->host.geom_pt_off[g];
+  }
+  return ED_OK;
+}
+
 int ed_plan_destroy(ed_plan* P) {
   if (!P) return ED_OK;
   plan_free(P);
@@ -330,10 +411,10 @@ int ed_plan_stats(const ed_plan* P, int64_t* out, int32_t n_out) {
       any = P->host.v_ptr[t + 1] > P->host.v_ptr[t];
     levels_with_m2l += any;
   }
-  const int64_t vals[12] = {P->n, P->n_nodes, P->n_leaves, P->host.depth, P->n_u,
+  const int64_t vals[13] = {P->n, P->n_nodes, P->n_leaves, P->host.depth, P->n_u,
                             P->n_p2p_interactions, P->n_v, P->n_off, P->p, P->ws,
-                            P->n_near, levels_with_m2l};
-  for (int i = 0; i < n_out && i < 12; ++i) out[i] = vals[i];
+                            P->n_near, levels_with_m2l, P->n_p2p_src};
+  for (int i = 0; i < n_out && i < 13; ++i) out[i] = vals[i];
   return ED_OK;
 }
 

@@ -227,12 +227,24 @@ struct M2LShape {
   static constexpr bool ENABLED = P <= 10;          // LDS budget (160 KiB)
 };
 
+// 16-bit MFMA (v_mfma_f32_32x32x16_{bf16,f16}, gfx950): K steps of 16.
+template <int P>
+struct M2LShapeH {
+  static constexpr int NC = (P + 1) * (P + 1);
+  static constexpr int NCP = (NC + 31) / 32 * 32;
+  static constexpr int KP = (NC + 15) / 16 * 16;
+  static constexpr int KS = KP + 8;  // row stride in elements (16-byte aligned)
+  static constexpr int COLS = 128;
+  static constexpr int THREADS = 256;
+};
+
 template <int P>
 __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t* __restrict__ tile_tab,
                                                   const int32_t* __restrict__ tile_beg,
                                                   const int32_t* __restrict__ tile_cnt,
                                                   const int32_t* __restrict__ pair_src,
                                                   const float* __restrict__ Tm,
+                                                  const float* __restrict__ ascale,
                                                   const float* __restrict__ M,
                                                   float* __restrict__ Y) {
   using S = M2LShape<P>;
@@ -250,7 +262,7 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
   for (int t = threadIdx.x; t < COLS * KP; t += NT) {
     const int c = t / KP, k = t - c * KP;
     float v = 0.0f;
-    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k];
+    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k] * ascale[k];
     bm[c * KS + k] = v;
   }
   __syncthreads();
@@ -289,11 +301,96 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
   }
 }
 
+// 16-bit M2L GEMM: table and gathered (scaled) multipoles rounded to bf16 or
+// fp16, FP32 accumulation.  Lane layout of v_mfma_f32_32x32x16: A[i][k] for
+// k = 8*(lane/32) .. +7 from lane i + 32*(k/8), B likewise with j = lane%32;
+// D as in the fp32 kernel.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T16> struct Mfma16;
+template <> struct Mfma16<__bf16> {
+  typedef bf16x8 vec;
+  static __device__ __forceinline__ f32x16 run(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+};
+template <> struct Mfma16<_Float16> {
+  typedef f16x8 vec;
+  static __device__ __forceinline__ f32x16 run(vec a, vec b, f32x16 c) {
+    return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+  }
+};
+
+template <int P, typename T16>
+__global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ tile_tab,
+                                                    const int32_t* __restrict__ tile_beg,
+                                                    const int32_t* __restrict__ tile_cnt,
+                                                    const int32_t* __restrict__ pair_src,
+                                                    const T16* __restrict__ Tm,
+                                                    const float* __restrict__ ascale,
+                                                    const float* __restrict__ M,
+                                                    float* __restrict__ Y) {
+  using S = M2LShapeH<P>;
+  using V = typename Mfma16<T16>::vec;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
+  __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
+  __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
+  const int tile = blockIdx.x;
+  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
+  const T16* Tt = Tm + (size_t)tab * NCP * KP;
+  for (int t = threadIdx.x; t < NCP * KP; t += 256) {
+    const int i = t / KP, k = t - i * KP;
+    ta[i * KS + k] = Tt[t];
+  }
+  for (int t = threadIdx.x; t < COLS * KP; t += 256) {
+    const int c = t / KP, k = t - c * KP;
+    float v = 0.0f;
+    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k] * ascale[k];
+    bm[c * KS + k] = (T16)v;
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col0 = wave * 32;
+  if (col0 >= cnt) return;
+  const int li = lane & 31, lk = lane >> 5;
+  constexpr int NRB = NCP / 32;
+  f32x16 acc[NRB];
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
+  const T16* brow = bm + (col0 + li) * KS + 8 * lk;
+#pragma unroll 2
+  for (int s = 0; s < KP / 16; ++s) {
+    const V b = *reinterpret_cast<const V*>(brow + 16 * s);
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+      const V a = *reinterpret_cast<const V*>(ta + (rb * 32 + li) * KS + 16 * s + 8 * lk);
+      acc[rb] = Mfma16<T16>::run(a, b, acc[rb]);
+    }
+  }
+  const int col = col0 + li;
+  if (col < cnt) {
+    float* yrow = Y + (size_t)(beg + col) * NCP;
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = rb * 32 + q * 8 + lk * 4;
+        float4 v = make_float4(acc[rb][4 * q + 0], acc[rb][4 * q + 1], acc[rb][4 * q + 2],
+                               acc[rb][4 * q + 3]);
+        *reinterpret_cast<float4*>(yrow + row) = v;
+      }
+  }
+}
+
 // Deterministic per-target sum of the GEMM rows: L[t] = inv_s * sum Y[row].
 template <int P>
 __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __restrict__ tptr,
                                                     const int32_t* __restrict__ trow,
                                                     const float4* __restrict__ ctr,
+                                                    const float* __restrict__ binv,
                                                     const float* __restrict__ Y,
                                                     float* __restrict__ L) {
   using S = M2LShape<P>;
@@ -304,7 +401,7 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
   for (int k = threadIdx.x; k < NC; k += blockDim.x) {
     float s = 0.0f;
     for (int q = b; q < e; ++q) s += Y[(size_t)trow[q] * NCP + k];
-    L[(size_t)t * NC + k] = s * is;
+    L[(size_t)t * NC + k] = s * (is * binv[k]);
   }
 }
 
@@ -333,6 +430,25 @@ __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restric
 // ---------------------------------------------------------------------------
 // Leaf evaluation: L2P + P2P (U list) + near CSR + self, scatter.
 // ---------------------------------------------------------------------------
+// One 64-lane wave per target leaf.  The lanes are split into S = 64 / n_t
+// slices (n_t = targets in the chunk, at most 64); slice s takes every S-th
+// source point of the stream, so small leaves still use the whole wave.
+// Sources are streamed through LDS 256 points at a time: first the target
+// leaf itself (with the j != i exclusion), then the concatenation of the other
+// U-list leaves.  Slice partial sums are combined in a fixed order.
+constexpr int kSrcChunk = 256;
+constexpr int kUSeg = 256;
+
+__device__ __forceinline__ int wave_incl_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  return v;
+}
+
 template <int P, typename TO>
 __global__ __launch_bounds__(kLeafBlock) void k_leaf(
     const int32_t* __restrict__ leaves, const int2* __restrict__ rng,
@@ -344,50 +460,102 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf(
     const int32_t* __restrict__ perm, int out_tree, float ke, TO* __restrict__ out) {
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float lc[NC];
-  __shared__ float4 src[kLeafBlock];
+  __shared__ float4 src[kSrcChunk];
+  __shared__ int ubeg[kUSeg];
+  __shared__ int upref[kUSeg + 1];
+  __shared__ float red[kLeafBlock];
+  const int lane = threadIdx.x;
   const int li = blockIdx.x;
   const int node = leaves[li];
   const int2 r = rng[node];
   const float4 c = ctr[node];
-  for (int j = threadIdx.x; j < NC; j += kLeafBlock) lc[j] = L[(size_t)node * NC + j];
+  for (int j = lane; j < NC; j += kLeafBlock) lc[j] = L[(size_t)node * NC + j];
   const int ub = u_ptr[li], ue = u_ptr[li + 1];
   for (int t0 = r.x; t0 < r.y; t0 += kLeafBlock) {
-    const int i = t0 + threadIdx.x;
-    const bool active = i < r.y;
-    float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (active) xi = pts[i];
-    __syncthreads();
-    float far = 0.0f;
-    if (active) far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w, lc);
-    float nearf = 0.0f;
-    for (int u = ub; u < ue; ++u) {
-      const int2 sr = rng[u_src[u]];
-      for (int s0 = sr.x; s0 < sr.y; s0 += kLeafBlock) {
-        const int ns = min(kLeafBlock, sr.y - s0);
+    const int ntc = min(kLeafBlock, r.y - t0);
+    const int S = kLeafBlock / ntc;
+    const int tl = lane % ntc, sl = lane / ntc;
+    const bool act = sl < S;
+    const int i = t0 + tl;
+    const float4 xi = pts[i];
+    float acc = 0.0f;
+    // (1) the target leaf itself, excluding j == i
+    for (int s0 = r.x; s0 < r.y; s0 += kSrcChunk) {
+      const int ns = min(kSrcChunk, r.y - s0);
+      __syncthreads();
+      for (int e = lane; e < ns; e += kLeafBlock) src[e] = pts[s0 + e];
+      __syncthreads();
+      if (act) {
+        const int me = i - s0;
+        for (int j = sl; j < ns; j += S) {
+          const float4 sj = src[j];
+          const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
+          const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
+          const float v = sj.w * __builtin_amdgcn_rsqf(r2);
+          acc += (j == me) ? 0.0f : v;
+        }
+      }
+    }
+    // (2) the other U-list leaves as one concatenated stream
+    for (int g0 = ub; g0 < ue; g0 += kUSeg) {
+      const int ng = min(kUSeg, ue - g0);
+      __syncthreads();
+      int part = 0;
+      int sz[kUSeg / kLeafBlock];
+#pragma unroll
+      for (int k = 0; k < kUSeg / kLeafBlock; ++k) {
+        const int q = lane * (kUSeg / kLeafBlock) + k;
+        int size = 0;
+        if (q < ng) {
+          const int sn = u_src[g0 + q];
+          const int2 sr = rng[sn];
+          size = (sn == node) ? 0 : sr.y - sr.x;
+          ubeg[q] = sr.x;
+        }
+        sz[k] = size;
+        part += size;
+      }
+      const int incl = wave_incl_scan(part);
+      int run = incl - part;
+#pragma unroll
+      for (int k = 0; k < kUSeg / kLeafBlock; ++k) {
+        const int q = lane * (kUSeg / kLeafBlock) + k;
+        if (q < ng) upref[q] = run;
+        run += sz[k];
+      }
+      const int total = __shfl(incl, 63, 64);
+      if (lane == 0) upref[ng] = total;
+      __syncthreads();
+      for (int e0 = 0; e0 < total; e0 += kSrcChunk) {
+        const int ns = min(kSrcChunk, total - e0);
         __syncthreads();
-        if (threadIdx.x < ns) src[threadIdx.x] = pts[s0 + threadIdx.x];
-        __syncthreads();
-        const bool selfblk = (s0 <= i) && (i < s0 + ns);
-        if (!selfblk) {
-          for (int j = 0; j < ns; ++j) {
-            const float4 sj = src[j];
-            const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
-            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-            nearf += sj.w * __builtin_amdgcn_rsqf(r2);
+        for (int e = lane; e < ns; e += kLeafBlock) {
+          const int ge = e0 + e;
+          int lo = 0, hi = ng - 1;  // last q with upref[q] <= ge
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (upref[mid] <= ge) lo = mid; else hi = mid - 1;
           }
-        } else {
-          const int me = i - s0;
-          for (int j = 0; j < ns; ++j) {
+          src[e] = pts[ubeg[lo] + (ge - upref[lo])];
+        }
+        __syncthreads();
+        if (act) {
+          for (int j = sl; j < ns; j += S) {
             const float4 sj = src[j];
             const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
             const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-            const float v = sj.w * __builtin_amdgcn_rsqf(r2);
-            nearf += (j == me) ? 0.0f : v;
+            acc += sj.w * __builtin_amdgcn_rsqf(r2);
           }
         }
       }
     }
-    if (active) {
+    __syncthreads();
+    red[lane] = act ? acc : 0.0f;
+    __syncthreads();
+    if (sl == 0) {
+      float nearf = 0.0f;
+      for (int k = 0; k < S; ++k) nearf += red[tl + k * ntc];
+      const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w, lc);
       float v = ke * (far + nearf) + selfc_t[i] * xi.w;
       const int nb = near_ptr[i], ne = near_ptr[i + 1];
       float corr = 0.0f;
@@ -409,11 +577,18 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf(
                                                          hipGetErrorString(e_)); \
   } while (0)
 
+// Stage boundary markers for ed_fmm_profile (no-op unless profiling).
+#define ED_MARK(k)                                                   \
+  do {                                                               \
+    if (Pn->prof_events) (void)hipEventRecord(Pn->prof_events[k], st); \
+  } while (0)
+
 template <int P>
 static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
                    void* out, int out64, int out_tree, hipStream_t st) {
   constexpr int NC = (P + 1) * (P + 1);
   const int n = (int)Pn->n;
+  ED_MARK(0);
   const int gb = (n + 255) / 256;
   if (sig64)
     hipLaunchKernelGGL(k_gather<double>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
@@ -424,6 +599,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
                        (const float*)sigma, sig_tree, (const float*)selfi, Pn->pts, Pn->area_t,
                        Pn->sig_t, Pn->selfc_t);
   ED_LAUNCH_CHECK("gather");
+  ED_MARK(1);
   const int nleaf = (int)Pn->n_leaves;
   const auto& H = Pn->host;
   const int depth = H.depth;
@@ -431,6 +607,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     hipLaunchKernelGGL(k_p2m<P>, dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
                        Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->M);
     ED_LAUNCH_CHECK("p2m");
+    ED_MARK(2);
     // upward pass: parents at level l (children at l+1), deepest first
     for (int l = depth - 1; l >= 2; --l) {
       const int64_t pb = Pn->parents_begin[l], pe = Pn->parents_begin[l + 1];
@@ -441,25 +618,40 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
         ED_LAUNCH_CHECK("m2m");
       }
     }
+    ED_MARK(3);
     // M2L for all nodes at levels >= 2
     const int node0 = (int)H.level_begin[2];
     const int nm = (int)(Pn->n_nodes - node0);
-    bool use_mfma = Pn->m2l_precision >= 0 && Pn->n_m2l_tiles > 0;
-    if constexpr (!M2LShape<P>::ENABLED) use_mfma = false;
-    if (!use_mfma) {
+    const int prec = Pn->n_m2l_tiles > 0 ? Pn->m2l_precision : -1;
+    bool valu = prec < 0;
+    if constexpr (!M2LShape<P>::ENABLED) valu = valu || prec == 0;
+    if (valu) {
       hipLaunchKernelGGL(k_m2l_valu<P>, dim3(nm), dim3(128), 0, st, node0, Pn->v_ptr,
                          Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Pn->M, Pn->L);
       ED_LAUNCH_CHECK("m2l_valu");
-    } else if constexpr (M2LShape<P>::ENABLED) {
-      hipLaunchKernelGGL(k_m2l_mfma<P>, dim3((unsigned)Pn->n_m2l_tiles),
-                         dim3(M2LShape<P>::THREADS), 0, st,
-                         Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
-                         Pn->m2l_pair_src, Pn->t_m2l_mfma, Pn->M, Pn->ypair);
+      ED_MARK(4);
+    } else {
+      const dim3 tiles((unsigned)Pn->n_m2l_tiles);
+      if (prec == 1) {
+        hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
+                           Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
+                           (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M, Pn->ypair);
+      } else if (prec == 2) {
+        hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
+                           Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
+                           (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M, Pn->ypair);
+      } else if constexpr (M2LShape<P>::ENABLED) {
+        hipLaunchKernelGGL(k_m2l_mfma<P>, tiles, dim3(M2LShape<P>::THREADS), 0, st,
+                           Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
+                           Pn->m2l_pair_src, Pn->t_m2l_mfma, Pn->m2l_ascale, Pn->M, Pn->ypair);
+      }
       ED_LAUNCH_CHECK("m2l_mfma");
+      ED_MARK(4);
       hipLaunchKernelGGL(k_m2l_reduce<P>, dim3(nm), dim3(128), 0, st, node0, Pn->m2l_tgt_ptr,
-                         Pn->m2l_tgt_row, Pn->node_ctr, Pn->ypair, Pn->L);
+                         Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Pn->ypair, Pn->L);
       ED_LAUNCH_CHECK("m2l_reduce");
     }
+    ED_MARK(5);
     // downward pass: levels 3..depth (levels 0..2 have no inherited local)
     for (int l = 3; l <= depth; ++l) {
       const int64_t b = H.level_begin[l], e = H.level_begin[l + 1];
@@ -467,7 +659,10 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
                          Pn->node_parent, Pn->node_octant, Pn->t_l2l, Pn->L);
       ED_LAUNCH_CHECK("l2l");
     }
+  } else {
+    for (int k = 2; k <= 5; ++k) ED_MARK(k);
   }
+  ED_MARK(6);
   const float ke = (float)kKE;
   if (out64)
     hipLaunchKernelGGL((k_leaf<P, double>), dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
@@ -480,6 +675,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
                        Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm,
                        out_tree, ke, (float*)out);
   ED_LAUNCH_CHECK("leaf");
+  ED_MARK(7);
   (void)NC;
   return ED_OK;
 }
@@ -496,12 +692,18 @@ int fmm_launch(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const vo
   }
 }
 
-int m2l_shape(int p, int* ncp, int* kp, int* cols, int* enabled) {
+int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled) {
   switch (p) {
 #define ED_SHAPE(PP)                                                          \
   case PP:                                                                    \
-    *ncp = M2LShape<PP>::NCP; *kp = M2LShape<PP>::KP;                         \
-    *cols = M2LShape<PP>::COLS; *enabled = M2LShape<PP>::ENABLED ? 1 : 0;     \
+    *ncp = M2LShape<PP>::NCP;                                                 \
+    if (prec == 0) {                                                          \
+      *kp = M2LShape<PP>::KP; *cols = M2LShape<PP>::COLS;                     \
+      *enabled = M2LShape<PP>::ENABLED ? 1 : 0;                               \
+    } else {                                                                  \
+      *kp = M2LShapeH<PP>::KP; *cols = M2LShapeH<PP>::COLS;                   \
+      *enabled = prec > 0 ? 1 : 0;                                            \
+    }                                                                         \
     return ED_OK;
     ED_SHAPE(4) ED_SHAPE(6) ED_SHAPE(8) ED_SHAPE(10) ED_SHAPE(12)
 #undef ED_SHAPE
@@ -510,3 +712,33 @@ int m2l_shape(int p, int* ncp, int* kp, int* cols, int* enabled) {
 }
 
 }  // namespace ed
+
+// Per-stage device time of the FMM matvec (FP64 in/out, original order),
+// averaged over `reps` launches: stage_ms[0..6] = gather, p2m, m2m, m2l (GEMM
+// or VALU), m2l reduce, l2l, leaf (L2P + P2P + near + self).
+extern "C" int ed_fmm_profile(ed_plan* P, const double* sigma, const double* selfi, double* out,
+                              int32_t reps, double* stage_ms, int32_t n_stages, ed_stream stream) {
+  using namespace ed;
+  if (!P || !sigma || !out || !stage_ms || reps < 1) return fail(ED_ERR_ARG, "ed_fmm_profile: bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  hipEvent_t ev[8];
+  for (int k = 0; k < 8; ++k)
+    if (hipEventCreate(&ev[k]) != hipSuccess) return fail(ED_ERR_DEVICE, "ed_fmm_profile: event");
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  int rc = ED_OK;
+  P->prof_events = ev;
+  for (int r = 0; r < reps && rc == ED_OK; ++r) {
+    rc = fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, st);
+    if (rc != ED_OK) break;
+    if (hipEventSynchronize(ev[7]) != hipSuccess) { rc = fail(ED_ERR_DEVICE, "ed_fmm_profile: sync"); break; }
+    for (int k = 0; k < 7; ++k) {
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      acc[k] += ms;
+    }
+  }
+  P->prof_events = nullptr;
+  for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
+  for (int k = 0; k < n_stages && k < 7; ++k) stage_ms[k] = acc[k] / reps;
+  return rc;
+}

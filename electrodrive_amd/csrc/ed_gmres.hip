@@ -385,7 +385,13 @@ struct Solver {
     return fmm_launch(P, in_tree, 1, 1, selfi_orig, out_tree, 1, 1, W->stream);
   }
   int accurate_op(const double* in_tree, double* out_tree) {
-    ED_T(dense_launch_f64(P->cent64_t, P->area64_t, in_tree, W->selfi_t, out_tree, n, 1, W->stream));
+    // block diagonal over the geometries of a batched plan
+    const auto& go = P->host.geom_pt_off;
+    for (size_t g = 0; g + 1 < go.size(); ++g) {
+      const int64_t b = go[g], ng = go[g + 1] - go[g];
+      ED_T(dense_launch_f64(P->cent64_t + 3 * b, P->area64_t + b, in_tree + b, W->selfi_t + b,
+                            out_tree + b, ng, 1, W->stream));
+    }
     if (P->n_near > 0) {
       hipLaunchKernelGGL(k_near_csr, dim3(gb), dim3(256), 0, W->stream, n, P->near_ptr,
                          P->near_col, P->near_w64, in_tree, out_tree);

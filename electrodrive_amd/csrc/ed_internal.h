@@ -43,11 +43,21 @@ struct HostTree {
   std::vector<int64_t> v_ptr, v_src;  // CSR over target nodes
   std::vector<int32_t> v_off;         // offset slot of each V entry
   std::vector<int64_t> u_ptr, u_src;  // CSR over all nodes (empty for non-leaves)
+  // node geometry in the frame of its own geometry (root centre at 0)
+  std::vector<double> node_c;         // 3 per node
+  std::vector<double> node_s;         // box side per node
+  // geometries (one for a single plan; several for a batched plan), in tree
+  // order each geometry's points form one contiguous range
+  std::vector<double> geom_center;    // 3 per geometry
+  std::vector<int64_t> geom_pt_off;   // n_geoms + 1
   int64_t n_leaves() const;
 };
 
 int build_host_tree(const double* centroids, int64_t n, int leaf_size,
                     int max_level, int ws, HostTree& t);
+// Concatenate per-geometry trees (point offsets = running sums of part.n)
+// into one tree whose nodes are ordered by level, then geometry.
+int merge_host_trees(std::vector<HostTree>& parts, HostTree& out);
 
 // Offset slot encoding: o in [-R, R]^3, R = 2 ws + 1.
 inline int offset_radius(int ws) { return 2 * ws + 1; }

@@ -34,7 +34,7 @@ struct ed_plan {
   int32_t* leaves = nullptr;
   int32_t* u_ptr = nullptr;
   int32_t* u_src = nullptr;  // node ids
-  int64_t n_u = 0, n_p2p_interactions = 0;
+  int64_t n_u = 0, n_p2p_interactions = 0, n_p2p_src = 0;
 
   // upward/downward passes: per level l, nodes of level l that have children
   std::vector<int64_t> parents_begin;  // host offsets into parents_list per level
@@ -62,8 +62,10 @@ struct ed_plan {
 
   // operator tables (column-major: T[tab][j][k] = out k <- in j)
   float* t_m2l = nullptr;       // n_off * nc * nc, fp32 (VALU path)
-  float* t_m2l_mfma = nullptr;  // n_off * ncp * ncp, row-major padded (MFMA A operand)
-  uint16_t* t_m2l_h = nullptr;  // n_off * ncp * ncp, bf16/fp16
+  float* t_m2l_mfma = nullptr;  // n_off * ncp * kp, row-major padded, scaled (MFMA A)
+  uint16_t* t_m2l_h = nullptr;  // n_off * ncp * kp16, bf16/fp16 (scaled)
+  float* m2l_ascale = nullptr;  // per input coefficient: sqrt((n+m)!(n-m)!)
+  float* m2l_binv = nullptr;    // per output coefficient: sqrt((n+m)!(n-m)!)
   float* t_m2m = nullptr;       // 8 * nc * nc
   float* t_l2l = nullptr;       // 8 * nc * nc
 
@@ -73,13 +75,14 @@ struct ed_plan {
   float* near_w = nullptr;
   int64_t n_near = 0;
 
-  // per-level inverse box side
-  std::vector<float> inv_side;
 
   // FP64 copies in tree order for the accurate (dense) residual operator
   double* cent64_t = nullptr;  // n x 3
   double* area64_t = nullptr;  // n
   double* near_w64 = nullptr;  // near CSR weights in FP64
+
+  // stage events while ed_fmm_profile runs (8 markers), else NULL
+  hipEvent_t* prof_events = nullptr;
 
   // GMRES workspace (allocated lazily by the solver, owned by the plan)
   void* gmres_ws = nullptr;

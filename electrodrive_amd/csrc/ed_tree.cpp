@@ -175,6 +175,15 @@ int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
   t.depth = lvl - 1;
   t.level_begin.push_back((int64_t)t.level.size());
   const int64_t M = (int64_t)t.level.size();
+  t.node_c.resize(3 * M);
+  t.node_s.resize(M);
+  for (int64_t i = 0; i < M; ++i) {
+    const double side = 2.0 * half / std::ldexp(1.0, t.level[i]);
+    t.node_s[i] = side;
+    for (int a = 0; a < 3; ++a) t.node_c[3 * i + a] = -half + (t.coord[3 * i + a] + 0.5) * side;
+  }
+  t.geom_center.assign(t.center, t.center + 3);
+  t.geom_pt_off = {0, n};
 
   // Locational-code hash: (1 << 3l) | prefix -> node.
   std::unordered_map<uint64_t, int64_t> where;
@@ -254,6 +263,93 @@ int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
     t.u_src.insert(t.u_src.end(), us.begin(), us.end());
     t.u_ptr[nidx + 1] = (int64_t)t.u_src.size();
   }
+  return ED_OK;
+}
+
+int merge_host_trees(std::vector<HostTree>& parts, HostTree& out) {
+  if (parts.empty()) return fail(ED_ERR_ARG, "merge: no geometries");
+  if (parts.size() == 1) { out = std::move(parts[0]); return ED_OK; }
+  out = HostTree();
+  const size_t G = parts.size();
+  out.leaf_size = parts[0].leaf_size;
+  out.max_level = parts[0].max_level;
+  out.ws = parts[0].ws;
+  std::vector<int64_t> pt_off(G + 1, 0);
+  int depth = 0;
+  for (size_t g = 0; g < G; ++g) {
+    pt_off[g + 1] = pt_off[g] + parts[g].n;
+    depth = std::max(depth, parts[g].depth);
+  }
+  out.n = pt_off[G];
+  out.depth = depth;
+  out.geom_pt_off = pt_off;
+  // new node index: level-major, then geometry, then the part's own order
+  std::vector<std::vector<int64_t>> nid(G);
+  int64_t next = 0;
+  for (size_t g = 0; g < G; ++g) nid[g].assign(parts[g].level.size(), -1);
+  for (int l = 0; l <= depth; ++l) {
+    out.level_begin.push_back(next);
+    for (size_t g = 0; g < G; ++g) {
+      const HostTree& t = parts[g];
+      if (l > t.depth) continue;
+      for (int64_t i = t.level_begin[l]; i < t.level_begin[l + 1]; ++i) nid[g][i] = next++;
+    }
+  }
+  out.level_begin.push_back(next);
+  const int64_t M = next;
+  out.level.assign(M, 0);
+  out.begin.assign(M, 0);
+  out.end.assign(M, 0);
+  out.parent.assign(M, -1);
+  out.first_child.assign(M, -1);
+  out.n_child.assign(M, 0);
+  out.coord.assign(3 * M, 0);
+  out.is_leaf.assign(M, 0);
+  out.node_c.assign(3 * M, 0.0);
+  out.node_s.assign(M, 0.0);
+  for (size_t g = 0; g < G; ++g) {
+    const HostTree& t = parts[g];
+    for (int64_t i = 0; i < (int64_t)t.level.size(); ++i) {
+      const int64_t k = nid[g][i];
+      out.level[k] = t.level[i];
+      out.begin[k] = t.begin[i] + pt_off[g];
+      out.end[k] = t.end[i] + pt_off[g];
+      out.parent[k] = t.parent[i] < 0 ? -1 : nid[g][t.parent[i]];
+      out.first_child[k] = t.first_child[i] < 0 ? -1 : (int32_t)nid[g][t.first_child[i]];
+      out.n_child[k] = t.n_child[i];
+      out.is_leaf[k] = t.is_leaf[i];
+      out.node_s[k] = t.node_s[i];
+      for (int a = 0; a < 3; ++a) {
+        out.coord[3 * k + a] = t.coord[3 * i + a];
+        out.node_c[3 * k + a] = t.node_c[3 * i + a];
+      }
+    }
+    for (int64_t i = 0; i < t.n; ++i) out.perm.push_back(t.perm[i] + pt_off[g]);
+    out.geom_center.insert(out.geom_center.end(), t.center, t.center + 3);
+  }
+  // lists follow their target node
+  std::vector<int64_t> owner_g(M), owner_i(M);
+  for (size_t g = 0; g < G; ++g)
+    for (int64_t i = 0; i < (int64_t)parts[g].level.size(); ++i) {
+      owner_g[nid[g][i]] = (int64_t)g;
+      owner_i[nid[g][i]] = i;
+    }
+  out.v_ptr.assign(M + 1, 0);
+  out.u_ptr.assign(M + 1, 0);
+  for (int64_t k = 0; k < M; ++k) {
+    const HostTree& t = parts[owner_g[k]];
+    const auto& map = nid[owner_g[k]];
+    const int64_t i = owner_i[k];
+    for (int64_t q = t.v_ptr[i]; q < t.v_ptr[i + 1]; ++q) {
+      out.v_src.push_back(map[t.v_src[q]]);
+      out.v_off.push_back(t.v_off[q]);
+    }
+    out.v_ptr[k + 1] = (int64_t)out.v_src.size();
+    for (int64_t q = t.u_ptr[i]; q < t.u_ptr[i + 1]; ++q) out.u_src.push_back(map[t.u_src[q]]);
+    out.u_ptr[k + 1] = (int64_t)out.u_src.size();
+  }
+  out.center[0] = out.center[1] = out.center[2] = 0.0;
+  out.half = 0.0;
   return ED_OK;
 }
 

@@ -9,7 +9,7 @@ from .config import M2L_PRECISIONS
 
 STAT_NAMES = ("n_points", "n_nodes", "n_leaves", "depth", "n_p2p_leaf_pairs",
               "n_p2p_interactions", "n_m2l_pairs", "n_m2l_offsets", "order", "ws",
-              "n_near_pairs", "n_levels_with_m2l")
+              "n_near_pairs", "n_levels_with_m2l", "n_p2p_source_points")
 
 
 class FmmPlan:
@@ -22,7 +22,7 @@ class FmmPlan:
     """
 
     def __init__(self, centroids, areas, *, leaf_size=64, order=8, ws=1, max_level=16,
-                 m2l_precision="fp32"):
+                 m2l_precision="fp32", geom_offsets=None):
         nat.require_gpu()
         L = nat.lib()
         C = np.ascontiguousarray(np.asarray(centroids, dtype=np.float64).reshape(-1, 3))
@@ -39,9 +39,17 @@ class FmmPlan:
         opts.m2l_precision = int(M2L_PRECISIONS[m2l_precision]
                                  if isinstance(m2l_precision, str) else m2l_precision)
         h = ctypes.c_void_p()
-        nat.check(L.ed_plan_create(C.ctypes.data_as(ctypes.c_void_p),
-                                   A.ctypes.data_as(ctypes.c_void_p), self.n,
-                                   ctypes.byref(opts), ctypes.byref(h)), "ed_plan_create")
+        if geom_offsets is None:
+            nat.check(L.ed_plan_create(C.ctypes.data_as(ctypes.c_void_p),
+                                       A.ctypes.data_as(ctypes.c_void_p), self.n,
+                                       ctypes.byref(opts), ctypes.byref(h)), "ed_plan_create")
+        else:
+            go = np.ascontiguousarray(np.asarray(geom_offsets, dtype=np.int64))
+            nat.check(L.ed_plan_create_batched(C.ctypes.data_as(ctypes.c_void_p),
+                                               A.ctypes.data_as(ctypes.c_void_p), self.n,
+                                               go.ctypes.data_as(ctypes.c_void_p), go.size - 1,
+                                               ctypes.byref(opts), ctypes.byref(h)),
+                      "ed_plan_create_batched")
         self._h = h
         self.order, self.ws, self.leaf_size = int(order), int(ws), int(leaf_size)
         self.m2l_precision = m2l_precision
@@ -67,6 +75,17 @@ class FmmPlan:
         out = (ctypes.c_int64 * len(STAT_NAMES))()
         nat.check(nat.lib().ed_plan_stats(self.handle, out, len(STAT_NAMES)), "ed_plan_stats")
         return {k: int(v) for k, v in zip(STAT_NAMES, out)}
+
+    def geometries(self):
+        """Panel offsets of the geometries of this plan (length n_geoms + 1)."""
+        g = ctypes.c_int64(0)
+        nat.check(nat.lib().ed_plan_geometries(self.handle, ctypes.byref(g), None, 0),
+                  "ed_plan_geometries")
+        off = np.zeros(g.value + 1, dtype=np.int64)
+        nat.check(nat.lib().ed_plan_geometries(self.handle, ctypes.byref(g),
+                                               off.ctypes.data_as(ctypes.c_void_p), off.size),
+                  "ed_plan_geometries")
+        return off
 
     def permutation(self):
         perm = np.empty(self.n, dtype=np.int64)
@@ -102,6 +121,19 @@ class FmmPlan:
         nat.check(fn(self.handle, nat.ptr(sigma), nat.ptr(self_integrals), nat.ptr(out), st),
                   "ed_fmm_matvec")
         return out
+
+    STAGES = ("gather", "p2m", "m2m", "m2l", "m2l_reduce", "l2l", "leaf")
+
+    def profile(self, sigma, self_integrals=None, reps=10):
+        """Average device milliseconds per matvec stage (HIP events)."""
+        if not sigma.is_cuda or sigma.dtype != torch.float64:
+            raise ValueError("profile takes a float64 device sigma")
+        out = torch.empty_like(sigma)
+        ms = (ctypes.c_double * len(self.STAGES))()
+        nat.check(nat.lib().ed_fmm_profile(self.handle, nat.ptr(sigma), nat.ptr(self_integrals),
+                                           nat.ptr(out), int(reps), ms, len(self.STAGES),
+                                           nat.stream_ptr(sigma.device)), "ed_fmm_profile")
+        return dict(zip(self.STAGES, list(ms)))
 
     def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
               tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,

@@ -54,11 +54,26 @@ int ed_plan_create(const double* centroids_host, const double* areas_host,
                    int64_t n, const ed_plan_opts* opts, ed_plan** out);
 int ed_plan_destroy(ed_plan* plan);
 
+/* Batched plan over several independent geometries (no reference
+ * counterpart: the reference batches right-hand sides only,
+ * core/bem_kernel.py:715-810).  Panels of geometry g are
+ * [geom_offsets[g], geom_offsets[g+1]) of the input arrays.  Each geometry
+ * gets its own octree and lists; the trees are merged level-major so each FMM
+ * stage is one launch over all geometries.  Matvecs and solves are
+ * block-diagonal: geometries never interact. */
+int ed_plan_create_batched(const double* centroids_host, const double* areas_host,
+                           int64_t n, const int64_t* geom_offsets, int64_t n_geoms,
+                           const ed_plan_opts* opts, ed_plan** out);
+/* Number of geometries and (if offsets_host non-NULL) their panel offsets. */
+int ed_plan_geometries(const ed_plan* plan, int64_t* n_geoms_out, int64_t* offsets_host,
+                       int64_t capacity);
+
 /* Counters (TreeStatistics / InteractionStats analogue, fmm3d/tree.py:120-162,
  * fmm3d/interaction_lists.py:59-88).  out[i] for i < n_out:
  * 0 n_points 1 n_nodes 2 n_leaves 3 depth 4 n_p2p_leaf_pairs
  * 5 n_p2p_interactions 6 n_m2l_pairs 7 n_m2l_offsets 8 order 9 ws
- * 10 n_near_pairs 11 n_levels_with_m2l */
+ * 10 n_near_pairs 11 n_levels_with_m2l 12 n_p2p_source_points (sum over P2P
+ * leaf pairs of the source leaf size) */
 int ed_plan_stats(const ed_plan* plan, int64_t* out, int32_t n_out);
 
 /* Tree-order permutation (index_map, fmm3d/tree.py:428-485): perm[i] is the
@@ -76,6 +91,15 @@ int ed_fmm_matvec_f32(ed_plan* plan, const float* sigma, const float* self_integ
                       float* out, ed_stream stream);
 int ed_fmm_matvec_f64(ed_plan* plan, const double* sigma, const double* self_integrals,
                       double* out, ed_stream stream);
+
+/* Per-stage device time of the FMM matvec (no reference counterpart: the
+ * reference logs MultipoleOpStats counters, fmm3d/multipole_operators.py:
+ * 984-1041, not times).  Runs the f64 matvec `reps` times with HIP events
+ * between stages on `stream`; stage_ms[0..6] = gather, p2m, m2m, m2l,
+ * m2l reduce, l2l, leaf (L2P + P2P + near + self), averaged. */
+int ed_fmm_profile(ed_plan* plan, const double* sigma, const double* self_integrals,
+                   double* out, int32_t reps, double* stage_ms, int32_t n_stages,
+                   ed_stream stream);
 
 /* Attach the near-singular correction (core/bem.py:1838-1957): pairs (P,2)
  * int64 (i, j) and FP64 weights delta_ij, host arrays in original order.  They
