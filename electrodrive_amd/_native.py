@@ -70,10 +70,21 @@ _SIGS = {
     "ed_gmres_opts_default": (None, [ctypes.POINTER(GmresOpts)]),
     "ed_gmres_solve": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(GmresOpts),
                                ctypes.POINTER(GmresInfo), c_vp]),
+    "ed_plan_set_coarse": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp]),
     "ed_vec_dot_f64": (c_i32, [c_vp, c_vp, c_i64, c_vp, ctypes.POINTER(c_dbl), c_vp]),
     "ed_vec_axpy_f64": (c_i32, [c_dbl, c_vp, c_vp, c_i64, c_vp]),
     "ed_vec_scale_f64": (c_i32, [c_dbl, c_vp, c_vp, c_i64, c_vp]),
     "ed_vec_mul_f64": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "ed_gemm_f32": (c_i32, [c_i32, c_i32, c_i64, c_i64, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp,
+                            c_i64, c_vp, c_i32, c_vp, ctypes.c_float, c_vp]),
+    "ed_act_backward_f32": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_i32, c_vp]),
+    "ed_colsum_f32": (c_i32, [c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]),
+    "ed_adam_f32": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_float, ctypes.c_float,
+                            ctypes.c_float, ctypes.c_float, c_i32, ctypes.c_float, c_vp]),
+    "ed_wmse_f32": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, ctypes.POINTER(ctypes.c_float),
+                            c_vp]),
+    "ed_panel_features": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "ed_panel_feature_count": (c_i32, []),
     "ed_host_tree_build": (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                    c_vp, c_vp, c_vp, c_vp]),
     "ed_host_operator": (c_i32, [c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp]),

@@ -68,7 +68,8 @@ void plan_free(ed_plan* P) {
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair, P->t_m2l,
                   P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
                   P->near_w, P->cent64_t, P->area64_t, P->near_w64, P->m2l_ascale,
-                  P->m2l_binv};
+                  P->m2l_binv, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
+                  P->coarse_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   gmres_free(P);

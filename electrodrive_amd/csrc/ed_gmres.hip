@@ -209,7 +209,8 @@ __global__ __launch_bounds__(kRedThreads) void k_dots1(int64_t n, const double* 
   const int j = *dj;
   const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
   const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) w[i] *= dinv[i];
+  if (dinv)
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) w[i] *= dinv[i];
   __syncthreads();
   for (int k = 0; k <= j; ++k) {
     const double* vk = V + (size_t)k * n;
@@ -367,6 +368,160 @@ __global__ void k_near_csr(int64_t n, const int32_t* __restrict__ ptr, const int
 }
 
 // ---------------------------------------------------------------------------
+// coarse space: M^-1 u = D^-1 (u - AZ c) + Z c,  c = E^-1 Z^T u,  E = Z^T A Z
+// (Z orthonormalised at set-up; exact on span(Z), Jacobi elsewhere)
+// ---------------------------------------------------------------------------
+constexpr int kMaxCoarse = 32;
+
+// partial[c][block] = sum_i Z_c[i] u[i] over the block's chunk
+__global__ __launch_bounds__(kRedThreads) void k_coarse_dots(int64_t n, int k,
+                                                             const double* __restrict__ Z,
+                                                             const double* __restrict__ u,
+                                                             double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
+  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  for (int c = 0; c < k; ++c) {
+    const double* zc = Z + (size_t)c * n;
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += zc[i] * u[i];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[(size_t)c * kRedBlocks + blockIdx.x] = s;
+  }
+}
+
+// cy = Einv (Z^T u), one block
+__global__ void k_coarse_solve(int k, const double* __restrict__ partial,
+                               const double* __restrict__ Einv, double* __restrict__ cy) {
+  __shared__ double zu[kMaxCoarse];
+  __shared__ double sh[4];
+  for (int c = 0; c < k; ++c) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[(size_t)c * kRedBlocks + b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) zu[c] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < k) {
+    double s = 0.0;
+    for (int c = 0; c < k; ++c) s += Einv[threadIdx.x * k + c] * zu[c];
+    cy[threadIdx.x] = s;
+  }
+}
+
+// out = dinv .* (u - AZ cy) + Z cy   (out may alias u)
+__global__ void k_coarse_apply(int64_t n, int k, const double* __restrict__ u,
+                               const double* __restrict__ dinv, const double* __restrict__ Z,
+                               const double* __restrict__ AZ, const double* __restrict__ cy,
+                               double* out) {
+  __shared__ double c[kMaxCoarse];
+  if (threadIdx.x < k) c[threadIdx.x] = cy[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double a = u[i], zc = 0.0;
+    for (int q = 0; q < k; ++q) {
+      a -= c[q] * AZ[(size_t)q * n + i];
+      zc += c[q] * Z[(size_t)q * n + i];
+    }
+    out[i] = dinv[i] * a + zc;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_norm2_part(int64_t n, const double* __restrict__ x,
+                                                            double* __restrict__ partial) {
+  __shared__ double sh[4];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    s += x[i] * x[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// Z_c -= sum_{q<c} (Z_q . Z_c) Z_q for one column (coefficients in cy), then
+// the caller normalises; used twice per column at set-up (CGS2).
+__global__ void k_coarse_orth(int64_t n, int c, double* __restrict__ Z, const double* __restrict__ cy) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = Z[(size_t)c * n + i];
+    for (int q = 0; q < c; ++q) v -= cy[q] * Z[(size_t)q * n + i];
+    Z[(size_t)c * n + i] = v;
+  }
+}
+
+__global__ void k_coarse_coef(int c, const double* __restrict__ partial, double* __restrict__ cy) {
+  __shared__ double sh[4];
+  for (int q = 0; q < c; ++q) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[(size_t)q * kRedBlocks + b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) cy[q] = v;
+  }
+}
+
+__global__ void k_scale_col(int64_t n, double* __restrict__ x, const double* __restrict__ scal, int slot) {
+  const double nr = scal[slot];
+  const double inv = nr > 0.0 ? 1.0 / nr : 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] *= inv;
+}
+
+// E (k x k, row-major E[r][c] = Z_r . AZ_c) from partial dots, then its
+// inverse by Gauss-Jordan with partial pivoting (pivots below 1e-13 max|E|
+// drop that direction).  One block.
+__global__ void k_coarse_einv(int k, const double* __restrict__ partial, double* __restrict__ Einv) {
+  __shared__ double M[kMaxCoarse][2 * kMaxCoarse];
+  __shared__ double sh[4];
+  __shared__ int piv;
+  __shared__ double pmax;
+  for (int e = 0; e < k * k; ++e) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[(size_t)e * kRedBlocks + b];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) M[e % k][e / k] = v;  // e = c * k + r holds Z_r . AZ_c
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < k * k; t += blockDim.x) M[t / k][k + t % k] = (t / k == t % k) ? 1.0 : 0.0;
+  __syncthreads();
+  double scale = 0.0;
+  for (int r = 0; r < k; ++r)
+    for (int c = 0; c < k; ++c) scale = fmax(scale, fabs(M[r][c]));
+  for (int col = 0; col < k; ++col) {
+    if (threadIdx.x == 0) {
+      int best = col;
+      for (int r = col + 1; r < k; ++r)
+        if (fabs(M[r][col]) > fabs(M[best][col])) best = r;
+      piv = best;
+      pmax = M[best][col];
+    }
+    __syncthreads();
+    const bool drop = fabs(pmax) <= 1e-13 * scale;
+    if (!drop && piv != col && threadIdx.x < 2 * k) {
+      const double a = M[col][threadIdx.x];
+      M[col][threadIdx.x] = M[piv][threadIdx.x];
+      M[piv][threadIdx.x] = a;
+    }
+    __syncthreads();
+    if (drop) {
+      if (threadIdx.x < 2 * k) M[col][threadIdx.x] = 0.0;
+      __syncthreads();
+      continue;
+    }
+    if (threadIdx.x < 2 * k) M[col][threadIdx.x] /= pmax;
+    __syncthreads();
+    for (int r = threadIdx.x; r < k; r += blockDim.x) {
+      if (r == col) continue;
+      const double f = M[r][col];
+      for (int c = 0; c < 2 * k; ++c) M[r][c] -= f * M[col][c];
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < k * k; t += blockDim.x) Einv[t] = M[t / k][k + t % k];
+}
+
+// ---------------------------------------------------------------------------
 // host driver
 // ---------------------------------------------------------------------------
 int dense_launch_f64(const double* C, const double* A, const double* sigma, const double* selfi,
@@ -400,11 +555,30 @@ struct Solver {
     return ED_OK;
   }
   // norms of (b - t) and dinv (b - t); r, z optional outputs
+  // z = M^-1 u in place (coarse-corrected Jacobi); u may alias z
+  void coarse_apply(const double* u, double* z) {
+    const int k = P->coarse_k;
+    hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, k,
+                       P->coarse_z, u, P->coarse_part);
+    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), 0, W->stream, k, P->coarse_part,
+                       P->coarse_einv, P->coarse_cy);
+    hipLaunchKernelGGL(k_coarse_apply, dim3(gb), dim3(256), 0, W->stream, n, k, u, W->dinv,
+                       P->coarse_z, P->coarse_az, P->coarse_cy, z);
+  }
   int resid_norms(const double* b, const double* t, double* r, double* z, double* nr, double* nz) {
+    const bool coarse = P->coarse_k > 0 && z != nullptr;
+    double* rr = (coarse && r == nullptr) ? W->vin : r;
     hipLaunchKernelGGL(k_resid, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, b, t,
-                       W->dinv, r, z, W->partial);
+                       W->dinv, rr, coarse ? nullptr : z, W->partial);
     hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, W->stream, W->partial, 2, W->scal,
                        S_NRM_R);
+    if (coarse) {
+      coarse_apply(rr, z);
+      hipLaunchKernelGGL(k_norm2_part, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, z,
+                         W->partial);
+      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, W->stream, W->partial, 1, W->scal,
+                         S_NRM_Z);
+    }
     ED_KCHECK("resid");
     ED_G(hipMemcpyAsync(W->hscal, W->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, W->stream));
     ED_G(hipStreamSynchronize(W->stream));
@@ -416,8 +590,10 @@ struct Solver {
     const int m = W->m;
     hipLaunchKernelGGL(k_copy_vj, dim3(gb), dim3(256), 0, W->stream, n, W->V, W->dj, W->vin);
     ED_T(approx_op(W->vin, W->w));
-    hipLaunchKernelGGL(k_dots1, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->dinv,
-                       W->w, W->V, W->dj, W->partial);
+    const bool coarse = P->coarse_k > 0;
+    if (coarse) coarse_apply(W->w, W->w);
+    hipLaunchKernelGGL(k_dots1, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n,
+                       coarse ? nullptr : W->dinv, W->w, W->V, W->dj, W->partial);
     hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(256), 0, W->stream, W->partial, W->dj, W->h, 0);
     hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->w, W->V,
                        W->dj, W->h, 0, W->partial);
@@ -609,5 +785,65 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
   ED_G(hipStreamWaitEvent(caller, ev, 0));
   ED_G(hipStreamSynchronize(W->stream));
   (void)hipEventDestroy(ev);
+  return ED_OK;
+}
+
+// Coarse space for the native solver (no reference counterpart).  Z holds k
+// columns of length n (column-major, original panel order).  The columns are
+// orthonormalised (CGS2), A Z is formed with the plan's FMM operator (with
+// `self_integrals`), E = Z^T A Z is inverted on the device.  k = 0 removes it.
+extern "C" int ed_plan_set_coarse(ed_plan* P, const double* Z, int32_t k,
+                                  const double* self_integrals, ed_stream stream) {
+  using namespace ed;
+  if (!P || k < 0 || k > kMaxCoarse || (k > 0 && !Z))
+    return fail(ED_ERR_ARG, "ed_plan_set_coarse: k must be in [0, 32] with Z given");
+  gmres_free(P);  // a captured Arnoldi graph refers to the old coarse arrays
+  void* olds[] = {P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy, P->coarse_part};
+  for (void* q : olds)
+    if (q) (void)hipFree(q);
+  P->coarse_z = P->coarse_az = P->coarse_einv = P->coarse_cy = P->coarse_part = nullptr;
+  P->coarse_k = 0;
+  if (k == 0) return ED_OK;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = P->n;
+  const unsigned gn = (unsigned)((n + 255) / 256);
+  ED_G(hipMalloc((void**)&P->coarse_z, sizeof(double) * n * k));
+  ED_G(hipMalloc((void**)&P->coarse_az, sizeof(double) * n * k));
+  ED_G(hipMalloc((void**)&P->coarse_einv, sizeof(double) * k * k));
+  ED_G(hipMalloc((void**)&P->coarse_cy, sizeof(double) * kMaxCoarse));
+  ED_G(hipMalloc((void**)&P->coarse_part, sizeof(double) * kRedBlocks * k * k));
+  double* scal = nullptr;
+  ED_G(hipMalloc((void**)&scal, sizeof(double) * 8));
+  for (int c = 0; c < k; ++c)
+    hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, st, n, P->perm, Z + (size_t)c * n,
+                       P->coarse_z + (size_t)c * n);
+  for (int c = 0; c < k; ++c) {
+    double* zc = P->coarse_z + (size_t)c * n;
+    for (int pass = 0; pass < 2 && c > 0; ++pass) {
+      hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, c,
+                         P->coarse_z, zc, P->coarse_part);
+      hipLaunchKernelGGL(k_coarse_coef, dim3(1), dim3(256), 0, st, c, P->coarse_part, P->coarse_cy);
+      hipLaunchKernelGGL(k_coarse_orth, dim3(gn), dim3(256), 0, st, n, c, P->coarse_z, P->coarse_cy);
+    }
+    hipLaunchKernelGGL(k_norm2_part, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, zc,
+                       P->coarse_part);
+    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, st, P->coarse_part, 1, scal, 0);
+    hipLaunchKernelGGL(k_scale_col, dim3(gn), dim3(256), 0, st, n, zc, scal, 0);
+  }
+  ED_KCHECK("coarse orthonormalisation");
+  for (int c = 0; c < k; ++c) {
+    int rc = fmm_launch(P, P->coarse_z + (size_t)c * n, 1, 1, self_integrals,
+                        P->coarse_az + (size_t)c * n, 1, 1, st);
+    if (rc != ED_OK) { (void)hipFree(scal); return rc; }
+  }
+  for (int c = 0; c < k; ++c)
+    hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, k,
+                       P->coarse_z, P->coarse_az + (size_t)c * n,
+                       P->coarse_part + (size_t)c * k * kRedBlocks);
+  hipLaunchKernelGGL(k_coarse_einv, dim3(1), dim3(64), 0, st, k, P->coarse_part, P->coarse_einv);
+  ED_KCHECK("coarse operator");
+  ED_G(hipStreamSynchronize(st));
+  (void)hipFree(scal);
+  P->coarse_k = k;
   return ED_OK;
 }

@@ -81,6 +81,14 @@ struct ed_plan {
   double* area64_t = nullptr;  // n
   double* near_w64 = nullptr;  // near CSR weights in FP64
 
+  // coarse space of the solver's preconditioner (tree order, k columns)
+  int coarse_k = 0;
+  double* coarse_z = nullptr;     // k x n, orthonormal
+  double* coarse_az = nullptr;    // k x n, A Z with the FMM operator
+  double* coarse_einv = nullptr;  // k x k, (Z^T A Z)^-1
+  double* coarse_cy = nullptr;
+  double* coarse_part = nullptr;
+
   // stage events while ed_fmm_profile runs (8 markers), else NULL
   hipEvent_t* prof_events = nullptr;
 

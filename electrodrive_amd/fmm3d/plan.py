@@ -135,6 +135,20 @@ class FmmPlan:
                                            nat.stream_ptr(sigma.device)), "ed_fmm_profile")
         return dict(zip(self.STAGES, list(ms)))
 
+    def set_coarse(self, Z, self_integrals=None):
+        """Coarse space (n, k) float64 device tensor (k <= 32) for the
+        solver's preconditioner; None removes it."""
+        if Z is None:
+            nat.check(nat.lib().ed_plan_set_coarse(self.handle, None, 0, None, None),
+                      "ed_plan_set_coarse")
+            return
+        if not Z.is_cuda or Z.dtype != torch.float64 or Z.dim() != 2 or Z.shape[0] != self.n:
+            raise ValueError("Z must be a float64 device tensor of shape (n, k)")
+        Zc = Z.t().contiguous()  # column-major
+        nat.check(nat.lib().ed_plan_set_coarse(self.handle, nat.ptr(Zc), int(Z.shape[1]),
+                                               nat.ptr(self_integrals),
+                                               nat.stream_ptr(Z.device)), "ed_plan_set_coarse")
+
     def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
               tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,
               use_graph=True, check_every=4):

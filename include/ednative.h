@@ -210,6 +210,14 @@ int ed_gmres_solve(ed_plan* plan, const double* b, double* x, const double* diag
                    const double* self_integrals, const ed_gmres_opts* opts,
                    ed_gmres_info* info, ed_stream stream);
 
+/* Coarse space of ed_gmres_solve's preconditioner (no reference counterpart):
+ * M^-1 u = D^-1 (u - A Z c) + Z c, c = (Z^T A Z)^-1 Z^T u, with Z (n x k,
+ * column-major, original order, device FP64) orthonormalised here and A the
+ * plan's FMM operator with `self_integrals`.  k <= 32; k = 0 removes it.
+ * Exact on span(Z), Jacobi elsewhere.  Used with the learned basis. */
+int ed_plan_set_coarse(ed_plan* plan, const double* Z, int32_t k,
+                       const double* self_integrals, ed_stream stream);
+
 /* FP64 level-1 kernels for GMRES over an arbitrary operator callable: they
  * replace the torch.dot / norm / axpy steps of gmres_restart's Arnoldi loop
  * (core/bem_solver.py:536-541, :771-773).  Deterministic reductions.
@@ -220,6 +228,35 @@ int ed_vec_axpy_f64(double a, const double* x, double* y, int64_t n, ed_stream s
 int ed_vec_scale_f64(double a, const double* x, double* y, int64_t n, ed_stream stream);
 /* y = d .* x (Jacobi apply, core/bem_solver.py:80-83) */
 int ed_vec_mul_f64(const double* d, const double* x, double* y, int64_t n, ed_stream stream);
+
+/* ------------------------------------------------------------------------ */
+/* Learned preconditioner network (no reference counterpart: the reference  */
+/* has no learned preconditioner).  FP32, row-major, device pointers.        */
+/* ------------------------------------------------------------------------ */
+/* C = act(op(A) op(B) + bias) (+ beta * C); op = transpose when trans_*;
+ * act 0 = identity, 1 = SiLU; pre_act (same layout as C) receives the
+ * pre-activation when non-NULL; bias (N) may be NULL. */
+int ed_gemm_f32(int32_t trans_a, int32_t trans_b, int64_t M, int64_t N, int64_t K,
+                const float* A, int64_t lda, const float* B, int64_t ldb, float* C,
+                int64_t ldc, const float* bias, int32_t act, float* pre_act, float beta,
+                ed_stream stream);
+int ed_act_backward_f32(const float* pre_act, const float* dy, float* dx, int64_t n,
+                        int32_t act, ed_stream stream);
+/* out[j] = sum_i X[i][j], fixed order; scratch >= 64 * cols floats */
+int ed_colsum_f32(const float* X, int64_t rows, int64_t cols, float* scratch, float* out,
+                  ed_stream stream);
+int ed_adam_f32(float* param, const float* grad, float* m, float* v, int64_t n, float lr,
+                float beta1, float beta2, float eps, int32_t step, float weight_decay,
+                ed_stream stream);
+/* grad = 2 w (y - t); *loss_host = sum w (y - t)^2 (synchronises); part >= 240 */
+int ed_wmse_f32(const float* y, const float* t, const float* w, int64_t n, float* grad,
+                float* part, float* loss_host, ed_stream stream);
+/* 12 per-panel features (geometry, RHS potential, Jacobi guess, Morton-leaf
+ * means) in original panel order; inputs device FP64 (n,3)/(n). */
+int ed_panel_features(ed_plan* plan, const double* centroids, const double* normals,
+                      const double* areas, const double* b, const double* diag, float* F,
+                      ed_stream stream);
+int32_t ed_panel_feature_count(void);
 
 /* ------------------------------------------------------------------------ */
 /* Host-only helpers (no device needed): expose the plan builder's octree,   */
