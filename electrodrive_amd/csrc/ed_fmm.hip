@@ -222,9 +222,9 @@ struct M2LShape {
   static constexpr int NCP = (NC + 31) / 32 * 32;  // output rows, padded
   static constexpr int KP = (NC + 1) / 2 * 2;      // inner dim, padded to K=2
   static constexpr int KS = KP + 1;                // LDS row stride
-  static constexpr int COLS = (P <= 8) ? 128 : 64;  // pairs per tile
-  static constexpr int THREADS = COLS * 2;          // one wave per 32 columns
-  static constexpr bool ENABLED = P <= 10;          // LDS budget (160 KiB)
+  static constexpr int COLS = 256;                  // pairs per tile (block)
+  static constexpr int THREADS = 256;               // 4 waves, 32 columns at a time
+  static constexpr bool ENABLED = P <= 12;          // table fits the 160 KiB LDS
 };
 
 // 16-bit MFMA (v_mfma_f32_32x32x16_{bf16,f16}, gfx950): K steps of 16.
@@ -248,10 +248,9 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
                                                   const float* __restrict__ M,
                                                   float* __restrict__ Y) {
   using S = M2LShape<P>;
-  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
-  constexpr int NT = S::THREADS;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
+  constexpr int NT = S::THREADS, NW = NT / 64, KH = KP / 2;
   __shared__ float ta[NCP * KS];
-  __shared__ float bm[COLS * KS];
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
   const float* Tt = Tm + (size_t)tab * NCP * KP;
@@ -259,45 +258,46 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
     const int i = t / KP, k = t - i * KP;
     ta[i * KS + k] = Tt[t];
   }
-  for (int t = threadIdx.x; t < COLS * KP; t += NT) {
-    const int c = t / KP, k = t - c * KP;
-    float v = 0.0f;
-    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k] * ascale[k];
-    bm[c * KS + k] = v;
-  }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int col0 = wave * 32;
-  if (col0 >= cnt) return;
   const int li = lane & 31, lk = lane >> 5;
   constexpr int NRB = NCP / 32;
-  f32x16 acc[NRB];
+  const int ntiles = (cnt + 31) / 32;
+  for (int ct = wave; ct < ntiles; ct += NW) {
+    const int col = ct * 32 + li;
+    const int src = col < cnt ? pair_src[beg + col] : -1;
+    // B[k][col] for k = 2s + lk, straight from the (L2-resident) multipoles
+    float bv[KH];
 #pragma unroll
-  for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
-  const float* brow = bm + (col0 + li) * KS + lk;
-#pragma unroll 4
-  for (int s = 0; s < KP / 2; ++s) {
-    const float b = brow[2 * s];
-#pragma unroll
-    for (int rb = 0; rb < NRB; ++rb) {
-      const float a = ta[(rb * 32 + li) * KS + 2 * s + lk];
-      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, b, acc[rb], 0, 0, 0);
+    for (int s = 0; s < KH; ++s) {
+      const int k = 2 * s + lk;
+      bv[s] = (src >= 0 && k < NC) ? M[(size_t)src * NC + k] * ascale[k] : 0.0f;
     }
-  }
-  const int col = col0 + li;
-  if (col < cnt) {
-    float* yrow = Y + (size_t)(beg + col) * NCP;
+    f32x16 acc[NRB];
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = rb * 32 + q * 8 + lk * 4;
-        float4 v = make_float4(acc[rb][4 * q + 0], acc[rb][4 * q + 1], acc[rb][4 * q + 2],
-                               acc[rb][4 * q + 3]);
-        *reinterpret_cast<float4*>(yrow + row) = v;
+      for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
+#pragma unroll
+    for (int s = 0; s < KH; ++s) {
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb) {
+        const float a = ta[(rb * 32 + li) * KS + 2 * s + lk];
+        acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv[s], acc[rb], 0, 0, 0);
       }
+    }
+    if (col < cnt) {
+      float* yrow = Y + (size_t)(beg + col) * NCP;
+#pragma unroll
+      for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int row = rb * 32 + q * 8 + lk * 4;
+          float4 v = make_float4(acc[rb][4 * q + 0], acc[rb][4 * q + 1], acc[rb][4 * q + 2],
+                                 acc[rb][4 * q + 3]);
+          *reinterpret_cast<float4*>(yrow + row) = v;
+        }
+    }
   }
 }
 
@@ -430,14 +430,17 @@ __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restric
 // ---------------------------------------------------------------------------
 // Leaf evaluation: L2P + P2P (U list) + near CSR + self, scatter.
 // ---------------------------------------------------------------------------
-// One 64-lane wave per target leaf.  The lanes are split into S = 64 / n_t
-// slices (n_t = targets in the chunk, at most 64); slice s takes every S-th
-// source point of the stream, so small leaves still use the whole wave.
-// Sources are streamed through LDS 256 points at a time: first the target
-// leaf itself (with the j != i exclusion), then the concatenation of the other
-// U-list leaves.  Slice partial sums are combined in a fixed order.
+// One 4-wave block per target leaf.  Inside a wave the lanes are split into
+// S = 64 / n_t slices (n_t = targets in the chunk, at most 64); the source
+// stream — first the target leaf itself (with the j != i exclusion), then the
+// concatenation of the other U-list leaves — is cut into 256-point chunks,
+// wave w taking chunks w, w+4, ... through its own LDS buffer, and slice s
+// taking every S-th point of a chunk.  Partial sums are combined in a fixed
+// order (wave-major, then slice).
 constexpr int kSrcChunk = 256;
 constexpr int kUSeg = 256;
+constexpr int kLeafWaves = 4;
+constexpr int kLeafThreads = 64 * kLeafWaves;
 
 __device__ __forceinline__ int wave_incl_scan(int v) {
   const int lane = threadIdx.x & 63;
@@ -450,7 +453,7 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 }
 
 template <int P, typename TO>
-__global__ __launch_bounds__(kLeafBlock) void k_leaf(
+__global__ __launch_bounds__(kLeafThreads) void k_leaf(
     const int32_t* __restrict__ leaves, const int2* __restrict__ rng,
     const float4* __restrict__ ctr, const float4* __restrict__ pts,
     const float* __restrict__ L, const int32_t* __restrict__ u_ptr,
@@ -460,35 +463,38 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf(
     const int32_t* __restrict__ perm, int out_tree, float ke, TO* __restrict__ out) {
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float lc[NC];
-  __shared__ float4 src[kSrcChunk];
+  __shared__ float4 src[kLeafWaves][kSrcChunk];
   __shared__ int ubeg[kUSeg];
   __shared__ int upref[kUSeg + 1];
-  __shared__ float red[kLeafBlock];
-  const int lane = threadIdx.x;
+  __shared__ int utotal;
+  __shared__ float red[kLeafThreads];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = blockIdx.x;
   const int node = leaves[li];
   const int2 r = rng[node];
   const float4 c = ctr[node];
-  for (int j = lane; j < NC; j += kLeafBlock) lc[j] = L[(size_t)node * NC + j];
+  for (int j = threadIdx.x; j < NC; j += kLeafThreads) lc[j] = L[(size_t)node * NC + j];
   const int ub = u_ptr[li], ue = u_ptr[li + 1];
-  for (int t0 = r.x; t0 < r.y; t0 += kLeafBlock) {
-    const int ntc = min(kLeafBlock, r.y - t0);
-    const int S = kLeafBlock / ntc;
+  float4* mysrc = src[wave];
+  for (int t0 = r.x; t0 < r.y; t0 += 64) {
+    const int ntc = min(64, r.y - t0);
+    const int S = 64 / ntc;
     const int tl = lane % ntc, sl = lane / ntc;
     const bool act = sl < S;
     const int i = t0 + tl;
     const float4 xi = pts[i];
     float acc = 0.0f;
     // (1) the target leaf itself, excluding j == i
-    for (int s0 = r.x; s0 < r.y; s0 += kSrcChunk) {
-      const int ns = min(kSrcChunk, r.y - s0);
+    for (int r0 = r.x; r0 < r.y; r0 += kLeafWaves * kSrcChunk) {
+      const int s0 = r0 + wave * kSrcChunk;
+      const int ns = max(0, min(kSrcChunk, r.y - s0));
       __syncthreads();
-      for (int e = lane; e < ns; e += kLeafBlock) src[e] = pts[s0 + e];
+      for (int e = lane; e < ns; e += 64) mysrc[e] = pts[s0 + e];
       __syncthreads();
       if (act) {
         const int me = i - s0;
         for (int j = sl; j < ns; j += S) {
-          const float4 sj = src[j];
+          const float4 sj = mysrc[j];
           const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
           const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
           const float v = sj.w * __builtin_amdgcn_rsqf(r2);
@@ -500,48 +506,51 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf(
     for (int g0 = ub; g0 < ue; g0 += kUSeg) {
       const int ng = min(kUSeg, ue - g0);
       __syncthreads();
-      int part = 0;
-      int sz[kUSeg / kLeafBlock];
+      if (wave == 0) {
+        int part = 0;
+        int sz[kUSeg / 64];
 #pragma unroll
-      for (int k = 0; k < kUSeg / kLeafBlock; ++k) {
-        const int q = lane * (kUSeg / kLeafBlock) + k;
-        int size = 0;
-        if (q < ng) {
-          const int sn = u_src[g0 + q];
-          const int2 sr = rng[sn];
-          size = (sn == node) ? 0 : sr.y - sr.x;
-          ubeg[q] = sr.x;
+        for (int k = 0; k < kUSeg / 64; ++k) {
+          const int q = lane * (kUSeg / 64) + k;
+          int size = 0;
+          if (q < ng) {
+            const int sn = u_src[g0 + q];
+            const int2 sr = rng[sn];
+            size = (sn == node) ? 0 : sr.y - sr.x;
+            ubeg[q] = sr.x;
+          }
+          sz[k] = size;
+          part += size;
         }
-        sz[k] = size;
-        part += size;
-      }
-      const int incl = wave_incl_scan(part);
-      int run = incl - part;
+        const int incl = wave_incl_scan(part);
+        int run = incl - part;
 #pragma unroll
-      for (int k = 0; k < kUSeg / kLeafBlock; ++k) {
-        const int q = lane * (kUSeg / kLeafBlock) + k;
-        if (q < ng) upref[q] = run;
-        run += sz[k];
+        for (int k = 0; k < kUSeg / 64; ++k) {
+          const int q = lane * (kUSeg / 64) + k;
+          if (q < ng) upref[q] = run;
+          run += sz[k];
+        }
+        if (lane == 63) { upref[ng] = incl; utotal = incl; }
       }
-      const int total = __shfl(incl, 63, 64);
-      if (lane == 0) upref[ng] = total;
       __syncthreads();
-      for (int e0 = 0; e0 < total; e0 += kSrcChunk) {
-        const int ns = min(kSrcChunk, total - e0);
+      const int total = utotal;
+      for (int r0 = 0; r0 < total; r0 += kLeafWaves * kSrcChunk) {
+        const int e0 = r0 + wave * kSrcChunk;
+        const int ns = max(0, min(kSrcChunk, total - e0));
         __syncthreads();
-        for (int e = lane; e < ns; e += kLeafBlock) {
+        for (int e = lane; e < ns; e += 64) {
           const int ge = e0 + e;
           int lo = 0, hi = ng - 1;  // last q with upref[q] <= ge
           while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
             if (upref[mid] <= ge) lo = mid; else hi = mid - 1;
           }
-          src[e] = pts[ubeg[lo] + (ge - upref[lo])];
+          mysrc[e] = pts[ubeg[lo] + (ge - upref[lo])];
         }
         __syncthreads();
         if (act) {
           for (int j = sl; j < ns; j += S) {
-            const float4 sj = src[j];
+            const float4 sj = mysrc[j];
             const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
             const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
             acc += sj.w * __builtin_amdgcn_rsqf(r2);
@@ -550,11 +559,12 @@ __global__ __launch_bounds__(kLeafBlock) void k_leaf(
       }
     }
     __syncthreads();
-    red[lane] = act ? acc : 0.0f;
+    red[threadIdx.x] = act ? acc : 0.0f;
     __syncthreads();
-    if (sl == 0) {
+    if (wave == 0 && sl == 0) {
       float nearf = 0.0f;
-      for (int k = 0; k < S; ++k) nearf += red[tl + k * ntc];
+      for (int w = 0; w < kLeafWaves; ++w)
+        for (int k = 0; k < S; ++k) nearf += red[w * 64 + tl + k * ntc];
       const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w, lc);
       float v = ke * (far + nearf) + selfc_t[i] * xi.w;
       const int nb = near_ptr[i], ne = near_ptr[i + 1];
@@ -665,12 +675,12 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   ED_MARK(6);
   const float ke = (float)kKE;
   if (out64)
-    hipLaunchKernelGGL((k_leaf<P, double>), dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
+    hipLaunchKernelGGL((k_leaf<P, double>), dim3(nleaf), dim3(kLeafThreads), 0, st, Pn->leaves,
                        Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->L, Pn->u_ptr, Pn->u_src,
                        Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm,
                        out_tree, ke, (double*)out);
   else
-    hipLaunchKernelGGL((k_leaf<P, float>), dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
+    hipLaunchKernelGGL((k_leaf<P, float>), dim3(nleaf), dim3(kLeafThreads), 0, st, Pn->leaves,
                        Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->L, Pn->u_ptr, Pn->u_src,
                        Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm,
                        out_tree, ke, (float*)out);
