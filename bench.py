@@ -12,7 +12,9 @@ operator.
 A step is one matvec of the solve operator (FMM far field + P2P + near
 correction + self term) over the 81,920 panels with sigma resident in HBM.
 ``value`` = panels / s of that matvec.  The line also reports the time to a
-1e-8 relative residual (set-up excluded and reported separately), per-stage
+1e-8 relative residual from the assembled geometry (set-up, first solve
+with its workspace and graph capture; the set-up share is also reported
+separately), per-stage
 times, the roofline of the dominant kernel and the CPU baseline (the oracle's
 C restatement of the dense reference matvec, OpenMP, timed on a bounded
 sample of rows on this host).
@@ -110,36 +112,49 @@ def main():
     u /= np.linalg.norm(u)
     r0 = rho * u
 
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    op = BemOperator(mesh, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
-                     near=True, self_depth=3, near_depth=3)
-    torch.cuda.synchronize()
-    setup_s = time.perf_counter() - t0
-    st = op.plan.stats()
-    b = op.rhs_point_charges([(1.0, tuple(r0))])
+    def build():
+        return BemOperator(mesh, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
+                           near=True, self_depth=3, near_depth=3)
 
+    # first construction loads the code objects; the timed path starts from a
+    # second one (plan, tree, quadrature set-up, solver workspace and graph
+    # capture all included)
+    op = build()
+    b = op.rhs_point_charges([(1.0, tuple(r0))])
     solve = None
     if not a.no_solve:
-        x, info = op.solve(b, tol=1e-8)  # warm (graph capture, workspace)
+        op.solve(b, tol=1e-8)
+        del op
         torch.cuda.synchronize()
         t0 = time.perf_counter()
+        op = build()
+        torch.cuda.synchronize()
+        setup_s = time.perf_counter() - t0
+        b = op.rhs_point_charges([(1.0, tuple(r0))])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
         x, info = op.solve(b, tol=1e-8)
         torch.cuda.synchronize()
-        solve_s = time.perf_counter() - t0
+        first_solve_s = time.perf_counter() - t1
+        t1 = time.perf_counter()
+        x, info = op.solve(b, tol=1e-8)
+        torch.cuda.synchronize()
+        solve_s = time.perf_counter() - t1
         from oracle import analytic  # checker only
         sig_k = analytic.kelvin_sigma(mesh.centroids, 1.0, r0, 1.0)
         err_k = float(np.linalg.norm(x.cpu().numpy() - sig_k) / np.linalg.norm(sig_k))
         r = b - op.accurate(x)
         rel = float(torch.linalg.norm(r) / torch.linalg.norm(b))
-        solve = dict(seconds=solve_s, setup_seconds=setup_s, gmres_iters=info["iters"],
-                     refine_steps=info["refine_steps"], code=info["code"],
-                     rel_resid_fp64_dense=rel, err_vs_kelvin=err_k,
+        solve = dict(time_to_solution_s=setup_s + first_solve_s, setup_s=setup_s,
+                     first_solve_s=first_solve_s, repeat_solve_s=solve_s,
+                     gmres_iters=info["iters"], refine_steps=info["refine_steps"],
+                     code=info["code"], rel_resid_fp64_dense=rel, err_vs_kelvin=err_k,
                      total_charge=op.total_charge(x), expected_charge=-1.0 / rho)
         sigma = x
     else:
         sigma = torch.as_tensor(rng.normal(size=n), dtype=torch.float64, device=dev)
 
+    st = op.plan.stats()
     out = torch.empty_like(sigma)
     for _ in range(a.warmup):
         op.plan.matvec(sigma, op.self_corr, out=out)
@@ -201,7 +216,7 @@ def main():
         "timed_wall_s": wall,
     }
     if solve is not None:
-        line["time_to_1e-8_s"] = solve["seconds"]
+        line["time_to_1e-8_s"] = solve["time_to_solution_s"]
     print(json.dumps(line))
     return 0
 

@@ -71,6 +71,9 @@ _SIGS = {
     "ed_gmres_solve": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(GmresOpts),
                                ctypes.POINTER(GmresInfo), c_vp]),
     "ed_plan_set_coarse": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp]),
+    "ed_plan_coarse_solve": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
+    "ed_plan_build_sai": (c_i32, [c_vp, c_vp, ctypes.c_double, ctypes.c_double, c_vp]),
+    "ed_plan_sai_apply": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
     "ed_vec_dot_f64": (c_i32, [c_vp, c_vp, c_i64, c_vp, ctypes.POINTER(c_dbl), c_vp]),
     "ed_vec_axpy_f64": (c_i32, [c_dbl, c_vp, c_vp, c_i64, c_vp]),
     "ed_vec_scale_f64": (c_i32, [c_dbl, c_vp, c_vp, c_i64, c_vp]),
@@ -85,6 +88,8 @@ _SIGS = {
                             c_vp]),
     "ed_panel_features": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
     "ed_panel_feature_count": (c_i32, []),
+    "ed_proj_loss_f32": (c_i32, [c_vp, c_i32, c_vp, c_i32, c_vp, c_i64, c_i32, ctypes.c_float,
+                                 c_vp, c_vp, c_vp]),
     "ed_host_tree_build": (c_i32, [c_vp, c_i64, c_i32, c_i32, c_i32, c_vp, c_vp, c_vp, c_vp,
                                    c_vp, c_vp, c_vp, c_vp]),
     "ed_host_operator": (c_i32, [c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp]),

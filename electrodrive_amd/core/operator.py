@@ -141,13 +141,23 @@ class BemOperator:
                                           weights=self.near_w)
         return v
 
+    def enable_sai(self, factor_pattern=2.0, factor_rows=4.0):
+        """Builds the sparse approximate inverse smoother (``precond="sai"``)."""
+        self.plan.build_sai(self.self_corr, factor_pattern, factor_rows)
+        self._sai = True
+
     def solve(self, b, x0=None, *, tol=1e-8, restart=50, maxiter=2000, refine=True,
               inner_tol=1e-4, max_refine=8, precond=True, use_graph=True, check_every=4):
         """FP64 GMRES on the FMM operator with FP64 iterative refinement against
-        the accurate operator (``refine=True``); returns (sigma, info)."""
+        the accurate operator (``refine=True``); returns (sigma, info).
+        ``precond``: True / "jacobi" (core/bem.py:2141-2158), "sai" (sparse
+        approximate inverse, built on first use) or False / "none"."""
         if x0 is None:
             x0 = (b / self.diag).contiguous()  # core/bem.py:2060
-        return self.plan.solve(b, x0, self.diag if precond else None, self.self_corr,
+        if precond == "sai" and not getattr(self, "_sai", False):
+            self.enable_sai()
+        use_diag = precond not in (False, None, "none")
+        return self.plan.solve(b, x0, self.diag if use_diag else None, self.self_corr,
                                restart=restart, maxiter=maxiter, tol=tol, precond=precond,
                                refine=refine, max_refine=max_refine, inner_tol=inner_tol,
                                use_graph=use_graph, check_every=check_every)

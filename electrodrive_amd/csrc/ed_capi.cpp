@@ -69,9 +69,10 @@ void plan_free(ed_plan* P) {
                   P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
                   P->near_w, P->cent64_t, P->area64_t, P->near_w64, P->m2l_ascale,
                   P->m2l_binv, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
-                  P->coarse_part};
+                  P->coarse_part, P->coarse_tmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  sai_free(P);
   gmres_free(P);
 }
 
@@ -453,6 +454,7 @@ int ed_plan_attach_near(ed_plan* P, const int64_t* pairs, const double* w, int64
   }
   std::stable_sort(es.begin(), es.end(), [](const E& a, const E& b) { return a.row < b.row; });
   gmres_free(P);  // a captured Arnoldi graph holds the old near arrays
+  sai_free(P);    // the sparse approximate inverse was fitted to the old operator
   std::vector<int32_t> ptr(n + 1, 0), col(es.size());
   std::vector<float> ww(es.size());
   std::vector<double> ww64(es.size());

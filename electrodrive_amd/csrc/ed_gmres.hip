@@ -27,7 +27,8 @@ struct GmresWS {
   int64_t n = 0;
   int m = 0;
   double *V = nullptr, *w = nullptr, *vin = nullptr, *x = nullptr, *b = nullptr, *r = nullptr,
-         *t = nullptr, *dinv = nullptr, *selfi_t = nullptr, *d = nullptr, *rhs = nullptr;
+         *t = nullptr, *dinv = nullptr, *selfi_t = nullptr, *d = nullptr, *rhs = nullptr,
+         *u2 = nullptr;
   double *H = nullptr, *g = nullptr, *cs = nullptr, *sn = nullptr, *h = nullptr, *h2 = nullptr;
   double* partial = nullptr;  // kRedBlocks x (m + 2)
   double* scal = nullptr;     // device scalars
@@ -37,6 +38,7 @@ struct GmresWS {
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
   const double* graph_selfi = nullptr;
+  int graph_mode = -1;  // preconditioner the captured graph was recorded with
 };
 
 // device scalar slots
@@ -48,7 +50,7 @@ void gmres_free(ed_plan* P) {
   if (W->gexec) (void)hipGraphExecDestroy(W->gexec);
   if (W->graph) (void)hipGraphDestroy(W->graph);
   void* ptrs[] = {W->V, W->w, W->vin, W->x, W->b, W->r, W->t, W->dinv, W->selfi_t, W->d, W->rhs,
-                  W->H, W->g, W->cs, W->sn, W->h, W->h2, W->partial, W->scal, W->dj};
+                  W->u2, W->H, W->g, W->cs, W->sn, W->h, W->h2, W->partial, W->scal, W->dj};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (W->hscal) (void)hipHostFree(W->hscal);
@@ -87,7 +89,7 @@ static int ws_get(ed_plan* P, int m, GmresWS** out) {
   const size_t n = (size_t)P->n;
   ED_G(hipMalloc((void**)&W->V, sizeof(double) * n * (m + 1)));
   double** vecs[] = {&W->w, &W->vin, &W->x, &W->b, &W->r, &W->t, &W->dinv, &W->selfi_t, &W->d,
-                     &W->rhs};
+                     &W->rhs, &W->u2};
   for (double** v : vecs) ED_G(hipMalloc((void**)v, sizeof(double) * n));
   ED_G(hipMalloc((void**)&W->H, sizeof(double) * (m + 1) * m));
   ED_G(hipMalloc((void**)&W->g, sizeof(double) * (m + 1)));
@@ -428,6 +430,34 @@ __global__ void k_coarse_apply(int64_t n, int k, const double* __restrict__ u,
   }
 }
 
+// out = Z cy
+__global__ void k_coarse_combine(int64_t n, int k, const double* __restrict__ Z,
+                                 const double* __restrict__ cy, double* __restrict__ out) {
+  __shared__ double c[kMaxCoarse];
+  if (threadIdx.x < k) c[threadIdx.x] = cy[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = 0.0;
+    for (int q = 0; q < k; ++q) v += c[q] * Z[(size_t)q * n + i];
+    out[i] = v;
+  }
+}
+
+// u -= AZ cy (in place)
+__global__ void k_coarse_sub(int64_t n, int k, const double* __restrict__ AZ,
+                             const double* __restrict__ cy, double* __restrict__ u) {
+  __shared__ double c[kMaxCoarse];
+  if (threadIdx.x < k) c[threadIdx.x] = cy[threadIdx.x];
+  __syncthreads();
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double v = u[i];
+    for (int q = 0; q < k; ++q) v -= c[q] * AZ[(size_t)q * n + i];
+    u[i] = v;
+  }
+}
+
 __global__ __launch_bounds__(kRedThreads) void k_norm2_part(int64_t n, const double* __restrict__ x,
                                                             double* __restrict__ partial) {
   __shared__ double sh[4];
@@ -554,7 +584,9 @@ struct Solver {
     }
     return ED_OK;
   }
-  // norms of (b - t) and dinv (b - t); r, z optional outputs
+  bool use_sai() const { return o->precond == 2; }
+  bool general() const { return P->coarse_k > 0 || use_sai(); }
+  int mode() const { return (use_sai() ? 4 : 0) + (o->precond ? 2 : 0) + (P->coarse_k > 0 ? 1 : 0); }
   // z = M^-1 u in place (coarse-corrected Jacobi); u may alias z
   void coarse_apply(const double* u, double* z) {
     const int k = P->coarse_k;
@@ -565,15 +597,35 @@ struct Solver {
     hipLaunchKernelGGL(k_coarse_apply, dim3(gb), dim3(256), 0, W->stream, n, k, u, W->dinv,
                        P->coarse_z, P->coarse_az, P->coarse_cy, z);
   }
+  // z = M^-1 u with the SAI smoother, M^-1 u = S (u - AZ c) + Z c; u is
+  // overwritten (in-place subtraction) and must not alias z
+  void sai_precond(double* u, double* z) {
+    const int k = P->coarse_k;
+    if (k > 0) {
+      hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, k,
+                         P->coarse_z, u, P->coarse_part);
+      hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), 0, W->stream, k, P->coarse_part,
+                         P->coarse_einv, P->coarse_cy);
+      hipLaunchKernelGGL(k_coarse_sub, dim3(gb), dim3(256), 0, W->stream, n, k, P->coarse_az,
+                         P->coarse_cy, u);
+    }
+    sai_apply_launch(P, u, k, P->coarse_z, P->coarse_cy, z, W->stream);
+  }
+  // norms of (b - t) and M^-1 (b - t); r, z optional outputs
   int resid_norms(const double* b, const double* t, double* r, double* z, double* nr, double* nz) {
-    const bool coarse = P->coarse_k > 0 && z != nullptr;
-    double* rr = (coarse && r == nullptr) ? W->vin : r;
+    const bool gen = general() && z != nullptr;
+    double* rr = (gen && r == nullptr) ? W->vin : r;
     hipLaunchKernelGGL(k_resid, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, b, t,
-                       W->dinv, rr, coarse ? nullptr : z, W->partial);
+                       W->dinv, rr, gen ? nullptr : z, W->partial);
     hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, W->stream, W->partial, 2, W->scal,
                        S_NRM_R);
-    if (coarse) {
-      coarse_apply(rr, z);
+    if (gen) {
+      if (use_sai()) {
+        ED_G(hipMemcpyAsync(W->u2, rr, sizeof(double) * n, hipMemcpyDeviceToDevice, W->stream));
+        sai_precond(W->u2, z);
+      } else {
+        coarse_apply(rr, z);
+      }
       hipLaunchKernelGGL(k_norm2_part, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, z,
                          W->partial);
       hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, W->stream, W->partial, 1, W->scal,
@@ -589,11 +641,15 @@ struct Solver {
   int arnoldi_step() {
     const int m = W->m;
     hipLaunchKernelGGL(k_copy_vj, dim3(gb), dim3(256), 0, W->stream, n, W->V, W->dj, W->vin);
-    ED_T(approx_op(W->vin, W->w));
-    const bool coarse = P->coarse_k > 0;
-    if (coarse) coarse_apply(W->w, W->w);
+    if (use_sai()) {
+      ED_T(approx_op(W->vin, W->u2));
+      sai_precond(W->u2, W->w);
+    } else {
+      ED_T(approx_op(W->vin, W->w));
+      if (P->coarse_k > 0) coarse_apply(W->w, W->w);
+    }
     hipLaunchKernelGGL(k_dots1, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n,
-                       coarse ? nullptr : W->dinv, W->w, W->V, W->dj, W->partial);
+                       general() ? nullptr : W->dinv, W->w, W->V, W->dj, W->partial);
     hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(256), 0, W->stream, W->partial, W->dj, W->h, 0);
     hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->w, W->V,
                        W->dj, W->h, 0, W->partial);
@@ -639,7 +695,7 @@ struct Solver {
         const int steps = std::min(std::min(ce, m - j), maxiter - local);
         for (int s = 0; s < steps; ++s) {
           if (use_graph) {
-            if (!W->gexec || W->graph_selfi != selfi_orig) {
+            if (!W->gexec || W->graph_selfi != selfi_orig || W->graph_mode != mode()) {
               if (W->gexec) { (void)hipGraphExecDestroy(W->gexec); W->gexec = nullptr; }
               if (W->graph) { (void)hipGraphDestroy(W->graph); W->graph = nullptr; }
               ED_G(hipStreamBeginCapture(W->stream, hipStreamCaptureModeThreadLocal));
@@ -651,6 +707,7 @@ struct Solver {
               W->graph = gr;
               ED_G(hipGraphInstantiate(&W->gexec, gr, nullptr, nullptr, 0));
               W->graph_selfi = selfi_orig;
+              W->graph_mode = mode();
             }
             ED_G(hipGraphLaunch(W->gexec, W->stream));
           } else {
@@ -714,6 +771,9 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
   if (opts) o = *opts;
   if (o.restart <= 0 || o.restart > 255) return fail(ED_ERR_ARG, "restart must be in [1, 255]");
   if (o.maxiter <= 0) return fail(ED_ERR_ARG, "maxiter must be positive");
+  if (o.precond < 0 || o.precond > 2) return fail(ED_ERR_ARG, "precond must be 0, 1 or 2");
+  if (o.precond == 2 && !P->sai_cnt)
+    return fail(ED_ERR_ARG, "precond 2 needs the sparse approximate inverse (ed_plan_build_sai)");
   std::memset(info, 0, sizeof(*info));
   info->first_iter_below_tol = -1;
   GmresWS* W = nullptr;
@@ -767,7 +827,11 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
       ED_G(hipMemcpyAsync(W->rhs, W->r, sizeof(double) * n, hipMemcpyDeviceToDevice, W->stream));
       int icode = 1;
       double ir0 = 0, ir1 = 0;
-      ED_T(S.gmres(W->rhs, W->d, o.inner_tol, o.maxiter - S.total_iters, &icode, &ir0, &ir1));
+      // the inner solve only has to take the residual down to the target: the
+      // last refinement step does not over-solve (0.25 leaves room for the
+      // approximate operator's own error)
+      const double itol = std::min(0.5, std::max(o.inner_tol, 0.25 * info->tol_abs / rn));
+      ED_T(S.gmres(W->rhs, W->d, itol, o.maxiter - S.total_iters, &icode, &ir0, &ir1));
       if (icode == 3 || icode == 5 || icode == 6 || icode == 7) { code = icode; break; }
       hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, W->stream, n, 1.0, W->d, W->x);
       ED_KCHECK("refine update");
@@ -798,10 +862,12 @@ extern "C" int ed_plan_set_coarse(ed_plan* P, const double* Z, int32_t k,
   if (!P || k < 0 || k > kMaxCoarse || (k > 0 && !Z))
     return fail(ED_ERR_ARG, "ed_plan_set_coarse: k must be in [0, 32] with Z given");
   gmres_free(P);  // a captured Arnoldi graph refers to the old coarse arrays
-  void* olds[] = {P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy, P->coarse_part};
+  void* olds[] = {P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy, P->coarse_part,
+                  P->coarse_tmp};
   for (void* q : olds)
     if (q) (void)hipFree(q);
   P->coarse_z = P->coarse_az = P->coarse_einv = P->coarse_cy = P->coarse_part = nullptr;
+  P->coarse_tmp = nullptr;
   P->coarse_k = 0;
   if (k == 0) return ED_OK;
   hipStream_t st = (hipStream_t)stream;
@@ -812,6 +878,7 @@ extern "C" int ed_plan_set_coarse(ed_plan* P, const double* Z, int32_t k,
   ED_G(hipMalloc((void**)&P->coarse_einv, sizeof(double) * k * k));
   ED_G(hipMalloc((void**)&P->coarse_cy, sizeof(double) * kMaxCoarse));
   ED_G(hipMalloc((void**)&P->coarse_part, sizeof(double) * kRedBlocks * k * k));
+  ED_G(hipMalloc((void**)&P->coarse_tmp, sizeof(double) * n));
   double* scal = nullptr;
   ED_G(hipMalloc((void**)&scal, sizeof(double) * 8));
   for (int c = 0; c < k; ++c)
@@ -845,5 +912,28 @@ extern "C" int ed_plan_set_coarse(ed_plan* P, const double* Z, int32_t k,
   ED_G(hipStreamSynchronize(st));
   (void)hipFree(scal);
   P->coarse_k = k;
+  return ED_OK;
+}
+
+// Coarse (Petrov-Galerkin) solve on span(Z): out = Z E^-1 Z^T u, E = Z^T A Z,
+// original panel order (no reference counterpart).  With u = b it is the
+// initial guess whose residual is orthogonal to span(Z).
+extern "C" int ed_plan_coarse_solve(ed_plan* P, const double* u, double* out, ed_stream stream) {
+  using namespace ed;
+  if (!P || !u || !out) return fail(ED_ERR_ARG, "ed_plan_coarse_solve: NULL argument");
+  if (P->coarse_k <= 0) return fail(ED_ERR_ARG, "ed_plan_coarse_solve: no coarse space set");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = P->n;
+  const int k = P->coarse_k;
+  const unsigned gn = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, st, n, P->perm, u, P->coarse_tmp);
+  hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, st, n, k,
+                     P->coarse_z, P->coarse_tmp, P->coarse_part);
+  hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), 0, st, k, P->coarse_part,
+                     P->coarse_einv, P->coarse_cy);
+  hipLaunchKernelGGL(k_coarse_combine, dim3(gn), dim3(256), 0, st, n, k, P->coarse_z,
+                     P->coarse_cy, P->coarse_tmp);
+  hipLaunchKernelGGL(k_perm_out, dim3(gn), dim3(256), 0, st, n, P->perm, P->coarse_tmp, out);
+  ED_KCHECK("coarse solve");
   return ED_OK;
 }

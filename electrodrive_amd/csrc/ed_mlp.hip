@@ -199,6 +199,96 @@ __global__ void k_features_leaf(const int32_t* __restrict__ leaves, const int2* 
   }
 }
 
+// Projection loss of a learned basis.  Solve s owns rows s*m .. s*m+m-1;
+// its basis is Z = [Y | X] (ky network columns, kx fixed columns, k <= 32).
+// c = argmin |t - Z c|^2 + lam |c|^2 with lam = ridge * trace(Z^T Z) / k,
+// r = t - Z c, loss_s = |r|^2 / |t|^2 and dY = -2 r c_Y^T / |t|^2 (lam held
+// fixed).  One block per solve; the k x k system is solved in FP64 by
+// Cholesky in LDS.
+constexpr int kProjMaxK = 32;
+__global__ __launch_bounds__(256) void k_proj_loss(const float* __restrict__ Y, int ky,
+                                                   const float* __restrict__ X, int kx,
+                                                   const float* __restrict__ T, int m, float ridge,
+                                                   float* __restrict__ dY,
+                                                   float* __restrict__ loss) {
+  __shared__ double G[kProjMaxK][kProjMaxK + 1];
+  __shared__ double h[kProjMaxK], c[kProjMaxK];
+  __shared__ double red[256];
+  const int k = ky + kx;
+  const int64_t r0 = (int64_t)blockIdx.x * m;
+  auto z = [&](int64_t i, int q) -> double {
+    return q < ky ? (double)Y[i * ky + q] : (double)X[i * kx + (q - ky)];
+  };
+  for (int e = threadIdx.x; e < k * k + k; e += blockDim.x) {
+    double acc = 0.0;
+    if (e < k * k) {
+      const int a = e / k, b = e % k;
+      if (b >= a)
+        for (int i = 0; i < m; ++i) acc += z(r0 + i, a) * z(r0 + i, b);
+      if (b >= a) { G[a][b] = acc; G[b][a] = acc; }
+    } else {
+      const int a = e - k * k;
+      for (int i = 0; i < m; ++i) acc += z(r0 + i, a) * (double)T[r0 + i];
+      h[a] = acc;
+    }
+  }
+  double tt = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) tt += (double)T[r0 + i] * T[r0 + i];
+  red[threadIdx.x] = tt;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  tt = red[0];
+  if (threadIdx.x == 0) {
+    double tr = 0.0;
+    for (int a = 0; a < k; ++a) tr += G[a][a];
+    const double lam = (double)ridge * (tr > 0.0 ? tr : 1.0) / k;
+    for (int a = 0; a < k; ++a) G[a][a] += lam;
+    // Cholesky G = L L^T in place (lower triangle)
+    for (int j = 0; j < k; ++j) {
+      double d = G[j][j];
+      for (int q = 0; q < j; ++q) d -= G[j][q] * G[j][q];
+      d = d > 1e-300 ? sqrt(d) : 1e-150;
+      G[j][j] = d;
+      for (int i = j + 1; i < k; ++i) {
+        double v = G[i][j];
+        for (int q = 0; q < j; ++q) v -= G[i][q] * G[j][q];
+        G[i][j] = v / d;
+      }
+    }
+    for (int i = 0; i < k; ++i) {  // L y = h
+      double v = h[i];
+      for (int q = 0; q < i; ++q) v -= G[i][q] * c[q];
+      c[i] = v / G[i][i];
+    }
+    for (int i = k - 1; i >= 0; --i) {  // L^T c = y
+      double v = c[i];
+      for (int q = i + 1; q < k; ++q) v -= G[q][i] * c[q];
+      c[i] = v / G[i][i];
+    }
+  }
+  __syncthreads();
+  const double inv_tt = tt > 0.0 ? 1.0 / tt : 0.0;
+  double rr = 0.0;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int64_t row = r0 + i;
+    double v = (double)T[row];
+    for (int q = 0; q < k; ++q) v -= z(row, q) * c[q];
+    rr += v * v;
+    for (int q = 0; q < ky; ++q) dY[row * ky + q] = (float)(-2.0 * v * c[q] * inv_tt);
+  }
+  __syncthreads();
+  red[threadIdx.x] = rr;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) loss[blockIdx.x] = (float)(red[0] * inv_tt);
+}
+
 #define ED_MLP_CHECK(msg)                                                          \
   do {                                                                             \
     hipError_t e_ = hipGetLastError();                                             \
@@ -330,6 +420,22 @@ int ed_panel_features(ed_plan* P, const double* C, const double* Nrm, const doub
   (void)hipFree(dgs);
   (void)hipFree(dgeom);
   if (e != hipSuccess) return fail(ED_ERR_LAUNCH, std::string("ed_panel_features: ") + hipGetErrorString(e));
+  return ED_OK;
+}
+
+// Projection loss of a learned coarse basis (no reference counterpart), see
+// k_proj_loss.  Y (n_sol*m x ky), X (n_sol*m x kx), T (n_sol*m), dY like Y,
+// loss (n_sol): device FP32.
+int ed_proj_loss_f32(const float* Y, int32_t ky, const float* X, int32_t kx, const float* T,
+                     int64_t n_sol, int32_t m, float ridge, float* dY, float* loss,
+                     ed_stream stream) {
+  if (ky < 1 || kx < 0 || ky + kx > kProjMaxK || n_sol < 0 || m < 1 || !(ridge >= 0.0f))
+    return fail(ED_ERR_ARG, "ed_proj_loss_f32: need 1 <= ky, ky + kx <= 32, m >= 1, ridge >= 0");
+  if (n_sol == 0) return ED_OK;
+  if (!Y || (kx > 0 && !X) || !T || !dY || !loss) return fail(ED_ERR_ARG, "ed_proj_loss_f32: NULL");
+  hipLaunchKernelGGL(k_proj_loss, dim3((unsigned)n_sol), dim3(256), 0, (hipStream_t)stream, Y, ky,
+                     X, kx, T, m, ridge, dY, loss);
+  ED_MLP_CHECK("ed_proj_loss_f32");
   return ED_OK;
 }
 

@@ -1,0 +1,37 @@
+// Near-pair search shared by the quadrature set-up and the sparse approximate
+// inverse: a host-built uniform cell grid and device count / fill passes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+namespace ed {
+
+struct NearGrid {
+  double lo[3];
+  double inv_h;
+  int32_t dim[3];
+};
+
+// Pairs (i, j), i != j, 0 < |c_i - c_j| <= df (R_i + R_j), R = sqrt(A / pi),
+// rows i-major with j ascending.  C (n, 3) and A (n) are device FP64.
+struct NearSearch {
+  const double* C = nullptr;
+  const double* A = nullptr;
+  int64_t n = 0;
+  double df = 0.0;
+  NearGrid g{};
+  int32_t* d_start = nullptr;
+  int32_t* d_pts = nullptr;
+  bool any = false;  // false: no pair can exist
+  ~NearSearch();
+  int build(const double* C, const double* A, int64_t n, double df, hipStream_t st);
+  // row offsets (n + 1, host, exclusive prefix of the row counts)
+  int count(hipStream_t st, std::vector<int64_t>& ptr);
+  // pairs (int64 (i, j)) and/or int32 columns j + col_off at the row offsets
+  int fill(hipStream_t st, const std::vector<int64_t>& ptr, int64_t* pairs, int32_t* cols32,
+           int64_t col_off);
+};
+
+}  // namespace ed

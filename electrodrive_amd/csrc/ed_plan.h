@@ -88,6 +88,11 @@ struct ed_plan {
   double* coarse_einv = nullptr;  // k x k, (Z^T A Z)^-1
   double* coarse_cy = nullptr;
   double* coarse_part = nullptr;
+  double* coarse_tmp = nullptr;   // n, coarse solve staging
+  // sparse approximate inverse smoother (column-major ELL, 32 slots per row)
+  int32_t* sai_col = nullptr;
+  double* sai_val = nullptr;
+  int32_t* sai_cnt = nullptr;
 
   // stage events while ed_fmm_profile runs (8 markers), else NULL
   hipEvent_t* prof_events = nullptr;
@@ -104,4 +109,9 @@ void plan_free(ed_plan* P);
 int fmm_launch(ed_plan* P, const void* sigma, int sigma_is_f64, int sigma_tree_order,
                const void* self_integrals, void* out, int out_is_f64,
                int out_tree_order, hipStream_t stream);
+void gmres_free(ed_plan* P);
+// z = M_sai u (+ Z cy when k > 0), tree order
+void sai_apply_launch(ed_plan* P, const double* u, int k, const double* Z, const double* cy,
+                      double* z, hipStream_t st);
+void sai_free(ed_plan* P);
 }  // namespace ed

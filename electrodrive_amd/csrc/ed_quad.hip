@@ -10,11 +10,20 @@
 // off, so the selected index set matches the reference bit for bit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <vector>
 
 #include "ed_internal.h"
+#include "ed_near.h"
 
 namespace ed {
+
+#define ED_TRY(call)               \
+  do {                             \
+    const int rc_ = (call);        \
+    if (rc_ != ED_OK) return rc_;  \
+  } while (0)
 
 #define ED_CHECK_LAUNCH(msg)                                                      \
   do {                                                                            \
@@ -121,51 +130,100 @@ __global__ void k_self_integrals(const double* __restrict__ PV, const double* __
   out[i] = acc;
 }
 
-// Near pairs: phase 0 counts per row, phase 1 fills at row offsets.
-constexpr int kNearBlock = 256;
+// Near pairs.  Candidates come from a uniform cell grid built on the host
+// (cell edge h >= the largest possible cutoff 2 * df * R_max, so a partner j
+// of i always lies in one of the 27 cells around i's cell); the acceptance
+// test itself is the reference's, evaluated with the same operations as a
+// brute-force scan.  Phase 0 counts per row, phase 1 fills at the row offsets
+// and sorts each row by j, so the output equals the i-major, j-ascending scan.
+// R = sqrt(clip(A, 0, inf) / pi): a NaN area stays NaN (no partner at all)
+__device__ __forceinline__ double near_radius(double a) {
+#pragma clang fp contract(off)
+  a = a < 0.0 ? 0.0 : a;
+  return sqrt(a / 3.141592653589793);
+}
+
+__device__ __forceinline__ int near_cell_coord(double x, double lo, double inv_h, int dim) {
+  int c = (int)floor((x - lo) * inv_h);
+  return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
+}
+
+constexpr int kNearBlock = 128;
 __global__ __launch_bounds__(kNearBlock) void k_near_pairs(const double* __restrict__ C,
                                                            const double* __restrict__ A,
-                                                           int64_t n, double df,
+                                                           int64_t n, double df, NearGrid g,
+                                                           const int32_t* __restrict__ cell_start,
+                                                           const int32_t* __restrict__ cell_pts,
                                                            int64_t* __restrict__ counts,
                                                            const int64_t* __restrict__ offs,
-                                                           int64_t* __restrict__ pairs) {
+                                                           int64_t* __restrict__ pairs,
+                                                           int32_t* __restrict__ cols32,
+                                                           int64_t col_off) {
 #pragma clang fp contract(off)
-  __shared__ double sx[kNearBlock], sy[kNearBlock], sz[kNearBlock], sr[kNearBlock];
   const int64_t i = (int64_t)blockIdx.x * kNearBlock + threadIdx.x;
-  const bool active = i < n;
-  double xi = 0, yi = 0, zi = 0, ri = 0;
-  const double pi = 3.141592653589793;
-  if (active) {
-    xi = C[3 * i]; yi = C[3 * i + 1]; zi = C[3 * i + 2];
-    double a = A[i];
-    a = a > 0.0 ? a : 0.0;
-    ri = sqrt(a / pi);
-  }
+  if (i >= n) return;
+  const double xi = C[3 * i], yi = C[3 * i + 1], zi = C[3 * i + 2];
+  const double ri = near_radius(A[i]);
   int64_t cnt = 0;
-  int64_t w = (offs && active) ? offs[i] : 0;
-  for (int64_t j0 = 0; j0 < n; j0 += kNearBlock) {
-    const int64_t j = j0 + threadIdx.x;
-    __syncthreads();
-    if (j < n) {
-      sx[threadIdx.x] = C[3 * j]; sy[threadIdx.x] = C[3 * j + 1]; sz[threadIdx.x] = C[3 * j + 2];
-      double a = A[j];
-      a = a > 0.0 ? a : 0.0;
-      sr[threadIdx.x] = sqrt(a / pi);
-    }
-    __syncthreads();
-    if (!active) continue;
-    const int nt = (int)((n - j0) < kNearBlock ? (n - j0) : kNearBlock);
-    for (int t = 0; t < nt; ++t) {
-      const double dx = xi - sx[t], dy = yi - sy[t], dz = zi - sz[t];
-      const double d = sqrt(dx * dx + dy * dy + dz * dz);
-      const double th = df * (ri + sr[t]);
-      if (isfinite(d) && isfinite(th) && d > 0.0 && d <= th) {
-        if (pairs) { pairs[2 * w] = i; pairs[2 * w + 1] = j0 + t; ++w; }
-        ++cnt;
+  const int64_t w0 = offs ? offs[i] : 0;
+  if (isfinite(xi) && isfinite(yi) && isfinite(zi) && isfinite(ri)) {
+    const int cx = near_cell_coord(xi, g.lo[0], g.inv_h, g.dim[0]);
+    const int cy = near_cell_coord(yi, g.lo[1], g.inv_h, g.dim[1]);
+    const int cz = near_cell_coord(zi, g.lo[2], g.inv_h, g.dim[2]);
+    for (int dz = -1; dz <= 1; ++dz) {
+      const int z = cz + dz;
+      if (z < 0 || z >= g.dim[2]) continue;
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int y = cy + dy;
+        if (y < 0 || y >= g.dim[1]) continue;
+        for (int dx = -1; dx <= 1; ++dx) {
+          const int x = cx + dx;
+          if (x < 0 || x >= g.dim[0]) continue;
+          const int64_t cell = ((int64_t)z * g.dim[1] + y) * g.dim[0] + x;
+          for (int32_t q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
+            const int64_t j = cell_pts[q];
+            const double ex = xi - C[3 * j], ey = yi - C[3 * j + 1], ez = zi - C[3 * j + 2];
+            const double d = sqrt(ex * ex + ey * ey + ez * ez);
+            const double th = df * (ri + near_radius(A[j]));
+            if (isfinite(d) && isfinite(th) && d > 0.0 && d <= th) {
+              if (pairs) {
+                pairs[2 * (w0 + cnt)] = i;
+                pairs[2 * (w0 + cnt) + 1] = j;
+              }
+              if (cols32) cols32[w0 + cnt] = (int32_t)(j + col_off);
+              ++cnt;
+            }
+          }
+        }
       }
     }
   }
-  if (active && counts) counts[i] = cnt;
+  if (counts) counts[i] = cnt;
+  // insertion sort of the row by j (rows hold tens of entries)
+  if (pairs) {
+    int64_t* row = pairs + 2 * w0;
+    for (int64_t a = 1; a < cnt; ++a) {
+      const int64_t v = row[2 * a + 1];
+      int64_t b = a - 1;
+      while (b >= 0 && row[2 * b + 1] > v) {
+        row[2 * (b + 1) + 1] = row[2 * b + 1];
+        --b;
+      }
+      row[2 * (b + 1) + 1] = v;
+    }
+  }
+  if (cols32) {
+    int32_t* row = cols32 + w0;
+    for (int64_t a = 1; a < cnt; ++a) {
+      const int32_t v = row[a];
+      int64_t b = a - 1;
+      while (b >= 0 && row[b] > v) {
+        row[b + 1] = row[b];
+        --b;
+      }
+      row[b + 1] = v;
+    }
+  }
 }
 
 __global__ void k_near_weights(const double* __restrict__ C, const double* __restrict__ A,
@@ -340,6 +398,137 @@ static int morton_launch(const T* P, int64_t n, int level, const T* bc, const T*
   return ED_OK;
 }
 
+// Host cell grid of the near-pair search.  Points with a non-finite
+// coordinate or radius can have no partner and are left out.  Returns false
+// when no pair can exist (df or every radius zero / non-finite).
+static bool near_grid_build(const double* C, const double* A, int64_t n, double df, NearGrid& g,
+                            std::vector<int32_t>& start, std::vector<int32_t>& pts) {
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double rmax = 0.0;
+  std::vector<char> ok((size_t)n, 0);
+  int64_t n_ok = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    const double a = A[i] < 0.0 ? 0.0 : A[i];
+    const double r = std::sqrt(a / 3.141592653589793);
+    const double* c = C + 3 * i;
+    if (!std::isfinite(c[0]) || !std::isfinite(c[1]) || !std::isfinite(c[2]) || !std::isfinite(r))
+      continue;
+    ok[i] = 1;
+    ++n_ok;
+    rmax = std::max(rmax, r);
+    for (int d = 0; d < 3; ++d) { lo[d] = std::min(lo[d], c[d]); hi[d] = std::max(hi[d], c[d]); }
+  }
+  const double cut = 2.0 * df * rmax;
+  if (n_ok < 2 || !(cut > 0.0) || !std::isfinite(cut)) return false;
+  // cell edge: the cutoff with a relative margin for the rounding of the
+  // cell coordinates, enlarged until the grid holds at most max(4n, 4096) cells
+  double h = cut * (1.0 + 1e-6) + 1e-300;
+  const double max_cells = std::max<double>(4.0 * (double)n, 4096.0);
+  for (;;) {
+    double cells = 1.0;
+    for (int d = 0; d < 3; ++d) cells *= std::floor((hi[d] - lo[d]) / h) + 1.0;
+    if (cells <= max_cells) break;
+    h *= std::max(1.01, std::cbrt(cells / max_cells));
+  }
+  for (int d = 0; d < 3; ++d) {
+    g.lo[d] = lo[d];
+    g.dim[d] = (int32_t)(std::floor((hi[d] - lo[d]) / h) + 1.0);
+  }
+  g.inv_h = 1.0 / h;
+  const int64_t ncell = (int64_t)g.dim[0] * g.dim[1] * g.dim[2];
+  auto coord = [&](double x, int d) {
+    int c = (int)std::floor((x - g.lo[d]) * g.inv_h);
+    return c < 0 ? 0 : (c >= g.dim[d] ? g.dim[d] - 1 : c);
+  };
+  std::vector<int32_t> cell((size_t)n, -1);
+  start.assign((size_t)ncell + 1, 0);
+  for (int64_t i = 0; i < n; ++i) {
+    if (!ok[i]) continue;
+    const double* c = C + 3 * i;
+    const int64_t id = ((int64_t)coord(c[2], 2) * g.dim[1] + coord(c[1], 1)) * g.dim[0] + coord(c[0], 0);
+    cell[i] = (int32_t)id;
+    ++start[id + 1];
+  }
+  for (int64_t k = 0; k < ncell; ++k) start[k + 1] += start[k];
+  pts.assign((size_t)n_ok, 0);
+  std::vector<int32_t> cur(start.begin(), start.end() - 1);
+  for (int64_t i = 0; i < n; ++i)
+    if (cell[i] >= 0) pts[cur[cell[i]]++] = (int32_t)i;
+  return true;
+}
+
+NearSearch::~NearSearch() {
+  if (d_start) (void)hipFree(d_start);
+  if (d_pts) (void)hipFree(d_pts);
+}
+
+int NearSearch::build(const double* Cd, const double* Ad, int64_t n_, double df_, hipStream_t st) {
+  C = Cd;
+  A = Ad;
+  n = n_;
+  df = df_;
+  any = false;
+  if (n > (int64_t)INT32_MAX) return fail(ED_ERR_ARG, "near pairs: n exceeds int32 range");
+  if (n <= 1) return ED_OK;
+  std::vector<double> hc((size_t)n * 3), ha((size_t)n);
+  hipError_t e = hipMemcpyAsync(hc.data(), C, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(ha.data(), A, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) return fail(ED_ERR_DEVICE, std::string("near pairs: ") + hipGetErrorString(e));
+  std::vector<int32_t> start, pts;
+  if (!near_grid_build(hc.data(), ha.data(), n, df, g, start, pts)) return ED_OK;
+  const int64_t ncell = (int64_t)start.size() - 1;
+  if (hipMalloc((void**)&d_start, sizeof(int32_t) * (ncell + 1)) != hipSuccess ||
+      hipMalloc((void**)&d_pts, sizeof(int32_t) * std::max<size_t>(pts.size(), 1)) != hipSuccess)
+    return fail(ED_ERR_ALLOC, "near pairs: alloc");
+  e = hipMemcpyAsync(d_start, start.data(), sizeof(int32_t) * (ncell + 1), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess && !pts.empty())
+    e = hipMemcpyAsync(d_pts, pts.data(), sizeof(int32_t) * pts.size(), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);  // host vectors go out of scope
+  if (e != hipSuccess) return fail(ED_ERR_DEVICE, std::string("near pairs: ") + hipGetErrorString(e));
+  any = true;
+  return ED_OK;
+}
+
+int NearSearch::count(hipStream_t st, std::vector<int64_t>& ptr) {
+  ptr.assign((size_t)std::max<int64_t>(n, 0) + 1, 0);
+  if (!any) return ED_OK;
+  int64_t* counts = nullptr;
+  if (hipMalloc((void**)&counts, sizeof(int64_t) * n) != hipSuccess)
+    return fail(ED_ERR_ALLOC, "near pairs: alloc");
+  const unsigned grid = (unsigned)((n + kNearBlock - 1) / kNearBlock);
+  hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, df, g,
+                     (const int32_t*)d_start, (const int32_t*)d_pts, counts,
+                     (const int64_t*)nullptr, (int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpyAsync(ptr.data() + 1, counts, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(counts);
+  if (e != hipSuccess) return fail(ED_ERR_DEVICE, std::string("near pairs: ") + hipGetErrorString(e));
+  for (int64_t i = 0; i < n; ++i) ptr[i + 1] += ptr[i];
+  return ED_OK;
+}
+
+int NearSearch::fill(hipStream_t st, const std::vector<int64_t>& ptr, int64_t* pairs,
+                     int32_t* cols32, int64_t col_off) {
+  if (!any || ptr[n] == 0) return ED_OK;
+  int64_t* offs = nullptr;
+  if (hipMalloc((void**)&offs, sizeof(int64_t) * n) != hipSuccess)
+    return fail(ED_ERR_ALLOC, "near pairs: alloc");
+  hipError_t e = hipMemcpyAsync(offs, ptr.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) {
+    const unsigned grid = (unsigned)((n + kNearBlock - 1) / kNearBlock);
+    hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, df, g,
+                       (const int32_t*)d_start, (const int32_t*)d_pts, (int64_t*)nullptr,
+                       (const int64_t*)offs, pairs, cols32, col_off);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(offs);
+  if (e != hipSuccess) return fail(ED_ERR_LAUNCH, std::string("near pairs: ") + hipGetErrorString(e));
+  return ED_OK;
+}
+
 }  // namespace ed
 
 using namespace ed;
@@ -364,39 +553,15 @@ int ed_near_pairs_f64(const double* C, const double* A, int64_t n, double df, in
   if (n <= 1) return ED_OK;
   if (!C || !A) return fail(ED_ERR_ARG, "near pairs: NULL geometry");
   hipStream_t st = (hipStream_t)stream;
-  int64_t *counts = nullptr, *offs = nullptr;
-  if (hipMalloc((void**)&counts, sizeof(int64_t) * n) != hipSuccess ||
-      hipMalloc((void**)&offs, sizeof(int64_t) * n) != hipSuccess) {
-    if (counts) (void)hipFree(counts);
-    return fail(ED_ERR_ALLOC, "near pairs: alloc");
-  }
-  const unsigned grid = (unsigned)((n + kNearBlock - 1) / kNearBlock);
-  hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, df, counts,
-                     (const int64_t*)nullptr, (int64_t*)nullptr);
-  std::vector<int64_t> hc(n);
-  hipError_t e = hipMemcpyAsync(hc.data(), counts, sizeof(int64_t) * n, hipMemcpyDeviceToHost, st);
-  if (e == hipSuccess) e = hipStreamSynchronize(st);
-  int rc = ED_OK;
-  if (e != hipSuccess) rc = fail(ED_ERR_DEVICE, std::string("near pairs: ") + hipGetErrorString(e));
-  int64_t total = 0;
-  if (rc == ED_OK) {
-    for (int64_t i = 0; i < n; ++i) { const int64_t c = hc[i]; hc[i] = total; total += c; }
-    *n_pairs_out = total;
-    if (pairs) {
-      if (capacity < total) rc = fail(ED_ERR_ARG, "near pairs: capacity too small");
-      else {
-        (void)hipMemcpyAsync(offs, hc.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st);
-        hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, df,
-                           (int64_t*)nullptr, (const int64_t*)offs, pairs);
-        e = hipGetLastError();
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-        if (e != hipSuccess) rc = fail(ED_ERR_LAUNCH, std::string("near pairs: ") + hipGetErrorString(e));
-      }
-    }
-  }
-  (void)hipFree(counts);
-  (void)hipFree(offs);
-  return rc;
+  NearSearch ns;
+  std::vector<int64_t> ptr;
+  ED_TRY(ns.build(C, A, n, df, st));
+  ED_TRY(ns.count(st, ptr));
+  const int64_t total = ptr[n];
+  *n_pairs_out = total;
+  if (!pairs || total == 0) return ED_OK;
+  if (capacity < total) return fail(ED_ERR_ARG, "near pairs: capacity too small");
+  return ns.fill(st, ptr, pairs, nullptr, 0);
 }
 
 int ed_near_weights_f64(const double* C, const double* A, const double* PV, const int64_t* pairs,

@@ -1,0 +1,301 @@
+// Sparse approximate inverse (SAI) smoother of the native solver (no
+// reference counterpart; the reference offers Jacobi and mass scaling only,
+// core/bem_solver.py:36-117).
+//
+// Left SAI: row i of M minimises |m^T A[J, I] - e_i^T| over m supported on the
+// pattern J(i) = {i} + panels within factor_pattern (R_i + R_j) of i, fitted
+// on the rows I(i) = {i} + panels within factor_rows (R_i + R_j).  A is the
+// collocation operator: K_E A_j / r_ij off the diagonal plus the near-field
+// quadrature correction, self_integral_i A_i on it.  Everything runs in tree
+// order; patterns never cross geometries of a batched plan.
+//
+// Layout: column-major ELL with kSaiJ slots per row (col[q * n + i],
+// val[q * n + i], cnt[i]), so consecutive rows read consecutive addresses.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "ed_internal.h"
+#include "ed_near.h"
+#include "ed_plan.h"
+
+namespace ed {
+
+constexpr int kSaiJ = 32;   // pattern slots per row
+constexpr int kSaiI = 128;  // fitting rows per row
+constexpr int kSaiCand = 512;
+
+__device__ __forceinline__ double sai_dist2(const double* C, int64_t a, int64_t b) {
+  const double dx = C[3 * a] - C[3 * b], dy = C[3 * a + 1] - C[3 * b + 1], dz = C[3 * a + 2] - C[3 * b + 2];
+  return dx * dx + dy * dy + dz * dz;
+}
+
+// list[0] = i, then the (cap - 1) nearest candidates of the CSR row (ties by
+// index; rows beyond kSaiCand candidates keep their first kSaiCand).
+__device__ void sai_select(int64_t i, const int32_t* __restrict__ col, int64_t beg, int64_t end,
+                           const double* __restrict__ C, int cap, int* list, int* count) {
+  const int c = (int)std::min<int64_t>(end - beg, kSaiCand);
+  if (threadIdx.x == 0) list[0] = (int)i;
+  if (c <= cap - 1) {
+    for (int t = threadIdx.x; t < c; t += blockDim.x) list[1 + t] = col[beg + t];
+    if (threadIdx.x == 0) *count = 1 + c;
+  } else {
+    for (int t = threadIdx.x; t < c; t += blockDim.x) {
+      const int ct = col[beg + t];
+      const double dt = sai_dist2(C, i, ct);
+      int rank = 0;
+      for (int u = 0; u < c; ++u) {
+        const int cu = col[beg + u];
+        const double du = sai_dist2(C, i, cu);
+        rank += (du < dt || (du == dt && cu < ct)) ? 1 : 0;
+      }
+      if (rank < cap - 1) list[1 + rank] = ct;
+    }
+    if (threadIdx.x == 0) *count = cap;
+  }
+  __syncthreads();
+}
+
+// One 64-lane block per row (rows row0 + blockIdx.x of a geometry segment).
+__global__ __launch_bounds__(64) void k_sai_build(
+    int64_t n, int64_t row0, const int64_t* __restrict__ jptr, const int32_t* __restrict__ jcol,
+    const int64_t* __restrict__ iptr, const int32_t* __restrict__ icol,
+    const double* __restrict__ C, const double* __restrict__ A, const double* __restrict__ selfi,
+    const int32_t* __restrict__ near_ptr, const int32_t* __restrict__ near_col,
+    const double* __restrict__ near_w, double ridge, int32_t* __restrict__ out_col,
+    double* __restrict__ out_val, int32_t* __restrict__ out_cnt) {
+  __shared__ int Jl[kSaiJ];
+  __shared__ int Il[kSaiI];
+  __shared__ double B[kSaiI][kSaiJ + 1];
+  __shared__ double G[kSaiJ][kSaiJ + 1];
+  __shared__ double h[kSaiJ];
+  __shared__ int nJ, nI, li;
+  const int64_t loc = blockIdx.x;
+  const int64_t i = row0 + loc;
+  sai_select(i, jcol, jptr[loc], jptr[loc + 1], C, kSaiJ, Jl, &nJ);
+  sai_select(i, icol, iptr[loc], iptr[loc + 1], C, kSaiI, Il, &nI);
+  const int nj = nJ, ni = nI;
+  // B[l][k] = A[J_k][I_l]
+  for (int e = threadIdx.x; e < ni * nj; e += blockDim.x) {
+    const int l = e / nj, k = e - l * nj;
+    const int a = Jl[k], b = Il[l];
+    double v;
+    if (a == b) {
+      v = selfi[a] * A[a];
+    } else {
+      const double r2 = fmax(sai_dist2(C, a, b), 1e-24);
+      v = kKE * A[b] / sqrt(r2);
+    }
+    B[l][k] = v;
+  }
+  __syncthreads();
+  // near-field corrections A[J_k][col] += w for the near entries of row J_k
+  if (near_ptr) {
+    for (int k = threadIdx.x; k < nj; k += blockDim.x) {
+      const int a = Jl[k];
+      for (int q = near_ptr[a]; q < near_ptr[a + 1]; ++q) {
+        const int b = near_col[q];
+        for (int l = 0; l < ni; ++l)
+          if (Il[l] == b) { B[l][k] += near_w[q]; break; }
+      }
+    }
+  }
+  if (threadIdx.x == 0) li = 0;  // Il[0] == i
+  __syncthreads();
+  // normal equations G = B^T B, h = B^T e_i = B[li][:]
+  for (int e = threadIdx.x; e < nj * nj; e += blockDim.x) {
+    const int a = e / nj, b = e - a * nj;
+    if (b < a) continue;
+    double s = 0.0;
+    for (int l = 0; l < ni; ++l) s += B[l][a] * B[l][b];
+    G[a][b] = s;
+    G[b][a] = s;
+  }
+  for (int k = threadIdx.x; k < nj; k += blockDim.x) h[k] = B[li][k];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double tr = 0.0;
+    for (int a = 0; a < nj; ++a) tr += G[a][a];
+    const double lam = ridge * (tr > 0.0 ? tr : 1.0) / nj;
+    for (int a = 0; a < nj; ++a) G[a][a] += lam;
+  }
+  __syncthreads();
+  // right-looking Cholesky G = L L^T (lower triangle, in place)
+  for (int j = 0; j < nj; ++j) {
+    if (threadIdx.x == 0) {
+      const double d = G[j][j];
+      G[j][j] = d > 1e-300 ? sqrt(d) : 1e-150;
+    }
+    __syncthreads();
+    const double d = G[j][j];
+    for (int r = j + 1 + threadIdx.x; r < nj; r += blockDim.x) G[r][j] /= d;
+    __syncthreads();
+    const int w = nj - j - 1;
+    for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+      const int r = j + 1 + e / w, c = j + 1 + e % w;
+      if (c <= r) G[r][c] -= G[r][j] * G[c][j];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    for (int r = 0; r < nj; ++r) {  // L y = h
+      double v = h[r];
+      for (int q = 0; q < r; ++q) v -= G[r][q] * h[q];
+      h[r] = v / G[r][r];
+    }
+    for (int r = nj - 1; r >= 0; --r) {  // L^T m = y
+      double v = h[r];
+      for (int q = r + 1; q < nj; ++q) v -= G[q][r] * h[q];
+      h[r] = v / G[r][r];
+    }
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kSaiJ; q += blockDim.x) {
+    const bool on = q < nj;
+    out_col[(size_t)q * n + i] = on ? Jl[q] : (int32_t)i;
+    out_val[(size_t)q * n + i] = on ? h[q] : 0.0;
+  }
+  if (threadIdx.x == 0) out_cnt[i] = nj;
+}
+
+// z = M u (+ Z cy when k > 0)
+__global__ void k_sai_apply(int64_t n, const int32_t* __restrict__ col,
+                            const double* __restrict__ val, const int32_t* __restrict__ cnt,
+                            const double* __restrict__ u, int k, const double* __restrict__ Z,
+                            const double* __restrict__ cy, double* __restrict__ z) {
+  __shared__ double c[32];
+  if (threadIdx.x < k) c[threadIdx.x] = cy[threadIdx.x];
+  __syncthreads();
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  const int m = cnt[i];
+  for (int q = 0; q < m; ++q) s += val[(size_t)q * n + i] * u[col[(size_t)q * n + i]];
+  for (int q = 0; q < k; ++q) s += c[q] * Z[(size_t)q * n + i];
+  z[i] = s;
+}
+
+__global__ void k_sai_perm(int64_t n, const int32_t* __restrict__ perm, const double* __restrict__ src,
+                           double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[perm[i]];
+}
+
+__global__ void k_sai_unperm(int64_t n, const int32_t* __restrict__ perm,
+                             const double* __restrict__ src, double* __restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[perm[i]] = src[i];
+}
+
+void sai_apply_launch(ed_plan* P, const double* u, int k, const double* Z, const double* cy,
+                      double* z, hipStream_t st) {
+  const int64_t n = P->n;
+  hipLaunchKernelGGL(k_sai_apply, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n,
+                     P->sai_col, P->sai_val, P->sai_cnt, u, k, Z, cy, z);
+}
+
+void sai_free(ed_plan* P) {
+  if (P->sai_col) (void)hipFree(P->sai_col);
+  if (P->sai_val) (void)hipFree(P->sai_val);
+  if (P->sai_cnt) (void)hipFree(P->sai_cnt);
+  P->sai_col = nullptr;
+  P->sai_val = nullptr;
+  P->sai_cnt = nullptr;
+}
+
+}  // namespace ed
+
+using namespace ed;
+
+extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, double factor_pattern,
+                                 double factor_rows, ed_stream stream) {
+  if (!P || !self_integrals) return fail(ED_ERR_ARG, "ed_plan_build_sai: NULL argument");
+  if (!(factor_pattern > 0.0) || !(factor_rows >= factor_pattern) || !std::isfinite(factor_rows))
+    return fail(ED_ERR_ARG, "ed_plan_build_sai: need 0 < factor_pattern <= factor_rows < inf");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = P->n;
+  gmres_free(P);  // a captured Arnoldi graph refers to the preconditioner arrays
+  sai_free(P);
+  double* selfi_t = nullptr;
+  if (hipMalloc((void**)&P->sai_col, sizeof(int32_t) * kSaiJ * n) != hipSuccess ||
+      hipMalloc((void**)&P->sai_val, sizeof(double) * kSaiJ * n) != hipSuccess ||
+      hipMalloc((void**)&P->sai_cnt, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc((void**)&selfi_t, sizeof(double) * n) != hipSuccess) {
+    sai_free(P);
+    if (selfi_t) (void)hipFree(selfi_t);
+    return fail(ED_ERR_ALLOC, "ed_plan_build_sai: alloc");
+  }
+  hipLaunchKernelGGL(k_sai_perm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, P->perm,
+                     self_integrals, selfi_t);
+  int rc = ED_OK;
+  const auto& go = P->host.geom_pt_off;
+  for (size_t gi = 0; gi + 1 < go.size() && rc == ED_OK; ++gi) {
+    const int64_t b = go[gi], ng = go[gi + 1] - go[gi];
+    if (ng <= 0) continue;
+    NearSearch sj, si;
+    std::vector<int64_t> pj, pi;
+    int32_t *cj = nullptr, *ci = nullptr;
+    int64_t *dpj = nullptr, *dpi = nullptr;
+    rc = sj.build(P->cent64_t + 3 * b, P->area64_t + b, ng, factor_pattern, st);
+    if (rc == ED_OK) rc = sj.count(st, pj);
+    if (rc == ED_OK) rc = si.build(P->cent64_t + 3 * b, P->area64_t + b, ng, factor_rows, st);
+    if (rc == ED_OK) rc = si.count(st, pi);
+    if (rc == ED_OK &&
+        (hipMalloc((void**)&cj, sizeof(int32_t) * std::max<int64_t>(pj[ng], 1)) != hipSuccess ||
+         hipMalloc((void**)&ci, sizeof(int32_t) * std::max<int64_t>(pi[ng], 1)) != hipSuccess ||
+         hipMalloc((void**)&dpj, sizeof(int64_t) * (ng + 1)) != hipSuccess ||
+         hipMalloc((void**)&dpi, sizeof(int64_t) * (ng + 1)) != hipSuccess))
+      rc = fail(ED_ERR_ALLOC, "ed_plan_build_sai: alloc");
+    if (rc == ED_OK) rc = sj.fill(st, pj, nullptr, cj, b);
+    if (rc == ED_OK) rc = si.fill(st, pi, nullptr, ci, b);
+    if (rc == ED_OK) {
+      hipError_t e = hipMemcpyAsync(dpj, pj.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) e = hipMemcpyAsync(dpi, pi.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
+      if (e == hipSuccess) {
+        hipLaunchKernelGGL(k_sai_build, dim3((unsigned)ng), dim3(64), 0, st, n, b,
+                           (const int64_t*)dpj, (const int32_t*)cj, (const int64_t*)dpi,
+                           (const int32_t*)ci, P->cent64_t, P->area64_t, (const double*)selfi_t,
+                           P->n_near > 0 ? P->near_ptr : nullptr, P->near_col, P->near_w64,
+                           1e-12, P->sai_col, P->sai_val, P->sai_cnt);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      if (e != hipSuccess) rc = fail(ED_ERR_LAUNCH, std::string("ed_plan_build_sai: ") + hipGetErrorString(e));
+    }
+    if (cj) (void)hipFree(cj);
+    if (ci) (void)hipFree(ci);
+    if (dpj) (void)hipFree(dpj);
+    if (dpi) (void)hipFree(dpi);
+  }
+  (void)hipFree(selfi_t);
+  if (rc != ED_OK) sai_free(P);
+  return rc;
+}
+
+// z = M_sai u in original panel order (device FP64); the smoother alone, for
+// tests and for callers that run their own Krylov loop.
+extern "C" int ed_plan_sai_apply(ed_plan* P, const double* u, double* z, ed_stream stream) {
+  if (!P || !u || !z) return fail(ED_ERR_ARG, "ed_plan_sai_apply: NULL argument");
+  if (!P->sai_cnt) return fail(ED_ERR_ARG, "ed_plan_sai_apply: sparse approximate inverse not built");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = P->n;
+  double *ut = nullptr, *zt = nullptr;
+  if (hipMalloc((void**)&ut, sizeof(double) * n) != hipSuccess ||
+      hipMalloc((void**)&zt, sizeof(double) * n) != hipSuccess) {
+    if (ut) (void)hipFree(ut);
+    return fail(ED_ERR_ALLOC, "ed_plan_sai_apply: alloc");
+  }
+  const unsigned gn = (unsigned)((n + 255) / 256);
+  hipLaunchKernelGGL(k_sai_perm, dim3(gn), dim3(256), 0, st, n, P->perm, u, ut);
+  sai_apply_launch(P, ut, 0, nullptr, nullptr, zt, st);
+  hipLaunchKernelGGL(k_sai_unperm, dim3(gn), dim3(256), 0, st, n, P->perm, zt, z);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  (void)hipFree(ut);
+  (void)hipFree(zt);
+  if (e != hipSuccess) return fail(ED_ERR_LAUNCH, std::string("ed_plan_sai_apply: ") + hipGetErrorString(e));
+  return ED_OK;
+}

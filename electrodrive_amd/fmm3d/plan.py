@@ -149,6 +149,35 @@ class FmmPlan:
                                                nat.ptr(self_integrals),
                                                nat.stream_ptr(Z.device)), "ed_plan_set_coarse")
 
+    def build_sai(self, self_integrals, factor_pattern=2.0, factor_rows=4.0):
+        """Sparse approximate inverse smoother (solver ``precond=2``)."""
+        if not self_integrals.is_cuda or self_integrals.dtype != torch.float64:
+            raise ValueError("self_integrals must be a float64 device tensor")
+        nat.check(nat.lib().ed_plan_build_sai(self.handle, nat.ptr(self_integrals.contiguous()),
+                                              float(factor_pattern), float(factor_rows),
+                                              nat.stream_ptr(self_integrals.device)),
+                  "ed_plan_build_sai")
+
+    def sai_apply(self, u):
+        """z = M_sai u (the smoother alone)."""
+        if not u.is_cuda or u.dtype != torch.float64 or u.shape != (self.n,):
+            raise ValueError("u must be a float64 device tensor of length n")
+        u = u.contiguous()
+        z = torch.empty_like(u)
+        nat.check(nat.lib().ed_plan_sai_apply(self.handle, nat.ptr(u), nat.ptr(z),
+                                              nat.stream_ptr(u.device)), "ed_plan_sai_apply")
+        return z
+
+    def coarse_solve(self, u):
+        """Z (Z^T A Z)^-1 Z^T u on the coarse space set by ``set_coarse``."""
+        if not u.is_cuda or u.dtype != torch.float64 or u.shape != (self.n,):
+            raise ValueError("u must be a float64 device tensor of length n")
+        u = u.contiguous()
+        out = torch.empty_like(u)
+        nat.check(nat.lib().ed_plan_coarse_solve(self.handle, nat.ptr(u), nat.ptr(out),
+                                                 nat.stream_ptr(u.device)), "ed_plan_coarse_solve")
+        return out
+
     def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
               tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,
               use_graph=True, check_every=4):
@@ -166,7 +195,11 @@ class FmmPlan:
         o = nat.GmresOpts()
         L.ed_gmres_opts_default(ctypes.byref(o))
         o.restart, o.maxiter, o.tol = int(restart), int(maxiter), float(tol)
-        o.precond, o.refine, o.max_refine = int(bool(precond)), int(bool(refine)), int(max_refine)
+        pc = {"none": 0, "jacobi": 1, "sai": 2}.get(precond, None) if isinstance(precond, str) \
+            else int(bool(precond))
+        if pc is None:
+            raise ValueError(f"unknown preconditioner {precond!r} (none, jacobi, sai)")
+        o.precond, o.refine, o.max_refine = pc, int(bool(refine)), int(max_refine)
         o.inner_tol, o.use_graph, o.check_every = float(inner_tol), int(bool(use_graph)), int(check_every)
         info = nat.GmresInfo()
         nat.check(L.ed_gmres_solve(self.handle, nat.ptr(b), nat.ptr(x), nat.ptr(diag),

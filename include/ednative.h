@@ -180,7 +180,8 @@ typedef struct ed_gmres_opts {
   int32_t restart;        /* Krylov dimension per cycle                      */
   int32_t maxiter;        /* total Arnoldi steps                             */
   double tol;             /* relative tolerance                              */
-  int32_t precond;        /* 0 none, 1 Jacobi on diag (1/diag, bad -> 1)     */
+  int32_t precond;        /* 0 none, 1 Jacobi on diag (1/diag, bad -> 1),    */
+                          /* 2 sparse approximate inverse (ed_plan_build_sai) */
   int32_t refine;         /* 1: outer FP64 iterative refinement with the     */
                           /*    dense FP64 operator as the residual operator */
   int32_t max_refine;     /* outer refinement steps                          */
@@ -210,13 +211,33 @@ int ed_gmres_solve(ed_plan* plan, const double* b, double* x, const double* diag
                    const double* self_integrals, const ed_gmres_opts* opts,
                    ed_gmres_info* info, ed_stream stream);
 
+/* Sparse approximate inverse smoother for ed_gmres_solve (precond = 2; no
+ * reference counterpart, whose setup_preconditioner offers Jacobi and mass
+ * scaling, core/bem_solver.py:36-117).  Row i of M minimises
+ * |m^T A[J(i), I(i)] - e_i^T| with J(i) the panels within
+ * factor_pattern (R_i + R_j) of panel i (at most 32, nearest first) and I(i)
+ * those within factor_rows (R_i + R_j) (at most 128); A is the plan's
+ * operator (near correction included when attached) with self_integrals
+ * (n, device FP64, original order) on the diagonal.  Rebuild after
+ * ed_plan_attach_near. */
+int ed_plan_build_sai(ed_plan* plan, const double* self_integrals, double factor_pattern,
+                      double factor_rows, ed_stream stream);
+/* z = M u with the built smoother alone (original order, device FP64). */
+int ed_plan_sai_apply(ed_plan* plan, const double* u, double* z, ed_stream stream);
+
 /* Coarse space of ed_gmres_solve's preconditioner (no reference counterpart):
- * M^-1 u = D^-1 (u - A Z c) + Z c, c = (Z^T A Z)^-1 Z^T u, with Z (n x k,
+ * M^-1 u = S (u - A Z c) + Z c, c = (Z^T A Z)^-1 Z^T u (S the Jacobi or the
+ * sparse approximate inverse smoother), with Z (n x k,
  * column-major, original order, device FP64) orthonormalised here and A the
  * plan's FMM operator with `self_integrals`.  k <= 32; k = 0 removes it.
  * Exact on span(Z), Jacobi elsewhere.  Used with the learned basis. */
 int ed_plan_set_coarse(ed_plan* plan, const double* Z, int32_t k,
                        const double* self_integrals, ed_stream stream);
+
+/* Coarse solve on the plan's coarse space (no reference counterpart):
+ * out = Z (Z^T A Z)^-1 Z^T u, original order, device FP64; the initial guess
+ * of the learned-preconditioned solve (u = b).  Needs ed_plan_set_coarse. */
+int ed_plan_coarse_solve(ed_plan* plan, const double* u, double* out, ed_stream stream);
 
 /* FP64 level-1 kernels for GMRES over an arbitrary operator callable: they
  * replace the torch.dot / norm / axpy steps of gmres_restart's Arnoldi loop
@@ -257,6 +278,13 @@ int ed_panel_features(ed_plan* plan, const double* centroids, const double* norm
                       const double* areas, const double* b, const double* diag, float* F,
                       ed_stream stream);
 int32_t ed_panel_feature_count(void);
+/* Projection loss for training a learned coarse basis (no reference
+ * counterpart): per solve s (rows s*m .. s*m+m-1), Z = [Y | X],
+ * c = argmin |t - Zc|^2 + ridge tr(Z^T Z)/k |c|^2, loss[s] = |t - Zc|^2/|t|^2,
+ * dY = -2 (t - Zc) c_Y^T / |t|^2.  ky >= 1, ky + kx <= 32; device FP32. */
+int ed_proj_loss_f32(const float* Y, int32_t ky, const float* X, int32_t kx, const float* T,
+                     int64_t n_sol, int32_t m, float ridge, float* dY, float* loss,
+                     ed_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Host-only helpers (no device needed): expose the plan builder's octree,   */

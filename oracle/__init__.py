@@ -24,7 +24,9 @@ to the Electrodrive repository root) in plain numpy (FP64) or C.  Modules:
                   summation, not against a reference file)
 - ``lattice_tree`` restatement of the native lattice octree and its U/V lists,
                   with a brute-force partition check
-- ``ref_tree``    the reference adaptive tight-bbox octree (fmm3d/tree.py:801-960)
+- ``mlp``         FP64 forward/backward of the learned-preconditioner MLP and
+                  its projection loss (no reference counterpart)
+- ``sai``         sparse approximate inverse smoother (no reference counterpart)
 
 Parity status per function is recorded in each module header.
 """

@@ -65,3 +65,33 @@ def loss_and_grad(params, dims, X, T, w=None):
         flat.append(gW.ravel())
         flat.append(gb.ravel())
     return loss, np.concatenate(flat)
+
+
+def proj_loss_and_grad(Y, X, T, m, ridge=1e-6):
+    """Projection loss of a learned coarse basis, per solve of m rows:
+    Z = [Y | X], c = argmin |t - Zc|^2 + lam |c|^2 with lam = ridge tr(Z^T Z)/k,
+    loss_s = |t - Zc|^2 / |t|^2, dloss_s/dY = -2 (t - Zc) c_Y^T / |t|^2 (the
+    envelope theorem, lam held fixed).  Returns (loss (n_sol,), dY like Y).
+    No reference counterpart; pinned in tests by finite differences."""
+    Y = np.asarray(Y, dtype=np.float64)
+    X = np.zeros((Y.shape[0], 0)) if X is None else np.asarray(X, dtype=np.float64)
+    T = np.asarray(T, dtype=np.float64).reshape(-1)
+    ky = Y.shape[1]
+    n_sol = Y.shape[0] // m
+    loss = np.zeros(n_sol)
+    dY = np.zeros_like(Y)
+    for s in range(n_sol):
+        sl = slice(s * m, (s + 1) * m)
+        Z = np.concatenate([Y[sl], X[sl]], axis=1)
+        t = T[sl]
+        G = Z.T @ Z
+        k = G.shape[0]
+        tr = np.trace(G)
+        lam = ridge * (tr if tr > 0 else 1.0) / k
+        c = np.linalg.solve(G + lam * np.eye(k), Z.T @ t)
+        r = t - Z @ c
+        tt = float(t @ t)
+        inv = 1.0 / tt if tt > 0 else 0.0
+        loss[s] = float(r @ r) * inv
+        dY[sl] = -2.0 * np.outer(r, c[:ky]) * inv
+    return loss, dY

@@ -314,3 +314,26 @@ def test_m2m_l2l_exact():
         t = cc + rng.uniform(-0.5, 0.5, (10, 3))
         assert np.allclose(fm.l2p(Lc, t, cc, 1.0, p), fm.l2p(L, t, np.zeros(3), 2.0, p),
                            rtol=1e-11, atol=1e-11)
+
+
+def test_sai_oracle_exact_inverse_and_improves_on_jacobi():
+    """Pins oracle.sai: with J = I = every panel (20-panel icosahedron, huge
+    factors) each row solves m^T A = e_i^T exactly, so M A = I; on a larger
+    mesh |I - M A|_F is far below Jacobi's |I - D^-1 A|_F."""
+    import numpy as np
+    from oracle import sai, quadrature as oq
+    from electrodrive_amd.core import bem_mesh
+    m = bem_mesh.icosphere(0)
+    I_self, corr = oq.self_integrals(m.centroids, m.panel_vertices, m.areas, depth=2)
+    A = sai.operator_entries(m.centroids, m.areas, corr)
+    M = sai.sai_matrix(m.centroids, m.areas, corr, factor_pattern=1e3, factor_rows=1e3)
+    assert np.abs(M @ A - np.eye(A.shape[0])).max() < 1e-8
+    m = bem_mesh.icosphere(2)
+    I_self, corr = oq.self_integrals(m.centroids, m.panel_vertices, m.areas, depth=2)
+    A = sai.operator_entries(m.centroids, m.areas, corr)
+    M = sai.sai_matrix(m.centroids, m.areas, corr)
+    eye = np.eye(A.shape[0])
+    e_sai = np.linalg.norm(eye - M @ A)
+    e_jac = np.linalg.norm(eye - A / np.diag(A)[:, None])
+    print("SAI", e_sai, "Jacobi", e_jac)
+    assert e_sai < 0.5 * e_jac

@@ -35,3 +35,29 @@ def test_native_mlp_matches_oracle(dims, B):
     after = net.params.cpu().numpy()
     nz = np.abs(gn) > 1e-3  # |g| >> eps: the first Adam step is lr * sign(g)
     assert np.allclose((before - after)[nz], 1e-3 * np.sign(gn[nz]), rtol=1e-3, atol=1e-7)
+
+
+@pytest.mark.parametrize("ky,kx,m,ns", [(8, 11, 512, 5), (3, 0, 77, 2), (16, 16, 300, 3)])
+def test_native_proj_loss_matches_oracle(ky, kx, m, ns):
+    """ed_proj_loss_f32 (loss and dY) against oracle.mlp.proj_loss_and_grad."""
+    import ctypes
+    from electrodrive_amd import _native as nat
+    rng = np.random.default_rng(ky + kx)
+    Y = rng.normal(size=(ns * m, ky)).astype(np.float32)
+    X = rng.normal(size=(ns * m, kx)).astype(np.float32)
+    T = (Y[:, :1] * 0.5 + rng.normal(size=(ns * m, 1))).astype(np.float32).reshape(-1)
+    ridge = 1e-6
+    lo, dYo = om.proj_loss_and_grad(Y, X, T, m, ridge=ridge)
+    dev = torch.device("cuda")
+    Yd, Xd, Td = (torch.tensor(a, device=dev) for a in (Y, X, T))
+    dY = torch.empty_like(Yd)
+    loss = torch.empty(ns, dtype=torch.float32, device=dev)
+    nat.check(nat.lib().ed_proj_loss_f32(nat.ptr(Yd), ky, nat.ptr(Xd) if kx else None, kx,
+                                         nat.ptr(Td), ns, m, ctypes.c_float(ridge),
+                                         nat.ptr(dY), nat.ptr(loss),
+                                         nat.stream_ptr(dev)), "ed_proj_loss_f32")
+    l = loss.cpu().numpy()
+    g = dY.cpu().numpy()
+    print("loss", l, "oracle", lo)
+    assert np.allclose(l, lo, rtol=1e-4, atol=1e-7)
+    assert np.linalg.norm(g - dYo) <= 1e-4 * np.linalg.norm(dYo)

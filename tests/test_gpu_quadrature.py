@@ -43,6 +43,41 @@ def test_near_pairs_bit_exact():
             assert np.array_equal(got, ref)
 
 
+def _cloud(seed, n):
+    rng = np.random.default_rng(seed)
+    C = rng.uniform(-1.0, 1.0, (n, 3))
+    A = np.exp(rng.normal(np.log(4.0 / n), 1.0, n))  # radii spread over a decade
+    return C, A
+
+
+@pytest.mark.parametrize("case", ["cloud", "edge_values", "far_clusters", "one_big_panel"])
+def test_near_pairs_bit_exact_edge_cases(case):
+    """Cell-grid search against the brute-force scan of core/bem.py:475-555 on
+    inputs that stress the grid: mixed radii, NaN / zero / negative / inf areas
+    and non-finite centroids, clusters 1e6 apart (cell-count cap), and one panel
+    whose radius sets a cell larger than the whole cloud."""
+    from electrodrive_amd.core import bem_quadrature as bq
+    C, A = _cloud(7, 3000)
+    if case == "edge_values":
+        A[[3, 40]] = np.nan
+        A[[5, 41]] = 0.0
+        A[[7, 42]] = -1e-3
+        A[9] = np.inf
+        C[11] = [np.nan, 0.0, 0.0]
+        C[13] = [np.inf, 0.0, 0.0]
+        C[15] = C[16]  # coincident centroids are never paired
+    elif case == "far_clusters":
+        C[1500:] += 1e6
+    elif case == "one_big_panel":
+        A[17] = 50.0
+    for f in (1.5, 0.0, -1.0) if case == "edge_values" else (1.5, 3.0):
+        ref = oq.near_pairs(C, A, f)
+        got = bq.build_near_pairs(C, A, f).cpu().numpy()
+        print(case, f, "pairs", ref.shape[0])
+        assert got.shape == ref.shape
+        assert np.array_equal(got, ref)
+
+
 def test_near_weights_and_apply():
     from electrodrive_amd.core import bem_quadrature as bq
     m = _mesh()
