@@ -6,16 +6,16 @@ icosphere, subdivision 6 (81,920 flat panels, R = 1), point charge q = 1 at
 r0 = rho * u with rho ~ U[1.2, 2.0] and u uniform on the sphere drawn from
 numpy.random.default_rng(0); refined self integrals (depth 3) and near
 quadrature correction (distance factor 1.5, depth 3); FMM in FP32 with FP32
-MFMA M2L; FP64 GMRES with FP64 iterative refinement against the dense FP64
-operator.
+MFMA M2L; FP64 GMRES (sparse approximate inverse smoother) with FP64
+iterative refinement against the dense FP64 operator.
 
 A step is one matvec of the solve operator (FMM far field + P2P + near
 correction + self term) over the 81,920 panels with sigma resident in HBM.
 ``value`` = panels / s of that matvec.  The line also reports the time to a
 1e-8 relative residual from the assembled geometry (set-up, first solve
 with its workspace and graph capture; the set-up share is also reported
-separately), per-stage
-times, the roofline of the dominant kernel and the CPU baseline (the oracle's
+separately), the GMRES iterations of the plain (Jacobi) and the
+learned-preconditioned solve at the same accuracy, per-stage times, the roofline of the dominant kernel and the CPU baseline (the oracle's
 C restatement of the dense reference matvec, OpenMP, timed on a bounded
 sample of rows on this host).
 """
@@ -47,16 +47,25 @@ def parse():
     ap.add_argument("--no-solve", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precond", default="sai", choices=["sai", "jacobi"])
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "batch", "torus"],
+                    help="sphere: 81,920-panel icosphere (default); batch: 64 geometries x 8 "
+                         "RHS; torus: ~300k-panel torus with 16-bit M2L")
+    ap.add_argument("--n-geoms", type=int, default=64)
+    ap.add_argument("--rhs", type=int, default=8)
+    ap.add_argument("--torus-panels", type=int, default=300312)
     return ap.parse_args()
 
 
-def stage_roofline(name, ms, st, n, order):
-    """Algorithmic work of one launch of a stage (documented in DESIGN.md)."""
+def stage_roofline(name, ms, st, n, order, m2l_peak=MFMA_FP32_PEAK_TFLOPS):
+    """Algorithmic work of one launch of a stage: M2L 2 (p+1)^4 flops per
+    translation pair; leaf bytes below; other stages 16 B per panel."""
     nc = (order + 1) ** 2
     s = ms * 1e-3
     if name == "m2l":
         flops = 2.0 * nc * nc * st["n_m2l_pairs"]
-        return dict(bound="mfma", achieved=flops / s / 1e12, peak=MFMA_FP32_PEAK_TFLOPS,
+        return dict(bound="mfma", achieved=flops / s / 1e12, peak=m2l_peak,
                     unit="TFLOP/s", algorithmic=flops)
     if name == "leaf":
         # targets (x,y,z,q) + source tiles of every P2P leaf pair + local
@@ -91,19 +100,79 @@ def cpu_baseline(mesh, sigma_np, self_corr_np, budget_s):
                        f"{n / (rows / t):.1f} s"), n / (rows / t)
 
 
+def timed(fn):
+    import torch
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    res = fn()
+    torch.cuda.synchronize()
+    return res, time.perf_counter() - t
+
+
+def matvec_measure(op, sigma, a, n_units, nrhs=1):
+    """Event-timed matvecs (``nrhs`` per step), stage times and roofline."""
+    import torch
+    st = op.plan.stats()
+    sig = [sigma] if nrhs == 1 else list(sigma)
+    out = torch.empty_like(sig[0])
+    for _ in range(a.warmup):
+        for sg in sig:
+            op.plan.matvec(sg, op.self_corr, out=out)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(a.steps):
+        for sg in sig:
+            op.plan.matvec(sg, op.self_corr, out=out)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = e0.elapsed_time(e1) / a.steps
+    stages = op.plan.profile(sig[0], op.self_corr, reps=max(5, a.steps // 2))
+    dom = max(stages, key=stages.get)
+    peak = MFMA_BF16_PEAK_TFLOPS if op.plan.m2l_precision in ("bf16", "fp16") else MFMA_FP32_PEAK_TFLOPS
+    rf = stage_roofline(dom, stages[dom], st, op.n, op.plan.order, peak)
+    roofline = dict(kernel=dom, bound=rf["bound"], achieved=rf["achieved"], peak=rf["peak"],
+                    unit=rf["unit"], frac=rf["achieved"] / rf["peak"], traffic=None,
+                    launch_ms=stages[dom], algorithmic_per_launch=rf["algorithmic"])
+    m2l = stage_roofline("m2l", stages["m2l"], st, op.n, op.plan.order, peak)
+    extra = dict(m2l_mfma=dict(achieved_tflops=m2l["achieved"], peak_tflops=peak,
+                               frac=m2l["achieved"] / peak, launch_ms=stages["m2l"]))
+    return dict(ms=ms, wall=wall, stages=stages, roofline=roofline, extra=extra, stats=st,
+                value=n_units / (ms * 1e-3))
+
+
+def base_line(a, value, ms, dtype, config):
+    return {
+        "metric": "time-to-1e-8 residual + FMM matvec panels/s; learned-precond vs plain GMRES",
+        "value": value, "unit": "panels/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic", "config": config,
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported (every box here has one "
               f"GPU and the path's multi-GPU sharding is design-only)", file=sys.stderr)
         return 2
+    import torch
+    torch.cuda.set_device(torch.device("cuda", 0))
+    line = {"sphere": run_sphere, "batch": run_batch, "torus": run_torus}[a.workload](a)
+    print(json.dumps(line))
+    return 0
+
+
+def run_sphere(a):
     import numpy as np
     import torch
     from electrodrive_amd.core import bem_mesh
     from electrodrive_amd.core.operator import BemOperator
 
     dev = torch.device("cuda", 0)
-    torch.cuda.set_device(dev)
     mesh = bem_mesh.icosphere(a.subdiv)
     n = mesh.n_panels
     rng = np.random.default_rng(a.seed)
@@ -117,68 +186,66 @@ def main():
                            near=True, self_depth=3, near_depth=3)
 
     # first construction loads the code objects; the timed path starts from a
-    # second one (plan, tree, quadrature set-up, solver workspace and graph
-    # capture all included)
+    # second one (plan, tree, quadrature and SAI set-up, solver workspace and
+    # graph capture all included)
     op = build()
     b = op.rhs_point_charges([(1.0, tuple(r0))])
     solve = None
     if not a.no_solve:
-        op.solve(b, tol=1e-8)
+        op.solve(b, tol=1e-8, precond=a.precond)
         del op
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         op = build()
+        if a.precond == "sai":
+            op.enable_sai()
         torch.cuda.synchronize()
         setup_s = time.perf_counter() - t0
         b = op.rhs_point_charges([(1.0, tuple(r0))])
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        x, info = op.solve(b, tol=1e-8)
-        torch.cuda.synchronize()
-        first_solve_s = time.perf_counter() - t1
-        t1 = time.perf_counter()
-        x, info = op.solve(b, tol=1e-8)
-        torch.cuda.synchronize()
-        solve_s = time.perf_counter() - t1
+
+        (x, info), first_solve_s = timed(lambda: op.solve(b, tol=1e-8, precond=a.precond))
+        (x, info), solve_s = timed(lambda: op.solve(b, tol=1e-8, precond=a.precond))
         from oracle import analytic  # checker only
         sig_k = analytic.kelvin_sigma(mesh.centroids, 1.0, r0, 1.0)
         err_k = float(np.linalg.norm(x.cpu().numpy() - sig_k) / np.linalg.norm(sig_k))
         r = b - op.accurate(x)
         rel = float(torch.linalg.norm(r) / torch.linalg.norm(b))
         solve = dict(time_to_solution_s=setup_s + first_solve_s, setup_s=setup_s,
-                     first_solve_s=first_solve_s, repeat_solve_s=solve_s,
+                     first_solve_s=first_solve_s, repeat_solve_s=solve_s, precond=a.precond,
                      gmres_iters=info["iters"], refine_steps=info["refine_steps"],
                      code=info["code"], rel_resid_fp64_dense=rel, err_vs_kelvin=err_k,
                      total_charge=op.total_charge(x), expected_charge=-1.0 / rho)
+        if not a.no_learned:
+            # learned-preconditioned vs plain GMRES at the same 1e-8 accuracy
+            from electrodrive_amd.learn import precond as lp_mod
+            lp = lp_mod.LearnedPreconditioner()
+            ce = 1  # exact iteration counts
+            (xp, ip), tp = timed(lambda: op.solve(b, tol=1e-8, precond=True, check_every=ce))
+            (xs, is_), ts = timed(lambda: op.solve(b, tol=1e-8, precond="sai", check_every=ce))
+            (xf, if_), tf = timed(lambda: lp_mod.baseline_solve(op, b, tol=1e-8, precond="sai",
+                                                                check_every=ce))
+            (xl, il), tl = timed(lambda: lp.solve(op, charges=[(1.0, tuple(r0))], tol=1e-8,
+                                                  precond="sai", check_every=ce))
+            (xlj, ilj), tlj = timed(lambda: lp.solve(op, charges=[(1.0, tuple(r0))], tol=1e-8,
+                                                     precond=True, check_every=ce))
+
+            def leg(x_, i_, t_):
+                rr = float(torch.linalg.norm(b - op.accurate(x_)) / torch.linalg.norm(b))
+                return dict(iters=i_["iters"], seconds=t_, rel_resid_fp64_dense=rr,
+                            diff_vs_plain=float(torch.linalg.norm(x_ - xp) / torch.linalg.norm(xp)))
+            solve["learned_vs_plain"] = dict(
+                plain_jacobi=leg(xp, ip, tp), sai=leg(xs, is_, ts),
+                sai_fixed_coarse=leg(xf, if_, tf), sai_learned=leg(xl, il, tl),
+                jacobi_learned=leg(xlj, ilj, tlj),
+                iteration_ratio_plain_over_learned=ip["iters"] / max(1, il["iters"]),
+                iteration_ratio_without_over_with_network=if_["iters"] / max(1, il["iters"]),
+                weights="electrodrive_amd/learn/weights/precond_mlp.npz")
         sigma = x
     else:
         sigma = torch.as_tensor(rng.normal(size=n), dtype=torch.float64, device=dev)
 
-    st = op.plan.stats()
-    out = torch.empty_like(sigma)
-    for _ in range(a.warmup):
-        op.plan.matvec(sigma, op.self_corr, out=out)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    e0.record()
-    for _ in range(a.steps):
-        op.plan.matvec(sigma, op.self_corr, out=out)
-    e1.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = e0.elapsed_time(e1) / a.steps
-    stages = op.plan.profile(sigma, op.self_corr, reps=max(5, a.steps // 2))
-    dom = max(stages, key=stages.get)
-    rf = stage_roofline(dom, stages[dom], st, n, a.order)
-    roofline = dict(kernel=dom, bound=rf["bound"], achieved=rf["achieved"], peak=rf["peak"],
-                    unit=rf["unit"], frac=rf["achieved"] / rf["peak"], traffic=None,
-                    launch_ms=stages[dom], algorithmic_per_launch=rf["algorithmic"])
-    m2l = stage_roofline("m2l", stages["m2l"], st, n, a.order)
-    extra_roofline = dict(m2l_mfma=dict(achieved_tflops=m2l["achieved"],
-                                        frac=m2l["achieved"] / MFMA_FP32_PEAK_TFLOPS,
-                                        launch_ms=stages["m2l"]))
+    mm = matvec_measure(op, sigma, a, n)
+    ms, stages, st = mm["ms"], mm["stages"], mm["stats"]
 
     cpu = None
     cpu_full = None
@@ -186,39 +253,200 @@ def main():
         cpu, cpu_full = cpu_baseline(mesh, sigma.cpu().numpy(), op.self_corr.cpu().numpy(),
                                      a.cpu_seconds)
         if solve is not None:
-            cpu["time_to_solution_estimate_s"] = cpu_full * solve["gmres_iters"]
+            # the reference method: Jacobi GMRES, one dense matvec per iteration
+            lvp = solve.get("learned_vs_plain")
+            its = lvp["plain_jacobi"]["iters"] if lvp else solve["gmres_iters"]
+            cpu["time_to_solution_estimate_s"] = cpu_full * its
+            cpu["time_to_solution_estimate"] = (f"{its} Jacobi GMRES iterations x the full "
+                                                f"dense matvec time")
 
-    value = n / (ms * 1e-3)
-    line = {
-        "metric": "time-to-1e-8 residual + FMM matvec panels/s; learned-precond vs plain GMRES",
-        "value": value,
-        "unit": "panels/s",
-        "n_gpus": 1,
-        "steps": a.steps,
-        "warmup": a.warmup,
-        "ms_per_step": ms,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "fp32",
-        "data": "synthetic",
-        "config": {"workload": f"grounded icosphere s={a.subdiv} ({n} panels), point charge q=1 "
-                               f"at rho={rho:.4f} (seed {a.seed}); FMM p={a.order} ws={a.ws} "
-                               f"leaf<={a.leaf} M2L {a.m2l}; near quadrature on",
-                   "panels": n, "order": a.order, "ws": a.ws, "leaf": a.leaf,
-                   "m2l_precision": a.m2l, "krylov": "fp64 GMRES(50) + fp64 refinement"},
-        "roofline": roofline,
-        "roofline_extra": extra_roofline,
-        "cpu_baseline": cpu,
-        "stage_ms": stages,
-        "plan_stats": st,
-        "solve": solve,
-        "timed_wall_s": wall,
-    }
+    line = base_line(a, mm["value"], ms, "fp32", {
+        "workload": f"grounded icosphere s={a.subdiv} ({n} panels), point charge q=1 "
+                    f"at rho={rho:.4f} (seed {a.seed}); FMM p={a.order} ws={a.ws} "
+                    f"leaf<={a.leaf} M2L {a.m2l}; near quadrature on",
+        "panels": n, "order": a.order, "ws": a.ws, "leaf": a.leaf,
+        "m2l_precision": a.m2l, "krylov": "fp64 GMRES(50) + fp64 refinement"})
+    line.update({"roofline": mm["roofline"], "roofline_extra": mm["extra"], "cpu_baseline": cpu,
+                 "stage_ms": stages, "plan_stats": st, "solve": solve, "timed_wall_s": mm["wall"]})
     if solve is not None:
         line["time_to_1e-8_s"] = solve["time_to_solution_s"]
-    print(json.dumps(line))
-    return 0
+        if "learned_vs_plain" in solve:
+            line["gmres_iters_plain_vs_learned"] = [
+                solve["learned_vs_plain"]["plain_jacobi"]["iters"],
+                solve["learned_vs_plain"]["sai_learned"]["iters"]]
+    return line
+
+
+def cpu_pair_rate(mesh, budget_s):
+    """FP64 dense pair interactions per second of the oracle's C port on this
+    host (bounded row sample)."""
+    import numpy as np
+    rng = np.random.default_rng(0)
+    sig = rng.normal(size=mesh.n_panels)
+    info, _ = cpu_baseline(mesh, sig, np.zeros(mesh.n_panels), budget_s)
+    return info["value"] * mesh.n_panels, info
+
+
+def batch_geometry(rng, family):
+    """One geometry of the uneven batch: 2k-40k panels (log-uniform target)."""
+    import numpy as np
+    from electrodrive_amd.core import bem_mesh
+    target = float(np.exp(rng.uniform(np.log(2000.0), np.log(40000.0))))
+    if family == "sphere":
+        R = rng.uniform(0.5, 2.0)
+        return bem_mesh.icosphere(4 if target < 10000 else 5, R), dict(kind="sphere", R=R)
+    if family == "torus":
+        r = rng.uniform(0.2, 0.4)
+        return bem_mesh.torus_for_panels(1.0, r, int(target)), dict(kind="torus", R=1.0, r=r)
+    if family == "dimer":
+        ra = rng.uniform(0.4, 0.8)
+        gap = rng.uniform(0.05, 0.5)
+        sub = 3 if target < 5000 else (4 if target < 20000 else 5)
+        return bem_mesh.sphere_dimer(ra, gap, sub), dict(kind="dimer", a=ra, gap=gap)
+    L = rng.uniform(1.0, 2.0)
+    k = int(round((target / 2.0) ** 0.5))
+    return bem_mesh.square_plate(L, k), dict(kind="plate", L=L)
+
+
+def run_batch(a):
+    """64 uneven geometries x 8 point-charge right-hand sides on one batched plan."""
+    import numpy as np
+    import torch
+    from electrodrive_amd.core.operator import BemOperator
+    from electrodrive_amd.learn.train import FAMILIES, random_charges
+
+    rng = np.random.default_rng(a.seed)
+    meshes, geos = [], []
+    for g in range(a.n_geoms):
+        m, gd = batch_geometry(rng, FAMILIES[g % 4])
+        meshes.append(m)
+        geos.append(gd)
+    charges = [[random_charges(rng, gd, 1) for gd in geos] for _ in range(a.rhs)]
+    sizes = np.array([m.n_panels for m in meshes])
+    N = int(sizes.sum())
+
+    def build():
+        op = BemOperator(meshes, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
+                         near=True, self_depth=3, near_depth=3)
+        if a.precond == "sai":
+            op.enable_sai()
+        return op
+
+    op = build()
+    op.solve(op.rhs_point_charges(charges[0], per_geometry=True), tol=1e-8, precond=a.precond)
+    del op
+    op, setup_s = timed(build)
+    solves = []
+    t_solve = 0.0
+    for r in range(a.rhs):
+        b = op.rhs_point_charges(charges[r], per_geometry=True)
+        (x, info), t = timed(lambda: op.solve(b, tol=1e-8, precond=a.precond))
+        t_solve += t
+        solves.append(dict(iters=info["iters"], code=info["code"], seconds=t,
+                           max_geometry_rel_resid=info.get("max_geometry_rel_resid")))
+    # FMM operator against the FP64 dense operator, per geometry, random sigma
+    sig = torch.as_tensor(rng.normal(size=N), dtype=torch.float64, device=op.device)
+    fa, fd = op.approx(sig), op.accurate(sig)
+    errs = []
+    for g in range(len(meshes)):
+        lo, hi = int(op.geom_offsets[g]), int(op.geom_offsets[g + 1])
+        errs.append(float(torch.linalg.norm(fa[lo:hi] - fd[lo:hi]) / torch.linalg.norm(fd[lo:hi])))
+    sigmas = [torch.as_tensor(rng.normal(size=N), dtype=torch.float64, device=op.device)
+              for _ in range(a.rhs)]
+    mm = matvec_measure(op, sigmas, a, N * a.rhs, nrhs=a.rhs)
+    cpu = None
+    if not a.no_cpu:
+        rate, info = cpu_pair_rate(meshes[int(np.argmax(sizes))], a.cpu_seconds)
+        pairs = float((sizes.astype(np.float64) ** 2).sum())
+        cpu = dict(value=N / (pairs / rate), unit="panels/s", cores=info["cores"], kind="port",
+                   sample=f"dense FP64 pair rate {rate:.3e}/s of the oracle's C port measured on "
+                          f"the largest geometry ({info['sample']}); batch matvec = sum N_g^2 "
+                          f"pairs")
+    line = base_line(a, mm["value"], mm["ms"], "fp32", {
+        "workload": f"uneven batch: {a.n_geoms} seeded geometries (spheres, tori, sphere dimers, "
+                    f"plates; {int(sizes.min())}-{int(sizes.max())} panels, {N} total) x {a.rhs} "
+                    f"point-charge RHS; one batched launch per FMM stage; FMM p={a.order} "
+                    f"ws={a.ws} leaf<={a.leaf} M2L {a.m2l}; GMRES {a.precond} + fp64 refinement",
+        "geometries": a.n_geoms, "rhs": a.rhs, "panels_total": N, "order": a.order,
+        "leaf": a.leaf, "m2l_precision": a.m2l})
+    line.update({"roofline": mm["roofline"], "roofline_extra": mm["extra"], "cpu_baseline": cpu,
+                 "stage_ms": mm["stages"], "plan_stats": mm["stats"],
+                 "fmm_vs_fp64_dense_max_rel_err": max(errs),
+                 "solve": dict(time_to_solution_s=setup_s + t_solve, setup_s=setup_s,
+                               solve_all_rhs_s=t_solve, per_rhs=solves),
+                 "time_to_1e-8_s": setup_s + t_solve, "timed_wall_s": mm["wall"]})
+    return line
+
+
+def torus_charge(rng):
+    import numpy as np
+    while True:
+        p = rng.uniform(-2.0, 2.0, 3)
+        rho = np.hypot(p[0], p[1])
+        if np.hypot(rho - 1.0, p[2]) > 0.3 + 0.05:
+            return [(1.0, tuple(p))]
+
+
+def run_torus(a):
+    """~300k-panel torus, M2L in BF16 and FP16 (FP32 accumulation) with FP64
+    iterative refinement to 1e-8; errors against the FP32-M2L solve and the
+    FP64 dense operator."""
+    import numpy as np
+    import torch
+    from electrodrive_amd.core import bem_mesh
+    from electrodrive_amd.core.operator import BemOperator
+
+    rng = np.random.default_rng(a.seed)
+    mesh = bem_mesh.torus_for_panels(1.0, 0.3, a.torus_panels)
+    n = mesh.n_panels
+    ch = torus_charge(rng)
+    sig = torch.as_tensor(rng.normal(size=n), dtype=torch.float64, device="cuda")
+    res = {}
+    ref_x = None
+    dense = None
+    for prec in ("fp32", "bf16", "fp16"):
+        def build():
+            op = BemOperator(mesh, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=prec,
+                             near=True, self_depth=3, near_depth=3)
+            op.enable_sai()
+            return op
+        op, setup_s = timed(build)
+        b = op.rhs_point_charges(ch)
+        (x, info), t = timed(lambda: op.solve(b, tol=1e-8, precond="sai"))
+        if dense is None:
+            dense = op.accurate(sig)
+        fa = op.approx(sig)
+        rr = float(torch.linalg.norm(b - op.accurate(x)) / torch.linalg.norm(b))
+        if ref_x is None:
+            ref_x = x
+        mm = matvec_measure(op, sig, a, n)
+        res[prec] = dict(setup_s=setup_s, solve_s=t, time_to_solution_s=setup_s + t,
+                         iters=info["iters"], refine_steps=info["refine_steps"], code=info["code"],
+                         rel_resid_fp64_dense=rr,
+                         solution_diff_vs_fp32_m2l=float(torch.linalg.norm(x - ref_x)
+                                                          / torch.linalg.norm(ref_x)),
+                         matvec_rel_err_vs_fp64_dense=float(torch.linalg.norm(fa - dense)
+                                                            / torch.linalg.norm(dense)),
+                         ms_per_matvec=mm["ms"], panels_per_s=mm["value"], stage_ms=mm["stages"],
+                         m2l=mm["extra"]["m2l_mfma"])
+        if prec == "bf16":
+            head = mm
+        del op
+    cpu = None
+    if not a.no_cpu:
+        rate, info = cpu_pair_rate(mesh, a.cpu_seconds)
+        cpu = dict(value=n / (float(n) ** 2 / rate), unit="panels/s", cores=info["cores"],
+                   kind="port", sample=info["sample"])
+    line = base_line(a, head["value"], head["ms"], "bf16", {
+        "workload": f"torus R=1 r=0.3 ({n} panels), point charge ~U[-2,2]^3 outside the "
+                    f"surface (seed {a.seed}); FMM p={a.order} ws={a.ws} leaf<={a.leaf}; M2L "
+                    f"BF16/FP16 MFMA (FP32 accumulation), FP64 GMRES (SAI) + FP64 refinement",
+        "panels": n, "order": a.order, "leaf": a.leaf, "m2l_precision": "bf16"})
+    line.update({"roofline": head["roofline"], "roofline_extra": head["extra"],
+                 "cpu_baseline": cpu, "stage_ms": head["stages"], "plan_stats": head["stats"],
+                 "precisions": res, "time_to_1e-8_s": res["bf16"]["time_to_solution_s"],
+                 "timed_wall_s": head["wall"]})
+    return line
 
 
 if __name__ == "__main__":

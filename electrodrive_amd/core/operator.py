@@ -157,6 +157,35 @@ class BemOperator:
         if precond == "sai" and not getattr(self, "_sai", False):
             self.enable_sai()
         use_diag = precond not in (False, None, "none")
+        G = len(self.meshes)
+        if G > 1:
+            # batches: the operator is block diagonal, so each geometry's block
+            # is scaled to a unit right-hand side and the tolerance divided by
+            # sqrt(G); every block then meets tol relative to its own |b_g|
+            scale = torch.ones_like(b)
+            for g in range(G):
+                lo, hi = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
+                nb = float(torch.linalg.norm(b[lo:hi]))
+                scale[lo:hi] = 1.0 / nb if nb > 0.0 else 1.0
+            x, info = self.plan.solve((b * scale).contiguous(), (x0 * scale).contiguous(),
+                                      self.diag if use_diag else None, self.self_corr,
+                                      restart=restart, maxiter=maxiter, tol=tol / math.sqrt(G),
+                                      precond=precond, refine=refine, max_refine=max_refine,
+                                      inner_tol=inner_tol, use_graph=use_graph,
+                                      check_every=check_every)
+            x = x / scale
+            r = b - self.accurate(x) if refine else None
+            if r is not None:
+                worst = 0.0
+                for g in range(G):
+                    lo, hi = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
+                    nb = float(torch.linalg.norm(b[lo:hi]))
+                    if nb > 0.0:
+                        worst = max(worst, float(torch.linalg.norm(r[lo:hi])) / nb)
+                info["max_geometry_rel_resid"] = worst
+                info["resid"] = float(torch.linalg.norm(r))
+                info["rel_resid"] = info["resid"] / max(float(torch.linalg.norm(b)), 1e-300)
+            return x, info
         return self.plan.solve(b, x0, self.diag if use_diag else None, self.self_corr,
                                restart=restart, maxiter=maxiter, tol=tol, precond=precond,
                                refine=refine, max_refine=max_refine, inner_tol=inner_tol,

@@ -54,4 +54,14 @@ def test_batched_solve():
     r = b - op.accurate(x)
     for g in range(len(ms)):
         s, e = int(op.geom_offsets[g]), int(op.geom_offsets[g + 1])
-        assert float(torch.linalg.norm(r[s:e]) / torch.linalg.norm(b)) < 1e-8
+        assert float(torch.linalg.norm(r[s:e]) / torch.linalg.norm(b[s:e])) <= 1e-8
+    assert info["max_geometry_rel_resid"] <= 1e-8
+    xs, infos = op.solve(b, tol=1e-8, precond="sai")
+    assert infos["code"] == 0 and infos["max_geometry_rel_resid"] <= 1e-8
+    assert float(torch.linalg.norm(xs - x) / torch.linalg.norm(x)) < 1e-6
+    # each block equals its own single-geometry solve
+    for g, (m, ch) in enumerate(zip(ms, charges)):
+        s, e = int(op.geom_offsets[g]), int(op.geom_offsets[g + 1])
+        og = BemOperator(m, order=10, ws=1, leaf_size=32)
+        xg, ig = og.solve(og.rhs_point_charges(ch), tol=1e-8)
+        assert float(torch.linalg.norm(x[s:e] - xg) / torch.linalg.norm(xg)) < 1e-6
