@@ -31,6 +31,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_BF16_PEAK_TFLOPS = 2500.0
+VALU_FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
+# WRITE_SIZE x 1.0, the factors a 256 MiB device copy reads), keyed by stage
+# and panel count of the measured workload; see profiles/
+TRAFFIC_BYTES = {}
 
 
 def parse():
@@ -52,6 +57,10 @@ def parse():
     ap.add_argument("--workload", default="sphere", choices=["sphere", "batch", "torus"],
                     help="sphere: 81,920-panel icosphere (default); batch: 64 geometries x 8 "
                          "RHS; torus: ~300k-panel torus with 16-bit M2L")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="use --order/--leaf as given instead of the autotuner's choice")
+    ap.add_argument("--tune-target", type=float, default=1e-5,
+                    help="autotuner bound on the FMM's relative L2 error vs FP64 dense")
     ap.add_argument("--n-geoms", type=int, default=64)
     ap.add_argument("--rhs", type=int, default=8)
     ap.add_argument("--torus-panels", type=int, default=300312)
@@ -67,11 +76,17 @@ def stage_roofline(name, ms, st, n, order, m2l_peak=MFMA_FP32_PEAK_TFLOPS):
         flops = 2.0 * nc * nc * st["n_m2l_pairs"]
         return dict(bound="mfma", achieved=flops / s / 1e12, peak=m2l_peak,
                     unit="TFLOP/s", algorithmic=flops)
-    if name == "leaf":
-        # targets (x,y,z,q) + source tiles of every P2P leaf pair + local
-        # coefficients + near CSR (col int32 + weight fp32) + sigma/self/perm/out
-        byts = (16.0 * n + 16.0 * st["n_p2p_source_points"] + 4.0 * nc * st["n_leaves"]
-                + 8.0 * st["n_near_pairs"] + 24.0 * n)
+    if name == "p2p":
+        # target points + the source points of every P2P leaf pair (x, y, z, q)
+        # + the partial-sum write
+        byts = 16.0 * n + 16.0 * st["n_p2p_source_points"] + 4.0 * n
+        return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                    algorithmic=byts, interactions_per_s=st["n_p2p_interactions"] / s,
+                    valu_tflops=20.0 * st["n_p2p_interactions"] / s / 1e12)
+    if name == "l2p_out":
+        # local coefficients once per leaf + near CSR (weight, column, gathered
+        # sigma) + per panel: point, P2P sum, self term, row pointer, perm, out
+        byts = (4.0 * nc * st["n_leaves"] + 12.0 * st["n_near_pairs"] + 40.0 * n)
         return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     algorithmic=byts)
     if name == "m2l_reduce":
@@ -137,9 +152,18 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
     roofline = dict(kernel=dom, bound=rf["bound"], achieved=rf["achieved"], peak=rf["peak"],
                     unit=rf["unit"], frac=rf["achieved"] / rf["peak"], traffic=None,
                     launch_ms=stages[dom], algorithmic_per_launch=rf["algorithmic"])
+    if dom in TRAFFIC_BYTES and op.n == TRAFFIC_BYTES[dom][0]:
+        roofline["traffic"] = TRAFFIC_BYTES[dom][1]
+        roofline["traffic_source"] = TRAFFIC_BYTES[dom][2]
     m2l = stage_roofline("m2l", stages["m2l"], st, op.n, op.plan.order, peak)
+    p2p = stage_roofline("p2p", stages["p2p"], st, op.n, op.plan.order)
     extra = dict(m2l_mfma=dict(achieved_tflops=m2l["achieved"], peak_tflops=peak,
-                               frac=m2l["achieved"] / peak, launch_ms=stages["m2l"]))
+                               frac=m2l["achieved"] / peak, launch_ms=stages["m2l"]),
+                 p2p=dict(achieved_gbs=p2p["achieved"], frac_hbm=p2p["achieved"] / HBM_PEAK_GBS,
+                          interactions_per_s=p2p["interactions_per_s"],
+                          valu_tflops=p2p["valu_tflops"],
+                          frac_valu=p2p["valu_tflops"] / VALU_FP32_PEAK_TFLOPS,
+                          launch_ms=stages["p2p"]))
     return dict(ms=ms, wall=wall, stages=stages, roofline=roofline, extra=extra, stats=st,
                 value=n_units / (ms * 1e-3))
 
@@ -180,6 +204,18 @@ def run_sphere(a):
     u = rng.normal(size=3)
     u /= np.linalg.norm(u)
     r0 = rho * u
+
+    tune = None
+    if not a.no_autotune:
+        from electrodrive_amd.fmm3d import FmmConfig, autotune_for_problem
+        base = FmmConfig(expansion_order=a.order, leaf_size=a.leaf,
+                         mac_theta=0.8 if a.ws == 1 else 0.5, m2l_precision=a.m2l)
+        tr, tune_s = timed(lambda: autotune_for_problem(
+            base, n, dry_run=False, target_rel_error=a.tune_target, centroids=mesh.centroids,
+            areas=mesh.areas, use_cache=False))
+        a.order, a.leaf = tr.config.native_order, tr.config.leaf_size
+        tune = dict(order=a.order, leaf=a.leaf, seconds=tune_s, target_rel_error=a.tune_target,
+                    **{k: v for k, v in tr.metrics.items() if k in ("ms", "rel_err", "trials")})
 
     def build():
         return BemOperator(mesh, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
@@ -263,11 +299,13 @@ def run_sphere(a):
     line = base_line(a, mm["value"], ms, "fp32", {
         "workload": f"grounded icosphere s={a.subdiv} ({n} panels), point charge q=1 "
                     f"at rho={rho:.4f} (seed {a.seed}); FMM p={a.order} ws={a.ws} "
-                    f"leaf<={a.leaf} M2L {a.m2l}; near quadrature on",
+                    f"leaf<={a.leaf} M2L {a.m2l} (order and leaf from the autotuner at "
+                    f"rel err <= {a.tune_target:g} vs FP64 dense); near quadrature on",
         "panels": n, "order": a.order, "ws": a.ws, "leaf": a.leaf,
         "m2l_precision": a.m2l, "krylov": "fp64 GMRES(50) + fp64 refinement"})
     line.update({"roofline": mm["roofline"], "roofline_extra": mm["extra"], "cpu_baseline": cpu,
-                 "stage_ms": stages, "plan_stats": st, "solve": solve, "timed_wall_s": mm["wall"]})
+                 "stage_ms": stages, "plan_stats": st, "solve": solve, "autotune": tune,
+                 "timed_wall_s": mm["wall"]})
     if solve is not None:
         line["time_to_1e-8_s"] = solve["time_to_solution_s"]
         if "learned_vs_plain" in solve:

@@ -63,12 +63,12 @@ static int upload(T** dst, const std::vector<T>& src) {
 void plan_free(ed_plan* P) {
   void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->perm, P->node_ctr, P->node_rng,
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
-                  P->leaves, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
+                  P->leaves, P->pt_node, P->p2p, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
                   P->m2l_pair_src, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair, P->t_m2l,
                   P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
                   P->near_w, P->cent64_t, P->area64_t, P->near_w64, P->m2l_ascale,
-                  P->m2l_binv, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
+                  P->m2l_binv, P->m2l_mmax, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
                   P->coarse_part, P->coarse_tmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -130,6 +130,13 @@ int plan_upload(ed_plan* P) {
   P->n_p2p_interactions = p2p_int;
   P->n_p2p_src = p2p_src;
   ED_TRY(upload(&P->leaves, leaves));
+  {
+    std::vector<int32_t> ptn((size_t)H.n, 0);
+    for (int32_t li : leaves)
+      for (int64_t t = H.begin[li]; t < H.end[li]; ++t) ptn[(size_t)t] = li;
+    ED_TRY(upload(&P->pt_node, ptn));
+    ED_HIP(hipMalloc((void**)&P->p2p, sizeof(float) * (size_t)std::max<int64_t>(H.n, 1)));
+  }
   ED_TRY(upload(&P->u_ptr, uptr));
   ED_TRY(upload(&P->u_src, usrc));
 
@@ -207,6 +214,7 @@ int plan_upload(ed_plan* P) {
   for (int k = 0; k < nc; ++k) { af[k] = (float)asc[k]; bf[k] = (float)asc[k]; }
   ED_TRY(upload(&P->m2l_ascale, af));
   ED_TRY(upload(&P->m2l_binv, bf));
+  ED_TRY(upload(&P->m2l_mmax, std::vector<float>(256, 0.0f)));
   std::vector<float> t8((size_t)8 * nc * nc), l8((size_t)8 * nc * nc);
   for (int o = 0; o < 8; ++o) {
     m2m_matrix(p, o, Td.data());

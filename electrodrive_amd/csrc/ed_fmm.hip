@@ -238,6 +238,32 @@ struct M2LShapeH {
   static constexpr int THREADS = 256;
 };
 
+// 16-bit inputs are scaled by one power of two per matvec so that the largest
+// scaled multipole is 2^14: FP16 keeps its ~11 decades of range where the
+// values are (the corrections of iterative refinement are orders of magnitude
+// below the first right-hand side) and the scale is undone exactly on Y.
+constexpr int kM2LScaleExp = 14;
+constexpr int kAbsMaxBlocks = 256;
+
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_absmax(int64_t total, const float* __restrict__ M,
+                                                   const float* __restrict__ ascale,
+                                                   float* __restrict__ part) {
+  constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float red[256];
+  float m = 0.0f;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    m = fmaxf(m, fabsf(M[e] * ascale[e % NC]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 template <int P>
 __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t* __restrict__ tile_tab,
                                                   const int32_t* __restrict__ tile_beg,
@@ -330,12 +356,26 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
                                                     const T16* __restrict__ Tm,
                                                     const float* __restrict__ ascale,
                                                     const float* __restrict__ M,
+                                                    const float* __restrict__ mmax,
                                                     float* __restrict__ Y) {
   using S = M2LShapeH<P>;
   using V = typename Mfma16<T16>::vec;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
   __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
+  __shared__ float red[256];
+  // power-of-two input scale from the per-block maxima of |M * ascale|
+  float mx = 0.0f;
+  for (int t = threadIdx.x; t < kAbsMaxBlocks; t += blockDim.x) mx = fmaxf(mx, mmax[t]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float gmax = red[0];
+  const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
+  const float inv_sc = 1.0f / sc;
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
   const T16* Tt = Tm + (size_t)tab * NCP * KP;
@@ -343,10 +383,12 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
     const int i = t / KP, k = t - i * KP;
     ta[i * KS + k] = Tt[t];
   }
+  // gathered, scaled multipoles: consecutive threads read consecutive
+  // coefficients of one source row
   for (int t = threadIdx.x; t < COLS * KP; t += 256) {
     const int c = t / KP, k = t - c * KP;
     float v = 0.0f;
-    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k] * ascale[k];
+    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k] * ascale[k] * sc;
     bm[c * KS + k] = (T16)v;
   }
   __syncthreads();
@@ -362,11 +404,11 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
     for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
   const T16* brow = bm + (col0 + li) * KS + 8 * lk;
 #pragma unroll 2
-  for (int s = 0; s < KP / 16; ++s) {
-    const V b = *reinterpret_cast<const V*>(brow + 16 * s);
+  for (int sv = 0; sv < KP / 16; ++sv) {
+    const V b = *reinterpret_cast<const V*>(brow + 16 * sv);
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb) {
-      const V a = *reinterpret_cast<const V*>(ta + (rb * 32 + li) * KS + 16 * s + 8 * lk);
+      const V a = *reinterpret_cast<const V*>(ta + (rb * 32 + li) * KS + 16 * sv + 8 * lk);
       acc[rb] = Mfma16<T16>::run(a, b, acc[rb]);
     }
   }
@@ -378,8 +420,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = rb * 32 + q * 8 + lk * 4;
-        float4 v = make_float4(acc[rb][4 * q + 0], acc[rb][4 * q + 1], acc[rb][4 * q + 2],
-                               acc[rb][4 * q + 3]);
+        float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
+                               acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
         *reinterpret_cast<float4*>(yrow + row) = v;
       }
   }
@@ -452,17 +494,48 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
   return v;
 }
 
-template <int P, typename TO>
-__global__ __launch_bounds__(kLeafThreads) void k_leaf(
+// Sum of q_j / r_ij over one LDS tile for a lane of slice `sl` of `S` (points
+// j = sl, sl + S, ...).  The step count is uniform across the wave and the
+// loop is unrolled 8-wide so the LDS reads of a group are in flight together;
+// out-of-range steps read a clamped index and j == me (the target itself)
+// gets r^2 = inf, so both add q * rsq(inf) = 0 without a branch.
+__device__ __forceinline__ float p2p_tile(const float4* __restrict__ tile, int ns, int sl, int S,
+                                          float4 xi, int me) {
+  constexpr int U = 8;
+  float acc = 0.0f;
+  const int nsteps = (ns + S - 1) / S;
+  int t = 0;
+  for (; t + U <= nsteps; t += U) {
+    float4 sv[U];
+    int jv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      jv[u] = sl + (t + u) * S;
+      sv[u] = tile[min(jv[u], ns - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const float dx = xi.x - sv[u].x, dy = xi.y - sv[u].y, dz = xi.z - sv[u].z;
+      float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
+      r2 = (jv[u] < ns && jv[u] != me) ? r2 : INFINITY;  // rsq(inf) = 0
+      acc = fmaf(sv[u].w, __builtin_amdgcn_rsqf(r2), acc);
+    }
+  }
+  for (; t < nsteps; ++t) {
+    const int j = sl + t * S;
+    const float4 sj = tile[min(j, ns - 1)];
+    const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
+    float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
+    r2 = (j < ns && j != me) ? r2 : INFINITY;
+    acc = fmaf(sj.w, __builtin_amdgcn_rsqf(r2), acc);
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(kLeafThreads) void k_p2p(
     const int32_t* __restrict__ leaves, const int2* __restrict__ rng,
-    const float4* __restrict__ ctr, const float4* __restrict__ pts,
-    const float* __restrict__ L, const int32_t* __restrict__ u_ptr,
-    const int32_t* __restrict__ u_src, const int32_t* __restrict__ near_ptr,
-    const int32_t* __restrict__ near_col, const float* __restrict__ near_w,
-    const float* __restrict__ sig_t, const float* __restrict__ selfc_t,
-    const int32_t* __restrict__ perm, int out_tree, float ke, TO* __restrict__ out) {
-  constexpr int NC = (P + 1) * (P + 1);
-  __shared__ float lc[NC];
+    const float4* __restrict__ pts, const int32_t* __restrict__ u_ptr,
+    const int32_t* __restrict__ u_src, float* __restrict__ p2p) {
   __shared__ float4 src[kLeafWaves][kSrcChunk];
   __shared__ int ubeg[kUSeg];
   __shared__ int upref[kUSeg + 1];
@@ -472,8 +545,6 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(
   const int li = blockIdx.x;
   const int node = leaves[li];
   const int2 r = rng[node];
-  const float4 c = ctr[node];
-  for (int j = threadIdx.x; j < NC; j += kLeafThreads) lc[j] = L[(size_t)node * NC + j];
   const int ub = u_ptr[li], ue = u_ptr[li + 1];
   float4* mysrc = src[wave];
   for (int t0 = r.x; t0 < r.y; t0 += 64) {
@@ -491,16 +562,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(
       __syncthreads();
       for (int e = lane; e < ns; e += 64) mysrc[e] = pts[s0 + e];
       __syncthreads();
-      if (act) {
-        const int me = i - s0;
-        for (int j = sl; j < ns; j += S) {
-          const float4 sj = mysrc[j];
-          const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
-          const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-          const float v = sj.w * __builtin_amdgcn_rsqf(r2);
-          acc += (j == me) ? 0.0f : v;
-        }
-      }
+      if (ns > 0) acc += p2p_tile(mysrc, ns, sl, S, xi, i - s0);
     }
     // (2) the other U-list leaves as one concatenated stream
     for (int g0 = ub; g0 < ue; g0 += kUSeg) {
@@ -548,14 +610,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(
           mysrc[e] = pts[ubeg[lo] + (ge - upref[lo])];
         }
         __syncthreads();
-        if (act) {
-          for (int j = sl; j < ns; j += S) {
-            const float4 sj = mysrc[j];
-            const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
-            const float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-            acc += sj.w * __builtin_amdgcn_rsqf(r2);
-          }
-        }
+        if (ns > 0) acc += p2p_tile(mysrc, ns, sl, S, xi, -1);
       }
     }
     __syncthreads();
@@ -565,16 +620,37 @@ __global__ __launch_bounds__(kLeafThreads) void k_leaf(
       float nearf = 0.0f;
       for (int w = 0; w < kLeafWaves; ++w)
         for (int k = 0; k < S; ++k) nearf += red[w * 64 + tl + k * ntc];
-      const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w, lc);
-      float v = ke * (far + nearf) + selfc_t[i] * xi.w;
-      const int nb = near_ptr[i], ne = near_ptr[i + 1];
-      float corr = 0.0f;
-      for (int q = nb; q < ne; ++q) corr += near_w[q] * sig_t[near_col[q]];
-      v += corr;
-      if (!isfinite(v)) v = 0.0f;
-      out[out_tree ? i : perm[i]] = (TO)v;
+      p2p[i] = nearf;
     }
   }
+}
+
+// Per panel (tree order): far field by L2P from its leaf's local expansion,
+// the P2P sum, the near-field correction (CSR row), the self term; output
+// in tree or original order.  Non-finite results become 0
+// (fmm3d/bem_fmm.py:741-749).
+template <int P, typename TO>
+__global__ __launch_bounds__(256) void k_l2p_out(
+    int n, const int32_t* __restrict__ pt_node, const float4* __restrict__ ctr,
+    const float4* __restrict__ pts, const float* __restrict__ L, const float* __restrict__ p2p,
+    const int32_t* __restrict__ near_ptr, const int32_t* __restrict__ near_col,
+    const float* __restrict__ near_w, const float* __restrict__ sig_t,
+    const float* __restrict__ selfc_t, const int32_t* __restrict__ perm, int out_tree, float ke,
+    TO* __restrict__ out) {
+  constexpr int NC = (P + 1) * (P + 1);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int node = pt_node[i];
+  const float4 c = ctr[node];
+  const float4 xi = pts[i];
+  const int nb = near_ptr[i], ne = near_ptr[i + 1];
+  float corr = 0.0f;
+  for (int q = nb; q < ne; ++q) corr += near_w[q] * sig_t[near_col[q]];
+  const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w,
+                                L + (size_t)node * NC);
+  float v = ke * (far + p2p[i]) + selfc_t[i] * xi.w + corr;
+  if (!isfinite(v)) v = 0.0f;
+  out[out_tree ? i : perm[i]] = (TO)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -642,14 +718,20 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
       ED_MARK(4);
     } else {
       const dim3 tiles((unsigned)Pn->n_m2l_tiles);
-      if (prec == 1) {
-        hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
-                           Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
-                           (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M, Pn->ypair);
-      } else if (prec == 2) {
-        hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
-                           Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
-                           (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M, Pn->ypair);
+      if (prec == 1 || prec == 2) {
+        constexpr int NC = (P + 1) * (P + 1);
+        hipLaunchKernelGGL(k_m2l_absmax<P>, dim3(kAbsMaxBlocks), dim3(256), 0, st,
+                           (int64_t)Pn->n_nodes * NC, Pn->M, Pn->m2l_ascale, Pn->m2l_mmax);
+        if (prec == 1)
+          hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16>), tiles, dim3(256), 0, st,
+                             Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
+                             Pn->m2l_pair_src, (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M,
+                             Pn->m2l_mmax, Pn->ypair);
+        else
+          hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16>), tiles, dim3(256), 0, st,
+                             Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
+                             Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale,
+                             Pn->M, Pn->m2l_mmax, Pn->ypair);
       } else if constexpr (M2LShape<P>::ENABLED) {
         hipLaunchKernelGGL(k_m2l_mfma<P>, tiles, dim3(M2LShape<P>::THREADS), 0, st,
                            Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
@@ -674,18 +756,20 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   }
   ED_MARK(6);
   const float ke = (float)kKE;
-  if (out64)
-    hipLaunchKernelGGL((k_leaf<P, double>), dim3(nleaf), dim3(kLeafThreads), 0, st, Pn->leaves,
-                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->L, Pn->u_ptr, Pn->u_src,
-                       Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm,
-                       out_tree, ke, (double*)out);
-  else
-    hipLaunchKernelGGL((k_leaf<P, float>), dim3(nleaf), dim3(kLeafThreads), 0, st, Pn->leaves,
-                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->L, Pn->u_ptr, Pn->u_src,
-                       Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm,
-                       out_tree, ke, (float*)out);
-  ED_LAUNCH_CHECK("leaf");
+  hipLaunchKernelGGL(k_p2p, dim3(nleaf), dim3(kLeafThreads), 0, st, Pn->leaves, Pn->node_rng,
+                     Pn->pts, Pn->u_ptr, Pn->u_src, Pn->p2p);
+  ED_LAUNCH_CHECK("p2p");
   ED_MARK(7);
+  if (out64)
+    hipLaunchKernelGGL((k_l2p_out<P, double>), dim3(gb), dim3(256), 0, st, n, Pn->pt_node,
+                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p, Pn->near_ptr, Pn->near_col,
+                       Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm, out_tree, ke, (double*)out);
+  else
+    hipLaunchKernelGGL((k_l2p_out<P, float>), dim3(gb), dim3(256), 0, st, n, Pn->pt_node,
+                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p, Pn->near_ptr, Pn->near_col,
+                       Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm, out_tree, ke, (float*)out);
+  ED_LAUNCH_CHECK("l2p_out");
+  ED_MARK(8);
   (void)NC;
   return ED_OK;
 }
@@ -724,31 +808,32 @@ int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled) {
 }  // namespace ed
 
 // Per-stage device time of the FMM matvec (FP64 in/out, original order),
-// averaged over `reps` launches: stage_ms[0..6] = gather, p2m, m2m, m2l (GEMM
-// or VALU), m2l reduce, l2l, leaf (L2P + P2P + near + self).
+// averaged over `reps` launches: stage_ms[0..7] = gather, p2m, m2m, m2l (GEMM
+// or VALU), m2l reduce, l2l, p2p, l2p_out (L2P + near + self + output).
 extern "C" int ed_fmm_profile(ed_plan* P, const double* sigma, const double* selfi, double* out,
                               int32_t reps, double* stage_ms, int32_t n_stages, ed_stream stream) {
   using namespace ed;
   if (!P || !sigma || !out || !stage_ms || reps < 1) return fail(ED_ERR_ARG, "ed_fmm_profile: bad argument");
   hipStream_t st = (hipStream_t)stream;
-  hipEvent_t ev[8];
-  for (int k = 0; k < 8; ++k)
+  constexpr int NE = 9;  // stage boundaries
+  hipEvent_t ev[NE];
+  for (int k = 0; k < NE; ++k)
     if (hipEventCreate(&ev[k]) != hipSuccess) return fail(ED_ERR_DEVICE, "ed_fmm_profile: event");
-  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  double acc[NE - 1] = {0, 0, 0, 0, 0, 0, 0, 0};
   int rc = ED_OK;
   P->prof_events = ev;
   for (int r = 0; r < reps && rc == ED_OK; ++r) {
     rc = fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, st);
     if (rc != ED_OK) break;
-    if (hipEventSynchronize(ev[7]) != hipSuccess) { rc = fail(ED_ERR_DEVICE, "ed_fmm_profile: sync"); break; }
-    for (int k = 0; k < 7; ++k) {
+    if (hipEventSynchronize(ev[NE - 1]) != hipSuccess) { rc = fail(ED_ERR_DEVICE, "ed_fmm_profile: sync"); break; }
+    for (int k = 0; k < NE - 1; ++k) {
       float ms = 0.0f;
       (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
       acc[k] += ms;
     }
   }
   P->prof_events = nullptr;
-  for (int k = 0; k < 8; ++k) (void)hipEventDestroy(ev[k]);
-  for (int k = 0; k < n_stages && k < 7; ++k) stage_ms[k] = acc[k] / reps;
+  for (int k = 0; k < NE; ++k) (void)hipEventDestroy(ev[k]);
+  for (int k = 0; k < n_stages && k < NE - 1; ++k) stage_ms[k] = acc[k] / reps;
   return rc;
 }

@@ -831,7 +831,10 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
       // last refinement step does not over-solve (0.25 leaves room for the
       // approximate operator's own error)
       const double itol = std::min(0.5, std::max(o.inner_tol, 0.25 * info->tol_abs / rn));
-      ED_T(S.gmres(W->rhs, W->d, itol, o.maxiter - S.total_iters, &icode, &ir0, &ir1));
+      // an inner solve that stalls on the approximate operator's rounding
+      // floor is cut after 4 restart cycles; the refinement continues from it
+      const int inner_max = std::min(o.maxiter - S.total_iters, 4 * o.restart);
+      ED_T(S.gmres(W->rhs, W->d, itol, inner_max, &icode, &ir0, &ir1));
       if (icode == 3 || icode == 5 || icode == 6 || icode == 7) { code = icode; break; }
       hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, W->stream, n, 1.0, W->d, W->x);
       ED_KCHECK("refine update");

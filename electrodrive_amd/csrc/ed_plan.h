@@ -31,6 +31,8 @@ struct ed_plan {
   float* L = nullptr;  // n_nodes * nc
 
   // leaves (node ids) and their U lists (CSR over leaf ordinal)
+  int32_t* pt_node = nullptr;   // n, leaf node of each tree-ordered panel
+  float* p2p = nullptr;         // n, P2P partial sums (tree order)
   int32_t* leaves = nullptr;
   int32_t* u_ptr = nullptr;
   int32_t* u_src = nullptr;  // node ids
@@ -66,6 +68,7 @@ struct ed_plan {
   uint16_t* t_m2l_h = nullptr;  // n_off * ncp * kp16, bf16/fp16 (scaled)
   float* m2l_ascale = nullptr;  // per input coefficient: sqrt((n+m)!(n-m)!)
   float* m2l_binv = nullptr;    // per output coefficient: sqrt((n+m)!(n-m)!)
+  float* m2l_mmax = nullptr;       // 256 per-block maxima of |M * ascale| (16-bit M2L scale)
   float* t_m2m = nullptr;       // 8 * nc * nc
   float* t_l2l = nullptr;       // 8 * nc * nc
 

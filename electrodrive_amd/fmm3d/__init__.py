@@ -3,6 +3,7 @@ from .config import FmmConfig  # noqa: F401
 from .bem_fmm import LaplaceFmm3D, make_laplace_fmm_backend  # noqa: F401
 from .geometry import morton_keys_from_points  # noqa: F401
 from .plan import FmmPlan  # noqa: F401
+from .autotune import TuningResult, autotune_for_problem  # noqa: F401
 
 __all__ = ["FmmConfig", "LaplaceFmm3D", "make_laplace_fmm_backend", "morton_keys_from_points",
-           "FmmPlan"]
+           "FmmPlan", "TuningResult", "autotune_for_problem"]

@@ -1,0 +1,199 @@
+"""Leaf size / expansion order autotuner (mirror of fmm3d/autotune.py:66-230).
+
+Same public names and arguments as the reference: ``autotune_for_problem(
+base_cfg, problem_size, *, dry_run=True, target_rel_error=1e-2,
+max_trials=24, device=None, use_cache=True) -> TuningResult(config,
+metrics)`` with a JSON cache keyed by a hardware fingerprint and a problem-size
+bucket (fmm3d/autotune.py:670-830).  What differs is what is measured: every
+candidate is a native plan on the device, timed with HIP events, and its error
+is the relative L2 difference to the FP64 dense operator on the same panels
+(the reference times its CPU pipeline and compares with a direct sum).
+
+Candidates: expansion order in ``SUPPORTED_ORDERS`` at or above 6 and leaf
+sizes 32 .. 256, with ``ws`` and the M2L precision of ``base_cfg``.  The fastest
+candidate whose error meets ``target_rel_error`` wins; when none does, the most
+accurate one.  Pass ``centroids`` / ``areas`` to tune on the actual geometry
+(otherwise a unit sphere of ``problem_size`` panels is used).
+
+The cache file (``$ELECTRODRIVE_AMD_AUTOTUNE_CACHE`` or
+``~/.cache/electrodrive_amd/fmm3d_autotune.json``) is the only shared mutable
+state; writers hold an exclusive ``flock`` on it.
+"""
+import dataclasses
+import fcntl
+import json
+import math
+import os
+from dataclasses import dataclass
+from pathlib import Path
+from typing import Any, Dict, List, Optional
+
+import numpy as np
+import torch
+
+from .. import _native as nat
+from .config import SUPPORTED_ORDERS, FmmConfig
+
+AUTOTUNE_CACHE_ENV_VAR = "ELECTRODRIVE_AMD_AUTOTUNE_CACHE"
+AUTOTUNE_CACHE_FILENAME = "fmm3d_autotune.json"
+LEAF_SIZES = (32, 64, 128, 256)
+
+
+@dataclass
+class TuningResult:
+    config: FmmConfig
+    metrics: Dict[str, float]
+
+
+def _bucket_problem_size(n: int) -> int:
+    """Power-of-two bucket (fmm3d/autotune.py:670-674)."""
+    return 1 << max(0, int(math.ceil(math.log2(max(1, n)))))
+
+
+def _hardware_fingerprint(device=None) -> str:
+    dev = torch.device(device or "cuda")
+    props = torch.cuda.get_device_properties(dev)
+    arch = getattr(props, "gcnArchName", "") or ""
+    return f"{props.name}|{arch}|cu{props.multi_processor_count}|lib{nat.lib().ed_abi_version()}"
+
+
+def _get_cache_path() -> Path:
+    env = os.environ.get(AUTOTUNE_CACHE_ENV_VAR)
+    if env:
+        return Path(env)
+    return Path.home() / ".cache" / "electrodrive_amd" / AUTOTUNE_CACHE_FILENAME
+
+
+def _cache_key(fp, bucket, cfg, target):
+    return f"{fp}|n{bucket}|ws{cfg.ws}|{cfg.m2l_precision}|err{target:.1e}"
+
+
+def _load_best_from_cache(key: str) -> Optional[Dict[str, Any]]:
+    path = _get_cache_path()
+    if not path.exists():
+        return None
+    try:
+        with open(path, "r") as f:
+            fcntl.flock(f, fcntl.LOCK_SH)
+            data = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return data.get(key)
+
+
+def _append_record_to_cache(key: str, record: Dict[str, Any]) -> None:
+    path = _get_cache_path()
+    try:
+        path.parent.mkdir(parents=True, exist_ok=True)
+        with open(path, "a+") as f:
+            fcntl.flock(f, fcntl.LOCK_EX)
+            f.seek(0)
+            txt = f.read()
+            try:
+                data = json.loads(txt) if txt.strip() else {}
+            except ValueError:
+                data = {}
+            data[key] = record
+            f.seek(0)
+            f.truncate()
+            json.dump(data, f, indent=1, sort_keys=True)
+    except OSError:
+        pass  # best effort, as the reference (fmm3d/autotune.py:767-803)
+
+
+def _synthetic_geometry(n):
+    """Unit sphere with about n panels (icosphere of the nearest size)."""
+    from ..core import bem_mesh
+    s = max(1, int(round(math.log(max(n, 20) / 20.0, 4))))
+    m = bem_mesh.icosphere(s)
+    return m.centroids, m.areas
+
+
+def _measure(cfg, C, A, sigma, ref, reps, dev):
+    from .plan import FmmPlan
+    plan = FmmPlan(C, A, leaf_size=cfg.leaf_size, order=cfg.native_order, ws=cfg.ws,
+                   max_level=cfg.max_level, m2l_precision=cfg.m2l_precision)
+    out = torch.empty_like(sigma)
+    for _ in range(2):
+        plan.matvec(sigma, out=out)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        plan.matvec(sigma, out=out)
+    e1.record()
+    torch.cuda.synchronize(dev)
+    ms = e0.elapsed_time(e1) / reps
+    err = float(torch.linalg.norm(out - ref) / torch.linalg.norm(ref))
+    st = plan.stats()
+    plan.close()
+    return dict(ms=ms, rel_err=err, n_leaves=float(st["n_leaves"]),
+                n_m2l_pairs=float(st["n_m2l_pairs"]),
+                n_p2p_interactions=float(st["n_p2p_interactions"]))
+
+
+def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: bool = True,
+                         target_rel_error: float = 1e-2, max_trials: int = 24, device=None,
+                         use_cache: bool = True, centroids=None, areas=None,
+                         reps: int = 5) -> TuningResult:
+    if problem_size <= 0:
+        raise ValueError("problem_size must be positive")
+    base_cfg.validate()
+    if dry_run:
+        return TuningResult(config=base_cfg,
+                            metrics={"mode": 0.0, "problem_size": float(problem_size)})
+    nat.require_gpu()
+    dev = torch.device(device or "cuda")
+    if dev.type != "cuda":
+        raise ValueError("the native autotuner measures on a GPU device")
+    fp = _hardware_fingerprint(dev)
+    bucket = _bucket_problem_size(problem_size)
+    key = _cache_key(fp, bucket, base_cfg, target_rel_error)
+    if use_cache and centroids is None:
+        rec = _load_best_from_cache(key)
+        if rec is not None:
+            cfg = dataclasses.replace(base_cfg, expansion_order=int(rec["expansion_order"]),
+                                      leaf_size=int(rec["leaf_size"]))
+            m = dict(rec["metrics"])
+            m["from_cache"] = 1.0
+            return TuningResult(config=cfg, metrics=m)
+
+    if centroids is None:
+        C, A = _synthetic_geometry(problem_size)
+    else:
+        C = np.asarray(centroids, dtype=np.float64).reshape(-1, 3)
+        A = np.asarray(areas, dtype=np.float64).reshape(-1)
+    n = C.shape[0]
+    rng = np.random.default_rng(0)
+    sigma = torch.as_tensor(rng.normal(size=n), dtype=torch.float64, device=dev)
+    from ..core.bem_kernel import dense_matvec
+    ref = dense_matvec(sigma, torch.as_tensor(C, device=dev), torch.as_tensor(A, device=dev))
+
+    cands: List[FmmConfig] = []
+    for p in (o for o in SUPPORTED_ORDERS if o >= 6):
+        for leaf in LEAF_SIZES:
+            cands.append(dataclasses.replace(base_cfg, expansion_order=p, leaf_size=leaf))
+    cands = cands[:max_trials]
+    trials = []
+    for cfg in cands:
+        try:
+            m = _measure(cfg, C, A, sigma, ref, reps, dev)
+        except RuntimeError:
+            continue
+        trials.append((cfg, m))
+    if not trials:
+        return TuningResult(config=base_cfg, metrics={"mode": 1.0, "problem_size": float(n)})
+    ok = [t for t in trials if t[1]["rel_err"] <= target_rel_error]
+    best_cfg, best = (min(ok, key=lambda t: t[1]["ms"]) if ok
+                      else min(trials, key=lambda t: t[1]["rel_err"]))
+    metrics = dict(best)
+    metrics.update(mode=2.0, problem_size=float(n), trials=float(len(trials)),
+                   met_target=1.0 if ok else 0.0,
+                   panels_per_s=n / (best["ms"] * 1e-3))
+    if use_cache:
+        _append_record_to_cache(key, dict(expansion_order=best_cfg.expansion_order,
+                                          leaf_size=best_cfg.leaf_size, metrics=metrics))
+    return TuningResult(config=best_cfg, metrics=metrics)
+
+
+__all__ = ["TuningResult", "autotune_for_problem", "AUTOTUNE_CACHE_ENV_VAR"]

@@ -122,7 +122,7 @@ class FmmPlan:
                   "ed_fmm_matvec")
         return out
 
-    STAGES = ("gather", "p2m", "m2m", "m2l", "m2l_reduce", "l2l", "leaf")
+    STAGES = ("gather", "p2m", "m2m", "m2l", "m2l_reduce", "l2l", "p2p", "l2p_out")
 
     def profile(self, sigma, self_integrals=None, reps=10):
         """Average device milliseconds per matvec stage (HIP events)."""

@@ -123,3 +123,23 @@ def test_host_morton_keys_bit_exact():
             assert lib.ed_host_morton_keys_f64(P.ctypes.data_as(vp), P.shape[0], level, None,
                                                None, k.ctypes.data_as(vp)) == 0
             assert np.array_equal(k, om.morton_keys(P, level))
+
+
+def test_autotune_api_without_device(tmp_path, monkeypatch):
+    """autotune_for_problem keeps the reference's dry-run and argument rules
+    (fmm3d/autotune.py:100-160); the JSON cache round-trips under a file lock."""
+    import pytest
+    from electrodrive_amd.fmm3d import FmmConfig, autotune_for_problem
+    from electrodrive_amd.fmm3d import autotune as at
+    cfg = FmmConfig(expansion_order=8, leaf_size=64)
+    r = autotune_for_problem(cfg, 1000)
+    assert r.config is cfg and r.metrics["mode"] == 0.0
+    with pytest.raises(ValueError):
+        autotune_for_problem(cfg, 0)
+    assert at._bucket_problem_size(1000) == 1024 and at._bucket_problem_size(1024) == 1024
+    monkeypatch.setenv(at.AUTOTUNE_CACHE_ENV_VAR, str(tmp_path / "c.json"))
+    at._append_record_to_cache("k1", {"expansion_order": 10, "leaf_size": 128, "metrics": {"ms": 1.0}})
+    at._append_record_to_cache("k2", {"expansion_order": 8, "leaf_size": 64, "metrics": {"ms": 2.0}})
+    assert at._load_best_from_cache("k1")["leaf_size"] == 128
+    assert at._load_best_from_cache("k2")["expansion_order"] == 8
+    assert at._load_best_from_cache("missing") is None

@@ -149,3 +149,26 @@ def test_m2l_precisions(prec, bound, order):
         assert _rel(pv.matvec(torch.tensor(sig).cuda()).cpu().numpy(), out) < 2e-6
         bound = 1e-4 if order == 8 else 2e-5
     assert err < bound
+
+
+def test_autotuner_meets_target_and_caches(tmp_path, monkeypatch):
+    """The tuner measures native plans and returns the fastest candidate whose
+    error vs the FP64 dense operator meets the target; a second call reuses
+    the cache."""
+    from electrodrive_amd.core import bem_mesh
+    from electrodrive_amd.fmm3d import FmmConfig, autotune_for_problem
+    from electrodrive_amd.fmm3d import autotune as at
+    monkeypatch.setenv(at.AUTOTUNE_CACHE_ENV_VAR, str(tmp_path / "tune.json"))
+    m = bem_mesh.icosphere(4)
+    cfg = FmmConfig(expansion_order=8, leaf_size=64, mac_theta=0.8)
+    r = autotune_for_problem(cfg, m.n_panels, dry_run=False, target_rel_error=1e-5,
+                             centroids=m.centroids, areas=m.areas)
+    print("tuned", r.config.expansion_order, r.config.leaf_size, r.metrics)
+    assert r.metrics["met_target"] == 1.0 and r.metrics["rel_err"] <= 1e-5
+    assert r.config.leaf_size in at.LEAF_SIZES and r.config.expansion_order >= 6
+    r2 = autotune_for_problem(cfg, m.n_panels, dry_run=False, target_rel_error=1e-3)
+    assert r2.metrics["mode"] == 2.0
+    r3 = autotune_for_problem(cfg, m.n_panels, dry_run=False, target_rel_error=1e-3)
+    assert r3.metrics.get("from_cache") == 1.0
+    assert (r3.config.expansion_order, r3.config.leaf_size) == (r2.config.expansion_order,
+                                                                r2.config.leaf_size)
