@@ -104,7 +104,7 @@ class BemOperator:
                                 max_level=max_level, m2l_precision=m2l_precision,
                                 geom_offsets=None if len(meshes) == 1 else self.geom_offsets)
             if self.near_pairs is not None and self.near_pairs.shape[0] > 0:
-                self.plan.attach_near(self.near_pairs.cpu().numpy(), self.near_w.cpu().numpy())
+                self.plan.attach_near_device(self.near_pairs, self.near_w)
         self.fmm = FmmOperator(self.plan, self.self_corr)
         # Jacobi diagonal of bem_solve: the raw self integral (core/bem.py:2059)
         self.diag = self.I_self.clamp_min(1e-18)

@@ -7,6 +7,8 @@
 #include <map>
 #include <string>
 #include <vector>
+#include <tuple>
+#include <mutex>
 
 #include "ed_plan.h"
 
@@ -22,6 +24,7 @@ int fail(int code, const std::string& msg) {
 
 int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled);
 void gmres_free(ed_plan* P);
+void gmres_invalidate_graph(ed_plan* P);
 
 #define ED_HIP(call)                                                                   \
   do {                                                                                 \
@@ -74,6 +77,45 @@ void plan_free(ed_plan* P) {
     if (p) (void)hipFree(p);
   sai_free(P);
   gmres_free(P);
+}
+
+// M2L translation tables depend only on the order, the lattice offset and
+// the storage precision, never on the geometry: they are computed once per
+// process (FP64 on the host) and reused by every later plan.
+struct OperatorTables {
+  std::vector<float> valu;     // nc x nc, column-major (VALU path)
+  std::vector<float> f32;      // ncp x kp scaled table (FP32 MFMA)
+  std::vector<uint16_t> h16;   // ncp x kp scaled table (BF16 / FP16 MFMA)
+};
+
+static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, int oz, int ncp,
+                                               int kp, const std::vector<double>& asc) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int, int>, OperatorTables> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_tuple(p, prec, ox, oy, oz);
+  auto it = cache.find(key);
+  if (it != cache.end()) return it->second;
+  const int nc = (p + 1) * (p + 1);
+  std::vector<double> Td((size_t)nc * nc);
+  m2l_matrix(p, ox, oy, oz, Td.data());
+  OperatorTables ot;
+  if (prec < 0) {
+    ot.valu.assign((size_t)nc * nc, 0.0f);
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < nc; ++j) ot.valu[(size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
+  } else {
+    if (prec == 0) ot.f32.assign((size_t)ncp * kp, 0.0f);
+    else ot.h16.assign((size_t)ncp * kp, 0);
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < nc; ++j) {
+        const double sc = Td[(size_t)k * nc + j] / (asc[k] * asc[j]);
+        const size_t at = (size_t)k * kp + j;
+        if (prec == 0) ot.f32[at] = (float)sc;
+        else ot.h16[at] = prec == 1 ? f32_to_bf16((float)sc) : f32_to_f16((float)sc);
+      }
+  }
+  return cache.emplace(key, std::move(ot)).first->second;
 }
 
 int plan_upload(ed_plan* P) {
@@ -170,7 +212,6 @@ int plan_upload(ed_plan* P) {
 
   const int R = offset_radius(P->ws), W = 2 * R + 1;
   std::vector<double> Td((size_t)nc * nc);
-  std::vector<float> tv((size_t)std::max(ntab, 1) * nc * nc, 0.0f);
   const int prec = P->m2l_precision;
   int ncp = 0, kp = 0, cols = 0, enabled = 0;
   ED_TRY(m2l_shape(p, prec < 0 ? 0 : prec, &ncp, &kp, &cols, &enabled));
@@ -187,22 +228,19 @@ int plan_upload(ed_plan* P) {
       if (m2 > 0) asc[n2 * n2 + 2 * m2] = f;
     }
   const size_t tsz = (size_t)std::max(ntab, 1) * ncp * kp;
+  const bool need_valu = !enabled || prec < 0;
+  std::vector<float> tv(need_valu ? (size_t)std::max(ntab, 1) * nc * nc : 1, 0.0f);
   std::vector<float> tm(enabled && prec == 0 ? tsz : 1, 0.0f);
   std::vector<uint16_t> th(enabled && prec > 0 ? tsz : 1, 0);
   for (auto& kv : slot_map) {
     const int slot = kv.first, tab = kv.second;
     const int ox = slot % W - R, oy = (slot / W) % W - R, oz = slot / (W * W) - R;
-    m2l_matrix(p, ox, oy, oz, Td.data());
-    for (int k = 0; k < nc; ++k)
-      for (int j = 0; j < nc; ++j) {
-        const double t = Td[(size_t)k * nc + j];
-        tv[(size_t)tab * nc * nc + (size_t)j * nc + k] = (float)t;
-        if (!enabled) continue;
-        const double sc = t / (asc[k] * asc[j]);
-        const size_t at = (size_t)tab * ncp * kp + (size_t)k * kp + j;
-        if (prec == 0) tm[at] = (float)sc;
-        else th[at] = prec == 1 ? f32_to_bf16((float)sc) : f32_to_f16((float)sc);
-      }
+    const OperatorTables& ot = cached_m2l_tables(p, need_valu ? -1 : prec, ox, oy, oz, ncp, kp, asc);
+    if (need_valu) std::memcpy(tv.data() + (size_t)tab * nc * nc, ot.valu.data(), sizeof(float) * nc * nc);
+    else if (prec == 0)
+      std::memcpy(tm.data() + (size_t)tab * ncp * kp, ot.f32.data(), sizeof(float) * ncp * kp);
+    else
+      std::memcpy(th.data() + (size_t)tab * ncp * kp, ot.h16.data(), sizeof(uint16_t) * ncp * kp);
   }
   ED_TRY(upload(&P->t_m2l, tv));
   ED_TRY(upload(&P->t_m2l_mfma, tm));
@@ -461,7 +499,7 @@ int ed_plan_attach_near(ed_plan* P, const int64_t* pairs, const double* w, int64
     es.push_back({inv[i], j, (int32_t)inv[j], w[q]});
   }
   std::stable_sort(es.begin(), es.end(), [](const E& a, const E& b) { return a.row < b.row; });
-  gmres_free(P);  // a captured Arnoldi graph holds the old near arrays
+  gmres_invalidate_graph(P);  // a captured Arnoldi graph holds the old near arrays
   sai_free(P);    // the sparse approximate inverse was fitted to the old operator
   std::vector<int32_t> ptr(n + 1, 0), col(es.size());
   std::vector<float> ww(es.size());

@@ -222,8 +222,11 @@ struct M2LShape {
   static constexpr int NCP = (NC + 31) / 32 * 32;  // output rows, padded
   static constexpr int KP = (NC + 1) / 2 * 2;      // inner dim, padded to K=2
   static constexpr int KS = KP + 1;                // LDS row stride
-  static constexpr int COLS = 256;                  // pairs per tile (block)
-  static constexpr int THREADS = 256;               // 4 waves, 32 columns at a time
+  static constexpr int COLS = 64;                   // pairs per tile (block)
+  static constexpr int THREADS = 256;               // 4 waves: 2 column tiles x 2 row halves
+  // row parts per column tile: at most 2 row blocks (64 accumulators) per wave
+  static constexpr int NRBT = NCP / 32;
+  static constexpr int RSPLIT = NRBT <= 2 ? 1 : (NRBT % 2 == 0 ? NRBT / 2 : NRBT);
   static constexpr bool ENABLED = P <= 12;          // table fits the 160 KiB LDS
 };
 
@@ -279,25 +282,44 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
   __shared__ float ta[NCP * KS];
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
-  const float* Tt = Tm + (size_t)tab * NCP * KP;
-  for (int t = threadIdx.x; t < NCP * KP; t += NT) {
-    const int i = t / KP, k = t - i * KP;
-    ta[i * KS + k] = Tt[t];
+  // table -> LDS with 16-byte loads (NCP * KP is a multiple of 4 and every
+  // table starts 16-byte aligned), unrolled so the loads are in flight together
+  const float4* T4 = reinterpret_cast<const float4*>(Tm + (size_t)tab * NCP * KP);
+  constexpr int N4 = NCP * KP / 4;
+  static_assert((NCP * KP) % 4 == 0, "table size must be a multiple of 4 floats");
+#pragma unroll 8
+  for (int t = threadIdx.x; t < N4; t += NT) {
+    const float4 v = T4[t];
+    const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int e = 4 * t + q, i = e / KP, k = e - i * KP;
+      ta[i * KS + k] = vv[q];
+    }
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
-  constexpr int NRB = NCP / 32;
+  constexpr int RS = S::RSPLIT;
+  constexpr int NRB = NCP / 32 / RS;  // row blocks per wave
   const int ntiles = (cnt + 31) / 32;
-  for (int ct = wave; ct < ntiles; ct += NW) {
+  // work item = (column tile, row part); waves take items round robin
+  for (int item = wave; item < ntiles * RS; item += NW) {
+    const int ct = item / RS, rb0 = (item % RS) * NRB;
     const int col = ct * 32 + li;
-    const int src = col < cnt ? pair_src[beg + col] : -1;
-    // B[k][col] for k = 2s + lk, straight from the (L2-resident) multipoles
+    // B[k][col] for k = 2s + lk, straight from the (L2-resident) multipoles.
+    // Every load is unconditional (a clamped, in-bounds address; masked after)
+    // so the compiler issues all of them before the first wait.
+    const bool cv = col < cnt;
+    const int srow = cv ? pair_src[beg + min(col, cnt - 1)] : pair_src[beg];
+    const float* mrow = M + (size_t)srow * NC;
     float bv[KH];
 #pragma unroll
     for (int s = 0; s < KH; ++s) {
       const int k = 2 * s + lk;
-      bv[s] = (src >= 0 && k < NC) ? M[(size_t)src * NC + k] * ascale[k] : 0.0f;
+      const int kk = min(k, NC - 1);
+      const float v = mrow[kk] * ascale[kk];
+      bv[s] = (cv && k < NC) ? v : 0.0f;
     }
     f32x16 acc[NRB];
 #pragma unroll
@@ -308,7 +330,7 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
     for (int s = 0; s < KH; ++s) {
 #pragma unroll
       for (int rb = 0; rb < NRB; ++rb) {
-        const float a = ta[(rb * 32 + li) * KS + 2 * s + lk];
+        const float a = ta[((rb0 + rb) * 32 + li) * KS + 2 * s + lk];
         acc[rb] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, bv[s], acc[rb], 0, 0, 0);
       }
     }
@@ -318,7 +340,7 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
       for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-          const int row = rb * 32 + q * 8 + lk * 4;
+          const int row = (rb0 + rb) * 32 + q * 8 + lk * 4;
           float4 v = make_float4(acc[rb][4 * q + 0], acc[rb][4 * q + 1], acc[rb][4 * q + 2],
                                  acc[rb][4 * q + 3]);
           *reinterpret_cast<float4*>(yrow + row) = v;
@@ -378,18 +400,24 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   const float inv_sc = 1.0f / sc;
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
-  const T16* Tt = Tm + (size_t)tab * NCP * KP;
-  for (int t = threadIdx.x; t < NCP * KP; t += 256) {
-    const int i = t / KP, k = t - i * KP;
-    ta[i * KS + k] = Tt[t];
+  // table -> LDS, 8 elements (16 bytes) per load; rows are KP (a multiple of
+  // 16) elements long, so a chunk never crosses a row
+  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * NCP * KP);
+  constexpr int N8 = NCP * KP / 8;
+#pragma unroll 4
+  for (int t = threadIdx.x; t < N8; t += 256) {
+    const int e = 8 * t, i = e / KP, k = e - i * KP;
+    *reinterpret_cast<uint4*>(ta + i * KS + k) = T8[t];
   }
   // gathered, scaled multipoles: consecutive threads read consecutive
   // coefficients of one source row
+  // (unconditional clamped loads, masked after: no serialising branches)
+#pragma unroll 8
   for (int t = threadIdx.x; t < COLS * KP; t += 256) {
     const int c = t / KP, k = t - c * KP;
-    float v = 0.0f;
-    if (c < cnt && k < NC) v = M[(size_t)pair_src[beg + c] * NC + k] * ascale[k] * sc;
-    bm[c * KS + k] = (T16)v;
+    const int cc = min(c, cnt - 1), kk = min(k, NC - 1);
+    const float v = M[(size_t)pair_src[beg + cc] * NC + kk] * ascale[kk] * sc;
+    bm[c * KS + k] = (T16)((c < cnt && k < NC) ? v : 0.0f);
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -442,6 +470,7 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
   const float is = ctr[t].w;
   for (int k = threadIdx.x; k < NC; k += blockDim.x) {
     float s = 0.0f;
+#pragma unroll 8
     for (int q = b; q < e; ++q) s += Y[(size_t)trow[q] * NCP + k];
     L[(size_t)t * NC + k] = s * (is * binv[k]);
   }
@@ -464,6 +493,7 @@ __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restric
   const float* Tc = T + (size_t)octant[c] * NC * NC;
   for (int k = threadIdx.x; k < NC; k += blockDim.x) {
     float s = 0.0f;
+#pragma unroll 16
     for (int j = 0; j < NC; ++j) s += Tc[(size_t)j * NC + k] * lp[j];
     L[(size_t)c * NC + k] += s;
   }
@@ -560,7 +590,14 @@ __global__ __launch_bounds__(kLeafThreads) void k_p2p(
       const int s0 = r0 + wave * kSrcChunk;
       const int ns = max(0, min(kSrcChunk, r.y - s0));
       __syncthreads();
-      for (int e = lane; e < ns; e += 64) mysrc[e] = pts[s0 + e];
+      if (ns > 0) {  // ns = 0: s0 may lie past the last panel, no load at all
+        float4 v[kSrcChunk / 64];
+#pragma unroll
+        for (int u = 0; u < kSrcChunk / 64; ++u) v[u] = pts[s0 + min(lane + 64 * u, max(ns - 1, 0))];
+#pragma unroll
+        for (int u = 0; u < kSrcChunk / 64; ++u)
+          if (lane + 64 * u < ns) mysrc[lane + 64 * u] = v[u];
+      }
       __syncthreads();
       if (ns > 0) acc += p2p_tile(mysrc, ns, sl, S, xi, i - s0);
     }
@@ -600,14 +637,26 @@ __global__ __launch_bounds__(kLeafThreads) void k_p2p(
         const int e0 = r0 + wave * kSrcChunk;
         const int ns = max(0, min(kSrcChunk, total - e0));
         __syncthreads();
-        for (int e = lane; e < ns; e += 64) {
-          const int ge = e0 + e;
-          int lo = 0, hi = ng - 1;  // last q with upref[q] <= ge
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (upref[mid] <= ge) lo = mid; else hi = mid - 1;
+        if (ns > 0) {
+          // locate every element of the wave's chunk first, then issue all the
+          // (clamped, unconditional) loads together
+          int gi[kSrcChunk / 64];
+#pragma unroll
+          for (int u = 0; u < kSrcChunk / 64; ++u) {
+            const int ge = e0 + min(lane + 64 * u, max(ns - 1, 0));
+            int lo = 0, hi = ng - 1;  // last q with upref[q] <= ge
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (upref[mid] <= ge) lo = mid; else hi = mid - 1;
+            }
+            gi[u] = ubeg[lo] + (ge - upref[lo]);
           }
-          mysrc[e] = pts[ubeg[lo] + (ge - upref[lo])];
+          float4 v[kSrcChunk / 64];
+#pragma unroll
+          for (int u = 0; u < kSrcChunk / 64; ++u) v[u] = pts[gi[u]];
+#pragma unroll
+          for (int u = 0; u < kSrcChunk / 64; ++u)
+            if (lane + 64 * u < ns) mysrc[lane + 64 * u] = v[u];
         }
         __syncthreads();
         if (ns > 0) acc += p2p_tile(mysrc, ns, sl, S, xi, -1);

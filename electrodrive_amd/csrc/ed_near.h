@@ -27,11 +27,12 @@ struct NearSearch {
   bool any = false;  // false: no pair can exist
   ~NearSearch();
   int build(const double* C, const double* A, int64_t n, double df, hipStream_t st);
-  // row offsets (n + 1, host, exclusive prefix of the row counts)
-  int count(hipStream_t st, std::vector<int64_t>& ptr);
+  // row offsets (n + 1, host, exclusive prefix of the row counts); a
+  // factor df_use <= the build's df searches the same grid (0: the build's)
+  int count(hipStream_t st, std::vector<int64_t>& ptr, double df_use = 0.0);
   // pairs (int64 (i, j)) and/or int32 columns j + col_off at the row offsets
   int fill(hipStream_t st, const std::vector<int64_t>& ptr, int64_t* pairs, int32_t* cols32,
-           int64_t col_off);
+           int64_t col_off, double df_use = 0.0);
 };
 
 }  // namespace ed

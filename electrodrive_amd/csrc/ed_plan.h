@@ -113,6 +113,7 @@ int fmm_launch(ed_plan* P, const void* sigma, int sigma_is_f64, int sigma_tree_o
                const void* self_integrals, void* out, int out_is_f64,
                int out_tree_order, hipStream_t stream);
 void gmres_free(ed_plan* P);
+void gmres_invalidate_graph(ed_plan* P);
 // z = M_sai u (+ Z cy when k > 0), tree order
 void sai_apply_launch(ed_plan* P, const double* u, int k, const double* Z, const double* cy,
                       double* z, hipStream_t st);

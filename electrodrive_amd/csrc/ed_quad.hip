@@ -490,14 +490,15 @@ int NearSearch::build(const double* Cd, const double* Ad, int64_t n_, double df_
   return ED_OK;
 }
 
-int NearSearch::count(hipStream_t st, std::vector<int64_t>& ptr) {
+int NearSearch::count(hipStream_t st, std::vector<int64_t>& ptr, double df_use) {
   ptr.assign((size_t)std::max<int64_t>(n, 0) + 1, 0);
   if (!any) return ED_OK;
+  const double dfu = df_use > 0.0 ? std::min(df_use, df) : df;
   int64_t* counts = nullptr;
   if (hipMalloc((void**)&counts, sizeof(int64_t) * n) != hipSuccess)
     return fail(ED_ERR_ALLOC, "near pairs: alloc");
   const unsigned grid = (unsigned)((n + kNearBlock - 1) / kNearBlock);
-  hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, df, g,
+  hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, dfu, g,
                      (const int32_t*)d_start, (const int32_t*)d_pts, counts,
                      (const int64_t*)nullptr, (int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
   hipError_t e = hipGetLastError();
@@ -510,15 +511,16 @@ int NearSearch::count(hipStream_t st, std::vector<int64_t>& ptr) {
 }
 
 int NearSearch::fill(hipStream_t st, const std::vector<int64_t>& ptr, int64_t* pairs,
-                     int32_t* cols32, int64_t col_off) {
+                     int32_t* cols32, int64_t col_off, double df_use) {
   if (!any || ptr[n] == 0) return ED_OK;
+  const double dfu = df_use > 0.0 ? std::min(df_use, df) : df;
   int64_t* offs = nullptr;
   if (hipMalloc((void**)&offs, sizeof(int64_t) * n) != hipSuccess)
     return fail(ED_ERR_ALLOC, "near pairs: alloc");
   hipError_t e = hipMemcpyAsync(offs, ptr.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) {
     const unsigned grid = (unsigned)((n + kNearBlock - 1) / kNearBlock);
-    hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, df, g,
+    hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, dfu, g,
                        (const int32_t*)d_start, (const int32_t*)d_pts, (int64_t*)nullptr,
                        (const int64_t*)offs, pairs, cols32, col_off);
     e = hipGetLastError();

@@ -217,7 +217,7 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
     return fail(ED_ERR_ARG, "ed_plan_build_sai: need 0 < factor_pattern <= factor_rows < inf");
   hipStream_t st = (hipStream_t)stream;
   const int64_t n = P->n;
-  gmres_free(P);  // a captured Arnoldi graph refers to the preconditioner arrays
+  gmres_invalidate_graph(P);  // a captured Arnoldi graph refers to the preconditioner arrays
   sai_free(P);
   double* selfi_t = nullptr;
   if (hipMalloc((void**)&P->sai_col, sizeof(int32_t) * kSaiJ * n) != hipSuccess ||
@@ -235,13 +235,13 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
   for (size_t gi = 0; gi + 1 < go.size() && rc == ED_OK; ++gi) {
     const int64_t b = go[gi], ng = go[gi + 1] - go[gi];
     if (ng <= 0) continue;
-    NearSearch sj, si;
+    // one cell grid (sized for the larger factor) serves both searches
+    NearSearch si;
     std::vector<int64_t> pj, pi;
     int32_t *cj = nullptr, *ci = nullptr;
     int64_t *dpj = nullptr, *dpi = nullptr;
-    rc = sj.build(P->cent64_t + 3 * b, P->area64_t + b, ng, factor_pattern, st);
-    if (rc == ED_OK) rc = sj.count(st, pj);
-    if (rc == ED_OK) rc = si.build(P->cent64_t + 3 * b, P->area64_t + b, ng, factor_rows, st);
+    rc = si.build(P->cent64_t + 3 * b, P->area64_t + b, ng, factor_rows, st);
+    if (rc == ED_OK) rc = si.count(st, pj, factor_pattern);
     if (rc == ED_OK) rc = si.count(st, pi);
     if (rc == ED_OK &&
         (hipMalloc((void**)&cj, sizeof(int32_t) * std::max<int64_t>(pj[ng], 1)) != hipSuccess ||
@@ -249,7 +249,7 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
          hipMalloc((void**)&dpj, sizeof(int64_t) * (ng + 1)) != hipSuccess ||
          hipMalloc((void**)&dpi, sizeof(int64_t) * (ng + 1)) != hipSuccess))
       rc = fail(ED_ERR_ALLOC, "ed_plan_build_sai: alloc");
-    if (rc == ED_OK) rc = sj.fill(st, pj, nullptr, cj, b);
+    if (rc == ED_OK) rc = si.fill(st, pj, nullptr, cj, b, factor_pattern);
     if (rc == ED_OK) rc = si.fill(st, pi, nullptr, ci, b);
     if (rc == ED_OK) {
       hipError_t e = hipMemcpyAsync(dpj, pj.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
@@ -297,5 +297,131 @@ extern "C" int ed_plan_sai_apply(ed_plan* P, const double* u, double* z, ed_stre
   (void)hipFree(ut);
   (void)hipFree(zt);
   if (e != hipSuccess) return fail(ED_ERR_LAUNCH, std::string("ed_plan_sai_apply: ") + hipGetErrorString(e));
+  return ED_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Near-correction CSR built on the device from device pairs (no reference
+// counterpart: the host path ed_plan_attach_near takes host arrays).  Pairs
+// must be i-major (the order every near-pair builder produces) with valid
+// indices; the result is identical to the host path's.
+// ---------------------------------------------------------------------------
+namespace ed {
+
+__global__ void k_inv_perm(int64_t n, const int32_t* __restrict__ perm, int32_t* __restrict__ inv) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) inv[perm[t]] = (int32_t)t;
+}
+
+__global__ void k_row_bounds(int64_t np, int64_t n, const int64_t* __restrict__ pairs,
+                             int32_t* __restrict__ first, int32_t* __restrict__ last,
+                             int32_t* __restrict__ bad) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int64_t i = pairs[2 * q], j = pairs[2 * q + 1];
+  if (i < 0 || i >= n || j < 0 || j >= n) { atomicOr(bad, 1); return; }
+  const int64_t ip = q > 0 ? pairs[2 * (q - 1)] : -1;
+  const int64_t in = q + 1 < np ? pairs[2 * (q + 1)] : -1;
+  if (q > 0 && ip > i) atomicOr(bad, 2);
+  if (ip != i) first[i] = (int32_t)q;
+  if (in != i) last[i] = (int32_t)(q + 1);
+}
+
+__global__ void k_row_counts(int64_t n, const int32_t* __restrict__ perm,
+                             const int32_t* __restrict__ first, const int32_t* __restrict__ last,
+                             int32_t* __restrict__ cnt) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) cnt[t] = last[perm[t]] - first[perm[t]];
+}
+
+__global__ void k_csr_fill(int64_t np, const int64_t* __restrict__ pairs,
+                           const double* __restrict__ w, const int32_t* __restrict__ inv,
+                           const int32_t* __restrict__ first, const int32_t* __restrict__ ptr,
+                           int32_t* __restrict__ col, float* __restrict__ w32,
+                           double* __restrict__ w64) {
+  const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= np) return;
+  const int64_t i = pairs[2 * q], j = pairs[2 * q + 1];
+  const int64_t dst = ptr[inv[i]] + (q - first[i]);
+  col[dst] = inv[j];
+  w32[dst] = (float)w[q];
+  w64[dst] = w[q];
+}
+
+}  // namespace ed
+
+extern "C" int ed_plan_attach_near_device(ed_plan* P, const int64_t* pairs, const double* w,
+                                          int64_t np, ed_stream stream) {
+  if (!P) return fail(ED_ERR_ARG, "ed_plan_attach_near_device: NULL plan");
+  if (np < 0 || (np > 0 && (!pairs || !w)) || np > (int64_t)INT32_MAX)
+    return fail(ED_ERR_ARG, "ed_plan_attach_near_device: bad pairs");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = P->n;
+  gmres_invalidate_graph(P);
+  sai_free(P);
+  (void)hipFree(P->near_ptr);
+  (void)hipFree(P->near_col);
+  (void)hipFree(P->near_w);
+  (void)hipFree(P->near_w64);
+  P->near_ptr = nullptr; P->near_col = nullptr; P->near_w = nullptr; P->near_w64 = nullptr;
+  P->n_near = 0;
+  int32_t *inv = nullptr, *first = nullptr, *last = nullptr, *cnt = nullptr, *bad = nullptr;
+  int rc = ED_OK;
+  const size_t m = (size_t)std::max<int64_t>(np, 1);
+  if (hipMalloc((void**)&inv, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc((void**)&first, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc((void**)&last, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc((void**)&cnt, sizeof(int32_t) * n) != hipSuccess ||
+      hipMalloc((void**)&bad, sizeof(int32_t)) != hipSuccess ||
+      hipMalloc((void**)&P->near_ptr, sizeof(int32_t) * (n + 1)) != hipSuccess ||
+      hipMalloc((void**)&P->near_col, sizeof(int32_t) * m) != hipSuccess ||
+      hipMalloc((void**)&P->near_w, sizeof(float) * m) != hipSuccess ||
+      hipMalloc((void**)&P->near_w64, sizeof(double) * m) != hipSuccess)
+    rc = fail(ED_ERR_ALLOC, "ed_plan_attach_near_device: alloc");
+  std::vector<int32_t> hcnt((size_t)n), hptr((size_t)n + 1, 0);
+  int32_t hbad = 0;
+  const unsigned gn = (unsigned)((n + 255) / 256), gp = (unsigned)((np + 255) / 256);
+  hipError_t e = hipSuccess;
+  if (rc == ED_OK) {
+    e = hipMemsetAsync(first, 0, sizeof(int32_t) * n, st);
+    if (e == hipSuccess) e = hipMemsetAsync(last, 0, sizeof(int32_t) * n, st);
+    if (e == hipSuccess) e = hipMemsetAsync(bad, 0, sizeof(int32_t), st);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(k_inv_perm, dim3(gn), dim3(256), 0, st, n, P->perm, inv);
+      if (np > 0)
+        hipLaunchKernelGGL(k_row_bounds, dim3(gp), dim3(256), 0, st, np, n, pairs, first, last, bad);
+      hipLaunchKernelGGL(k_row_counts, dim3(gn), dim3(256), 0, st, n, P->perm, first, last, cnt);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipMemcpyAsync(hcnt.data(), cnt, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(&hbad, bad, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = fail(ED_ERR_DEVICE, std::string("ed_plan_attach_near_device: ") + hipGetErrorString(e));
+  }
+  if (rc == ED_OK && hbad)
+    rc = fail(ED_ERR_ARG, (hbad & 1) ? "ed_plan_attach_near_device: index out of range"
+                                     : "ed_plan_attach_near_device: pairs are not i-major");
+  if (rc == ED_OK) {
+    for (int64_t t = 0; t < n; ++t) hptr[t + 1] = hptr[t] + hcnt[t];
+    e = hipMemcpyAsync(P->near_ptr, hptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && np > 0) {
+      hipLaunchKernelGGL(k_csr_fill, dim3(gp), dim3(256), 0, st, np, pairs, w, inv, first,
+                         P->near_ptr, P->near_col, P->near_w, P->near_w64);
+      e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) rc = fail(ED_ERR_LAUNCH, std::string("ed_plan_attach_near_device: ") + hipGetErrorString(e));
+  }
+  (void)hipFree(inv); (void)hipFree(first); (void)hipFree(last); (void)hipFree(cnt); (void)hipFree(bad);
+  if (rc != ED_OK) {
+    // leave the plan with an empty (all-zero) correction
+    (void)hipFree(P->near_ptr); (void)hipFree(P->near_col); (void)hipFree(P->near_w); (void)hipFree(P->near_w64);
+    P->near_ptr = nullptr; P->near_col = nullptr; P->near_w = nullptr; P->near_w64 = nullptr;
+    std::vector<int32_t> z((size_t)n + 1, 0);
+    if (hipMalloc((void**)&P->near_ptr, sizeof(int32_t) * (n + 1)) == hipSuccess)
+      (void)hipMemcpy(P->near_ptr, z.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice);
+    return rc;
+  }
+  P->n_near = np;
   return ED_OK;
 }

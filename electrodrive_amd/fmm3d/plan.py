@@ -102,6 +102,19 @@ class FmmPlan:
                                                 w.ctypes.data_as(ctypes.c_void_p), pairs.shape[0]),
                   "ed_plan_attach_near")
 
+    def attach_near_device(self, pairs, weights):
+        """Near CSR from device tensors: pairs (P, 2) int64 i-major, weights (P,)
+        float64 (the output of ``build_near_pairs`` / ``near_correction_weights``)."""
+        if (not pairs.is_cuda or pairs.dtype != torch.int64 or pairs.dim() != 2
+                or pairs.shape[1] != 2):
+            raise ValueError("pairs must be an int64 device tensor of shape (P, 2)")
+        if not weights.is_cuda or weights.dtype != torch.float64 or weights.shape != (pairs.shape[0],):
+            raise ValueError("weights must be a float64 device tensor of length P")
+        pairs, weights = pairs.contiguous(), weights.contiguous()
+        nat.check(nat.lib().ed_plan_attach_near_device(self.handle, nat.ptr(pairs), nat.ptr(weights),
+                                                       pairs.shape[0], nat.stream_ptr(pairs.device)),
+                  "ed_plan_attach_near_device")
+
     def matvec(self, sigma, self_integrals=None, out=None, stream=None):
         if not sigma.is_cuda or sigma.dim() != 1 or sigma.shape[0] != self.n:
             raise ValueError("sigma must be a 1-D device tensor of length n")

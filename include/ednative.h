@@ -107,6 +107,13 @@ int ed_fmm_profile(ed_plan* plan, const double* sigma, const double* self_integr
  * detaches. */
 int ed_plan_attach_near(ed_plan* plan, const int64_t* pairs_host,
                         const double* weights_host, int64_t n_pairs);
+/* Same CSR from device arrays (no host round trip; no reference counterpart):
+ * pairs (n_pairs, 2) int64 i-major with valid indices (what ed_near_pairs_f64
+ * produces), weights (n_pairs) FP64, both device memory in original order.
+ * Unsorted rows or out-of-range indices -> ED_ERR_ARG and an empty
+ * correction. */
+int ed_plan_attach_near_device(ed_plan* plan, const int64_t* pairs, const double* weights,
+                               int64_t n_pairs, ed_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Dense single-layer matvec: replaces _bem_matvec_core_torch and            */

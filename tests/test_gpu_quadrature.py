@@ -123,3 +123,28 @@ def test_morton_keys_bit_exact(dtype):
         ref = omorton.morton_keys(p, 16, c, h)
         got, _ = morton_keys_from_points(torch.tensor(p).cuda(), level=16, global_bbox=(c, h))
         assert np.array_equal(got.cpu().numpy(), ref)
+
+
+def test_attach_near_device_equals_host_path():
+    """ed_plan_attach_near_device builds the same tree-ordered CSR as the host
+    path: the FMM matvec with either attached correction is bit-identical;
+    unsorted pairs are rejected."""
+    from electrodrive_amd.core import bem_mesh
+    from electrodrive_amd.core import bem_quadrature as bq
+    from electrodrive_amd.fmm3d.plan import FmmPlan
+    m = bem_mesh.torus(1.0, 0.3, 40, 12)
+    dev = torch.device("cuda")
+    C = torch.tensor(m.centroids, device=dev)
+    A = torch.tensor(m.areas, device=dev)
+    PV = torch.tensor(m.panel_vertices, device=dev)
+    pairs = bq.build_near_pairs(C, A, 1.5)
+    w = bq.near_correction_weights(C, A, PV, pairs, quad_order=2, max_depth=2)
+    p1 = FmmPlan(m.centroids, m.areas, leaf_size=32, order=8)
+    p2 = FmmPlan(m.centroids, m.areas, leaf_size=32, order=8)
+    p1.attach_near(pairs.cpu().numpy(), w.cpu().numpy())
+    p2.attach_near_device(pairs, w)
+    sig = torch.tensor(np.random.default_rng(2).normal(size=m.n_panels), device=dev)
+    assert torch.equal(p1.matvec(sig), p2.matvec(sig))
+    assert p1.stats()["n_near_pairs"] == p2.stats()["n_near_pairs"] == pairs.shape[0]
+    with pytest.raises(ValueError):
+        p2.attach_near_device(pairs.flip(0).contiguous(), w)
