@@ -72,7 +72,7 @@ void plan_free(ed_plan* P) {
                   P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
                   P->near_w, P->cent64_t, P->area64_t, P->near_w64, P->m2l_ascale,
                   P->m2l_binv, P->m2l_mmax, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
-                  P->coarse_part, P->coarse_tmp};
+                  P->coarse_part, P->coarse_tmp, P->dense_items, P->dense_rns, P->dense_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   sai_free(P);
@@ -295,6 +295,43 @@ int plan_upload(ed_plan* P) {
     for (int64_t t = 0; t < M; ++t) tptr[t + 1] += tptr[t];
     std::vector<int32_t> fillp(tptr.begin(), tptr.end() - 1);
     for (size_t q = 0; q < ptgt.size(); ++q) trow[fillp[ptgt[q]]++] = (int32_t)q;
+    // XCD-aware launch order: workgroups go to the 8 XCDs round robin
+    // (block b -> XCD b % 8), each XCD with its own L2.  Tables are binned to
+    // XCDs (largest tile count first, into the lightest bin) and block
+    // 8 k + x runs the k-th tile of bin x, so every tile of a table reads it
+    // from the same L2.  Bins are padded with empty tiles (count 0).
+    {
+      constexpr int kXcd = 8;
+      std::vector<std::vector<int32_t>> tiles_of(ntab);
+      for (size_t t = 0; t < ttab.size(); ++t) tiles_of[ttab[t]].push_back((int32_t)t);
+      std::vector<int32_t> order(ntab);
+      for (int t = 0; t < ntab; ++t) order[t] = t;
+      std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+        return tiles_of[a].size() > tiles_of[b].size();
+      });
+      std::vector<std::vector<int32_t>> bins(kXcd);
+      for (int32_t tab : order) {
+        int best = 0;
+        for (int x = 1; x < kXcd; ++x)
+          if (bins[x].size() < bins[best].size()) best = x;
+        for (int32_t t : tiles_of[tab]) bins[best].push_back(t);
+      }
+      size_t depth = 0;
+      for (auto& b : bins) depth = std::max(depth, b.size());
+      std::vector<int32_t> xt, xb, xc;
+      for (size_t k = 0; k < depth; ++k)
+        for (int x = 0; x < kXcd; ++x) {
+          if (k < bins[x].size()) {
+            const int32_t t = bins[x][k];
+            xt.push_back(ttab[t]); xb.push_back(tbeg[t]); xc.push_back(tcnt[t]);
+          } else {
+            xt.push_back(0); xb.push_back(0); xc.push_back(0);
+          }
+        }
+      ttab.swap(xt);
+      tbeg.swap(xb);
+      tcnt.swap(xc);
+    }
     ED_TRY(upload(&P->m2l_pair_src, psrc));
     ED_TRY(upload(&P->m2l_pair_tgt, ptgt));
     ED_TRY(upload(&P->m2l_tile_tab, ttab));

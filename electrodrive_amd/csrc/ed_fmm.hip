@@ -282,6 +282,7 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
   __shared__ float ta[NCP * KS];
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
+  if (cnt == 0) return;  // padding tile of the XCD-aware order
   // table -> LDS with 16-byte loads (NCP * KP is a multiple of 4 and every
   // table starts 16-byte aligned), unrolled so the loads are in flight together
   const float4* T4 = reinterpret_cast<const float4*>(Tm + (size_t)tab * NCP * KP);
@@ -386,6 +387,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
   __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
   __shared__ float red[256];
+  if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
   // power-of-two input scale from the per-block maxima of |M * ascale|
   float mx = 0.0f;
   for (int t = threadIdx.x; t < kAbsMaxBlocks; t += blockDim.x) mx = fmaxf(mx, mmax[t]);

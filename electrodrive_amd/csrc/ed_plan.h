@@ -6,6 +6,10 @@
 
 #include "ed_internal.h"
 
+namespace ed {
+struct DenseItem;
+}
+
 struct ed_plan {
   int device = 0;
   int p = 8, nc = 81, ws = 1, leaf_size = 64, max_level = 16;
@@ -92,6 +96,11 @@ struct ed_plan {
   double* coarse_cy = nullptr;
   double* coarse_part = nullptr;
   double* coarse_tmp = nullptr;   // n, coarse solve staging
+  // split-form dense FP64 operator of the refinement (built on first use)
+  ed::DenseItem* dense_items = nullptr;
+  int64_t dense_nitems = 0;
+  int32_t* dense_rns = nullptr;   // n, source splits of each row
+  double* dense_part = nullptr;   // max_split x n partial sums
   // sparse approximate inverse smoother (column-major ELL, 32 slots per row)
   int32_t* sai_col = nullptr;
   double* sai_val = nullptr;
@@ -105,6 +114,20 @@ struct ed_plan {
 };
 
 namespace ed {
+// Work item of the split dense operator: up to 512 target rows and one
+// contiguous source range of their geometry.
+struct DenseItem {
+  int64_t row0;
+  int32_t nrows;
+  int32_t split;
+  int64_t src0, src1;
+};
+void dense_schedule(const std::vector<int64_t>& off, std::vector<DenseItem>& items,
+                    std::vector<int32_t>& row_nsplit, int* max_split);
+template <typename T>
+int dense_run(const DenseItem* items, int64_t n_items, const int32_t* row_nsplit, int max_split,
+              const T* C, const T* A, const T* sigma, const T* selfi, T* out, int64_t n,
+              int64_t nrhs, T* partial, hipStream_t st);
 int plan_upload(ed_plan* P);
 void plan_free(ed_plan* P);
 // FMM matvec with tree-order q/sigma/self already in place; writes the result
