@@ -66,7 +66,7 @@ static int upload(T** dst, const std::vector<T>& src) {
 void plan_free(ed_plan* P) {
   void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->perm, P->node_ctr, P->node_rng,
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
-                  P->leaves, P->pt_node, P->p2p, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
+                  P->leaves, P->pt_node, P->p2p, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
                   P->m2l_pair_src, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair, P->t_m2l,
                   P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
@@ -152,6 +152,7 @@ int plan_upload(ed_plan* P) {
   ED_HIP(hipMalloc((void**)&P->L, sizeof(float) * (size_t)M * nc));
   ED_HIP(hipMemset(P->M, 0, sizeof(float) * (size_t)M * nc));
   ED_HIP(hipMemset(P->L, 0, sizeof(float) * (size_t)M * nc));
+  ED_HIP(hipMalloc((void**)&P->ym2m, sizeof(float) * (size_t)M * nc));
 
   // leaves and U lists
   std::vector<int32_t> leaves, uptr{0}, usrc;

@@ -145,6 +145,44 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
 // M2M: one block per parent; M_parent = sum_c T_m2m[oct(c)] M_c
 // T stored column-major: T[oct][j][k] (input j, output k)
 // ---------------------------------------------------------------------------
+// M2M in two passes per level: one block per child applies its octant's
+// operator (Y_c = T_oct(c) M_c; 121 loads per thread instead of 8 x 121),
+// then one block per parent sums its children in a fixed order.
+template <int P>
+__global__ __launch_bounds__(128) void k_m2m_child(int child0, const int32_t* __restrict__ octant,
+                                                   const float* __restrict__ T,
+                                                   const float* __restrict__ M,
+                                                   float* __restrict__ Y) {
+  constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float mc[NC];
+  const int c = child0 + blockIdx.x;
+  for (int j = threadIdx.x; j < NC; j += blockDim.x) mc[j] = M[(size_t)c * NC + j];
+  __syncthreads();
+  const float* Tc = T + (size_t)octant[c] * NC * NC;
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
+    float s = 0.0f;
+#pragma unroll 16
+    for (int j = 0; j < NC; ++j) s += Tc[(size_t)j * NC + k] * mc[j];
+    Y[(size_t)blockIdx.x * NC + k] = s;
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(128) void k_m2m_sum(const int32_t* __restrict__ parents,
+                                                 const int32_t* __restrict__ first_child,
+                                                 const int32_t* __restrict__ n_child, int child0,
+                                                 const float* __restrict__ Y,
+                                                 float* __restrict__ M) {
+  constexpr int NC = (P + 1) * (P + 1);
+  const int par = parents[blockIdx.x];
+  const int c0 = first_child[par] - child0, nch = n_child[par];
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
+    float s = 0.0f;
+    for (int c = 0; c < nch; ++c) s += Y[(size_t)(c0 + c) * NC + k];
+    M[(size_t)par * NC + k] = s;
+  }
+}
+
 template <int P>
 __global__ __launch_bounds__(128) void k_m2m(const int32_t* __restrict__ parents,
                                              const int32_t* __restrict__ first_child,
@@ -528,38 +566,49 @@ __device__ __forceinline__ int wave_incl_scan(int v) {
 
 // Sum of q_j / r_ij over one LDS tile for a lane of slice `sl` of `S` (points
 // j = sl, sl + S, ...).  The step count is uniform across the wave and the
-// loop is unrolled 8-wide so the LDS reads of a group are in flight together;
-// out-of-range steps read a clamped index and j == me (the target itself)
-// gets r^2 = inf, so both add q * rsq(inf) = 0 without a branch.
+// loop is unrolled U-wide so the LDS reads of a group are in flight together.
+// U-list tiles (SELF = false) are padded past `ns` with zero-charge points, so
+// the loop needs neither index clamps nor masks.  The target's own tile
+// (SELF) clamps the index and sets r^2 = inf for j >= ns and j == me, so both
+// add q * rsq(inf) = 0 without a branch.
+constexpr int kP2PUnroll = 8;
+constexpr int kP2PMaxSlices = 8;
+constexpr int kP2PPad = (kP2PUnroll + 1) * kP2PMaxSlices;
+
+template <bool SELF>
 __device__ __forceinline__ float p2p_tile(const float4* __restrict__ tile, int ns, int sl, int S,
                                           float4 xi, int me) {
-  constexpr int U = 8;
+  constexpr int U = kP2PUnroll;
   float acc = 0.0f;
   const int nsteps = (ns + S - 1) / S;
   int t = 0;
-  for (; t + U <= nsteps; t += U) {
+  for (; t + U <= nsteps + (SELF ? 0 : U - 1); t += U) {
     float4 sv[U];
-    int jv[U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      jv[u] = sl + (t + u) * S;
-      sv[u] = tile[min(jv[u], ns - 1)];
+      const int j = sl + (t + u) * S;
+      sv[u] = tile[SELF ? min(j, ns - 1) : j];
     }
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       const float dx = xi.x - sv[u].x, dy = xi.y - sv[u].y, dz = xi.z - sv[u].z;
       float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-      r2 = (jv[u] < ns && jv[u] != me) ? r2 : INFINITY;  // rsq(inf) = 0
+      if (SELF) {
+        const int j = sl + (t + u) * S;
+        r2 = (j < ns && j != me) ? r2 : INFINITY;  // rsq(inf) = 0
+      }
       acc = fmaf(sv[u].w, __builtin_amdgcn_rsqf(r2), acc);
     }
   }
-  for (; t < nsteps; ++t) {
-    const int j = sl + t * S;
-    const float4 sj = tile[min(j, ns - 1)];
-    const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
-    float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-    r2 = (j < ns && j != me) ? r2 : INFINITY;
-    acc = fmaf(sj.w, __builtin_amdgcn_rsqf(r2), acc);
+  if (SELF) {
+    for (; t < nsteps; ++t) {
+      const int j = sl + t * S;
+      const float4 sj = tile[min(j, ns - 1)];
+      const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
+      float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
+      r2 = (j < ns && j != me) ? r2 : INFINITY;
+      acc = fmaf(sj.w, __builtin_amdgcn_rsqf(r2), acc);
+    }
   }
   return acc;
 }
@@ -568,7 +617,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_p2p(
     const int32_t* __restrict__ leaves, const int2* __restrict__ rng,
     const float4* __restrict__ pts, const int32_t* __restrict__ u_ptr,
     const int32_t* __restrict__ u_src, float* __restrict__ p2p) {
-  __shared__ float4 src[kLeafWaves][kSrcChunk];
+  __shared__ float4 src[kLeafWaves][kSrcChunk + kP2PPad];
   __shared__ int ubeg[kUSeg];
   __shared__ int upref[kUSeg + 1];
   __shared__ int utotal;
@@ -581,7 +630,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_p2p(
   float4* mysrc = src[wave];
   for (int t0 = r.x; t0 < r.y; t0 += 64) {
     const int ntc = min(64, r.y - t0);
-    const int S = 64 / ntc;
+    const int S = min(64 / ntc, kP2PMaxSlices);
     const int tl = lane % ntc, sl = lane / ntc;
     const bool act = sl < S;
     const int i = t0 + tl;
@@ -601,7 +650,7 @@ __global__ __launch_bounds__(kLeafThreads) void k_p2p(
           if (lane + 64 * u < ns) mysrc[lane + 64 * u] = v[u];
       }
       __syncthreads();
-      if (ns > 0) acc += p2p_tile(mysrc, ns, sl, S, xi, i - s0);
+      if (ns > 0) acc += p2p_tile<true>(mysrc, ns, act ? sl : 0, S, xi, i - s0);
     }
     // (2) the other U-list leaves as one concatenated stream
     for (int g0 = ub; g0 < ue; g0 += kUSeg) {
@@ -659,9 +708,11 @@ __global__ __launch_bounds__(kLeafThreads) void k_p2p(
 #pragma unroll
           for (int u = 0; u < kSrcChunk / 64; ++u)
             if (lane + 64 * u < ns) mysrc[lane + 64 * u] = v[u];
+          // zero-charge padding read by the unmasked loop past ns
+          for (int e = ns + lane; e < ns + kP2PPad; e += 64) mysrc[e] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         __syncthreads();
-        if (ns > 0) acc += p2p_tile(mysrc, ns, sl, S, xi, -1);
+        if (ns > 0) acc += p2p_tile<false>(mysrc, ns, act ? sl : 0, S, xi, -1);
       }
     }
     __syncthreads();
@@ -749,9 +800,12 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     for (int l = depth - 1; l >= 2; --l) {
       const int64_t pb = Pn->parents_begin[l], pe = Pn->parents_begin[l + 1];
       if (pe > pb) {
-        hipLaunchKernelGGL(k_m2m<P>, dim3((unsigned)(pe - pb)), dim3(128), 0, st,
-                           Pn->parents_list + pb, Pn->first_child, Pn->n_child,
-                           Pn->node_octant, Pn->t_m2m, Pn->M);
+        const int64_t cb = H.level_begin[l + 1], ce = H.level_begin[l + 2];
+        hipLaunchKernelGGL(k_m2m_child<P>, dim3((unsigned)(ce - cb)), dim3(128), 0, st, (int)cb,
+                           Pn->node_octant, Pn->t_m2m, Pn->M, Pn->ym2m);
+        hipLaunchKernelGGL(k_m2m_sum<P>, dim3((unsigned)(pe - pb)), dim3(128), 0, st,
+                           Pn->parents_list + pb, Pn->first_child, Pn->n_child, (int)cb,
+                           Pn->ym2m, Pn->M);
         ED_LAUNCH_CHECK("m2m");
       }
     }

@@ -37,6 +37,7 @@ struct ed_plan {
   // leaves (node ids) and their U lists (CSR over leaf ordinal)
   int32_t* pt_node = nullptr;   // n, leaf node of each tree-ordered panel
   float* p2p = nullptr;         // n, P2P partial sums (tree order)
+  float* ym2m = nullptr;        // n_nodes x (p+1)^2, per-child M2M products
   int32_t* leaves = nullptr;
   int32_t* u_ptr = nullptr;
   int32_t* u_src = nullptr;  // node ids
