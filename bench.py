@@ -33,9 +33,11 @@ MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 VALU_FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0, the factors a 256 MiB device copy reads), keyed by stage
-# and panel count of the measured workload; see profiles/
-TRAFFIC_BYTES = {}
+# WRITE_SIZE x 1.0, the factors a 256 MiB device copy reads) for the headline
+# workload (81,920 panels, order 10, leaf 128); summary and commands in
+# profiles/pmc_traffic_icosphere81920_p10_leaf128.md
+TRAFFIC_BYTES = {"p2p": (81920, 1.1373e7, "profiles/pmc_traffic_icosphere81920_p10_leaf128.md"),
+                 "m2l": (81920, 6.266e7, "profiles/pmc_traffic_icosphere81920_p10_leaf128.md")}
 
 
 def parse():
@@ -54,9 +56,11 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precond", default="sai", choices=["sai", "jacobi"])
     ap.add_argument("--no-learned", action="store_true")
-    ap.add_argument("--workload", default="sphere", choices=["sphere", "batch", "torus"],
+    ap.add_argument("--workload", default="sphere", choices=["sphere", "batch", "torus", "pmc"],
                     help="sphere: 81,920-panel icosphere (default); batch: 64 geometries x 8 "
-                         "RHS; torus: ~300k-panel torus with 16-bit M2L")
+                         "RHS; torus: ~300k-panel torus with 16-bit M2L; pmc: a short run for "
+                         "rocprofv3 --pmc (matvecs of the sphere workload, then a 256 MiB "
+                         "device copy as the calibration kernel of known size)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="use --order/--leaf as given instead of the autotuner's choice")
     ap.add_argument("--tune-target", type=float, default=1e-5,
@@ -185,7 +189,8 @@ def main():
         return 2
     import torch
     torch.cuda.set_device(torch.device("cuda", 0))
-    line = {"sphere": run_sphere, "batch": run_batch, "torus": run_torus}[a.workload](a)
+    line = {"sphere": run_sphere, "batch": run_batch, "torus": run_torus,
+            "pmc": run_pmc}[a.workload](a)
     print(json.dumps(line))
     return 0
 
@@ -256,14 +261,18 @@ def run_sphere(a):
             from electrodrive_amd.learn import precond as lp_mod
             lp = lp_mod.LearnedPreconditioner()
             ce = 1  # exact iteration counts
-            (xp, ip), tp = timed(lambda: op.solve(b, tol=1e-8, precond=True, check_every=ce))
-            (xs, is_), ts = timed(lambda: op.solve(b, tol=1e-8, precond="sai", check_every=ce))
-            (xf, if_), tf = timed(lambda: lp_mod.baseline_solve(op, b, tol=1e-8, precond="sai",
-                                                                check_every=ce))
-            (xl, il), tl = timed(lambda: lp.solve(op, charges=[(1.0, tuple(r0))], tol=1e-8,
-                                                  precond="sai", check_every=ce))
-            (xlj, ilj), tlj = timed(lambda: lp.solve(op, charges=[(1.0, tuple(r0))], tol=1e-8,
-                                                     precond=True, check_every=ce))
+
+            def warm(fn):  # second of two calls: warm workspace and graph
+                fn()
+                return timed(fn)
+            (xp, ip), tp = warm(lambda: op.solve(b, tol=1e-8, precond=True, check_every=ce))
+            (xs, is_), ts = warm(lambda: op.solve(b, tol=1e-8, precond="sai", check_every=ce))
+            (xf, if_), tf = warm(lambda: lp_mod.baseline_solve(op, b, tol=1e-8, precond="sai",
+                                                               check_every=ce))
+            (xl, il), tl = warm(lambda: lp.solve(op, charges=[(1.0, tuple(r0))], tol=1e-8,
+                                                 precond="sai", check_every=ce))
+            (xlj, ilj), tlj = warm(lambda: lp.solve(op, charges=[(1.0, tuple(r0))], tol=1e-8,
+                                                    precond=True, check_every=ce))
 
             def leg(x_, i_, t_):
                 rr = float(torch.linalg.norm(b - op.accurate(x_)) / torch.linalg.norm(b))
@@ -486,6 +495,27 @@ def run_torus(a):
                  "timed_wall_s": head["wall"]})
     return line
 
+
+def run_pmc(a):
+    """Counter run: --steps matvecs of the sphere workload (order/leaf as given)
+    and three copies of a 256 MiB device buffer, nothing else timed."""
+    import numpy as np
+    import torch
+    from electrodrive_amd.core import bem_mesh
+    from electrodrive_amd.core.operator import BemOperator
+    mesh = bem_mesh.icosphere(a.subdiv)
+    op = BemOperator(mesh, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
+                     near=True, self_depth=3, near_depth=3)
+    sig = torch.as_tensor(np.random.default_rng(a.seed).normal(size=op.n), dtype=torch.float64,
+                          device="cuda")
+    out = torch.empty_like(sig)
+    (_, t_mv) = timed(lambda: [op.plan.matvec(sig, op.self_corr, out=out) for _ in range(a.steps)])
+    src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    (_, t_cp) = timed(lambda: [dst.copy_(src) for _ in range(3)])
+    return {"workload": "pmc", "panels": op.n, "order": a.order, "leaf": a.leaf,
+            "matvecs": a.steps, "matvec_s": t_mv, "copy_bytes": 2 * src.numel() * 4,
+            "copies": 3, "copy_s": t_cp}
 
 if __name__ == "__main__":
     sys.exit(main())
