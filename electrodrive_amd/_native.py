@@ -74,6 +74,10 @@ _SIGS = {
     "ed_plan_coarse_solve": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
     "ed_plan_build_sai": (c_i32, [c_vp, c_vp, ctypes.c_double, ctypes.c_double, c_vp]),
     "ed_plan_attach_near_device": (c_i32, [c_vp, c_vp, c_vp, c_i64, c_vp]),
+    "ed_targets_f64": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "ed_targets_f32": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp]),
+    "ed_near_targets_f64": (c_i32, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp, c_i64, ctypes.c_double,
+                                    c_i32, c_i32, c_vp, c_vp]),
     "ed_plan_sai_apply": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
     "ed_vec_dot_f64": (c_i32, [c_vp, c_vp, c_i64, c_vp, ctypes.POINTER(c_dbl), c_vp]),
     "ed_vec_axpy_f64": (c_i32, [c_dbl, c_vp, c_vp, c_i64, c_vp]),

@@ -99,3 +99,55 @@ def bem_matvec_batched(sigma, src_centroids, areas, *, tile_size: int = 1024,
     if not isinstance(ker, LaplaceSingleLayerKernel):
         raise ValueError("only the Laplace single-layer kernel is implemented natively")
     return dense_matvec(sigma, src_centroids, areas, self_integrals)
+
+
+def _targets(targets, src_centroids, areas, sigma, want_v, want_e):
+    nat.require_gpu()
+    dt = sigma.dtype if sigma.dtype in (torch.float32, torch.float64) else torch.float64
+    dev = sigma.device if sigma.is_cuda else torch.device("cuda", torch.cuda.current_device())
+    X = torch.as_tensor(targets).to(dev, dt).reshape(-1, 3).contiguous()
+    C = torch.as_tensor(src_centroids).to(dev, dt).reshape(-1, 3).contiguous()
+    A = torch.as_tensor(areas).to(dev, dt).reshape(-1).contiguous()
+    s = sigma.to(dev, dt).reshape(-1).contiguous()
+    M, N = X.shape[0], s.shape[0]
+    if C.shape[0] != N or A.shape[0] != N:
+        raise ValueError("src_centroids, areas and sigma must have the same length")
+    V = torch.empty(M, dtype=dt, device=dev) if want_v else None
+    E = torch.empty((M, 3), dtype=dt, device=dev) if want_e else None
+    fn = nat.lib().ed_targets_f64 if dt == torch.float64 else nat.lib().ed_targets_f32
+    with torch.cuda.device(dev):
+        nat.check(fn(nat.ptr(X), M, nat.ptr(C), nat.ptr(A), nat.ptr(s), N, nat.ptr(V), nat.ptr(E),
+                     nat.stream_ptr(dev)), "ed_targets")
+    out_dev = sigma.device
+    return (None if V is None else V.to(out_dev)), (None if E is None else E.to(out_dev))
+
+
+def bem_potential_targets(targets, src_centroids, areas, sigma, tile_size: int = 1024, *,
+                          logger=None, use_keops: bool = False, kernel=None,
+                          backend: str = "auto"):
+    """Induced potential at arbitrary targets (core/bem_kernel.py:818-918):
+    V_i = sum_j K_E sigma_j A_j rsqrt(max(|x_i - c_j|^2, 1e-24)) on the native
+    target kernel; same backends and errors as the reference."""
+    if backend not in {"auto", "torch_tiled", "keops"}:
+        raise ValueError(f"Unknown backend for bem_potential_targets: {backend!r}")
+    ker = kernel or DEFAULT_SINGLE_LAYER_KERNEL
+    if not isinstance(ker, LaplaceSingleLayerKernel):
+        raise ValueError("only the Laplace single-layer kernel is implemented natively")
+    _ensure_tile_size(tile_size)
+    del use_keops, logger
+    return _targets(targets, src_centroids, areas, sigma, True, False)[0]
+
+
+def bem_E_field_targets(targets, src_centroids, areas, sigma, tile_size: int = 1024, *,
+                        logger=None, use_keops: bool = False, kernel=None,
+                        backend: str = "auto"):
+    """Induced E-field at arbitrary targets (core/bem_kernel.py:921-1008):
+    E_i = sum_j K_E sigma_j A_j (x_i - c_j) rsqrt(max(r^2, 1e-24))^3, shape (M, 3)."""
+    if backend not in {"auto", "torch_tiled", "keops"}:
+        raise ValueError(f"Unknown backend for bem_E_field_targets: {backend!r}")
+    ker = kernel or DEFAULT_SINGLE_LAYER_KERNEL
+    if not isinstance(ker, LaplaceSingleLayerKernel):
+        raise ValueError("only the Laplace single-layer kernel is implemented natively")
+    _ensure_tile_size(tile_size)
+    del use_keops, logger
+    return _targets(targets, src_centroids, areas, sigma, False, True)[1]

@@ -119,3 +119,32 @@ def near_quadrature_matvec(V_far, sigma, centroids, areas, panel_vertices, near_
                                           n, int(quad_order), nat.ptr(out), nat.stream_ptr(dev)),
               "ed_near_apply_f64")
     return out.to(V_far.device, V_far.dtype)
+
+
+def near_quadrature_potentials(V_far, targets, centroids, areas, sigma, panel_vertices, *,
+                               distance_factor: float, quad_order: int = 2, cache=None,
+                               max_depth=None):
+    """Near-field correction of target potentials (core/bem.py:940-1047): for
+    panels j with |c_j - x_i| <= distance_factor R_j, A_j > 0, sigma_j != 0,
+    V_i += sigma_j (I_near - K_E A_j / max(|c_j - x_i|, 1e-12)), I_near refined
+    ``max_depth`` times toward x_i (None: 4, the reference default).  ``cache``
+    is accepted for signature parity (the native path needs none)."""
+    del cache
+    if quad_order not in (1, 2):
+        raise ValueError("only quad_order 1 or 2 are supported")
+    depth = 4 if max_depth is None else int(max_depth)
+    dev = V_far.device if V_far.is_cuda else _dev()
+    V = V_far.to(dev, torch.float64).contiguous().clone()
+    X = _f64(targets, dev).reshape(-1, 3)
+    C = _f64(centroids, dev).reshape(-1, 3)
+    A = _f64(areas, dev).reshape(-1)
+    PV = _f64(panel_vertices, dev).reshape(-1, 3, 3)
+    s = _f64(sigma, dev).reshape(-1)
+    if X.shape[0] == 0 or s.shape[0] == 0:
+        return V_far
+    nat.check(nat.lib().ed_near_targets_f64(nat.ptr(X), X.shape[0], nat.ptr(C), nat.ptr(A),
+                                            nat.ptr(PV), nat.ptr(s), s.shape[0],
+                                            float(distance_factor), depth, int(quad_order),
+                                            nat.ptr(V), nat.stream_ptr(dev)),
+              "ed_near_targets_f64")
+    return V.to(V_far.device, V_far.dtype)

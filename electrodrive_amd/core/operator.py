@@ -203,6 +203,43 @@ class BemOperator:
                                refine=refine, max_refine=max_refine, inner_tol=inner_tol,
                                use_graph=use_graph, check_every=check_every)
 
+    # -- post-solve evaluation (core/bem.py:1441-1522) -------------------------
+    def free_space(self, points, charges, field=False):
+        """Potential (or field) of point charges at points: K_E q / max(r, 1e-12)
+        (core/bem.py:291-320) or K_E q (x - r_k) / max(r, 1e-12)^3 (:340-371)."""
+        P = torch.as_tensor(points, dtype=torch.float64).to(self.device).reshape(-1, 3)
+        out = torch.zeros((P.shape[0], 3) if field else P.shape[0], dtype=torch.float64,
+                          device=self.device)
+        for q, pos in charges:
+            p = torch.tensor(pos, dtype=torch.float64, device=self.device)
+            R = P - p[None, :]
+            r = torch.linalg.norm(R, dim=1).clamp_min(1e-12)
+            out = out + ((K_E * float(q)) * R / (r ** 3)[:, None] if field
+                         else (K_E * float(q)) / r)
+        return out
+
+    def potential_at(self, points, sigma, charges=(), near=True, near_depth=None,
+                     distance_factor=1.5, quad_order=2):
+        """V_total = V_free + V_induced at arbitrary points, the induced part by
+        the native target kernel plus (``near``) the near-singular correction
+        of core/bem.py:940-1047."""
+        from .bem_kernel import bem_potential_targets
+        P = torch.as_tensor(points, dtype=torch.float64).to(self.device).reshape(-1, 3)
+        s = sigma.to(self.device, torch.float64)
+        V = bem_potential_targets(P, self.C, self.A, s)
+        if near:
+            V = bq.near_quadrature_potentials(V, P, self.C, self.A, s, self.PV,
+                                              distance_factor=distance_factor,
+                                              quad_order=quad_order, max_depth=near_depth)
+        return V + self.free_space(P, charges)
+
+    def field_at(self, points, sigma, charges=()):
+        """E_total = E_free + E_induced at arbitrary points (core/bem.py:1497-1520)."""
+        from .bem_kernel import bem_E_field_targets
+        P = torch.as_tensor(points, dtype=torch.float64).to(self.device).reshape(-1, 3)
+        E = bem_E_field_targets(P, self.C, self.A, sigma.to(self.device, torch.float64))
+        return E + self.free_space(P, charges, field=True)
+
     def total_charge(self, sigma):
         return float((sigma.to(torch.float64) * self.A).sum())
 

@@ -131,6 +131,27 @@ int ed_dense_matvec_f32(const float* centroids, const float* areas,
 /* ------------------------------------------------------------------------ */
 /* Quadrature                                                                */
 /* ------------------------------------------------------------------------ */
+/* Induced potential V (m) and field E (m, 3) at arbitrary targets (m, 3) from
+ * panel charges: V_i = sum_j K_E sigma_j A_j rsqrt(max(r^2, 1e-24)),
+ * E_i = sum_j K_E sigma_j A_j (x_i - c_j) rsqrt(max(r^2, 1e-24))^3, non-finite
+ * results -> 0 (bem_potential_targets / bem_E_field_targets,
+ * core/bem_kernel.py:233-335, :818-1008).  V or E may be NULL (not
+ * computed).  Device arrays, original order. */
+int ed_targets_f64(const double* targets, int64_t m, const double* centroids,
+                   const double* areas, const double* sigma, int64_t n, double* V, double* E,
+                   ed_stream stream);
+int ed_targets_f32(const float* targets, int64_t m, const float* centroids, const float* areas,
+                   const float* sigma, int64_t n, float* V, float* E, ed_stream stream);
+/* Near-singular correction of target potentials (core/bem.py:940-1047): for
+ * panels with |c_j - x_i| <= factor R_j, A_j > 0, sigma_j != 0,
+ * V_i += sigma_j (I_near - K_E A_j / max(|c_j - x_i|, 1e-12)), I_near the
+ * panel integral refined `depth` (0..6) times toward x_i with the
+ * order-`order` rule.  V (m) is updated in place; device FP64. */
+int ed_near_targets_f64(const double* targets, int64_t m, const double* centroids,
+                        const double* areas, const double* panel_vertices, const double* sigma,
+                        int64_t n, double factor, int32_t depth, int32_t order, double* V,
+                        ed_stream stream);
+
 /* Self integrals I_i = sum_k K_E w_k / max(|c_i - y_k|, 1e-12) over the panel
  * refined `depth` times toward its centroid with the order-`order` rule
  * (core/bem.py:1777-1816, core/bem_quadrature.py:217-308).  panel_vertices

@@ -62,3 +62,18 @@ def direct_potential(targets, centroids, areas, sigma):
         r2 = np.maximum((d * d).sum(-1), 1e-24)
         out[i0:i0 + 1024] = (K_E / np.sqrt(r2)) @ sA
     return out
+
+
+def direct_E_field(targets, centroids, areas, sigma):
+    """core/bem_kernel.py:280-335:
+    E(x_i) = sum_j K_E sigma_j A_j (x_i - c_j) / max(|x_i - c_j|^2, 1e-24)^(3/2)."""
+    T = np.asarray(targets, dtype=float)
+    C = np.asarray(centroids, dtype=float)
+    sA = np.asarray(sigma, dtype=float) * np.asarray(areas, dtype=float)
+    out = np.zeros((T.shape[0], 3))
+    for i0 in range(0, T.shape[0], 512):
+        d = T[i0:i0 + 512, None, :] - C[None, :, :]
+        r2 = np.maximum((d * d).sum(-1), 1e-24)
+        w = K_E * sA[None, :] / (r2 * np.sqrt(r2))
+        out[i0:i0 + 512] = np.einsum("tn,tnk->tk", w, d)
+    return out

@@ -207,3 +207,30 @@ def near_apply(V_far, sigma, pairs, weights):
     s = np.asarray(sigma, dtype=float)
     np.add.at(V, pairs[:, 0], s[pairs[:, 1]] * np.asarray(weights, dtype=float))
     return V
+
+
+def near_target_potentials(V_far, targets, centroids, areas, sigma, panel_vertices,
+                           distance_factor, quad_order=2, max_depth=None):
+    """core/bem.py:940-1047: near-singular correction of target potentials.
+    For every target and every panel with |c_j - x| <= f R_j (R = sqrt(A/pi)),
+    A_j > 0 and sigma_j != 0, replace the lumped term K_E A_j / max(r, 1e-12)
+    by the refined panel integral toward the target."""
+    V = np.array(V_far, dtype=float, copy=True)
+    T = np.asarray(targets, dtype=float).reshape(-1, 3)
+    C = np.asarray(centroids, dtype=float)
+    A = np.asarray(areas, dtype=float).reshape(-1)
+    S = np.asarray(sigma, dtype=float).reshape(-1)
+    PV = np.asarray(panel_vertices, dtype=float).reshape(-1, 3, 3)
+    radii = np.sqrt(np.clip(A, 0.0, np.inf) / math.pi)
+    for i in range(T.shape[0]):
+        x = T[i]
+        d = np.linalg.norm(C - x[None, :], axis=1)
+        for j in np.nonzero(d <= distance_factor * radii)[0]:
+            if A[j] <= 0.0 or S[j] == 0.0:
+                continue
+            pts, w = near_singular_quadrature(x, PV[j], "telles", quad_order, max_depth)
+            r = np.maximum(np.linalg.norm(pts - x[None, :], axis=1), 1e-12)
+            i_near = float(np.sum((K_E / r) * w))
+            i_far = K_E * A[j] / max(d[j], 1e-12)
+            V[i] += S[j] * (i_near - i_far)
+    return V

@@ -337,3 +337,80 @@ def test_sai_oracle_exact_inverse_and_improves_on_jacobi():
     e_jac = np.linalg.norm(eye - A / np.diag(A)[:, None])
     print("SAI", e_sai, "Jacobi", e_jac)
     assert e_sai < 0.5 * e_jac
+
+
+def test_direct_E_field_oracle_is_minus_gradient():
+    """Pins oracle.dense.direct_E_field: E = -grad V of direct_potential
+    (central differences), and the exterior field of a uniformly charged
+    sphere is K_E Q / r^2 radially."""
+    import numpy as np
+    from oracle import dense as od
+    from oracle.constants import K_E
+    from electrodrive_amd.core import bem_mesh
+    rng = np.random.default_rng(4)
+    m = bem_mesh.icosphere(2)
+    sig = rng.normal(size=m.n_panels)
+    X = rng.uniform(-2, 2, (5, 3)) * 1.5
+    E = od.direct_E_field(X, m.centroids, m.areas, sig)
+    h = 1e-5
+    for k in range(3):
+        dx = np.zeros(3); dx[k] = h
+        g = (od.direct_potential(X + dx, m.centroids, m.areas, sig)
+             - od.direct_potential(X - dx, m.centroids, m.areas, sig)) / (2 * h)
+        assert np.allclose(E[:, k], -g, rtol=1e-6, atol=1e-9 * np.abs(E).max())
+    m = bem_mesh.icosphere(4)
+    x = np.array([[0.0, 0.0, 3.0], [2.0, 2.0, 1.0]])
+    E = od.direct_E_field(x, m.centroids, m.areas, np.ones(m.n_panels))
+    Q = m.areas.sum()
+    r = np.linalg.norm(x, axis=1)
+    ref = (K_E * Q / r ** 3)[:, None] * x
+    assert np.allclose(E, ref, rtol=2e-3, atol=1e-6 * np.abs(ref).max())
+
+
+def test_near_target_oracle_pins():
+    """Pins oracle.quadrature.near_target_potentials: no near panel -> V
+    unchanged; a target on a panel centroid swaps the lumped K_E A / 1e-12
+    term for that panel's refined self integral (the self-integral oracle);
+    an off-plane target's refined integral approaches the exact panel
+    integral (adaptive 2-D quadrature) as the depth grows."""
+    import numpy as np
+    from scipy import integrate
+    from oracle import quadrature as oq
+    from oracle.constants import K_E
+    from electrodrive_amd.core import bem_mesh
+    m = bem_mesh.icosphere(2)
+    rng = np.random.default_rng(9)
+    sig = rng.normal(size=m.n_panels)
+    far = np.array([[5.0, 0.0, 0.0]])
+    assert oq.near_target_potentials(np.array([1.0]), far, m.centroids, m.areas, sig,
+                                     m.panel_vertices, 1.5)[0] == 1.0
+    j = 17
+    x = m.centroids[j:j + 1]
+    V = oq.near_target_potentials(np.array([0.0]), x, m.centroids, m.areas, sig,
+                                  m.panel_vertices, 1.5, max_depth=3)
+    I_self, _ = oq.self_integrals(m.centroids[j:j + 1], m.panel_vertices[j:j + 1],
+                                  m.areas[j:j + 1], depth=3)
+    d = np.linalg.norm(m.centroids - x, axis=1)
+    rad = np.sqrt(m.areas / np.pi)
+    expect = sig[j] * (I_self[0] - K_E * m.areas[j] / 1e-12)
+    for k in np.nonzero((d <= 1.5 * rad) & (np.arange(m.n_panels) != j))[0]:
+        pts, w = oq.near_singular_quadrature(x[0], m.panel_vertices[k], "telles", 2, 3)
+        r = np.maximum(np.linalg.norm(pts - x, axis=1), 1e-12)
+        expect += sig[k] * (np.sum(K_E / r * w) - K_E * m.areas[k] / d[k])
+    assert abs(V[0] - expect) <= 1e-9 * abs(expect)
+    # flat right triangle: a distant target is integrated to quadrature accuracy
+    # at every depth; a target 0.05 above the panel stays within the few-percent
+    # plateau of the reference rule (it refines only toward the nearest sub-panel)
+    tri = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0], [0.0, 1.0, 0.0]])
+    for tgt, tol in ((np.array([0.3, 0.3, 3.0]), 1e-4), (np.array([0.3, 0.3, 0.05]), 0.05)):
+        exact, _ = integrate.dblquad(
+            lambda y, xx: K_E / np.sqrt((xx - tgt[0]) ** 2 + (y - tgt[1]) ** 2 + tgt[2] ** 2),
+            0.0, 1.0, lambda xx: 0.0, lambda xx: 1.0 - xx, epsabs=1e-9 * K_E)
+        errs = []
+        for depth in (0, 2, 4, 6):
+            pts, w = oq.near_singular_quadrature(tgt, tri, "telles", 2, depth)
+            val = np.sum(K_E / np.linalg.norm(pts - tgt, axis=1) * w)
+            errs.append(abs(val - exact) / exact)
+        assert max(errs[1:]) < tol
+        if tol > 1e-3:
+            assert errs[0] > 0.1  # unrefined: ~20 % off, the reason for the refinement
