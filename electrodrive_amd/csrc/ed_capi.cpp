@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -66,7 +67,7 @@ static int upload(T** dst, const std::vector<T>& src) {
 void plan_free(ed_plan* P) {
   void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->perm, P->node_ctr, P->node_rng,
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
-                  P->leaves, P->pt_node, P->p2p, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
+                  P->leaves, P->pt_node, P->p2p_items, P->p2p_seg, P->p2p_part, P->p2p_ref, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
                   P->m2l_pair_src, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair, P->t_m2l,
                   P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
@@ -75,6 +76,8 @@ void plan_free(ed_plan* P) {
                   P->coarse_part, P->coarse_tmp, P->dense_items, P->dense_rns, P->dense_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
+  if (P->ev_join) (void)hipEventDestroy(P->ev_join);
   sai_free(P);
   gmres_free(P);
 }
@@ -116,6 +119,83 @@ static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, 
       }
   }
   return cache.emplace(key, std::move(ot)).first->second;
+}
+
+// One side stream per device for the P2P branch of every plan on it, created
+// on first use and kept for the life of the process (a stream holds a
+// hardware queue; plans come and go, e.g. in the autotuner).
+static int side_stream(int device, hipStream_t* out) {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = streams.find(device);
+  if (it == streams.end()) {
+    hipStream_t s = nullptr;
+    ED_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    it = streams.emplace(device, s).first;
+  }
+  *out = it->second;
+  return ED_OK;
+}
+
+// P2P schedule: the interactions of every 64-target chunk of a leaf are cut
+// into items of about kP2PItemSteps wave steps (kP2PItemSteps * slices source
+// points), so that the launch holds many equal pieces of work instead of one
+// block per leaf whose cost varies 60-fold with leaf and U-list size.
+static int p2p_schedule(ed_plan* P, const std::vector<int32_t>& leaves) {
+  const auto& H = P->host;
+  int steps = ed::kP2PItemSteps;
+  if (const char* e = std::getenv("ED_P2P_ITEM_STEPS")) steps = std::max(16, std::atoi(e));
+  std::vector<int2> seg;
+  std::vector<ed::P2PItem> items;
+  std::vector<int2> ref((size_t)std::max<int64_t>(H.n, 1), make_int2(0, 0));
+  for (int32_t li : leaves) {
+    const int32_t b = (int32_t)H.begin[li], e = (int32_t)H.end[li];
+    const int32_t k0 = (int32_t)seg.size();
+    int64_t total = e - b;
+    seg.push_back(make_int2(b, e));
+    for (int64_t q = H.u_ptr[li]; q < H.u_ptr[li + 1]; ++q) {
+      const int64_t s = H.u_src[q];
+      if (s == li || H.end[s] <= H.begin[s]) continue;
+      seg.push_back(make_int2((int32_t)H.begin[s], (int32_t)H.end[s]));
+      total += H.end[s] - H.begin[s];
+    }
+    const int32_t k1 = (int32_t)seg.size();
+    for (int32_t t0 = b; t0 < e; t0 += 64) {
+      const int32_t ntc = std::min<int32_t>(64, e - t0);
+      const int32_t S = std::min<int32_t>(64 / ntc, 8);
+      const int64_t budget = (int64_t)steps * S;
+      const int64_t ib = (int64_t)items.size();
+      int32_t k = k0, off = 0;
+      int64_t left = total;
+      while (left > 0) {
+        const int32_t cnt = (int32_t)std::min<int64_t>(budget, left);
+        ed::P2PItem it{t0, ntc, k, off, cnt, (int32_t)(items.size() * 64), S, 0};
+        items.push_back(it);
+        int32_t c = cnt;
+        while (c > 0 && k < k1) {
+          const int32_t len = seg[k].y - seg[k].x - off;
+          const int32_t take = std::min(len, c);
+          off += take;
+          c -= take;
+          if (off == seg[k].y - seg[k].x) { ++k; off = 0; }
+        }
+        left -= cnt;
+      }
+      if ((int64_t)items.size() * 64 > INT32_MAX) return fail(ED_ERR_ARG, "too many P2P items");
+      for (int32_t t = 0; t < ntc; ++t)
+        ref[(size_t)(t0 + t)] = make_int2((int32_t)(ib * 64 + t), (int32_t)(items.size() - ib));
+    }
+  }
+  P->n_p2p_items = (int64_t)items.size();
+  ED_TRY(upload(&P->p2p_seg, seg));
+  ED_TRY(upload(&P->p2p_ref, ref));
+  ED_HIP(hipMalloc(&P->p2p_items, sizeof(ed::P2PItem) * std::max<size_t>(items.size(), 1)));
+  if (!items.empty())
+    ED_HIP(hipMemcpy(P->p2p_items, items.data(), sizeof(ed::P2PItem) * items.size(),
+                     hipMemcpyHostToDevice));
+  ED_HIP(hipMalloc((void**)&P->p2p_part, sizeof(float) * 64 * std::max<size_t>(items.size(), 1)));
+  return ED_OK;
 }
 
 int plan_upload(ed_plan* P) {
@@ -178,8 +258,11 @@ int plan_upload(ed_plan* P) {
     for (int32_t li : leaves)
       for (int64_t t = H.begin[li]; t < H.end[li]; ++t) ptn[(size_t)t] = li;
     ED_TRY(upload(&P->pt_node, ptn));
-    ED_HIP(hipMalloc((void**)&P->p2p, sizeof(float) * (size_t)std::max<int64_t>(H.n, 1)));
   }
+  ED_TRY(p2p_schedule(P, leaves));
+  ED_TRY(side_stream(P->device, &P->side));
+  ED_HIP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
+  ED_HIP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
   ED_TRY(upload(&P->u_ptr, uptr));
   ED_TRY(upload(&P->u_src, usrc));
 

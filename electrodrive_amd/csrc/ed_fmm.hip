@@ -10,8 +10,10 @@
 //   m2l         MFMA GEMM per offset table: Y[pair] = T_off * M[src] (fp32
 //               v_mfma_f32_32x32x2f32), then a deterministic per-target sum
 //   l2l         one block per node: L += T_oct * L_parent (level by level)
-//   leaf        one wave per leaf: L2P + P2P over the U list + near CSR +
-//               self term, K_E scaling, scatter to the output order
+//   p2p         one wave per work item (<= 64 targets x a piece of their
+//               leaf's U-list source stream), partial sums per item
+//   l2p_out     one thread per panel: L2P + P2P partials + near CSR + self
+//               term, K_E scaling, scatter to the output order
 // No float atomics anywhere: every sum has a fixed order.
 #include <hip/hip_runtime.h>
 
@@ -540,191 +542,123 @@ __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// Leaf evaluation: L2P + P2P (U list) + near CSR + self, scatter.
+// P2P: one wave per work item (ed::P2PItem), four independent waves a block.  A chunk of n_t <= 64 targets of
+// one leaf meets `cnt` consecutive points of the leaf's source stream: the
+// leaf itself first (with the j != i exclusion), then its other U-list
+// leaves.  Chunks of more than 32 targets (one slice) read each source at a
+// wave-uniform address, so the loads are scalar and the sources sit in SGPRs:
+// no LDS staging and no barriers.  Smaller chunks split the lanes into
+// S = min(64 / n_t, 8) slices, slice s taking every S-th point, and add the
+// slices in a fixed order at the end.  Every item writes its own row of
+// partial sums and k_l2p_out adds a panel's rows in item order, so the
+// result does not depend on scheduling.
 // ---------------------------------------------------------------------------
-// One 4-wave block per target leaf.  Inside a wave the lanes are split into
-// S = 64 / n_t slices (n_t = targets in the chunk, at most 64); the source
-// stream — first the target leaf itself (with the j != i exclusion), then the
-// concatenation of the other U-list leaves — is cut into 256-point chunks,
-// wave w taking chunks w, w+4, ... through its own LDS buffer, and slice s
-// taking every S-th point of a chunk.  Partial sums are combined in a fixed
-// order (wave-major, then slice).
-constexpr int kSrcChunk = 256;
-constexpr int kUSeg = 256;
-constexpr int kLeafWaves = 4;
-constexpr int kLeafThreads = 64 * kLeafWaves;
+constexpr int kP2PUnroll = 8;
+constexpr int kP2PWaves = 4;  // independent items per block
 
-__device__ __forceinline__ int wave_incl_scan(int v) {
-  const int lane = threadIdx.x & 63;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int t = __shfl_up(v, o, 64);
-    if (lane >= o) v += t;
-  }
-  return v;
+// r^2 + 1e-24 in place of max(r^2, 1e-24): equal for every r^2 above 3e-17
+// (the addend is below half an ulp there), and both give 1e12 for r = 0.
+__device__ __forceinline__ float p2p_r2(float4 xi, float4 s) {
+  const float dx = xi.x - s.x, dy = xi.y - s.y, dz = xi.z - s.z;
+  return fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, 1e-24f)));
 }
 
-// Sum of q_j / r_ij over one LDS tile for a lane of slice `sl` of `S` (points
-// j = sl, sl + S, ...).  The step count is uniform across the wave and the
-// loop is unrolled U-wide so the LDS reads of a group are in flight together.
-// U-list tiles (SELF = false) are padded past `ns` with zero-charge points, so
-// the loop needs neither index clamps nor masks.  The target's own tile
-// (SELF) clamps the index and sets r^2 = inf for j >= ns and j == me, so both
-// add q * rsq(inf) = 0 without a branch.
-constexpr int kP2PUnroll = 8;
-constexpr int kP2PMaxSlices = 8;
-constexpr int kP2PPad = (kP2PUnroll + 1) * kP2PMaxSlices;
-
+// acc + sum of q_j rsqrt(r_ij^2) over j in [j, end), j wave-uniform.  SELF
+// (the chunk's own leaf): r^2 = inf for j == me, and rsq(inf) = 0.
 template <bool SELF>
-__device__ __forceinline__ float p2p_tile(const float4* __restrict__ tile, int ns, int sl, int S,
-                                          float4 xi, int me) {
+__device__ __forceinline__ float p2p_uniform(const float4* __restrict__ pts, int j, int end,
+                                             float4 xi, int me, float acc) {
   constexpr int U = kP2PUnroll;
-  float acc = 0.0f;
-  const int nsteps = (ns + S - 1) / S;
-  int t = 0;
-  for (; t + U <= nsteps + (SELF ? 0 : U - 1); t += U) {
-    float4 sv[U];
+  for (; j + U <= end; j += U) {
+    float4 s[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s[u] = pts[j + u];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
-      const int j = sl + (t + u) * S;
-      sv[u] = tile[SELF ? min(j, ns - 1) : j];
-    }
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      const float dx = xi.x - sv[u].x, dy = xi.y - sv[u].y, dz = xi.z - sv[u].z;
-      float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-      if (SELF) {
-        const int j = sl + (t + u) * S;
-        r2 = (j < ns && j != me) ? r2 : INFINITY;  // rsq(inf) = 0
-      }
-      acc = fmaf(sv[u].w, __builtin_amdgcn_rsqf(r2), acc);
+      float r2 = p2p_r2(xi, s[u]);
+      if (SELF) r2 = (j + u == me) ? INFINITY : r2;
+      acc = fmaf(s[u].w, __builtin_amdgcn_rsqf(r2), acc);
     }
   }
-  if (SELF) {
-    for (; t < nsteps; ++t) {
-      const int j = sl + t * S;
-      const float4 sj = tile[min(j, ns - 1)];
-      const float dx = xi.x - sj.x, dy = xi.y - sj.y, dz = xi.z - sj.z;
-      float r2 = fmaxf(dx * dx + dy * dy + dz * dz, 1e-24f);
-      r2 = (j < ns && j != me) ? r2 : INFINITY;
-      acc = fmaf(sj.w, __builtin_amdgcn_rsqf(r2), acc);
-    }
+  for (; j < end; ++j) {
+    const float4 s = pts[j];
+    float r2 = p2p_r2(xi, s);
+    if (SELF) r2 = (j == me) ? INFINITY : r2;
+    acc = fmaf(s.w, __builtin_amdgcn_rsqf(r2), acc);
   }
   return acc;
 }
 
-__global__ __launch_bounds__(kLeafThreads) void k_p2p(
-    const int32_t* __restrict__ leaves, const int2* __restrict__ rng,
-    const float4* __restrict__ pts, const int32_t* __restrict__ u_ptr,
-    const int32_t* __restrict__ u_src, float* __restrict__ p2p) {
-  __shared__ float4 src[kLeafWaves][kSrcChunk + kP2PPad];
-  __shared__ int ubeg[kUSeg];
-  __shared__ int upref[kUSeg + 1];
-  __shared__ int utotal;
-  __shared__ float red[kLeafThreads];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int li = blockIdx.x;
-  const int node = leaves[li];
-  const int2 r = rng[node];
-  const int ub = u_ptr[li], ue = u_ptr[li + 1];
-  float4* mysrc = src[wave];
-  for (int t0 = r.x; t0 < r.y; t0 += 64) {
-    const int ntc = min(64, r.y - t0);
-    const int S = min(64 / ntc, kP2PMaxSlices);
-    const int tl = lane % ntc, sl = lane / ntc;
-    const bool act = sl < S;
-    const int i = t0 + tl;
-    const float4 xi = pts[i];
-    float acc = 0.0f;
-    // (1) the target leaf itself, excluding j == i
-    for (int r0 = r.x; r0 < r.y; r0 += kLeafWaves * kSrcChunk) {
-      const int s0 = r0 + wave * kSrcChunk;
-      const int ns = max(0, min(kSrcChunk, r.y - s0));
-      __syncthreads();
-      if (ns > 0) {  // ns = 0: s0 may lie past the last panel, no load at all
-        float4 v[kSrcChunk / 64];
+// The same over the points j0 + sl + t S of [j0, end) for slice sl of S; the
+// step count is uniform, indices past the end are clamped and get r^2 = inf.
+template <bool SELF>
+__device__ __forceinline__ float p2p_sliced(const float4* __restrict__ pts, int j0, int end,
+                                            int sl, int S, float4 xi, int me, float acc) {
+  constexpr int U = kP2PUnroll / 2;
+  const int nsteps = (end - j0 + S - 1) / S;
+  const int jb = j0 + sl;
+  int t = 0;
+  for (; t + U <= nsteps; t += U) {
+    float4 s[U];
 #pragma unroll
-        for (int u = 0; u < kSrcChunk / 64; ++u) v[u] = pts[s0 + min(lane + 64 * u, max(ns - 1, 0))];
+    for (int u = 0; u < U; ++u) s[u] = pts[min(jb + (t + u) * S, end - 1)];
 #pragma unroll
-        for (int u = 0; u < kSrcChunk / 64; ++u)
-          if (lane + 64 * u < ns) mysrc[lane + 64 * u] = v[u];
-      }
-      __syncthreads();
-      if (ns > 0) acc += p2p_tile<true>(mysrc, ns, act ? sl : 0, S, xi, i - s0);
-    }
-    // (2) the other U-list leaves as one concatenated stream
-    for (int g0 = ub; g0 < ue; g0 += kUSeg) {
-      const int ng = min(kUSeg, ue - g0);
-      __syncthreads();
-      if (wave == 0) {
-        int part = 0;
-        int sz[kUSeg / 64];
-#pragma unroll
-        for (int k = 0; k < kUSeg / 64; ++k) {
-          const int q = lane * (kUSeg / 64) + k;
-          int size = 0;
-          if (q < ng) {
-            const int sn = u_src[g0 + q];
-            const int2 sr = rng[sn];
-            size = (sn == node) ? 0 : sr.y - sr.x;
-            ubeg[q] = sr.x;
-          }
-          sz[k] = size;
-          part += size;
-        }
-        const int incl = wave_incl_scan(part);
-        int run = incl - part;
-#pragma unroll
-        for (int k = 0; k < kUSeg / 64; ++k) {
-          const int q = lane * (kUSeg / 64) + k;
-          if (q < ng) upref[q] = run;
-          run += sz[k];
-        }
-        if (lane == 63) { upref[ng] = incl; utotal = incl; }
-      }
-      __syncthreads();
-      const int total = utotal;
-      for (int r0 = 0; r0 < total; r0 += kLeafWaves * kSrcChunk) {
-        const int e0 = r0 + wave * kSrcChunk;
-        const int ns = max(0, min(kSrcChunk, total - e0));
-        __syncthreads();
-        if (ns > 0) {
-          // locate every element of the wave's chunk first, then issue all the
-          // (clamped, unconditional) loads together
-          int gi[kSrcChunk / 64];
-#pragma unroll
-          for (int u = 0; u < kSrcChunk / 64; ++u) {
-            const int ge = e0 + min(lane + 64 * u, max(ns - 1, 0));
-            int lo = 0, hi = ng - 1;  // last q with upref[q] <= ge
-            while (lo < hi) {
-              const int mid = (lo + hi + 1) >> 1;
-              if (upref[mid] <= ge) lo = mid; else hi = mid - 1;
-            }
-            gi[u] = ubeg[lo] + (ge - upref[lo]);
-          }
-          float4 v[kSrcChunk / 64];
-#pragma unroll
-          for (int u = 0; u < kSrcChunk / 64; ++u) v[u] = pts[gi[u]];
-#pragma unroll
-          for (int u = 0; u < kSrcChunk / 64; ++u)
-            if (lane + 64 * u < ns) mysrc[lane + 64 * u] = v[u];
-          // zero-charge padding read by the unmasked loop past ns
-          for (int e = ns + lane; e < ns + kP2PPad; e += 64) mysrc[e] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        __syncthreads();
-        if (ns > 0) acc += p2p_tile<false>(mysrc, ns, act ? sl : 0, S, xi, -1);
-      }
-    }
-    __syncthreads();
-    red[threadIdx.x] = act ? acc : 0.0f;
-    __syncthreads();
-    if (wave == 0 && sl == 0) {
-      float nearf = 0.0f;
-      for (int w = 0; w < kLeafWaves; ++w)
-        for (int k = 0; k < S; ++k) nearf += red[w * 64 + tl + k * ntc];
-      p2p[i] = nearf;
+    for (int u = 0; u < U; ++u) {
+      const int j = jb + (t + u) * S;
+      float r2 = p2p_r2(xi, s[u]);
+      r2 = (j >= end || (SELF && j == me)) ? INFINITY : r2;
+      acc = fmaf(s[u].w, __builtin_amdgcn_rsqf(r2), acc);
     }
   }
+  for (; t < nsteps; ++t) {
+    const int j = jb + t * S;
+    const float4 s = pts[min(j, end - 1)];
+    float r2 = p2p_r2(xi, s);
+    r2 = (j >= end || (SELF && j == me)) ? INFINITY : r2;
+    acc = fmaf(s.w, __builtin_amdgcn_rsqf(r2), acc);
+  }
+  return acc;
+}
+
+__global__ __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_p2p(const ed::P2PItem* __restrict__ items,
+                                                       int n_items,
+                                                       const int2* __restrict__ seg,
+                                                       const float4* __restrict__ pts,
+                                                       float* __restrict__ part) {
+  // the wave index is uniform: readfirstlane lets the item and every source
+  // address derived from it live in SGPRs (scalar loads)
+  const int item = blockIdx.x * kP2PWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= n_items) return;  // whole waves only
+  const ed::P2PItem it = items[item];
+  const int lane = threadIdx.x & 63;
+  const int ntc = it.ntc, S = it.slices;
+  const int tl = lane % ntc, sl = lane / ntc;
+  const bool act = sl < S;
+  const int me = it.tgt0 + tl;
+  const float4 xi = pts[me];
+  float acc = 0.0f;
+  int k = it.seg, off = it.off, left = it.cnt;
+  while (left > 0) {
+    const int2 r = seg[k];
+    const int j0 = r.x + off;
+    const int n = min(r.y - j0, left);
+    const bool self = r.x <= it.tgt0 && it.tgt0 < r.y;
+    if (S == 1)
+      acc = self ? p2p_uniform<true>(pts, j0, j0 + n, xi, me, acc)
+                 : p2p_uniform<false>(pts, j0, j0 + n, xi, me, acc);
+    else
+      acc = self ? p2p_sliced<true>(pts, j0, j0 + n, act ? sl : 0, S, xi, me, acc)
+                 : p2p_sliced<false>(pts, j0, j0 + n, act ? sl : 0, S, xi, me, acc);
+    left -= n;
+    ++k;
+    off = 0;
+  }
+  if (S > 1) {  // slices of target tl sit in lanes tl + q n_t, q < S
+    float v = 0.0f;
+    for (int q = 0; q < S; ++q) v += __shfl(acc, tl + q * ntc, 64);
+    acc = v;
+  }
+  if (lane < ntc) part[it.out + lane] = acc;
 }
 
 // Per panel (tree order): far field by L2P from its leaf's local expansion,
@@ -734,7 +668,8 @@ __global__ __launch_bounds__(kLeafThreads) void k_p2p(
 template <int P, typename TO>
 __global__ __launch_bounds__(256) void k_l2p_out(
     int n, const int32_t* __restrict__ pt_node, const float4* __restrict__ ctr,
-    const float4* __restrict__ pts, const float* __restrict__ L, const float* __restrict__ p2p,
+    const float4* __restrict__ pts, const float* __restrict__ L,
+    const int2* __restrict__ p2p_ref, const float* __restrict__ p2p_part,
     const int32_t* __restrict__ near_ptr, const int32_t* __restrict__ near_col,
     const float* __restrict__ near_w, const float* __restrict__ sig_t,
     const float* __restrict__ selfc_t, const int32_t* __restrict__ perm, int out_tree, float ke,
@@ -750,7 +685,18 @@ __global__ __launch_bounds__(256) void k_l2p_out(
   for (int q = nb; q < ne; ++q) corr += near_w[q] * sig_t[near_col[q]];
   const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w,
                                 L + (size_t)node * NC);
-  float v = ke * (far + p2p[i]) + selfc_t[i] * xi.w + corr;
+  const int2 pr = p2p_ref[i];
+  float near = 0.0f;
+  int k = 0;
+  for (; k + 4 <= pr.y; k += 4) {  // four loads in flight, added in order
+    float a[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = p2p_part[pr.x + 64 * (k + u)];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) near += a[u];
+  }
+  for (; k < pr.y; ++k) near += p2p_part[pr.x + 64 * k];
+  float v = ke * (far + near) + selfc_t[i] * xi.w + corr;
   if (!isfinite(v)) v = 0.0f;
   out[out_tree ? i : perm[i]] = (TO)v;
 }
@@ -771,6 +717,20 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     if (Pn->prof_events) (void)hipEventRecord(Pn->prof_events[k], st); \
   } while (0)
 
+#define ED_HIP_LAUNCH(call, msg)                                              \
+  do {                                                                        \
+    hipError_t e_ = (call);                                                   \
+    if (e_ != hipSuccess) return fail(ED_ERR_LAUNCH, std::string(msg) + ": " + \
+                                                         hipGetErrorString(e_)); \
+  } while (0)
+
+static void p2p_launch(ed_plan* Pn, hipStream_t s) {
+  const unsigned nb = (unsigned)((Pn->n_p2p_items + kP2PWaves - 1) / kP2PWaves);
+  hipLaunchKernelGGL(k_p2p, dim3(nb), dim3(64 * kP2PWaves), 0, s,
+                     (const ed::P2PItem*)Pn->p2p_items, (int)Pn->n_p2p_items, Pn->p2p_seg,
+                     Pn->pts, Pn->p2p_part);
+}
+
 template <int P>
 static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
                    void* out, int out64, int out_tree, hipStream_t st) {
@@ -788,6 +748,16 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
                        Pn->sig_t, Pn->selfc_t);
   ED_LAUNCH_CHECK("gather");
   ED_MARK(1);
+  // P2P depends only on the gathered points: off the profiling path it runs
+  // on the side stream while this stream does the far-field passes
+  const bool fork = Pn->prof_events == nullptr && Pn->n_p2p_items > 0;
+  if (fork) {
+    ED_HIP_LAUNCH(hipEventRecord(Pn->ev_fork, st), "p2p fork");
+    ED_HIP_LAUNCH(hipStreamWaitEvent(Pn->side, Pn->ev_fork, 0), "p2p fork");
+    p2p_launch(Pn, Pn->side);
+    ED_LAUNCH_CHECK("p2p");
+    ED_HIP_LAUNCH(hipEventRecord(Pn->ev_join, Pn->side), "p2p join");
+  }
   const int nleaf = (int)Pn->n_leaves;
   const auto& H = Pn->host;
   const int depth = H.depth;
@@ -860,18 +830,21 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     for (int k = 2; k <= 5; ++k) ED_MARK(k);
   }
   ED_MARK(6);
+  if (fork) {
+    ED_HIP_LAUNCH(hipStreamWaitEvent(st, Pn->ev_join, 0), "p2p join");
+  } else if (Pn->n_p2p_items > 0) {
+    p2p_launch(Pn, st);
+    ED_LAUNCH_CHECK("p2p");
+  }
   const float ke = (float)kKE;
-  hipLaunchKernelGGL(k_p2p, dim3(nleaf), dim3(kLeafThreads), 0, st, Pn->leaves, Pn->node_rng,
-                     Pn->pts, Pn->u_ptr, Pn->u_src, Pn->p2p);
-  ED_LAUNCH_CHECK("p2p");
   ED_MARK(7);
   if (out64)
     hipLaunchKernelGGL((k_l2p_out<P, double>), dim3(gb), dim3(256), 0, st, n, Pn->pt_node,
-                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p, Pn->near_ptr, Pn->near_col,
+                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p_ref, Pn->p2p_part, Pn->near_ptr, Pn->near_col,
                        Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm, out_tree, ke, (double*)out);
   else
     hipLaunchKernelGGL((k_l2p_out<P, float>), dim3(gb), dim3(256), 0, st, n, Pn->pt_node,
-                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p, Pn->near_ptr, Pn->near_col,
+                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p_ref, Pn->p2p_part, Pn->near_ptr, Pn->near_col,
                        Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm, out_tree, ke, (float*)out);
   ED_LAUNCH_CHECK("l2p_out");
   ED_MARK(8);

@@ -36,7 +36,14 @@ struct ed_plan {
 
   // leaves (node ids) and their U lists (CSR over leaf ordinal)
   int32_t* pt_node = nullptr;   // n, leaf node of each tree-ordered panel
-  float* p2p = nullptr;         // n, P2P partial sums (tree order)
+  // P2P work items (ed::P2PItem), their source segments (per leaf: the leaf
+  // itself, then its other U-list leaves), one row of 64 partial sums per
+  // item, and per tree-ordered panel (first partial, number of partials)
+  void* p2p_items = nullptr;
+  int2* p2p_seg = nullptr;
+  float* p2p_part = nullptr;
+  int2* p2p_ref = nullptr;
+  int64_t n_p2p_items = 0;
   float* ym2m = nullptr;        // n_nodes x (p+1)^2, per-child M2M products
   int32_t* leaves = nullptr;
   int32_t* u_ptr = nullptr;
@@ -109,12 +116,25 @@ struct ed_plan {
 
   // stage events while ed_fmm_profile runs (8 markers), else NULL
   hipEvent_t* prof_events = nullptr;
+  // P2P runs on a side stream beside the far-field passes (fork/join by
+  // events on the caller's stream, so graph capture records both branches);
+  // the stream is shared by all plans on the device and never destroyed
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
   // GMRES workspace (allocated lazily by the solver, owned by the plan)
   void* gmres_ws = nullptr;
 };
 
 namespace ed {
+// P2P work item: up to 64 consecutive targets of one leaf (tree positions
+// tgt0 .. tgt0 + ntc - 1) against `cnt` consecutive points of that leaf's
+// source stream, starting at offset `off` of segment `seg`; the item writes
+// its partial sums to p2p_part[out + t].  slices = min(64 / ntc, 8).
+struct P2PItem {
+  int32_t tgt0, ntc, seg, off, cnt, out, slices, pad;
+};
+constexpr int kP2PItemSteps = 256;
 // Work item of the split dense operator: up to 512 target rows and one
 // contiguous source range of their geometry.
 struct DenseItem {
