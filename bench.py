@@ -34,10 +34,13 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 VALU_FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, the factors a 256 MiB device copy reads) for the headline
-# workload (81,920 panels, order 10, leaf 128); summary and commands in
-# profiles/pmc_traffic_icosphere81920_p10_leaf128.md
-TRAFFIC_BYTES = {"p2p": (81920, 1.1373e7, "profiles/pmc_traffic_icosphere81920_p10_leaf128.md"),
-                 "m2l": (81920, 6.266e7, "profiles/pmc_traffic_icosphere81920_p10_leaf128.md")}
+# workload (81,920 panels, order 10, leaf 256), keyed by (panels, order, leaf),
+# and the VALU-busy fraction of k_p2p from SQ counters; summary and commands
+# in profiles/pmc_traffic_icosphere81920_p10_leaf256.md
+PMC_SOURCE = "profiles/pmc_traffic_icosphere81920_p10_leaf256.md"
+TRAFFIC_BYTES = {("p2p", 81920, 10, 256): 2.188e7,
+                 ("m2l", 81920, 10, 256): 5.269e7}
+P2P_VALU_BUSY = {(81920, 10, 256): 0.70}
 
 
 def parse():
@@ -156,9 +159,10 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
     roofline = dict(kernel=dom, bound=rf["bound"], achieved=rf["achieved"], peak=rf["peak"],
                     unit=rf["unit"], frac=rf["achieved"] / rf["peak"], traffic=None,
                     launch_ms=stages[dom], algorithmic_per_launch=rf["algorithmic"])
-    if dom in TRAFFIC_BYTES and op.n == TRAFFIC_BYTES[dom][0]:
-        roofline["traffic"] = TRAFFIC_BYTES[dom][1]
-        roofline["traffic_source"] = TRAFFIC_BYTES[dom][2]
+    key = (op.n, op.plan.order, op.plan.leaf_size)
+    if (dom,) + key in TRAFFIC_BYTES:
+        roofline["traffic"] = TRAFFIC_BYTES[(dom,) + key]
+        roofline["traffic_source"] = PMC_SOURCE
     m2l = stage_roofline("m2l", stages["m2l"], st, op.n, op.plan.order, peak)
     p2p = stage_roofline("p2p", stages["p2p"], st, op.n, op.plan.order)
     extra = dict(m2l_mfma=dict(achieved_tflops=m2l["achieved"], peak_tflops=peak,
@@ -167,6 +171,7 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
                           interactions_per_s=p2p["interactions_per_s"],
                           valu_tflops=p2p["valu_tflops"],
                           frac_valu=p2p["valu_tflops"] / VALU_FP32_PEAK_TFLOPS,
+                          valu_busy_pmc=P2P_VALU_BUSY.get(key),
                           launch_ms=stages["p2p"]))
     return dict(ms=ms, wall=wall, stages=stages, roofline=roofline, extra=extra, stats=st,
                 value=n_units / (ms * 1e-3))

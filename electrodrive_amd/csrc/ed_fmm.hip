@@ -6,7 +6,8 @@
 // Stages of one matvec (one launch each, level loops for M2M / L2L):
 //   gather      q, sigma, self term into tree order
 //   p2m         one wave per leaf: lanes stream points, LDS column reduction
-//   m2m         one block per parent: sum over children of T_oct * M_child
+//   m2m         per level: one block per child (T_oct * M_child), then one
+//               block per parent summing its children
 //   m2l         MFMA GEMM per offset table: Y[pair] = T_off * M[src] (fp32
 //               v_mfma_f32_32x32x2f32), then a deterministic per-target sum
 //   l2l         one block per node: L += T_oct * L_parent (level by level)
@@ -542,8 +543,9 @@ __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restric
 }
 
 // ---------------------------------------------------------------------------
-// P2P: one wave per work item (ed::P2PItem), four independent waves a block.  A chunk of n_t <= 64 targets of
-// one leaf meets `cnt` consecutive points of the leaf's source stream: the
+// P2P: one wave per work item (ed::P2PItem), four independent waves a
+// block.  A chunk of n_t <= 64 targets of one leaf meets `cnt` consecutive
+// points of the leaf's source stream: the
 // leaf itself first (with the j != i exclusion), then its other U-list
 // leaves.  Chunks of more than 32 targets (one slice) read each source at a
 // wave-uniform address, so the loads are scalar and the sources sit in SGPRs:
