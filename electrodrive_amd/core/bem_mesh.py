@@ -2,11 +2,14 @@
 
 ``TriMesh`` and the panel quantities follow core/bem_mesh.py:13-25 and :103-126
 (centroid = vertex mean, area = |e1 x e2|/2, panels with area <= 1e-16
-dropped, unit normal = cross / (2A + 1e-30)).  ``torus`` and ``plane_patch``
-follow core/bem_mesh.py:180-212 and :139-151.  ``icosphere``, ``sphere_dimer``
-and ``square_plate`` have no reference counterpart (the reference meshes
-spheres as UV grids, core/bem_mesh.py:154-177).
+dropped, unit normal = cross / (2A + 1e-30)).  ``torus``, ``plane_patch`` and
+``uv_sphere`` follow core/bem_mesh.py:180-212, :139-151 and :154-177, and
+``generate_mesh`` (meshing a parsed spec at a target edge length) follows
+core/bem_mesh.py:27-136.  ``icosphere``, ``sphere_dimer`` and ``square_plate``
+have no reference counterpart.
 """
+import math
+import os
 from dataclasses import dataclass
 
 import numpy as np
@@ -150,3 +153,100 @@ def torus_for_panels(R: float, a: float, target_panels: int) -> TriMesh:
     n_v = max(3, int(round((target_panels / (2.0 * ratio)) ** 0.5)))
     n_u = max(3, int(round(target_panels / (2.0 * n_v))))
     return torus(R, a, n_u, n_v)
+
+
+def uv_sphere(center, radius: float, n_theta: int, n_phi: int) -> TriMesh:
+    """Latitude-longitude sphere (core/bem_mesh.py:154-177): n_theta rings from
+    pole to pole (both poles included, n_phi copies each), n_phi meridians;
+    two triangles per cell, the degenerate ones at the poles dropped."""
+    c = np.asarray(center, dtype=float)
+    theta = np.linspace(0.0, np.pi, n_theta)
+    phi = np.linspace(0.0, 2.0 * np.pi, n_phi, endpoint=False)
+    tt, pp = np.meshgrid(theta, phi, indexing="ij")
+    rs = radius * np.sin(tt)
+    verts = np.stack([(c[0] + rs * np.cos(pp)).ravel(), (c[1] + rs * np.sin(pp)).ravel(),
+                      (c[2] + radius * np.cos(tt)).ravel()], axis=1)
+    i = np.arange(n_theta - 1)[:, None]
+    j = np.arange(n_phi)[None, :]
+    jn = (j + 1) % n_phi
+    a_idx = (i * n_phi + j).ravel()
+    b_idx = (i * n_phi + jn).ravel()
+    c_idx = ((i + 1) * n_phi + j).ravel()
+    d_idx = ((i + 1) * n_phi + jn).ravel()
+    tris = np.empty(((n_theta - 1) * n_phi * 2, 3), dtype=np.int64)
+    tris[0::2] = np.stack([a_idx, b_idx, c_idx], axis=1)
+    tris[1::2] = np.stack([b_idx, d_idx, c_idx], axis=1)
+    return make_trimesh(verts, tris)
+
+
+def _env_scale(name: str, default: float) -> float:
+    try:
+        return float(os.getenv(name, str(default)))
+    except ValueError:
+        return default
+
+
+def generate_mesh(spec, target_h: float = 0.1, logger=None) -> TriMesh:
+    """Mesh every conductor of a spec at edge length ~target_h
+    (core/bem_mesh.py:27-136), with the reference's sizing rules:
+
+    - plane at z: square patch of side L = s * max(8, 16 d), d = max(0.3,
+      largest |z_q - z| of the point charges), n = max(8, int(L / h)) cells a
+      side (s = EDE_BEM_PLANE_PATCH_SCALE, default 1);
+    - sphere: n_theta = max(10, ceil(k pi r / h)), n_phi = max(20,
+      ceil(2 k pi r / h)) with k = max(1, EDE_BEM_SPHERE_DENSITY_SCALE),
+      default 1.5;
+    - torus / toroid: n_u = max(24, ceil(s_u 2 pi R / h)), n_v = max(16,
+      ceil(s_v 2 pi a / h)), R = major_radius (or radius, default 1),
+      a = minor_radius (default R / 4).
+
+    h = max(1e-3, target_h).  Panels carry the conductor's ``id`` (default:
+    its index); other conductor types raise NotImplementedError."""
+    if logger:
+        logger.info("Mesh generation started.", target_h=target_h)
+    h = max(1e-3, float(target_h))
+    verts, tris, cids = [], [], []
+    off = 0
+    for idx, c in enumerate(spec.conductors):
+        kind = c.get("type")
+        cid = int(c.get("id", idx))
+        if kind == "plane":
+            z = float(c.get("z", 0.0))
+            dmin = 0.3
+            for ch in spec.charges:
+                if ch.get("type") == "point":
+                    dmin = max(dmin, abs(float(ch["pos"][2] - z)))
+            L = _env_scale("EDE_BEM_PLANE_PATCH_SCALE", 1.0) * max(8.0, 16.0 * dmin)
+            n = max(8, int(L / h))
+            part = plane_patch(z, L, n)
+        elif kind == "sphere":
+            ctr = np.array(c.get("center", [0.0, 0.0, 0.0]), dtype=float)
+            r = float(c.get("radius", 1.0))
+            k = max(1.0, _env_scale("EDE_BEM_SPHERE_DENSITY_SCALE", 1.5))
+            n_theta = max(10, int(math.ceil(k * math.pi * r / h)))
+            n_phi = max(20, int(math.ceil(k * 2.0 * math.pi * r / h)))
+            part = uv_sphere(ctr, r, n_theta, n_phi)
+        elif kind in ("torus", "toroid"):
+            ctr = np.array(c.get("center", [0.0, 0.0, 0.0]), dtype=float)
+            R = float(c.get("major_radius", c.get("radius", 1.0)))
+            a = float(c.get("minor_radius", 0.25 * R))
+            n_u = max(24, int(math.ceil(_env_scale("EDE_BEM_TORUS_MAJOR_SCALE", 1.0)
+                                        * (2.0 * math.pi * R) / h)))
+            n_v = max(16, int(math.ceil(_env_scale("EDE_BEM_TORUS_MINOR_SCALE", 1.0)
+                                        * (2.0 * math.pi * a) / h)))
+            part = torus(R, a, n_u, n_v, center=ctr)
+        else:
+            raise NotImplementedError(f"Unsupported conductor type: {kind}")
+        # the parts keep every vertex; only their triangle lists were filtered
+        verts.append(part.vertices)
+        tris.append(part.triangles + off)
+        cids.append(np.full(part.n_panels, cid, dtype=np.int32))
+        off += part.vertices.shape[0]
+    if not verts:
+        return TriMesh(np.empty((0, 3)), np.empty((0, 3), dtype=np.int64), np.empty((0, 3)),
+                       np.empty((0, 3)), np.empty(0), np.empty(0, dtype=np.int32))
+    mesh = make_trimesh(np.vstack(verts), np.vstack(tris), np.concatenate(cids))
+    if logger:
+        logger.info("Mesh generation complete.", n_panels=mesh.n_panels,
+                    total_area=float(mesh.areas.sum()))
+    return mesh
