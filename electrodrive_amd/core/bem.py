@@ -30,8 +30,10 @@ What is the reference's and what is this build's:
 
 Out of scope (reference control plane): run manifests, control.json polling,
 the BEM intercept recorder, the near_alpha neighbour blending (off by
-default), tile-size autotuning.  ``differentiable=True`` raises, as the
-reference does without its differentiable backend.
+default), tile-size autotuning.  ``differentiable=True`` solves the
+reference's differentiable operator (dense + self term, no near correction)
+with ``solve_diffbem`` (adjoint gradients, core/diffbem.py) and returns a
+solution whose sigma and evaluations carry gradients.
 """
 from __future__ import annotations
 
@@ -45,9 +47,12 @@ from .. import _native as nat
 from ..fmm3d.autotune import autotune_for_problem
 from ..fmm3d.config import FmmConfig
 from ..orchestration.parser import CanonicalSpec
-from .bem_kernel import bem_E_field_targets, bem_potential_targets
+from .bem_kernel import (_bem_E_field_targets_core_torch, _bem_matvec_core_torch,
+                         _bem_potential_targets_core_torch, bem_E_field_targets,
+                         bem_potential_targets)
 from .bem_mesh import TriMesh, generate_mesh
 from .bem_quadrature import near_quadrature_potentials
+from .diffbem import solve_diffbem
 from .kernels import K_E
 from .operator import BemOperator
 
@@ -172,7 +177,10 @@ class BEMSolution:
         Ps = _offset_from_boundary(self._spec, P, self._A, self._dtype, self._device)
         V = E = None
         if compute_V:
-            V_ind = bem_potential_targets(Ps, self._C, self._A, self._S)
+            if self._differentiable:
+                V_ind = _bem_potential_targets_core_torch(Ps, self._C, self._A, self._S)
+            else:
+                V_ind = bem_potential_targets(Ps, self._C, self._A, self._S)
             if self._near_quad_enabled and self._panel_vertices.numel() > 0:
                 V_ind = near_quadrature_potentials(
                     V_ind, Ps, self._C, self._A, self._S, self._panel_vertices,
@@ -180,8 +188,10 @@ class BEMSolution:
                     max_depth=self._near_quad_max_depth)
             V = _free_space_at_points(self._spec, Ps, self._dtype, self._device) + V_ind
         if compute_E:
+            field = (_bem_E_field_targets_core_torch if self._differentiable
+                     else bem_E_field_targets)
             E = (_free_space_E_field_at_points(self._spec, Ps, self._dtype, self._device)
-                 + bem_E_field_targets(Ps, self._C, self._A, self._S))
+                 + field(Ps, self._C, self._A, self._S))
         return V, E
 
     def eval(self, p) -> float:
@@ -223,9 +233,6 @@ def bem_solve(spec: CanonicalSpec, cfg, logger, differentiable: bool = False) ->
     """The reference's h-refined BEM solve; returns the result dict
     (boundary_samples, sample_points, solution, surface_charge_density,
     mesh_stats, gmres_stats, refinement_history) or {"error": ...}."""
-    if differentiable:
-        raise RuntimeError("differentiable=True requested for bem_solve, but the "
-                           "differentiable solve is not available in this build.")
     nat.require_gpu()
     device = torch.device("cuda", torch.cuda.current_device())
     dtype = torch.float64
@@ -301,11 +308,25 @@ def bem_solve(spec: CanonicalSpec, cfg, logger, differentiable: bool = False) ->
             x0 = torch.zeros_like(b)
         # the reference's tolerance is relative to ||b - A x0||; the native
         # solver's is relative to ||b||
-        r0 = float(torch.linalg.norm(b - op.approx(x0)))
-        bn = float(torch.linalg.norm(b))
-        tol_b = gtol * (r0 / bn if bn > 0.0 else 1.0)
-        sigma, info = op.solve(b, x0, tol=tol_b, restart=restart, maxiter=maxiter,
-                               precond=precond)
+        if differentiable:
+            # the reference's differentiable operator: the dense matvec with
+            # the self term, no near-field correction (core/bem.py:2015-2022)
+            C_d, A_d, sc_d = op.C, op.A, op.self_corr
+
+            def matvec_diff(sg, C_d=C_d, A_d=A_d, sc_d=sc_d):
+                return _bem_matvec_core_torch(C_d, A_d, sg, self_integrals=sc_d)
+
+            d_out = solve_diffbem(spec, cfg, logger, C=op.C, N=op.C, A=op.A, rhs=b,
+                                  matvec=matvec_diff, x0=x0)
+            sigma, info = d_out["sigma"], dict(d_out["stats"])
+            mv_final = matvec_diff
+        else:
+            r0 = float(torch.linalg.norm(b - op.approx(x0)))
+            bn = float(torch.linalg.norm(b))
+            tol_b = gtol * (r0 / bn if bn > 0.0 else 1.0)
+            sigma, info = op.solve(b, x0, tol=tol_b, restart=restart, maxiter=maxiter,
+                                   precond=precond)
+            mv_final = op.accurate
         info = dict(info)
         info["tol"] = gtol
         info["used_preconditioner"] = precond != "none"
@@ -318,9 +339,9 @@ def bem_solve(spec: CanonicalSpec, cfg, logger, differentiable: bool = False) ->
                                 gmres_resid=info.get("resid"))
         if not bool(torch.isfinite(sigma).all()):
             return error_result("BEM sigma contains NaN/Inf after linear solve.")
-        V_tot = torch.nan_to_num(V_free + op.accurate(sigma), nan=0.0, posinf=0.0, neginf=0.0)
-        bc_resid = float(torch.max(torch.abs(V_tot - bc)))
-        energy = compute_bem_capacitive_energy(V_tot, sigma, op.A)
+        V_tot = torch.nan_to_num(V_free + mv_final(sigma), nan=0.0, posinf=0.0, neginf=0.0)
+        bc_resid = float(torch.max(torch.abs(V_tot.detach() - bc)))
+        energy = compute_bem_capacitive_energy(V_tot.detach(), sigma.detach(), op.A)
         log("info", "Pass results.", bc_resid_linf=bc_resid, energy_A=f"{energy:.6e}")
         payload = {"pass": rp + 1, "h": current_h, "dof": N, "bc_resid_linf": bc_resid,
                    "energy_A": energy, "gmres_info": info, "tile_size": 0,
@@ -361,7 +382,8 @@ def bem_solve(spec: CanonicalSpec, cfg, logger, differentiable: bool = False) ->
         eval_f = max(eval_f, 3.0)
     pv = torch.as_tensor(mesh_b.panel_vertices, device=device, dtype=dtype) if near_eval else None
     solution = BEMSolution(spec_b, C_b, A_b, sigma_b, device, dtype, 0, normals=N_b_rm,
-                           differentiable=False, panel_vertices=pv, near_quadrature=near_eval,
+                           differentiable=differentiable, panel_vertices=pv,
+                           near_quadrature=near_eval and not differentiable,
                            near_quad_order=near_order, near_quad_dist_factor=eval_f,
                            near_quad_max_depth=self_depth)
     solution.meta["energy_A"] = best["energy_A"]
@@ -376,7 +398,7 @@ def bem_solve(spec: CanonicalSpec, cfg, logger, differentiable: bool = False) ->
                                   / 2 ** 20}
     if "plane" in {c.get("type") for c in spec_b.conductors} and mesh_stats["total_area"] > 0:
         mesh_stats["patch_L"] = math.sqrt(mesh_stats["total_area"])
-    return {"boundary_samples": Vtot_b[idx].cpu().numpy().tolist(),
+    return {"boundary_samples": Vtot_b[idx].detach().cpu().numpy().tolist(),
             "sample_points": C_b[idx].cpu().numpy().tolist(),
             "solution": solution,
             "surface_charge_density": sigma_b.detach().cpu().numpy(),

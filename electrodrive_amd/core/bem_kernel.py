@@ -20,7 +20,10 @@ from .. import _native as nat
 from .kernels import DEFAULT_SINGLE_LAYER_KERNEL, K_E, LaplaceSingleLayerKernel
 
 __all__ = ["bem_matvec_gpu", "bem_matvec_batched", "dense_matvec", "K_E",
-           "LaplaceSingleLayerKernel", "DEFAULT_SINGLE_LAYER_KERNEL"]
+           "bem_potential_targets", "bem_E_field_targets",
+           "_bem_matvec_core_torch", "_bem_potential_targets_core_torch",
+           "_bem_E_field_targets_core_torch", "bem_matvec_diff", "bem_potential_targets_diff",
+           "bem_E_field_targets_diff", "LaplaceSingleLayerKernel", "DEFAULT_SINGLE_LAYER_KERNEL"]
 
 
 def _ensure_tile_size(tile_size: Optional[int]) -> int:
@@ -151,3 +154,179 @@ def bem_E_field_targets(targets, src_centroids, areas, sigma, tile_size: int = 1
     _ensure_tile_size(tile_size)
     del use_keops, logger
     return _targets(targets, src_centroids, areas, sigma, False, True)[1]
+
+
+# -- differentiable kernels (core/bem_kernel.py:157-335, :1016-1080) ------------
+# Forward and backward both run on the native kernels.  With q = sigma * A and
+# rho_ij = rsqrt(max(|x_i - c_j|^2, 1e-24)), upstream gradient g:
+#   dense matvec  V_i = sum_{j != i} K_E q_j rho_ij + s_i q_i
+#     dL/dq_j = sum_{i != j} K_E g_i rho_ij + s_j g_j   (the dense kernel on g, unit areas)
+#     dL/dc_k = -g_k E_q(c_k) - q_k E_g(c_k)            (E_w(x) = sum K_E w (x - c) rho^3)
+#     dL/ds_i = g_i q_i
+#   target potential  V_i = sum_j K_E q_j rho_ij at x_i
+#     dL/dq_j = sum_i K_E g_i rho_ij                     (targets and sources swapped)
+#     dL/dx_i = -g_i E_q(x_i),  dL/dc_j = -q_j E_g(c_j) (g as charges at the targets)
+#   target field  E_i = sum_j K_E q_j (x_i - c_j) rho_ij^3
+#     dL/dq_j = -sum_k [field at c_j of charges G_.k at the targets]_k
+# dL/dsigma = A dL/dq and dL/dA = sigma dL/dq.  The r^2 clamp is constant
+# below 1e-24 (zero derivative there), as in the reference's torch cores.
+def _unit(n, like):
+    return torch.ones(n, dtype=like.dtype, device=like.device)
+
+
+class _DenseMatvecFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, sigma, centroids, areas, self_integrals):
+        ctx.save_for_backward(sigma, centroids, areas, self_integrals)
+        ctx.has_self = self_integrals is not None
+        return dense_matvec(sigma.detach(), centroids.detach(), areas.detach(),
+                            None if self_integrals is None else self_integrals.detach())
+
+    @staticmethod
+    def backward(ctx, g):
+        sigma, C, A, s = ctx.saved_tensors
+        g = g.detach().contiguous()
+        C, A, sig = C.detach(), A.detach(), sigma.detach()
+        n = sig.shape[0]
+        need_s, need_c, need_a, need_si = ctx.needs_input_grad[:4]
+        d_sig = d_c = d_a = d_si = None
+        q = sig * A
+        if need_s or need_a:
+            dq = dense_matvec(g, C, _unit(n, g))
+            if ctx.has_self:
+                dq = dq + s.detach() * g
+            d_sig = dq * A if need_s else None
+            d_a = dq * sig if need_a else None
+        if need_c:
+            Eq = _targets(C, C, A, sig, False, True)[1]
+            Eg = _targets(C, C, _unit(n, g), g, False, True)[1]
+            d_c = -(g[:, None] * Eq) - q[:, None] * Eg
+        if need_si and ctx.has_self:
+            d_si = g * q
+        return d_sig, d_c, d_a, d_si
+
+
+class _TargetPotentialFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, targets, centroids, areas, sigma):
+        ctx.save_for_backward(targets, centroids, areas, sigma)
+        return _targets(targets.detach(), centroids.detach(), areas.detach(), sigma.detach(),
+                        True, False)[0]
+
+    @staticmethod
+    def backward(ctx, g):
+        X, C, A, sig = (t.detach() for t in ctx.saved_tensors)
+        g = g.detach().contiguous()
+        need_x, need_c, need_a, need_s = ctx.needs_input_grad[:4]
+        d_x = d_c = d_a = d_s = None
+        q = sig * A
+        if need_s or need_a or need_c:
+            ones = _unit(X.shape[0], g)
+            if need_s or need_a:
+                dq = _targets(C, X, ones, g, True, False)[0]
+                d_s = dq * A if need_s else None
+                d_a = dq * sig if need_a else None
+            if need_c:
+                d_c = -q[:, None] * _targets(C, X, ones, g, False, True)[1]
+        if need_x:
+            d_x = -g[:, None] * _targets(X, C, A, sig, False, True)[1]
+        return d_x, d_c, d_a, d_s
+
+
+class _TargetFieldFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, targets, centroids, areas, sigma):
+        ctx.save_for_backward(targets, centroids, areas, sigma)
+        return _targets(targets.detach(), centroids.detach(), areas.detach(), sigma.detach(),
+                        False, True)[1]
+
+    @staticmethod
+    def backward(ctx, G):
+        X, C, A, sig = (t.detach() for t in ctx.saved_tensors)
+        G = G.detach().contiguous()
+        need_x, need_c, need_a, need_s = ctx.needs_input_grad[:4]
+        if need_x or need_c:
+            raise NotImplementedError("gradients of the target field with respect to target or "
+                                      "source positions are not provided")
+        d_a = d_s = None
+        if need_s or need_a:
+            ones = _unit(X.shape[0], G)
+            dq = torch.zeros_like(sig)
+            for k in range(3):
+                dq = dq - _targets(C, X, ones, G[:, k].contiguous(), False, True)[1][:, k]
+            d_s = dq * A if need_s else None
+            d_a = dq * sig if need_a else None
+        return None, None, d_a, d_s
+
+
+def _as_dev(t, like):
+    return torch.as_tensor(t).to(like.device, like.dtype)
+
+
+def _bem_matvec_core_torch(centroids, areas, sigma, self_integrals=None, tile_size: int = 1024,
+                           kernel=None):
+    """Differentiable dense single-layer matvec (core/bem_kernel.py:157-230):
+    same values as ``bem_matvec_gpu``, gradients to sigma, centroids, areas and
+    self_integrals (native kernels both ways)."""
+    ker = kernel or DEFAULT_SINGLE_LAYER_KERNEL
+    if not isinstance(ker, LaplaceSingleLayerKernel):
+        raise ValueError("only the Laplace single-layer kernel is implemented natively")
+    _ensure_tile_size(tile_size)
+    nat.require_gpu()
+    if sigma.dim() != 1:
+        raise ValueError("sigma must have shape (N,)")
+    dev = sigma.device if sigma.is_cuda else torch.device("cuda", torch.cuda.current_device())
+    s = sigma.to(dev)
+    C = _as_dev(centroids, s).reshape(-1, 3)
+    A = _as_dev(areas, s).reshape(-1)
+    si = None if self_integrals is None else _as_dev(self_integrals, s).reshape(-1)
+    return _DenseMatvecFn.apply(s, C, A, si).to(sigma.device)
+
+
+def _bem_potential_targets_core_torch(targets, src_centroids, areas, sigma, tile_size: int = 1024,
+                                      kernel=None):
+    """Differentiable induced potential at targets (core/bem_kernel.py:233-276)."""
+    ker = kernel or DEFAULT_SINGLE_LAYER_KERNEL
+    if not isinstance(ker, LaplaceSingleLayerKernel):
+        raise ValueError("only the Laplace single-layer kernel is implemented natively")
+    nat.require_gpu()
+    dev = sigma.device if sigma.is_cuda else torch.device("cuda", torch.cuda.current_device())
+    s = sigma.to(dev).reshape(-1)
+    return _TargetPotentialFn.apply(_as_dev(targets, s).reshape(-1, 3),
+                                    _as_dev(src_centroids, s).reshape(-1, 3),
+                                    _as_dev(areas, s).reshape(-1), s).to(sigma.device)
+
+
+def _bem_E_field_targets_core_torch(targets, src_centroids, areas, sigma, tile_size: int = 1024,
+                                    kernel=None):
+    """Differentiable induced E-field at targets (core/bem_kernel.py:279-335);
+    gradients to sigma and areas."""
+    ker = kernel or DEFAULT_SINGLE_LAYER_KERNEL
+    if not isinstance(ker, LaplaceSingleLayerKernel):
+        raise ValueError("only the Laplace single-layer kernel is implemented natively")
+    nat.require_gpu()
+    dev = sigma.device if sigma.is_cuda else torch.device("cuda", torch.cuda.current_device())
+    s = sigma.to(dev).reshape(-1)
+    return _TargetFieldFn.apply(_as_dev(targets, s).reshape(-1, 3),
+                                _as_dev(src_centroids, s).reshape(-1, 3),
+                                _as_dev(areas, s).reshape(-1), s).to(sigma.device)
+
+
+def bem_matvec_diff(sigma, src_centroids, areas, *, tile_size: int = 1024, self_integrals=None,
+                    kernel=None):
+    """core/bem_kernel.py:1016-1037."""
+    return _bem_matvec_core_torch(src_centroids, areas, sigma, self_integrals, tile_size, kernel)
+
+
+def bem_potential_targets_diff(targets, src_centroids, areas, sigma, tile_size: int = 1024,
+                               kernel=None):
+    """core/bem_kernel.py:1040-1058."""
+    return _bem_potential_targets_core_torch(targets, src_centroids, areas, sigma, tile_size,
+                                             kernel)
+
+
+def bem_E_field_targets_diff(targets, src_centroids, areas, sigma, tile_size: int = 1024,
+                             kernel=None):
+    """core/bem_kernel.py:1061-1079."""
+    return _bem_E_field_targets_core_torch(targets, src_centroids, areas, sigma, tile_size,
+                                           kernel)
