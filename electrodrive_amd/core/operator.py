@@ -172,8 +172,11 @@ class BemOperator:
         G = len(self.meshes)
         if G > 1:
             # batches: the operator is block diagonal, so each geometry's block
-            # is scaled to a unit right-hand side and the tolerance divided by
-            # sqrt(G); every block then meets tol relative to its own |b_g|
+            # is scaled to a unit right-hand side.  Without a coarse space the
+            # native solver runs one Krylov space per geometry and stops each
+            # at tol |b_g|; a coarse space couples the geometries into one
+            # system, solved to tol / sqrt(G) so that every block still meets
+            # tol relative to its own |b_g|
             scale = torch.ones_like(b)
             for g in range(G):
                 lo, hi = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
@@ -181,7 +184,8 @@ class BemOperator:
                 scale[lo:hi] = 1.0 / nb if nb > 0.0 else 1.0
             x, info = self.plan.solve((b * scale).contiguous(), (x0 * scale).contiguous(),
                                       self.diag if use_diag else None, self.self_corr,
-                                      restart=restart, maxiter=maxiter, tol=tol / math.sqrt(G),
+                                      restart=restart, maxiter=maxiter,
+                                      tol=tol if self.plan.coarse_k == 0 else tol / math.sqrt(G),
                                       precond=precond, refine=refine, max_refine=max_refine,
                                       inner_tol=inner_tol, use_graph=use_graph,
                                       check_every=check_every)

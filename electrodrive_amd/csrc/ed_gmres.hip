@@ -10,8 +10,15 @@
 // MGS, and a cycle ends as soon as the preconditioned residual estimate meets
 // its tolerance instead of always running `restart` steps (bem_solver.py:703-707).
 // All vectors live in tree order; b / x / diag are permuted on entry and exit.
+// A batched plan's operator is block diagonal over its geometries: without a
+// coarse space the Arnoldi process runs on every geometry independently (one
+// Krylov space, Hessenberg matrix and Givens sequence per geometry, all
+// advanced by the same launches) and geometry g is converged at
+// |r_g| <= tol |b_g|, so the step count is that of the slowest geometry,
+// not that of the joint spectrum.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -29,11 +36,23 @@ struct GmresWS {
   double *V = nullptr, *w = nullptr, *vin = nullptr, *x = nullptr, *b = nullptr, *r = nullptr,
          *t = nullptr, *dinv = nullptr, *selfi_t = nullptr, *d = nullptr, *rhs = nullptr,
          *u2 = nullptr;
+  // per segment: Hessenberg (m+1) x m, g (m+1), Givens cs / sn (m), CGS
+  // coefficients h / h2 (m+2; h also holds y after the back substitution)
   double *H = nullptr, *g = nullptr, *cs = nullptr, *sn = nullptr, *h = nullptr, *h2 = nullptr;
-  double* partial = nullptr;  // kRedBlocks x (m + 2)
+  double* partial = nullptr;  // (m + 2) x tiles
   double* scal = nullptr;     // device scalars
   double* hscal = nullptr;    // pinned host mirror
   int* dj = nullptr;          // device Arnoldi index
+  // Segments: the Arnoldi process runs independently on every geometry of a
+  // batched plan (the operator is block diagonal), sharing each launch.
+  // Tiling k = 0: one segment per geometry; k = 1: the whole vector as one
+  // segment (single geometries, or a coarse space that couples geometries).
+  int Gmax = 1;
+  int G_k[2] = {1, 1}, NT_k[2] = {0, 0};
+  int4* tiles_k[2] = {nullptr, nullptr};  // (segment, lo, hi, 0)
+  int* seg_t0_k[2] = {nullptr, nullptr};  // tile range of each segment
+  double* segv = nullptr;   // device per-segment values (see SV_*), 8 x Gmax
+  double* hsegv = nullptr;  // pinned host mirror
   hipStream_t stream = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -42,7 +61,9 @@ struct GmresWS {
 };
 
 // device scalar slots
-enum { S_EST = 0, S_BETA = 1, S_HNEXT = 2, S_NRM_R = 3, S_NRM_Z = 4, S_BREAK = 5, S_NSCAL = 8 };
+enum { S_EST = 0, S_BREAK = 5, S_NSCAL = 8 };
+// per-segment value rows of segv (each Gmax long)
+enum { SV_NRM_R = 0, SV_NRM_Z = 1, SV_EST_TOL = 2, SV_ACTIVE = 3, SV_HNEXT = 4, SV_N = 8 };
 
 void gmres_free(ed_plan* P) {
   GmresWS* W = (GmresWS*)P->gmres_ws;
@@ -50,10 +71,12 @@ void gmres_free(ed_plan* P) {
   if (W->gexec) (void)hipGraphExecDestroy(W->gexec);
   if (W->graph) (void)hipGraphDestroy(W->graph);
   void* ptrs[] = {W->V, W->w, W->vin, W->x, W->b, W->r, W->t, W->dinv, W->selfi_t, W->d, W->rhs,
-                  W->u2, W->H, W->g, W->cs, W->sn, W->h, W->h2, W->partial, W->scal, W->dj};
+                  W->u2, W->H, W->g, W->cs, W->sn, W->h, W->h2, W->partial, W->scal, W->dj,
+                  W->tiles_k[0], W->tiles_k[1], W->seg_t0_k[0], W->seg_t0_k[1], W->segv};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (W->hscal) (void)hipHostFree(W->hscal);
+  if (W->hsegv) (void)hipHostFree(W->hsegv);
   if (W->stream) (void)hipStreamDestroy(W->stream);
   delete W;
   P->gmres_ws = nullptr;
@@ -91,6 +114,18 @@ void gmres_invalidate_graph(ed_plan* P) {
       return fail(ED_ERR_LAUNCH, std::string(msg) + ": " + hipGetErrorString(e_));      \
   } while (0)
 
+// Tiles of at most `ts` entries inside each segment [off[g], off[g+1]).
+static void make_tiles(const std::vector<int64_t>& off, int64_t ts, std::vector<int4>& tiles,
+                       std::vector<int>& t0) {
+  tiles.clear();
+  t0.assign(1, 0);
+  for (size_t g = 0; g + 1 < off.size(); ++g) {
+    for (int64_t lo = off[g]; lo < off[g + 1]; lo += ts)
+      tiles.push_back(make_int4((int)g, (int)lo, (int)std::min(lo + ts, off[g + 1]), 0));
+    t0.push_back((int)tiles.size());
+  }
+}
+
 static int ws_get(ed_plan* P, int m, GmresWS** out) {
   GmresWS* W = (GmresWS*)P->gmres_ws;
   if (W && W->m == m && W->n == P->n) { *out = W; return ED_OK; }
@@ -100,20 +135,42 @@ static int ws_get(ed_plan* P, int m, GmresWS** out) {
   W->n = P->n;
   W->m = m;
   const size_t n = (size_t)P->n;
+  std::vector<int64_t> geo = P->host.geom_pt_off;
+  if (geo.size() < 2) geo = {0, (int64_t)n};
+  const std::vector<int64_t> whole = {0, (int64_t)n};
+  // ~512 tiles over the whole vector, 256..4096 entries each
+  const int64_t ts = std::min<int64_t>(4096, std::max<int64_t>(256, ((int64_t)n / 512 + 63) / 64 * 64));
+  int ntmax = 1;
+  for (int k = 0; k < 2; ++k) {
+    std::vector<int4> tl;
+    std::vector<int> t0;
+    make_tiles(k == 0 ? geo : whole, ts, tl, t0);
+    W->G_k[k] = (int)t0.size() - 1;
+    W->NT_k[k] = (int)tl.size();
+    ntmax = std::max(ntmax, W->NT_k[k]);
+    ED_G(hipMalloc((void**)&W->tiles_k[k], sizeof(int4) * std::max<size_t>(tl.size(), 1)));
+    ED_G(hipMalloc((void**)&W->seg_t0_k[k], sizeof(int) * t0.size()));
+    if (!tl.empty())
+      ED_G(hipMemcpy(W->tiles_k[k], tl.data(), sizeof(int4) * tl.size(), hipMemcpyHostToDevice));
+    ED_G(hipMemcpy(W->seg_t0_k[k], t0.data(), sizeof(int) * t0.size(), hipMemcpyHostToDevice));
+  }
+  const int G = W->Gmax = std::max(W->G_k[0], W->G_k[1]);
   ED_G(hipMalloc((void**)&W->V, sizeof(double) * n * (m + 1)));
   double** vecs[] = {&W->w, &W->vin, &W->x, &W->b, &W->r, &W->t, &W->dinv, &W->selfi_t, &W->d,
                      &W->rhs, &W->u2};
   for (double** v : vecs) ED_G(hipMalloc((void**)v, sizeof(double) * n));
-  ED_G(hipMalloc((void**)&W->H, sizeof(double) * (m + 1) * m));
-  ED_G(hipMalloc((void**)&W->g, sizeof(double) * (m + 1)));
-  ED_G(hipMalloc((void**)&W->cs, sizeof(double) * m));
-  ED_G(hipMalloc((void**)&W->sn, sizeof(double) * m));
-  ED_G(hipMalloc((void**)&W->h, sizeof(double) * (m + 2)));
-  ED_G(hipMalloc((void**)&W->h2, sizeof(double) * (m + 2)));
-  ED_G(hipMalloc((void**)&W->partial, sizeof(double) * kRedBlocks * (m + 2)));
+  ED_G(hipMalloc((void**)&W->H, sizeof(double) * G * (m + 1) * m));
+  ED_G(hipMalloc((void**)&W->g, sizeof(double) * G * (m + 1)));
+  ED_G(hipMalloc((void**)&W->cs, sizeof(double) * G * m));
+  ED_G(hipMalloc((void**)&W->sn, sizeof(double) * G * m));
+  ED_G(hipMalloc((void**)&W->h, sizeof(double) * G * (m + 2)));
+  ED_G(hipMalloc((void**)&W->h2, sizeof(double) * G * (m + 2)));
+  ED_G(hipMalloc((void**)&W->partial, sizeof(double) * (size_t)ntmax * (m + 2)));
   ED_G(hipMalloc((void**)&W->scal, sizeof(double) * S_NSCAL));
+  ED_G(hipMalloc((void**)&W->segv, sizeof(double) * SV_N * G));
   ED_G(hipMalloc((void**)&W->dj, sizeof(int)));
   ED_G(hipHostMalloc((void**)&W->hscal, sizeof(double) * S_NSCAL, hipHostMallocDefault));
+  ED_G(hipHostMalloc((void**)&W->hsegv, sizeof(double) * SV_N * G, hipHostMallocDefault));
   ED_G(hipStreamCreateWithFlags(&W->stream, hipStreamNonBlocking));
   *out = W;
   return ED_OK;
@@ -155,30 +212,6 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
   return s;  // valid in thread 0
 }
 
-// r = b - t, partial norms of r and of dinv .* r (slots 0, 1)
-__global__ __launch_bounds__(kRedThreads) void k_resid(int64_t n, const double* __restrict__ b,
-                                                       const double* __restrict__ t,
-                                                       const double* __restrict__ dinv,
-                                                       double* __restrict__ r,
-                                                       double* __restrict__ z,
-                                                       double* __restrict__ partial) {
-  __shared__ double sh[4];
-  double s0 = 0.0, s1 = 0.0;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const double ri = b[i] - (t ? t[i] : 0.0);
-    if (r) r[i] = ri;
-    const double zi = dinv[i] * ri;
-    if (z) z[i] = zi;
-    s0 += ri * ri;
-    s1 += zi * zi;
-  }
-  double a = block_sum(s0, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = a;
-  a = block_sum(s1, sh);
-  if (threadIdx.x == 0) partial[kRedBlocks + blockIdx.x] = a;
-}
-
 // Fixed-order reduction of `cnt` partial vectors; writes sqrt into scal slots.
 __global__ void k_finish_norms(const double* __restrict__ partial, int cnt, double* __restrict__ scal,
                                int slot0) {
@@ -191,20 +224,6 @@ __global__ void k_finish_norms(const double* __restrict__ partial, int cnt, doub
   }
 }
 
-// V0 = z / beta; g = [beta, 0...]; j = 0
-__global__ void k_start(int64_t n, int m, const double* __restrict__ z, double* __restrict__ V,
-                        double* __restrict__ g, const double* __restrict__ scal, int* dj) {
-  const double beta = scal[S_NRM_Z];
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    V[i] = z[i] / beta;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    *dj = 0;
-  }
-  if (blockIdx.x == 0)
-    for (int k = threadIdx.x; k <= m; k += blockDim.x) g[k] = (k == 0) ? beta : 0.0;
-}
-
 __global__ void k_copy_vj(int64_t n, const double* __restrict__ V, const int* __restrict__ dj,
                           double* __restrict__ vin) {
   const int j = *dj;
@@ -214,59 +233,140 @@ __global__ void k_copy_vj(int64_t n, const double* __restrict__ V, const int* __
     vin[i] = vj[i];
 }
 
-// w = dinv .* w; partial dots V_k . w for k <= j
-__global__ __launch_bounds__(kRedThreads) void k_dots1(int64_t n, const double* __restrict__ dinv,
-                                                       double* __restrict__ w,
-                                                       const double* __restrict__ V,
-                                                       const int* __restrict__ dj,
-                                                       double* __restrict__ partial) {
+// ---- segmented Arnoldi kernels --------------------------------------------
+// One block per tile (segment, lo, hi); per-tile partial sums are reduced per
+// segment over its tile range in a fixed order, so every result is
+// deterministic.  Segment values live in rows of segv (SV_*, stride G).
+
+// r = b - t (t may be NULL), z = dinv .* r; partial[0][tile] = |r|^2,
+// partial[1][tile] = |z|^2 (z, r optional outputs)
+__global__ __launch_bounds__(kRedThreads) void k_resid_seg(const int4* __restrict__ tiles, int nt,
+                                                           const double* __restrict__ b,
+                                                           const double* __restrict__ t,
+                                                           const double* __restrict__ dinv,
+                                                           double* __restrict__ r,
+                                                           double* __restrict__ z,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int4 tl = tiles[blockIdx.x];
+  double s0 = 0.0, s1 = 0.0;
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) {
+    const double ri = b[i] - (t ? t[i] : 0.0);
+    if (r) r[i] = ri;
+    const double zi = dinv[i] * ri;
+    if (z) z[i] = zi;
+    s0 += ri * ri;
+    s1 += zi * zi;
+  }
+  double a = block_sum(s0, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+  a = block_sum(s1, sh);
+  if (threadIdx.x == 0) partial[nt + blockIdx.x] = a;
+}
+
+// partial[tile] = |x|^2 over the tile
+__global__ __launch_bounds__(kRedThreads) void k_norm2_seg(const int4* __restrict__ tiles,
+                                                           const double* __restrict__ x,
+                                                           double* __restrict__ partial) {
+  __shared__ double sh[4];
+  const int4 tl = tiles[blockIdx.x];
+  double s = 0.0;
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) s += x[i] * x[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+// out[k * ldo + g] = sqrt(sum of partial[k][tiles of g]), k < cnt; one block
+// per segment
+__global__ void k_finish_seg_norms(const double* __restrict__ partial, int nt, int cnt,
+                                   const int* __restrict__ seg_t0, double* __restrict__ out,
+                                   int ldo) {
+  __shared__ double sh[4];
+  const int g = blockIdx.x;
+  const int t0 = seg_t0[g], t1 = seg_t0[g + 1];
+  for (int k = 0; k < cnt; ++k) {
+    double v = 0.0;
+    for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v += partial[(size_t)k * nt + t];
+    v = block_sum(v, sh);
+    if (threadIdx.x == 0) out[(size_t)k * ldo + g] = sqrt(v);
+  }
+}
+
+// V0 = z / beta_g (0 for an inactive segment); g_g = [beta_g, 0...]; j = 0
+__global__ void k_start_seg(const int4* __restrict__ tiles, int G, int m,
+                            const double* __restrict__ z, double* __restrict__ V,
+                            double* __restrict__ gv, const double* __restrict__ segv, int ldsv,
+                            int* dj) {
+  const int4 tl = tiles[blockIdx.x];
+  const double beta = segv[SV_NRM_Z * ldsv + tl.x];
+  const bool act = segv[SV_ACTIVE * ldsv + tl.x] != 0.0 && beta > 0.0;
+  const double inv = act ? 1.0 / beta : 0.0;
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) V[i] = z[i] * inv;
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) *dj = 0;
+    for (int e = threadIdx.x; e < G * (m + 1); e += blockDim.x) {
+      const int sg = e / (m + 1), k = e - sg * (m + 1);
+      const bool a = segv[SV_ACTIVE * ldsv + sg] != 0.0;
+      gv[e] = (k == 0 && a) ? segv[SV_NRM_Z * ldsv + sg] : 0.0;
+    }
+  }
+}
+
+// w = dinv .* w (optional); partial[k][tile] = V_k . w, k <= j
+__global__ __launch_bounds__(kRedThreads) void k_dots_seg(const int4* __restrict__ tiles, int nt,
+                                                          int64_t n, const double* __restrict__ dinv,
+                                                          double* __restrict__ w,
+                                                          const double* __restrict__ V,
+                                                          const int* __restrict__ dj,
+                                                          double* __restrict__ partial) {
   __shared__ double sh[4];
   const int j = *dj;
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
+  const int4 tl = tiles[blockIdx.x];
   if (dinv)
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) w[i] *= dinv[i];
+    for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) w[i] *= dinv[i];
   __syncthreads();
   for (int k = 0; k <= j; ++k) {
     const double* vk = V + (size_t)k * n;
     double s = 0.0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += vk[i] * w[i];
+    for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) s += vk[i] * w[i];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[(size_t)k * kRedBlocks + blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[(size_t)k * nt + blockIdx.x] = s;
   }
 }
 
-// h[k] (=, or += when accumulate) = sum_b partial[k][b], k <= j (+1 extra row
-// when `extra` is set: the norm slot at index j+1)
-__global__ void k_finish_dots(const double* __restrict__ partial, const int* __restrict__ dj,
-                              double* __restrict__ h, int accumulate) {
+// h[g * ldh + k] = sum of partial[k][tiles of g], k <= j; one block per segment
+__global__ void k_finish_dots_seg(const double* __restrict__ partial, int nt,
+                                  const int* __restrict__ seg_t0, const int* __restrict__ dj,
+                                  double* __restrict__ h, int ldh) {
   __shared__ double sh[4];
+  const int g = blockIdx.x;
+  const int t0 = seg_t0[g], t1 = seg_t0[g + 1];
   const int j = *dj;
   for (int k = 0; k <= j; ++k) {
     double v = 0.0;
-    for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[(size_t)k * kRedBlocks + b];
+    for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v += partial[(size_t)k * nt + t];
     v = block_sum(v, sh);
-    if (threadIdx.x == 0) h[k] = accumulate ? h[k] + v : v;
+    if (threadIdx.x == 0) h[(size_t)g * ldh + k] = v;
   }
 }
 
-// w -= sum_k c[k] V_k  (c = h, or c = h2 = h - h_prev stored by caller); then
-// partial dots of V_k . w (second CGS pass) or of w . w (norm pass).
-__global__ __launch_bounds__(kRedThreads) void k_update(int64_t n, double* __restrict__ w,
-                                                        const double* __restrict__ V,
-                                                        const int* __restrict__ dj,
-                                                        const double* __restrict__ c,
-                                                        int norm_pass,
-                                                        double* __restrict__ partial) {
+// w -= sum_k c[g][k] V_k over the tile; then partial dots V_k . w (second CGS
+// pass) or |w|^2 (norm pass)
+__global__ __launch_bounds__(kRedThreads) void k_update_seg(const int4* __restrict__ tiles, int nt,
+                                                            int64_t n, double* __restrict__ w,
+                                                            const double* __restrict__ V,
+                                                            const int* __restrict__ dj,
+                                                            const double* __restrict__ c, int ldc,
+                                                            int norm_pass,
+                                                            double* __restrict__ partial) {
   __shared__ double sh[4];
   __shared__ double cc[256];
   const int j = *dj;
-  for (int k = threadIdx.x; k <= j; k += blockDim.x) cc[k] = c[k];
+  const int4 tl = tiles[blockIdx.x];
+  for (int k = threadIdx.x; k <= j; k += blockDim.x) cc[k] = c[(size_t)tl.x * ldc + k];
   __syncthreads();
-  const int64_t chunk = (n + gridDim.x - 1) / gridDim.x;
-  const int64_t i0 = (int64_t)blockIdx.x * chunk, i1 = min(n, i0 + chunk);
   double nrm = 0.0;
-  for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) {
     double wi = w[i];
     for (int k = 0; k <= j; ++k) wi -= cc[k] * V[(size_t)k * n + i];
     w[i] = wi;
@@ -281,85 +381,124 @@ __global__ __launch_bounds__(kRedThreads) void k_update(int64_t n, double* __res
   for (int k = 0; k <= j; ++k) {
     const double* vk = V + (size_t)k * n;
     double s = 0.0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) s += vk[i] * w[i];
+    for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) s += vk[i] * w[i];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[(size_t)k * kRedBlocks + blockIdx.x] = s;
+    if (threadIdx.x == 0) partial[(size_t)k * nt + blockIdx.x] = s;
   }
 }
 
-// After CGS2: H[:, j] = h1 + h2, H[j+1, j] = ||w||; apply Givens rotations
-// (core/bem_solver.py:590-616); est = |g[j+1]|; j++.
-__global__ void k_givens(int m, const double* __restrict__ partial, double* __restrict__ h1,
-                         const double* __restrict__ h2, double* __restrict__ H,
-                         double* __restrict__ g, double* __restrict__ cs, double* __restrict__ sn,
-                         double* __restrict__ scal, int* dj) {
-  __shared__ double sh[4];
-  double v = 0.0;
-  for (int b = threadIdx.x; b < kRedBlocks; b += blockDim.x) v += partial[b];
-  v = block_sum(v, sh);
-  if (threadIdx.x != 0) return;
+// Per segment (one thread each): |w_g| from the norm-pass partials, column
+// H_g[:, j] = h1 + h2 with H[j+1, j] = |w_g|, the Givens rotations
+// (core/bem_solver.py:590-616) and est_g = |g_g[j+1]|.  Scalars: S_EST =
+// max over active segments of est_g / est_tol_g, S_BREAK = 1 when an active
+// segment breaks down.  j++.
+__global__ void k_givens_seg(int G, int m, const double* __restrict__ partial,
+                             const int* __restrict__ seg_t0, const double* __restrict__ h1,
+                             const double* __restrict__ h2, double* __restrict__ H,
+                             double* __restrict__ gv, double* __restrict__ cs,
+                             double* __restrict__ sn, double* __restrict__ segv, int ldsv,
+                             double* __restrict__ scal, int* dj) {
+  __shared__ double smax[256];
+  __shared__ int sbrk[256];
   const int j = *dj;
   const int ld = m + 1;
-  double* col = H + (size_t)j * ld;
-  for (int k = 0; k <= j; ++k) col[k] = h1[k] + h2[k];
-  const double hn = sqrt(v);
-  col[j + 1] = hn;
-  scal[S_HNEXT] = hn;
-  if (!isfinite(hn) || hn <= 0.0) scal[S_BREAK] = 1.0;
-  for (int k = 0; k < j; ++k) {
-    const double t = cs[k] * col[k] + sn[k] * col[k + 1];
-    col[k + 1] = -sn[k] * col[k] + cs[k] * col[k + 1];
-    col[k] = t;
+  double rmax = 0.0;
+  int brk = 0;
+  for (int sg = threadIdx.x; sg < G; sg += blockDim.x) {
+    double v = 0.0;
+    for (int t = seg_t0[sg]; t < seg_t0[sg + 1]; ++t) v += partial[t];
+    const double* a1 = h1 + (size_t)sg * (m + 2);
+    const double* a2 = h2 + (size_t)sg * (m + 2);
+    double* col = H + ((size_t)sg * m + j) * ld;
+    double* g = gv + (size_t)sg * ld;
+    double* c_ = cs + (size_t)sg * m;
+    double* s_ = sn + (size_t)sg * m;
+    for (int k = 0; k <= j; ++k) col[k] = a1[k] + a2[k];
+    const double hn = sqrt(v);
+    col[j + 1] = hn;
+    segv[SV_HNEXT * ldsv + sg] = hn;
+    const bool act = segv[SV_ACTIVE * ldsv + sg] != 0.0;
+    if (act && (!isfinite(hn) || hn <= 0.0)) brk = 1;
+    for (int k = 0; k < j; ++k) {
+      const double t = c_[k] * col[k] + s_[k] * col[k + 1];
+      col[k + 1] = -s_[k] * col[k] + c_[k] * col[k + 1];
+      col[k] = t;
+    }
+    const double den = sqrt(col[j] * col[j] + col[j + 1] * col[j + 1]);
+    double c = 1.0, s = 0.0;
+    if (den >= 1e-14 && isfinite(den)) { c = col[j] / den; s = col[j + 1] / den; }
+    c_[j] = c;
+    s_[j] = s;
+    col[j] = c * col[j] + s * col[j + 1];
+    col[j + 1] = 0.0;
+    const double gj = g[j];
+    g[j] = c * gj;
+    g[j + 1] = -s * gj;
+    if (act) {
+      const double et = segv[SV_EST_TOL * ldsv + sg];
+      const double ratio = fabs(g[j + 1]) / (et > 0.0 ? et : 1e-300);
+      rmax = isfinite(ratio) ? fmax(rmax, ratio) : INFINITY;
+    }
   }
-  const double den = sqrt(col[j] * col[j] + col[j + 1] * col[j + 1]);
-  double c = 1.0, s = 0.0;
-  if (den >= 1e-14 && isfinite(den)) { c = col[j] / den; s = col[j + 1] / den; }
-  cs[j] = c;
-  sn[j] = s;
-  col[j] = c * col[j] + s * col[j + 1];
-  col[j + 1] = 0.0;
-  const double gj = g[j];
-  g[j] = c * gj;
-  g[j + 1] = -s * gj;
-  scal[S_EST] = fabs(g[j + 1]);
-  *dj = j + 1;
+  smax[threadIdx.x] = rmax;
+  sbrk[threadIdx.x] = brk;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    int b = 0;
+    for (int k = 0; k < (int)blockDim.x; ++k) {
+      r = fmax(r, smax[k]);
+      b |= sbrk[k];
+    }
+    scal[S_EST] = r;
+    if (b) scal[S_BREAK] = 1.0;
+    *dj = j + 1;
+  }
 }
 
-// V_{j} = w / hnext  (j already incremented)
-__global__ void k_scale_next(int64_t n, const double* __restrict__ w, double* __restrict__ V,
-                             const int* __restrict__ dj, const double* __restrict__ scal) {
+// V_j = w / hnext_g over the tile (j already incremented; 0 after a breakdown)
+__global__ void k_scale_next_seg(const int4* __restrict__ tiles, int64_t n,
+                                 const double* __restrict__ w, double* __restrict__ V,
+                                 const int* __restrict__ dj, const double* __restrict__ segv,
+                                 int ldsv) {
   const int j = *dj;
-  const double hn = scal[S_HNEXT];
+  const int4 tl = tiles[blockIdx.x];
+  const double hn = segv[SV_HNEXT * ldsv + tl.x];
   const double inv = (hn > 0.0 && isfinite(hn)) ? 1.0 / hn : 0.0;
   double* vj = V + (size_t)j * n;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    vj[i] = w[i] * inv;
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) vj[i] = w[i] * inv;
 }
 
-// y = R^{-1} g (back substitution, m_use = *dj), stored in h
-__global__ void k_backsolve(int m, const double* __restrict__ H, const double* __restrict__ g,
-                            const int* __restrict__ dj, double* __restrict__ y) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// y_g = R_g^{-1} g_g (back substitution, m_use = *dj), one thread per segment,
+// stored in h
+__global__ void k_backsolve_seg(int G, int m, const double* __restrict__ H,
+                                const double* __restrict__ gv, const int* __restrict__ dj,
+                                double* __restrict__ y) {
   const int mu = *dj;
   const int ld = m + 1;
-  for (int i = mu - 1; i >= 0; --i) {
-    double s = g[i];
-    for (int k = i + 1; k < mu; ++k) s -= H[(size_t)k * ld + i] * y[k];
-    const double d = H[(size_t)i * ld + i];
-    y[i] = d != 0.0 ? s / d : 0.0;
+  for (int sg = blockIdx.x * blockDim.x + threadIdx.x; sg < G; sg += gridDim.x * blockDim.x) {
+    const double* Hs = H + (size_t)sg * m * ld;
+    const double* g = gv + (size_t)sg * ld;
+    double* ys = y + (size_t)sg * (m + 2);
+    for (int i = mu - 1; i >= 0; --i) {
+      double s = g[i];
+      for (int k = i + 1; k < mu; ++k) s -= Hs[(size_t)k * ld + i] * ys[k];
+      const double d = Hs[(size_t)i * ld + i];
+      ys[i] = (d != 0.0 && isfinite(d)) ? s / d : 0.0;
+    }
   }
 }
 
-// x += sum_{k < m_use} y_k V_k
-__global__ void k_xupdate(int64_t n, double* __restrict__ x, const double* __restrict__ V,
-                          const double* __restrict__ y, const int* __restrict__ dj) {
+// x += sum_{k < m_use} y_g[k] V_k over the tile
+__global__ void k_xupdate_seg(const int4* __restrict__ tiles, int64_t n, int m,
+                              double* __restrict__ x, const double* __restrict__ V,
+                              const double* __restrict__ y, const int* __restrict__ dj) {
   __shared__ double yy[256];
   const int mu = *dj;
-  for (int k = threadIdx.x; k < mu; k += blockDim.x) yy[k] = y[k];
+  const int4 tl = tiles[blockIdx.x];
+  for (int k = threadIdx.x; k < mu; k += blockDim.x) yy[k] = y[(size_t)tl.x * (m + 2) + k];
   __syncthreads();
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) {
     double s = x[i];
     for (int k = 0; k < mu; ++k) s += yy[k] * V[(size_t)k * n + i];
     x[i] = s;
@@ -634,14 +773,24 @@ struct Solver {
     }
     sai_apply_launch(P, u, k, P->coarse_z, P->coarse_cy, z, W->stream);
   }
-  // norms of (b - t) and M^-1 (b - t); r, z optional outputs
-  int resid_norms(const double* b, const double* t, double* r, double* z, double* nr, double* nz) {
+  // active tiling: per geometry unless a coarse space couples the geometries
+  int tk() const { return P->coarse_k > 0 ? 1 : 0; }
+  int G() const { return W->G_k[tk()]; }
+  int NT() const { return W->NT_k[tk()]; }
+  const int4* tiles() const { return W->tiles_k[tk()]; }
+  const int* seg_t0() const { return W->seg_t0_k[tk()]; }
+
+  // per-segment norms of (b - t) and M^-1 (b - t) into segv rows SV_NRM_R /
+  // SV_NRM_Z and the host vectors nr, nz; r, z optional outputs
+  int resid_norms(const double* b, const double* t, double* r, double* z, std::vector<double>& nr,
+                  std::vector<double>& nz) {
     const bool gen = general() && z != nullptr;
     double* rr = (gen && r == nullptr) ? W->vin : r;
-    hipLaunchKernelGGL(k_resid, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, b, t,
+    const int nt = NT(), g = G(), ld = W->Gmax;
+    hipLaunchKernelGGL(k_resid_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), nt, b, t,
                        W->dinv, rr, gen ? nullptr : z, W->partial);
-    hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, W->stream, W->partial, 2, W->scal,
-                       S_NRM_R);
+    hipLaunchKernelGGL(k_finish_seg_norms, dim3(g), dim3(256), 0, W->stream, W->partial, nt, 2,
+                       seg_t0(), W->segv + SV_NRM_R * ld, ld);
     if (gen) {
       if (use_sai()) {
         ED_G(hipMemcpyAsync(W->u2, rr, sizeof(double) * n, hipMemcpyDeviceToDevice, W->stream));
@@ -649,20 +798,21 @@ struct Solver {
       } else {
         coarse_apply(rr, z);
       }
-      hipLaunchKernelGGL(k_norm2_part, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, z,
+      hipLaunchKernelGGL(k_norm2_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), z,
                          W->partial);
-      hipLaunchKernelGGL(k_finish_norms, dim3(1), dim3(256), 0, W->stream, W->partial, 1, W->scal,
-                         S_NRM_Z);
+      hipLaunchKernelGGL(k_finish_seg_norms, dim3(g), dim3(256), 0, W->stream, W->partial, nt, 1,
+                         seg_t0(), W->segv + SV_NRM_Z * ld, ld);
     }
     ED_KCHECK("resid");
-    ED_G(hipMemcpyAsync(W->hscal, W->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, W->stream));
+    ED_G(hipMemcpyAsync(W->hsegv, W->segv, sizeof(double) * 2 * ld, hipMemcpyDeviceToHost,
+                        W->stream));
     ED_G(hipStreamSynchronize(W->stream));
-    *nr = W->hscal[S_NRM_R];
-    *nz = W->hscal[S_NRM_Z];
+    nr.assign(W->hsegv + SV_NRM_R * ld, W->hsegv + SV_NRM_R * ld + g);
+    nz.assign(W->hsegv + SV_NRM_Z * ld, W->hsegv + SV_NRM_Z * ld + g);
     return ED_OK;
   }
   int arnoldi_step() {
-    const int m = W->m;
+    const int m = W->m, nt = NT(), g = G(), ld = W->Gmax;
     hipLaunchKernelGGL(k_copy_vj, dim3(gb), dim3(256), 0, W->stream, n, W->V, W->dj, W->vin);
     if (use_sai()) {
       ED_T(approx_op(W->vin, W->u2));
@@ -671,44 +821,69 @@ struct Solver {
       ED_T(approx_op(W->vin, W->w));
       if (P->coarse_k > 0) coarse_apply(W->w, W->w);
     }
-    hipLaunchKernelGGL(k_dots1, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n,
+    hipLaunchKernelGGL(k_dots_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), nt, n,
                        general() ? nullptr : W->dinv, W->w, W->V, W->dj, W->partial);
-    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(256), 0, W->stream, W->partial, W->dj, W->h, 0);
-    hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->w, W->V,
-                       W->dj, W->h, 0, W->partial);
+    hipLaunchKernelGGL(k_finish_dots_seg, dim3(g), dim3(256), 0, W->stream, W->partial, nt,
+                       seg_t0(), W->dj, W->h, m + 2);
+    hipLaunchKernelGGL(k_update_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), nt, n,
+                       W->w, W->V, W->dj, W->h, m + 2, 0, W->partial);
     // second Gram-Schmidt pass: coefficients h2
-    hipLaunchKernelGGL(k_finish_dots, dim3(1), dim3(256), 0, W->stream, W->partial, W->dj, W->h2, 0);
-    hipLaunchKernelGGL(k_update, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, W->w, W->V,
-                       W->dj, W->h2, 1, W->partial);
-    hipLaunchKernelGGL(k_givens, dim3(1), dim3(256), 0, W->stream, m, W->partial, W->h, W->h2, W->H,
-                       W->g, W->cs, W->sn, W->scal, W->dj);
-    hipLaunchKernelGGL(k_scale_next, dim3(gb), dim3(256), 0, W->stream, n, W->w, W->V, W->dj,
-                       W->scal);
+    hipLaunchKernelGGL(k_finish_dots_seg, dim3(g), dim3(256), 0, W->stream, W->partial, nt,
+                       seg_t0(), W->dj, W->h2, m + 2);
+    hipLaunchKernelGGL(k_update_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), nt, n,
+                       W->w, W->V, W->dj, W->h2, m + 2, 1, W->partial);
+    hipLaunchKernelGGL(k_givens_seg, dim3(1), dim3(256), 0, W->stream, g, m, W->partial, seg_t0(),
+                       W->h, W->h2, W->H, W->g, W->cs, W->sn, W->segv, ld, W->scal, W->dj);
+    hipLaunchKernelGGL(k_scale_next_seg, dim3(nt), dim3(256), 0, W->stream, tiles(), n, W->w,
+                       W->V, W->dj, W->segv, ld);
     ED_KCHECK("arnoldi");
     return ED_OK;
   }
 
-  // Restarted GMRES on approx_op for A d = rhs (tree order), x updated in place.
-  // tol_abs applies to the true residual of approx_op.  Returns code.
-  int gmres(const double* rhs, double* x, double tol_rel, int maxiter, int* code,
-            double* r0_out, double* r_out) {
-    const int m = W->m;
-    double nr = 0.0, nz = 0.0;
+  // Restarted GMRES on approx_op for A d = rhs (tree order), x updated in
+  // place, independently per segment: segment g stops at
+  // |rhs_g - A x_g| <= tol_rel[g] |rhs_g - A x0_g| (tol_rel[g] >= 1 leaves it
+  // untouched).  The code is 0 when every segment converged; r0 / r1 return
+  // the per-segment initial and final true residual norms.
+  int gmres(const double* rhs, double* x, const std::vector<double>& tol_rel, int maxiter,
+            int* code, std::vector<double>* r0_out, std::vector<double>* r1_out) {
+    const int m = W->m, g = G(), ld = W->Gmax, nt = NT();
+    std::vector<double> nr, nz;
     ED_T(approx_op(x, W->t));
-    ED_T(resid_norms(rhs, W->t, W->r, W->w, &nr, &nz));
+    ED_T(resid_norms(rhs, W->t, W->r, W->w, nr, nz));
     if (r0_out) *r0_out = nr;
-    if (!std::isfinite(nr)) { *code = 3; if (r_out) *r_out = nr; return ED_OK; }
-    const double init_true = nr > 1e-300 ? nr : 1.0;
-    const double tol_abs = std::max(tol_rel * init_true, 1e-300);
-    if (nr <= tol_abs) { *code = 0; if (r_out) *r_out = nr; return ED_OK; }
+    auto finish = [&](int c) {
+      *code = c;
+      if (r1_out) *r1_out = nr;
+      return ED_OK;
+    };
+    std::vector<double> tol_abs(g);
+    bool all_ok = true;
+    for (int k = 0; k < g; ++k) {
+      if (!std::isfinite(nr[k])) return finish(3);
+      tol_abs[k] = std::max(tol_rel[k] * (nr[k] > 1e-300 ? nr[k] : 1.0), 1e-300);
+      all_ok = all_ok && nr[k] <= tol_abs[k];
+    }
+    if (all_ok) return finish(0);
     int local = 0;
     bool use_graph = o->use_graph != 0;
-    // target of the preconditioned estimate |g_{j+1}|: tol * ||D^-1 r0||
-    const double est_tol = tol_rel * nz;
+    // target of each segment's preconditioned estimate |g_{j+1}|: tol * |M^-1 r0|
+    std::vector<double> est_tol(g);
+    for (int k = 0; k < g; ++k) est_tol[k] = tol_rel[k] * nz[k];
     while (local < maxiter) {
-      if (!std::isfinite(nz) || nz == 0.0) { *code = nz == 0.0 ? 6 : 7; if (r_out) *r_out = nr; return ED_OK; }
-      hipLaunchKernelGGL(k_start, dim3(gb), dim3(256), 0, W->stream, n, m, W->w, W->V, W->g, W->scal,
-                         W->dj);
+      bool any = false;
+      for (int k = 0; k < g; ++k) {
+        if (!std::isfinite(nz[k])) return finish(7);
+        const bool act = nr[k] > tol_abs[k] && nz[k] > 0.0;
+        W->hsegv[SV_EST_TOL * ld + k] = est_tol[k];
+        W->hsegv[SV_ACTIVE * ld + k] = act ? 1.0 : 0.0;
+        any = any || act;
+      }
+      if (!any) return finish(6);  // unconverged segments with a zero preconditioned residual
+      ED_G(hipMemcpyAsync(W->segv + SV_EST_TOL * ld, W->hsegv + SV_EST_TOL * ld,
+                          sizeof(double) * 2 * ld, hipMemcpyHostToDevice, W->stream));
+      hipLaunchKernelGGL(k_start_seg, dim3(nt), dim3(256), 0, W->stream, tiles(), g, m, W->w, W->V,
+                         W->g, W->segv, ld, W->dj);
       ED_G(hipMemsetAsync(W->scal + S_BREAK, 0, sizeof(double), W->stream));
       ED_KCHECK("start");
       int j = 0;
@@ -742,28 +917,32 @@ struct Solver {
         total_iters += steps;
         ED_G(hipMemcpyAsync(W->hscal, W->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, W->stream));
         ED_G(hipStreamSynchronize(W->stream));
-        const double est = W->hscal[S_EST];
+        const double ratio = W->hscal[S_EST];
         if (W->hscal[S_BREAK] != 0.0) stop = true;
-        if (!std::isfinite(est)) { *code = 5; if (r_out) *r_out = est; return ED_OK; }
-        if (est <= est_tol) {
+        if (!std::isfinite(ratio)) return finish(5);
+        if (ratio <= 1.0) {
           if (first_below < 0) first_below = total_iters;
           stop = true;
         }
       }
       // with a breakdown at step j the last column is still valid (hn = 0 only)
-      hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, W->stream, m, W->H, W->g, W->dj, W->h);
-      hipLaunchKernelGGL(k_xupdate, dim3(gb), dim3(256), 0, W->stream, n, x, W->V, W->h, W->dj);
+      hipLaunchKernelGGL(k_backsolve_seg, dim3(std::max(1, (g + 63) / 64)), dim3(64), 0,
+                         W->stream, g, m, W->H, W->g, W->dj, W->h);
+      hipLaunchKernelGGL(k_xupdate_seg, dim3(nt), dim3(256), 0, W->stream, tiles(), n, m, x, W->V,
+                         W->h, W->dj);
       ED_KCHECK("update");
       ++cycles;
       ED_T(approx_op(x, W->t));
-      ED_T(resid_norms(rhs, W->t, W->r, W->w, &nr, &nz));
-      if (!std::isfinite(nr)) { *code = 6; if (r_out) *r_out = nr; return ED_OK; }
-      if (nr <= tol_abs) { *code = 0; if (r_out) *r_out = nr; return ED_OK; }
-      if (W->hscal[S_BREAK] != 0.0 && j <= 1) { *code = 2; if (r_out) *r_out = nr; return ED_OK; }
+      ED_T(resid_norms(rhs, W->t, W->r, W->w, nr, nz));
+      all_ok = true;
+      for (int k = 0; k < g; ++k) {
+        if (!std::isfinite(nr[k])) return finish(6);
+        all_ok = all_ok && nr[k] <= tol_abs[k];
+      }
+      if (all_ok) return finish(0);
+      if (W->hscal[S_BREAK] != 0.0 && j <= 1) return finish(2);
     }
-    *code = 1;
-    if (r_out) *r_out = nr;
-    return ED_OK;
+    return finish(1);
   }
 };
 
@@ -820,21 +999,29 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
     ED_G(hipMemsetAsync(W->selfi_t, 0, sizeof(double) * n, W->stream));
   ED_KCHECK("solver setup");
   Solver S{P, W, &o, selfi, n, gb};
+  const int G = S.G();
   int code = 1;
-  double bn = 0.0, dummy = 0.0;
-  ED_T(S.resid_norms(W->b, nullptr, nullptr, nullptr, &bn, &dummy));
+  std::vector<double> bn, dummy;
+  ED_T(S.resid_norms(W->b, nullptr, nullptr, nullptr, bn, dummy));
+  auto l2 = [](const std::vector<double>& v) {
+    double q = 0.0;
+    for (double e : v) q += e * e;
+    return std::sqrt(q);
+  };
+  const double bn_all = l2(bn);
   if (!o.refine) {
-    double r0 = 0, r1 = 0;
-    ED_T(S.gmres(W->b, W->x, o.tol, o.maxiter, &code, &r0, &r1));
-    info->resid0 = r0;
-    info->resid = r1;
-    info->tol_abs = o.tol * (r0 > 1e-300 ? r0 : 1.0);
+    std::vector<double> r0, r1;
+    ED_T(S.gmres(W->b, W->x, std::vector<double>(G, o.tol), o.maxiter, &code, &r0, &r1));
+    info->resid0 = l2(r0);
+    info->resid = l2(r1);
+    info->tol_abs = o.tol * (info->resid0 > 1e-300 ? info->resid0 : 1.0);
   } else {
-    // outer FP64 iterative refinement with the accurate operator
-    double rn = 0.0, zn = 0.0;
+    // outer FP64 iterative refinement with the accurate operator; segment g
+    // is done at |r_g| <= tol |b_g|
+    std::vector<double> rn, zn, tol_abs(G);
+    for (int k = 0; k < G; ++k) tol_abs[k] = o.tol * (bn[k] > 1e-300 ? bn[k] : 1.0);
+    info->tol_abs = o.tol * (bn_all > 1e-300 ? bn_all : 1.0);
     code = 1;
-    const double tol_abs = o.tol * (bn > 1e-300 ? bn : 1.0);
-    info->tol_abs = tol_abs;
     for (int it = 0; it <= o.max_refine; ++it) {
       // The first correction only has to bring the residual down by the inner
       // tolerance, so its right-hand side may come from the approximate
@@ -843,23 +1030,31 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
       bool approx_resid = false;
       if (it == 0) {
         ED_T(S.approx_op(W->x, W->t));
-        ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, &rn, &zn));
-        info->resid0 = rn;
-        // far from converged relative to the approximate operator's accuracy
-        // (inner_tol is set at or above that floor)
-        approx_resid = std::isfinite(rn) && rn > std::max(100.0 * tol_abs, 10.0 * o.inner_tol * bn);
+        ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
+        info->resid0 = l2(rn);
+        // every segment far from converged relative to the approximate
+        // operator's accuracy (inner_tol is set at or above that floor)
+        approx_resid = true;
+        for (int k = 0; k < G; ++k)
+          approx_resid = approx_resid && std::isfinite(rn[k]) &&
+                         rn[k] > std::max(100.0 * tol_abs[k], 10.0 * o.inner_tol * bn[k]);
       }
       if (!approx_resid) {
         ED_T(S.accurate_op(W->x, W->t));
-        ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, &rn, &zn));
+        ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
       }
       info->refine_steps = it;
-      if (!std::isfinite(rn)) { code = 6; break; }
-      if (!approx_resid && rn <= tol_abs) { code = 0; break; }
+      bool finite = true, done = true;
+      for (int k = 0; k < G; ++k) {
+        finite = finite && std::isfinite(rn[k]);
+        done = done && rn[k] <= tol_abs[k];
+      }
+      if (!finite) { code = 6; break; }
+      if (!approx_resid && done) { code = 0; break; }
       if (it == o.max_refine || S.total_iters >= o.maxiter) {
         if (approx_resid) {  // report an accurate residual
           ED_T(S.accurate_op(W->x, W->t));
-          ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, &rn, &zn));
+          ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
         }
         code = 1;
         break;
@@ -868,27 +1063,30 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
       // own scratch, so the outer residual is kept in `rhs`
       ED_G(hipMemsetAsync(W->d, 0, sizeof(double) * n, W->stream));
       ED_G(hipMemcpyAsync(W->rhs, W->r, sizeof(double) * n, hipMemcpyDeviceToDevice, W->stream));
-      int icode = 1;
-      double ir0 = 0, ir1 = 0;
-      // the inner solve only has to take the residual down to the target: the
-      // last refinement step does not over-solve (0.25 leaves room for the
-      // approximate operator's own error)
-      const double itol = std::min(0.5, std::max(o.inner_tol, 0.25 * tol_abs / rn));
+      // each segment's inner solve only has to take its residual down to its
+      // target (0.25 leaves room for the approximate operator's own error);
+      // converged segments are left alone (tolerance 1)
+      std::vector<double> itol(G);
+      for (int k = 0; k < G; ++k)
+        itol[k] = rn[k] <= tol_abs[k]
+                      ? 1.0
+                      : std::min(0.5, std::max(o.inner_tol, 0.25 * tol_abs[k] / rn[k]));
       // an inner solve that stalls on the approximate operator's rounding
       // floor is cut after 4 restart cycles; the refinement continues from it
       const int inner_max = std::min(o.maxiter - S.total_iters, 4 * o.restart);
-      ED_T(S.gmres(W->rhs, W->d, itol, inner_max, &icode, &ir0, &ir1));
+      int icode = 1;
+      ED_T(S.gmres(W->rhs, W->d, itol, inner_max, &icode, nullptr, nullptr));
       if (icode == 3 || icode == 5 || icode == 6 || icode == 7) { code = icode; break; }
       hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, W->stream, n, 1.0, W->d, W->x);
       ED_KCHECK("refine update");
     }
-    info->resid = rn;
+    info->resid = l2(rn);
   }
   info->code = code;
   info->iters = S.total_iters;
   info->cycles = S.cycles;
   info->first_iter_below_tol = S.first_below;
-  info->rel_resid = info->resid / (bn > 1e-300 ? bn : 1.0);
+  info->rel_resid = info->resid / (bn_all > 1e-300 ? bn_all : 1.0);
   hipLaunchKernelGGL(k_perm_out, dim3(gn), dim3(256), 0, W->stream, n, P->perm, W->x, x);
   ED_KCHECK("solver output");
   ED_G(hipEventRecord(ev, W->stream));

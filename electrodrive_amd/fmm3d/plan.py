@@ -52,6 +52,7 @@ class FmmPlan:
                       "ed_plan_create_batched")
         self._h = h
         self.order, self.ws, self.leaf_size = int(order), int(ws), int(leaf_size)
+        self.coarse_k = 0
         self.m2l_precision = m2l_precision
 
     @property
@@ -154,6 +155,7 @@ class FmmPlan:
         if Z is None:
             nat.check(nat.lib().ed_plan_set_coarse(self.handle, None, 0, None, None),
                       "ed_plan_set_coarse")
+            self.coarse_k = 0
             return
         if not Z.is_cuda or Z.dtype != torch.float64 or Z.dim() != 2 or Z.shape[0] != self.n:
             raise ValueError("Z must be a float64 device tensor of shape (n, k)")
@@ -161,6 +163,7 @@ class FmmPlan:
         nat.check(nat.lib().ed_plan_set_coarse(self.handle, nat.ptr(Zc), int(Z.shape[1]),
                                                nat.ptr(self_integrals),
                                                nat.stream_ptr(Z.device)), "ed_plan_set_coarse")
+        self.coarse_k = int(Z.shape[1])
 
     def build_sai(self, self_integrals, factor_pattern=2.0, factor_rows=4.0):
         """Sparse approximate inverse smoother (solver ``precond=2``)."""

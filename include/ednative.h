@@ -236,7 +236,9 @@ void ed_gmres_opts_default(ed_gmres_opts* opts);
 
 /* b (n), x (n, in: initial guess, out: solution), diag (n, Jacobi diagonal,
  * may be NULL), self_integrals (n, per-area diagonal of the operator) — all
- * device FP64 in original order. */
+ * device FP64 in original order.  On a batched plan without a coarse space
+ * every geometry gets its own Krylov space (same launches) and converges at
+ * ||r_g|| <= tol ||b_g||; `resid` / `rel_resid` are over the whole vector. */
 int ed_gmres_solve(ed_plan* plan, const double* b, double* x, const double* diag,
                    const double* self_integrals, const ed_gmres_opts* opts,
                    ed_gmres_info* info, ed_stream stream);
