@@ -59,9 +59,15 @@ def test_batched_solve():
     xs, infos = op.solve(b, tol=1e-8, precond="sai")
     assert infos["code"] == 0 and infos["max_geometry_rel_resid"] <= 1e-8
     assert float(torch.linalg.norm(xs - x) / torch.linalg.norm(x)) < 1e-6
-    # each block equals its own single-geometry solve
+    # each block equals its own single-geometry solve; one Krylov space per
+    # geometry: the batch takes about as many Arnoldi steps as its slowest
+    # geometry alone, not the steps of the joint spectrum
+    single = []
     for g, (m, ch) in enumerate(zip(ms, charges)):
         s, e = int(op.geom_offsets[g]), int(op.geom_offsets[g + 1])
         og = BemOperator(m, order=10, ws=1, leaf_size=32)
         xg, ig = og.solve(og.rhs_point_charges(ch), tol=1e-8)
         assert float(torch.linalg.norm(x[s:e] - xg) / torch.linalg.norm(xg)) < 1e-6
+        single.append(og.solve(og.rhs_point_charges(ch), tol=1e-8, precond="sai")[1]["iters"])
+    print("batched SAI steps", infos["iters"], "single-geometry SAI steps", single)
+    assert infos["iters"] <= 1.25 * max(single) + 4
