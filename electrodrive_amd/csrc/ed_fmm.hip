@@ -684,7 +684,24 @@ __global__ __launch_bounds__(256) void k_l2p_out(
   const float4 xi = pts[i];
   const int nb = near_ptr[i], ne = near_ptr[i + 1];
   float corr = 0.0f;
-  for (int q = nb; q < ne; ++q) corr += near_w[q] * sig_t[near_col[q]];
+  // groups of four with clamped indices and zero weights past the row end:
+  // every load of a group is issued before the first wait (the row's column
+  // and weight loads, then the four dependent sigma gathers)
+  for (int q = nb; q < ne; q += 4) {
+    int c[4];
+    float wq[4], sv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int e = min(q + u, ne - 1);
+      c[u] = near_col[e];
+      const float wv = near_w[e];
+      wq[u] = q + u < ne ? wv : 0.0f;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) sv[u] = sig_t[c[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) corr += wq[u] * sv[u];
+  }
   const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w,
                                 L + (size_t)node * NC);
   const int2 pr = p2p_ref[i];

@@ -517,7 +517,22 @@ __global__ void k_near_csr(int64_t n, const int32_t* __restrict__ ptr, const int
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double s = 0.0;
-  for (int q = ptr[i]; q < ptr[i + 1]; ++q) s += w[q] * x[col[q]];
+  const int b = ptr[i], e = ptr[i + 1];
+  for (int q = b; q < e; q += 4) {  // four loads in flight, clamped and masked
+    int c[4];
+    double wq[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int k = min(q + u, e - 1);
+      c[u] = col[k];
+      const double wv = w[k];
+      wq[u] = q + u < e ? wv : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xv[u] = x[c[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s += wq[u] * xv[u];
+  }
   out[i] += s;
 }
 

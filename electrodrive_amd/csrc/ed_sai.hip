@@ -33,10 +33,16 @@ __device__ __forceinline__ double sai_dist2(const double* C, int64_t a, int64_t 
   return dx * dx + dy * dy + dz * dz;
 }
 
+constexpr int kSaiThreads = 128;
+constexpr int kSaiHash = 256;  // LDS hash slots for the fitting rows (>= 2 kSaiI)
+
 // list[0] = i, then the (cap - 1) nearest candidates of the CSR row (ties by
-// index; rows beyond kSaiCand candidates keep their first kSaiCand).
+// index; rows beyond kSaiCand candidates keep their first kSaiCand).  The
+// candidates' distances are computed once into LDS (dsh, csh), then each
+// candidate's rank is counted there.
 __device__ void sai_select(int64_t i, const int32_t* __restrict__ col, int64_t beg, int64_t end,
-                           const double* __restrict__ C, int cap, int* list, int* count) {
+                           const double* __restrict__ C, int cap, int* list, int* count,
+                           double* dsh, int* csh) {
   const int c = (int)std::min<int64_t>(end - beg, kSaiCand);
   if (threadIdx.x == 0) list[0] = (int)i;
   if (c <= cap - 1) {
@@ -45,12 +51,17 @@ __device__ void sai_select(int64_t i, const int32_t* __restrict__ col, int64_t b
   } else {
     for (int t = threadIdx.x; t < c; t += blockDim.x) {
       const int ct = col[beg + t];
-      const double dt = sai_dist2(C, i, ct);
+      csh[t] = ct;
+      dsh[t] = sai_dist2(C, i, ct);
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < c; t += blockDim.x) {
+      const int ct = csh[t];
+      const double dt = dsh[t];
       int rank = 0;
       for (int u = 0; u < c; ++u) {
-        const int cu = col[beg + u];
-        const double du = sai_dist2(C, i, cu);
-        rank += (du < dt || (du == dt && cu < ct)) ? 1 : 0;
+        const double du = dsh[u];
+        rank += (du < dt || (du == dt && csh[u] < ct)) ? 1 : 0;
       }
       if (rank < cap - 1) list[1 + rank] = ct;
     }
@@ -59,8 +70,12 @@ __device__ void sai_select(int64_t i, const int32_t* __restrict__ col, int64_t b
   __syncthreads();
 }
 
-// One 64-lane block per row (rows row0 + blockIdx.x of a geometry segment).
-__global__ __launch_bounds__(64) void k_sai_build(
+__device__ __forceinline__ int sai_hash(int key) {
+  return (int)(((uint32_t)key * 2654435761u) >> 24) & (kSaiHash - 1);
+}
+
+// One 128-thread block per row (rows row0 + blockIdx.x of a geometry segment).
+__global__ __launch_bounds__(kSaiThreads) void k_sai_build(
     int64_t n, int64_t row0, const int64_t* __restrict__ jptr, const int32_t* __restrict__ jcol,
     const int64_t* __restrict__ iptr, const int32_t* __restrict__ icol,
     const double* __restrict__ C, const double* __restrict__ A, const double* __restrict__ selfi,
@@ -72,51 +87,91 @@ __global__ __launch_bounds__(64) void k_sai_build(
   __shared__ double B[kSaiI][kSaiJ + 1];
   __shared__ double G[kSaiJ][kSaiJ + 1];
   __shared__ double h[kSaiJ];
-  __shared__ int nJ, nI, li;
+  __shared__ double dsh[kSaiCand];
+  __shared__ int csh[kSaiCand];
+  __shared__ int hk[kSaiHash], hv[kSaiHash];
+  __shared__ int pre[kSaiJ + 1];
+  __shared__ int nJ, nI;
+  const int tid = threadIdx.x;
   const int64_t loc = blockIdx.x;
   const int64_t i = row0 + loc;
-  sai_select(i, jcol, jptr[loc], jptr[loc + 1], C, kSaiJ, Jl, &nJ);
-  sai_select(i, icol, iptr[loc], iptr[loc + 1], C, kSaiI, Il, &nI);
+  sai_select(i, jcol, jptr[loc], jptr[loc + 1], C, kSaiJ, Jl, &nJ, dsh, csh);
+  sai_select(i, icol, iptr[loc], iptr[loc + 1], C, kSaiI, Il, &nI, dsh, csh);
   const int nj = nJ, ni = nI;
-  // B[l][k] = A[J_k][I_l]
-  for (int e = threadIdx.x; e < ni * nj; e += blockDim.x) {
-    const int l = e / nj, k = e - l * nj;
-    const int a = Jl[k], b = Il[l];
-    double v;
-    if (a == b) {
-      v = selfi[a] * A[a];
-    } else {
-      const double r2 = fmax(sai_dist2(C, a, b), 1e-24);
-      v = kKE * A[b] / sqrt(r2);
+  // B[l][k] = A[J_k][I_l] (zero for k >= nj)
+  for (int e = tid; e < ni * kSaiJ; e += blockDim.x) {
+    const int l = e / kSaiJ, k = e - l * kSaiJ;
+    double v = 0.0;
+    if (k < nj) {
+      const int a = Jl[k], b = Il[l];
+      if (a == b) {
+        v = selfi[a] * A[a];
+      } else {
+        const double r2 = fmax(sai_dist2(C, a, b), 1e-24);
+        v = kKE * A[b] / sqrt(r2);
+      }
     }
     B[l][k] = v;
   }
-  __syncthreads();
-  // near-field corrections A[J_k][col] += w for the near entries of row J_k
+  // near-field corrections A[J_k][col] += w for the near entries of the rows
+  // J_k: the fitting rows go into an LDS hash (panel -> position), and the
+  // near entries of all pattern rows are spread over the threads
   if (near_ptr) {
-    for (int k = threadIdx.x; k < nj; k += blockDim.x) {
-      const int a = Jl[k];
-      for (int q = near_ptr[a]; q < near_ptr[a + 1]; ++q) {
-        const int b = near_col[q];
-        for (int l = 0; l < ni; ++l)
-          if (Il[l] == b) { B[l][k] += near_w[q]; break; }
+    for (int e = tid; e < kSaiHash; e += blockDim.x) hk[e] = -1;
+    if (tid == 0) {
+      pre[0] = 0;
+      for (int k = 0; k < nj; ++k) pre[k + 1] = pre[k] + (near_ptr[Jl[k] + 1] - near_ptr[Jl[k]]);
+    }
+    __syncthreads();
+    for (int l = tid; l < ni; l += blockDim.x) {
+      const int key = Il[l];
+      int sl = sai_hash(key);
+      while (true) {
+        const int prev = atomicCAS(&hk[sl], -1, key);
+        if (prev == -1) { hv[sl] = l; break; }
+        sl = (sl + 1) & (kSaiHash - 1);
       }
     }
+    __syncthreads();
+    const int tot = pre[nj];
+    for (int e = tid; e < tot; e += blockDim.x) {
+      int lo = 0, hi = nj - 1;  // last k with pre[k] <= e
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+      }
+      const int k = lo;
+      const int q = near_ptr[Jl[k]] + (e - pre[k]);
+      const int b = near_col[q];
+      int sl = sai_hash(b);
+      while (hk[sl] != -1 && hk[sl] != b) sl = (sl + 1) & (kSaiHash - 1);
+      if (hk[sl] == b) B[hv[sl]][k] += near_w[q];
+    }
   }
-  if (threadIdx.x == 0) li = 0;  // Il[0] == i
   __syncthreads();
-  // normal equations G = B^T B, h = B^T e_i = B[li][:]
-  for (int e = threadIdx.x; e < nj * nj; e += blockDim.x) {
-    const int a = e / nj, b = e - a * nj;
-    if (b < a) continue;
-    double s = 0.0;
-    for (int l = 0; l < ni; ++l) s += B[l][a] * B[l][b];
-    G[a][b] = s;
-    G[b][a] = s;
+  // normal equations G = B^T B (2 x 4 register blocks per thread over the
+  // full 32 x 32; entries beyond nj are zero), h = B^T e_i = B[0][:]
+  {
+    const int ra = (tid / 8) * 2, cb = (tid % 8) * 4;
+    double acc[2][4] = {{0.0, 0.0, 0.0, 0.0}, {0.0, 0.0, 0.0, 0.0}};
+    for (int l = 0; l < ni; ++l) {
+      const double a0 = B[l][ra], a1 = B[l][ra + 1];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double bv = B[l][cb + u];
+        acc[0][u] += a0 * bv;
+        acc[1][u] += a1 * bv;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      G[ra][cb + u] = acc[0][u];
+      G[ra + 1][cb + u] = acc[1][u];
+    }
   }
-  for (int k = threadIdx.x; k < nj; k += blockDim.x) h[k] = B[li][k];
+  for (int k = tid; k < nj; k += blockDim.x) h[k] = B[0][k];  // Il[0] == i
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     double tr = 0.0;
     for (int a = 0; a < nj; ++a) tr += G[a][a];
     const double lam = ridge * (tr > 0.0 ? tr : 1.0) / nj;
@@ -125,40 +180,40 @@ __global__ __launch_bounds__(64) void k_sai_build(
   __syncthreads();
   // right-looking Cholesky G = L L^T (lower triangle, in place)
   for (int j = 0; j < nj; ++j) {
-    if (threadIdx.x == 0) {
+    if (tid == 0) {
       const double d = G[j][j];
       G[j][j] = d > 1e-300 ? sqrt(d) : 1e-150;
     }
     __syncthreads();
     const double d = G[j][j];
-    for (int r = j + 1 + threadIdx.x; r < nj; r += blockDim.x) G[r][j] /= d;
+    for (int r = j + 1 + tid; r < nj; r += blockDim.x) G[r][j] /= d;
     __syncthreads();
     const int w = nj - j - 1;
-    for (int e = threadIdx.x; e < w * w; e += blockDim.x) {
+    for (int e = tid; e < w * w; e += blockDim.x) {
       const int r = j + 1 + e / w, c = j + 1 + e % w;
       if (c <= r) G[r][c] -= G[r][j] * G[c][j];
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) {
-    for (int r = 0; r < nj; ++r) {  // L y = h
-      double v = h[r];
-      for (int q = 0; q < r; ++q) v -= G[r][q] * h[q];
-      h[r] = v / G[r][r];
-    }
-    for (int r = nj - 1; r >= 0; --r) {  // L^T m = y
-      double v = h[r];
-      for (int q = r + 1; q < nj; ++q) v -= G[q][r] * h[q];
-      h[r] = v / G[r][r];
-    }
+  // L y = h, then L^T m = y, column by column
+  for (int r = 0; r < nj; ++r) {
+    if (tid == 0) h[r] /= G[r][r];
+    __syncthreads();
+    for (int q = r + 1 + tid; q < nj; q += blockDim.x) h[q] -= G[q][r] * h[r];
+    __syncthreads();
   }
-  __syncthreads();
-  for (int q = threadIdx.x; q < kSaiJ; q += blockDim.x) {
+  for (int r = nj - 1; r >= 0; --r) {
+    if (tid == 0) h[r] /= G[r][r];
+    __syncthreads();
+    for (int q = tid; q < r; q += blockDim.x) h[q] -= G[r][q] * h[r];
+    __syncthreads();
+  }
+  for (int q = tid; q < kSaiJ; q += blockDim.x) {
     const bool on = q < nj;
     out_col[(size_t)q * n + i] = on ? Jl[q] : (int32_t)i;
     out_val[(size_t)q * n + i] = on ? h[q] : 0.0;
   }
-  if (threadIdx.x == 0) out_cnt[i] = nj;
+  if (tid == 0) out_cnt[i] = nj;
 }
 
 // z = M u (+ Z cy when k > 0)
@@ -255,7 +310,7 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
       hipError_t e = hipMemcpyAsync(dpj, pj.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = hipMemcpyAsync(dpi, pi.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
       if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_sai_build, dim3((unsigned)ng), dim3(64), 0, st, n, b,
+        hipLaunchKernelGGL(k_sai_build, dim3((unsigned)ng), dim3(kSaiThreads), 0, st, n, b,
                            (const int64_t*)dpj, (const int32_t*)cj, (const int64_t*)dpi,
                            (const int32_t*)ci, P->cent64_t, P->area64_t, (const double*)selfi_t,
                            P->n_near > 0 ? P->near_ptr : nullptr, P->near_col, P->near_w64,

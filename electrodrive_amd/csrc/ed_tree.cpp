@@ -17,20 +17,27 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
-#include <unordered_map>
 
 #include "ed_internal.h"
 
 namespace ed {
 
+// Spreads the low 21 bits of v to every third bit (bit b -> bit 3b) with the
+// shift-and-mask ("magic numbers") method: five steps of doubling shifts.
+static inline uint64_t spread3(uint64_t v) {
+  v &= 0x1fffffULL;
+  v = (v | (v << 32)) & 0x1f00000000ffffULL;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffULL;
+  v = (v | (v << 8)) & 0x100f00f00f00f00fULL;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ULL;
+  v = (v | (v << 2)) & 0x1249249249249249ULL;
+  return v;
+}
+
+// Bit b of x, y, z goes to bit 3b, 3b+1, 3b+2 of the key, b < bits (<= 21).
 static inline uint64_t interleave3(uint64_t x, uint64_t y, uint64_t z, int bits) {
-  uint64_t k = 0;
-  for (int b = 0; b < bits; ++b) {
-    k |= ((x >> b) & 1ULL) << (3 * b);
-    k |= ((y >> b) & 1ULL) << (3 * b + 1);
-    k |= ((z >> b) & 1ULL) << (3 * b + 2);
-  }
-  return k;
+  const uint64_t mask = bits >= 21 ? 0x1fffffULL : ((1ULL << bits) - 1);
+  return spread3(x & mask) | (spread3(y & mask) << 1) | (spread3(z & mask) << 2);
 }
 
 // fmm3d/geometry.py:696-703 and :785-799 in FP64: normalised = (p - min)/extent
@@ -121,13 +128,29 @@ int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
   const double bh[3] = {half, half, half};
   std::vector<int64_t> keys(n);
   morton_keys_host(C, n, kKeyLevel, t.center, bh, keys.data());
-  t.perm.resize(n);
-  std::iota(t.perm.begin(), t.perm.end(), 0);
-  std::sort(t.perm.begin(), t.perm.end(), [&](int64_t a, int64_t b) {
-    return keys[a] != keys[b] ? keys[a] < keys[b] : a < b;
-  });
-  t.keys.resize(n);
-  for (int64_t i = 0; i < n; ++i) t.keys[i] = (uint64_t)keys[t.perm[i]];
+  // stable LSD radix sort of (key, index) on 16-bit digits: key order, ties
+  // in index order
+  {
+    std::vector<uint64_t> ka(n), kb(n);
+    std::vector<int64_t> ia(n), ib(n);
+    for (int64_t i = 0; i < n; ++i) { ka[i] = (uint64_t)keys[i]; ia[i] = i; }
+    std::vector<int64_t> cnt(65537);
+    for (int pass = 0; pass < 4; ++pass) {
+      const int sh = 16 * pass;
+      std::fill(cnt.begin(), cnt.end(), 0);
+      for (int64_t i = 0; i < n; ++i) ++cnt[((ka[i] >> sh) & 0xffff) + 1];
+      for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+      for (int64_t i = 0; i < n; ++i) {
+        const int64_t dst = cnt[(ka[i] >> sh) & 0xffff]++;
+        kb[dst] = ka[i];
+        ib[dst] = ia[i];
+      }
+      ka.swap(kb);
+      ia.swap(ib);
+    }
+    t.perm = std::move(ia);
+    t.keys = std::move(ka);
+  }
 
   // Level-by-level node construction.
   struct Pending { int64_t b, e, parent; };
@@ -185,14 +208,34 @@ int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
   t.geom_center.assign(t.center, t.center + 3);
   t.geom_pt_off = {0, n};
 
-  // Locational-code hash: (1 << 3l) | prefix -> node.
-  std::unordered_map<uint64_t, int64_t> where;
-  where.reserve((size_t)M * 2);
+  // Locational-code hash: (1 << 3l) | prefix -> node, open addressing with
+  // linear probing (codes are never 0, so 0 marks an empty slot).
   auto code_of = [&](int l, int64_t x, int64_t y, int64_t z) -> uint64_t {
     return (1ULL << (3 * l)) | interleave3((uint64_t)x, (uint64_t)y, (uint64_t)z, l);
   };
-  for (int64_t i = 0; i < M; ++i)
-    where[code_of(t.level[i], t.coord[3 * i], t.coord[3 * i + 1], t.coord[3 * i + 2])] = i;
+  int hbits = 4;
+  while ((1LL << hbits) < 4 * M) ++hbits;
+  const uint64_t hmask = (1ULL << hbits) - 1;
+  std::vector<uint64_t> hkey((size_t)1 << hbits, 0);
+  std::vector<int64_t> hval((size_t)1 << hbits, -1);
+  auto slot_of = [&](uint64_t code) {
+    return (size_t)((code * 0x9E3779B97F4A7C15ULL) >> (64 - hbits));
+  };
+  for (int64_t i = 0; i < M; ++i) {
+    const uint64_t c = code_of(t.level[i], t.coord[3 * i], t.coord[3 * i + 1], t.coord[3 * i + 2]);
+    size_t h = slot_of(c);
+    while (hkey[h] != 0 && hkey[h] != c) h = (h + 1) & hmask;
+    hkey[h] = c;
+    hval[h] = i;
+  }
+  auto find_node = [&](uint64_t c) -> int64_t {
+    size_t h = slot_of(c);
+    while (hkey[h] != 0) {
+      if (hkey[h] == c) return hval[h];
+      h = (h + 1) & hmask;
+    }
+    return -1;
+  };
 
   auto colleagues = [&](int64_t a, std::vector<int64_t>& out) {
     out.clear();
@@ -204,8 +247,8 @@ int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
           const int64_t x = t.coord[3 * a] + dx, y = t.coord[3 * a + 1] + dy,
                         z = t.coord[3 * a + 2] + dz;
           if (x < 0 || y < 0 || z < 0 || x >= lim || y >= lim || z >= lim) continue;
-          auto it = where.find(code_of(l, x, y, z));
-          if (it != where.end()) out.push_back(it->second);
+          const int64_t f = find_node(code_of(l, x, y, z));
+          if (f >= 0) out.push_back(f);
         }
   };
   auto adjacent = [&](int64_t a, int64_t b) {
