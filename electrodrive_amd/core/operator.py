@@ -25,7 +25,10 @@ from .bem_kernel import dense_matvec
 from .kernels import K_E
 
 
-INNER_TOL = {"bf16": 2e-2, "fp16": 3e-3}
+# Inner (FMM-operator) solve depth per M2L precision.  FP32: 1e-5, so that the
+# first correction reaches the FMM's own accuracy and two FP64 refinement
+# residuals (dense, the dominant solve cost) suffice for 1e-8 instead of three.
+INNER_TOL = {"fp32": 1e-5, "bf16": 2e-2, "fp16": 3e-3}
 
 
 class FmmOperator:
