@@ -10,7 +10,7 @@ is the relative L2 difference to the FP64 dense operator on the same panels
 (the reference times its CPU pipeline and compares with a direct sum).
 
 Candidates: expansion order in ``SUPPORTED_ORDERS`` at or above 6 and leaf
-sizes 32 .. 256, with ``ws`` and the M2L precision of ``base_cfg``.  The fastest
+sizes 32 .. 512, with ``ws`` and the M2L precision of ``base_cfg``.  The fastest
 candidate whose error meets ``target_rel_error`` wins; when none does, the most
 accurate one.  Pass ``centroids`` / ``areas`` to tune on the actual geometry
 (otherwise a unit sphere of ``problem_size`` panels is used).
@@ -36,7 +36,7 @@ from .config import SUPPORTED_ORDERS, FmmConfig
 
 AUTOTUNE_CACHE_ENV_VAR = "ELECTRODRIVE_AMD_AUTOTUNE_CACHE"
 AUTOTUNE_CACHE_FILENAME = "fmm3d_autotune.json"
-LEAF_SIZES = (32, 64, 128, 256)
+LEAF_SIZES = (32, 64, 128, 256, 384, 512)
 
 
 @dataclass
