@@ -3,7 +3,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -144,8 +143,7 @@ static int side_stream(int device, hipStream_t* out) {
 // block per leaf whose cost varies 60-fold with leaf and U-list size.
 static int p2p_schedule(ed_plan* P, const std::vector<int32_t>& leaves) {
   const auto& H = P->host;
-  int steps = ed::kP2PItemSteps;
-  if (const char* e = std::getenv("ED_P2P_ITEM_STEPS")) steps = std::max(16, std::atoi(e));
+  const int steps = ed::kP2PItemSteps;
   std::vector<int2> seg;
   std::vector<ed::P2PItem> items;
   std::vector<int2> ref((size_t)std::max<int64_t>(H.n, 1), make_int2(0, 0));
