@@ -220,7 +220,7 @@ def _operator_shape(cfg, n: int, mesh: TriMesh, logger) -> Tuple[int, int]:
     if n < int(getattr(cfg, "fmm_min_panels", 4096)):
         return int(order or 10), int(leaf or 64)
     base = FmmConfig(expansion_order=10, leaf_size=64, mac_theta=0.8,
-                     m2l_precision=getattr(cfg, "m2l_precision", "fp32"))
+                     m2l_precision=getattr(cfg, "m2l_precision", "fp16x3"))
     res = autotune_for_problem(base, n, dry_run=False, target_rel_error=1e-5,
                                centroids=mesh.centroids, areas=mesh.areas, use_cache=False)
     if logger:
@@ -292,7 +292,7 @@ def bem_solve(spec: CanonicalSpec, cfg, logger, differentiable: bool = False) ->
             break
         order, leaf = _operator_shape(cfg, N, mesh, logger)
         op = BemOperator(mesh, order=order, ws=1, leaf_size=leaf,
-                         m2l_precision=getattr(cfg, "m2l_precision", "fp32"),
+                         m2l_precision=getattr(cfg, "m2l_precision", "fp16x3"),
                          refined_self=near_eval or near_mv, self_depth=self_depth,
                          near=near_mv, near_depth=near_depth, near_order=near_order,
                          distance_factor=near_f, device=device)

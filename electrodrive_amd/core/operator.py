@@ -28,7 +28,7 @@ from .kernels import K_E
 # Inner (FMM-operator) solve depth per M2L precision.  FP32: 1e-5, so that the
 # first correction reaches the FMM's own accuracy and two FP64 refinement
 # residuals (dense, the dominant solve cost) suffice for 1e-8 instead of three.
-INNER_TOL = {"fp32": 1e-5, "bf16": 2e-2, "fp16": 3e-3}
+INNER_TOL = {"fp32": 1e-5, "fp16x3": 1e-5, "bf16": 2e-2, "fp16": 3e-3}
 
 
 class FmmOperator:
@@ -63,7 +63,7 @@ class BemOperator:
     geometries: one plan, block-diagonal operator, geometries never interact)."""
 
     def __init__(self, mesh, *, order=8, ws=1, leaf_size=64, max_level=16,
-                 m2l_precision="fp32", refined_self=True, self_depth=3, near=True,
+                 m2l_precision="fp16x3", refined_self=True, self_depth=3, near=True,
                  near_depth=3, near_order=2, distance_factor=1.5, device=None):
         nat.require_gpu()
         self.device = torch.device(device or "cuda")

@@ -47,6 +47,11 @@ static uint16_t f32_to_f16(float f) {
   std::memcpy(&u, &h, 2);
   return u;
 }
+static double f16_to_f64(uint16_t u) {
+  _Float16 h;
+  std::memcpy(&h, &u, 2);
+  return (double)(float)h;
+}
 
 template <typename T>
 static int upload(T** dst, const std::vector<T>& src) {
@@ -87,7 +92,8 @@ void plan_free(ed_plan* P) {
 struct OperatorTables {
   std::vector<float> valu;     // nc x nc, column-major (VALU path)
   std::vector<float> f32;      // ncp x kp scaled table (FP32 MFMA)
-  std::vector<uint16_t> h16;   // ncp x kp scaled table (BF16 / FP16 MFMA)
+  std::vector<uint16_t> h16;   // ncp x kp scaled table (BF16 / FP16 MFMA); for
+                               // the split form, hi then lo (2 x ncp x kp FP16)
 };
 
 static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, int oz, int ncp,
@@ -108,13 +114,20 @@ static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, 
       for (int j = 0; j < nc; ++j) ot.valu[(size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
   } else {
     if (prec == 0) ot.f32.assign((size_t)ncp * kp, 0.0f);
-    else ot.h16.assign((size_t)ncp * kp, 0);
+    else ot.h16.assign((size_t)ncp * kp * (prec == 3 ? 2 : 1), 0);
     for (int k = 0; k < nc; ++k)
       for (int j = 0; j < nc; ++j) {
         const double sc = Td[(size_t)k * nc + j] / (asc[k] * asc[j]);
         const size_t at = (size_t)k * kp + j;
-        if (prec == 0) ot.f32[at] = (float)sc;
-        else ot.h16[at] = prec == 1 ? f32_to_bf16((float)sc) : f32_to_f16((float)sc);
+        if (prec == 0) {
+          ot.f32[at] = (float)sc;
+        } else if (prec == 3) {  // sc = hi + lo to ~2^-22
+          const uint16_t hi = f32_to_f16((float)sc);
+          ot.h16[at] = hi;
+          ot.h16[(size_t)ncp * kp + at] = f32_to_f16((float)(sc - f16_to_f64(hi)));
+        } else {
+          ot.h16[at] = prec == 1 ? f32_to_bf16((float)sc) : f32_to_f16((float)sc);
+        }
       }
   }
   return cache.emplace(key, std::move(ot)).first->second;
@@ -313,7 +326,8 @@ int plan_upload(ed_plan* P) {
   const bool need_valu = !enabled || prec < 0;
   std::vector<float> tv(need_valu ? (size_t)std::max(ntab, 1) * nc * nc : 1, 0.0f);
   std::vector<float> tm(enabled && prec == 0 ? tsz : 1, 0.0f);
-  std::vector<uint16_t> th(enabled && prec > 0 ? tsz : 1, 0);
+  const size_t hsz = (size_t)ncp * kp * (prec == 3 ? 2 : 1);  // per table
+  std::vector<uint16_t> th(enabled && prec > 0 ? (size_t)std::max(ntab, 1) * hsz : 1, 0);
   for (auto& kv : slot_map) {
     const int slot = kv.first, tab = kv.second;
     const int ox = slot % W - R, oy = (slot / W) % W - R, oz = slot / (W * W) - R;
@@ -322,7 +336,7 @@ int plan_upload(ed_plan* P) {
     else if (prec == 0)
       std::memcpy(tm.data() + (size_t)tab * ncp * kp, ot.f32.data(), sizeof(float) * ncp * kp);
     else
-      std::memcpy(th.data() + (size_t)tab * ncp * kp, ot.h16.data(), sizeof(uint16_t) * ncp * kp);
+      std::memcpy(th.data() + (size_t)tab * hsz, ot.h16.data(), sizeof(uint16_t) * hsz);
   }
   ED_TRY(upload(&P->t_m2l, tv));
   ED_TRY(upload(&P->t_m2l_mfma, tm));
@@ -466,8 +480,8 @@ static int plan_create_impl(const double* centroids, const double* areas, int64_
   if (opts) o = *opts;
   if (o.order != 4 && o.order != 6 && o.order != 8 && o.order != 10 && o.order != 12)
     return fail(ED_ERR_ARG, "ed_plan_create: order must be one of 4, 6, 8, 10, 12");
-  if (o.m2l_precision < -1 || o.m2l_precision > 2)
-    return fail(ED_ERR_ARG, "ed_plan_create: m2l_precision must be -1, 0, 1 or 2");
+  if (o.m2l_precision < -1 || o.m2l_precision > 3)
+    return fail(ED_ERR_ARG, "ed_plan_create: m2l_precision must be -1, 0, 1, 2 or 3");
   if (n_geoms < 1) return fail(ED_ERR_ARG, "ed_plan_create: n_geoms must be >= 1");
   if (geom_off) {
     if (geom_off[0] != 0 || geom_off[n_geoms] != n)

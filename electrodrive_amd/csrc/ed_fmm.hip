@@ -498,6 +498,102 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   }
 }
 
+// Split-FP16 M2L with FP32-level accuracy: table and (scaled) multipoles are
+// each a sum hi + lo of two FP16 numbers (22 significant bits), and
+// A B ~ A_hi B_hi + A_hi B_lo + A_lo B_hi is three v_mfma_f32_32x32x16_f16
+// with FP32 accumulation (the dropped A_lo B_lo is ~2^-22 of the product).
+// Both table halves sit in LDS; each lane forms its column's B hi / lo in
+// registers from the gathered FP32 multipoles.  Four waves x 32 columns x all
+// rows, the same tile schedule and Y layout as the 16-bit kernel.
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict__ tile_tab,
+                                                      const int32_t* __restrict__ tile_beg,
+                                                      const int32_t* __restrict__ tile_cnt,
+                                                      const int32_t* __restrict__ pair_src,
+                                                      const _Float16* __restrict__ Tm,
+                                                      const float* __restrict__ ascale,
+                                                      const float* __restrict__ M,
+                                                      const float* __restrict__ mmax,
+                                                      float* __restrict__ Y) {
+  using S = M2LShapeH<P>;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
+  __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
+  __shared__ float red[256];
+  if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
+  float mx = 0.0f;
+  for (int t = threadIdx.x; t < kAbsMaxBlocks; t += blockDim.x) mx = fmaxf(mx, mmax[t]);
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  const float gmax = red[0];
+  const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
+  const float inv_sc = 1.0f / sc;
+  const int tile = blockIdx.x;
+  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
+  // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
+  // multiple of 16) elements long, so a chunk never crosses a row
+  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
+  constexpr int N8 = NCP * KP / 8;
+#pragma unroll 4
+  for (int t = threadIdx.x; t < 2 * N8; t += 256) {
+    const int hh = t >= N8 ? 1 : 0;
+    const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
+    *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
+  }
+  __syncthreads();
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col0 = wave * 32;
+  if (col0 >= cnt) return;  // no barrier below
+  const int li = lane & 31, lk = lane >> 5;
+  const int col = col0 + li;
+  const bool cv = col < cnt;
+  const float* mrow = M + (size_t)pair_src[beg + min(col, cnt - 1)] * NC;
+  constexpr int NRB = NCP / 32;
+  f32x16 acc[NRB];
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
+#pragma unroll 2
+  for (int sv = 0; sv < KP / 16; ++sv) {
+    f16x8 bh, bl;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 16 * sv + 8 * lk + j;
+      const int kk = min(k, NC - 1);
+      const float raw = mrow[kk] * ascale[kk] * sc;
+      const float v = (cv && k < NC) ? raw : 0.0f;
+      const _Float16 h = (_Float16)v;
+      bh[j] = h;
+      bl[j] = (_Float16)(v - (float)h);
+    }
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+      const int at = (rb * 32 + li) * KS + 16 * sv + 8 * lk;
+      const f16x8 ah = *reinterpret_cast<const f16x8*>(ta + at);
+      const f16x8 al = *reinterpret_cast<const f16x8*>(ta + NCP * KS + at);
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[rb], 0, 0, 0);
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[rb], 0, 0, 0);
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[rb], 0, 0, 0);
+    }
+  }
+  if (cv) {
+    float* yrow = Y + (size_t)(beg + col) * NCP;
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = rb * 32 + q * 8 + lk * 4;
+        float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
+                               acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
+        *reinterpret_cast<float4*>(yrow + row) = v;
+      }
+  }
+}
+
 // Deterministic per-target sum of the GEMM rows: L[t] = inv_s * sum Y[row].
 template <int P>
 __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __restrict__ tptr,
@@ -812,11 +908,16 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
       ED_MARK(4);
     } else {
       const dim3 tiles((unsigned)Pn->n_m2l_tiles);
-      if (prec == 1 || prec == 2) {
+      if (prec == 1 || prec == 2 || prec == 3) {
         constexpr int NC = (P + 1) * (P + 1);
         hipLaunchKernelGGL(k_m2l_absmax<P>, dim3(kAbsMaxBlocks), dim3(256), 0, st,
                            (int64_t)Pn->n_nodes * NC, Pn->M, Pn->m2l_ascale, Pn->m2l_mmax);
-        if (prec == 1)
+        if (prec == 3)
+          hipLaunchKernelGGL((k_m2l_mfma16x3<P>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
+                             Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
+                             (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M, Pn->m2l_mmax,
+                             Pn->ypair);
+        else if (prec == 1)
           hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16>), tiles, dim3(256), 0, st,
                              Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
                              Pn->m2l_pair_src, (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M,

@@ -30,7 +30,7 @@ class LaplaceFmm3D:
     def __init__(self, src_centroids, areas, *, max_leaf_size: int = 64, theta: float = 0.5,
                  use_dipole: bool = True, logger: Optional[Any] = None,
                  expansion_order: Optional[int] = None, backend: str = "auto",
-                 device=None, ws: Optional[int] = None, m2l_precision: str = "fp32",
+                 device=None, ws: Optional[int] = None, m2l_precision: str = "fp16x3",
                  max_level: int = 16):
         if src_centroids.ndim != 2 or src_centroids.shape[1] != 3:
             raise ValueError("src_centroids must have shape (N, 3)")

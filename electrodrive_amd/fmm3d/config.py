@@ -10,7 +10,9 @@ lattice FMM of this package:
   only for theta > 0.866, so theta >= 0.75 maps to ws = 1 and smaller theta to
   ws = 2.
 - ``m2l_precision``: "fp32" (MFMA fp32), "bf16" / "fp16" (MFMA with fp32
-  accumulation) or "valu" (plain fp32 vector path).
+  accumulation), "fp16x3" (split FP16: table and multipoles as hi + lo FP16
+  pairs, three FP16 MFMAs per product, FP32-level accuracy) or "valu" (plain
+  fp32 vector path).
 """
 from dataclasses import dataclass
 from typing import Literal, Optional
@@ -24,7 +26,7 @@ BackendKind = Literal["cpu", "gpu", "auto"]
 PrecisionKind = Literal["single", "double", "mixed"]
 
 SUPPORTED_ORDERS = (4, 6, 8, 10, 12)
-M2L_PRECISIONS = {"valu": -1, "fp32": 0, "bf16": 1, "fp16": 2}
+M2L_PRECISIONS = {"valu": -1, "fp32": 0, "bf16": 1, "fp16": 2, "fp16x3": 3}
 
 
 @dataclass
@@ -46,7 +48,7 @@ class FmmConfig:
 
     ws: Optional[int] = None
     max_level: int = 16
-    m2l_precision: str = "fp32"
+    m2l_precision: str = "fp16x3"
 
     def __post_init__(self) -> None:
         if self.dtype is None:

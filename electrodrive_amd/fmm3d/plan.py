@@ -22,7 +22,7 @@ class FmmPlan:
     """
 
     def __init__(self, centroids, areas, *, leaf_size=64, order=8, ws=1, max_level=16,
-                 m2l_precision="fp32", geom_offsets=None):
+                 m2l_precision="fp16x3", geom_offsets=None):
         nat.require_gpu()
         L = nat.lib()
         C = np.ascontiguousarray(np.asarray(centroids, dtype=np.float64).reshape(-1, 3))

@@ -53,5 +53,5 @@ class BEMConfig:
     fmm_order: Optional[int] = None
     fmm_leaf_size: Optional[int] = None
     fmm_min_panels: int = 4096
-    m2l_precision: str = "fp32"
+    m2l_precision: str = "fp16x3"
     precond: str = "sai"

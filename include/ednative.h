@@ -44,7 +44,9 @@ typedef struct ed_plan_opts {
   int32_t order;         /* expansion order p, 2..16 (reference default 8)   */
   int32_t ws;            /* well-separation radius in boxes: 1 or 2          */
   int32_t max_level;     /* deepest octree level, <= 20                      */
-  int32_t m2l_precision; /* 0 = fp32 MFMA, 1 = bf16 MFMA, 2 = fp16 MFMA      */
+  int32_t m2l_precision; /* 0 = fp32 MFMA, 1 = bf16 MFMA, 2 = fp16 MFMA,
+                            3 = split fp16 (hi + lo, three fp16 MFMAs),
+                            -1 = VALU                                       */
   int32_t reserved[3];
 } ed_plan_opts;
 

@@ -130,11 +130,13 @@ def test_laplace_fmm3d_dropin():
         bem_matvec_gpu(sig, C, A, backend="external")
 
 
-@pytest.mark.parametrize("prec,bound", [("fp32", 1e-5), ("bf16", 5e-3), ("fp16", 1e-3)])
+@pytest.mark.parametrize("prec,bound", [("fp32", 1e-5), ("fp16x3", 1e-5), ("bf16", 5e-3),
+                                        ("fp16", 1e-3)])
 @pytest.mark.parametrize("order", [8, 10, 12])
 def test_m2l_precisions(prec, bound, order):
     """16-bit MFMA M2L (scaled tables, FP32 accumulation) against the dense
-    FP64 result; fp32 MFMA vs the VALU path to FP32 rounding."""
+    FP64 result; fp32 MFMA and split fp16 (hi + lo, three MFMAs) vs the VALU
+    path to FP32 rounding."""
     from electrodrive_amd.fmm3d.plan import FmmPlan
     m = _mesh(4)
     rng = np.random.default_rng(21)
@@ -144,7 +146,7 @@ def test_m2l_precisions(prec, bound, order):
     out = plan.matvec(torch.tensor(sig).cuda()).cpu().numpy()
     err = _rel(out, ref)
     print(f"p={order} {prec} rel err {err:.3e}")
-    if prec == "fp32":
+    if prec in ("fp32", "fp16x3"):
         pv = FmmPlan(m.centroids, m.areas, leaf_size=32, order=order, ws=1, m2l_precision="valu")
         assert _rel(pv.matvec(torch.tensor(sig).cuda()).cpu().numpy(), out) < 2e-6
         bound = 1e-4 if order == 8 else 2e-5
