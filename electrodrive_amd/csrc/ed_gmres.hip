@@ -334,20 +334,19 @@ __global__ __launch_bounds__(kRedThreads) void k_dots_seg(const int4* __restrict
   }
 }
 
-// h[g * ldh + k] = sum of partial[k][tiles of g], k <= j; one block per segment
+// h[g * ldh + k] = sum of partial[k][tiles of g]; one block per (segment g =
+// blockIdx.x, row k = blockIdx.y), rows k > j exit
 __global__ void k_finish_dots_seg(const double* __restrict__ partial, int nt,
                                   const int* __restrict__ seg_t0, const int* __restrict__ dj,
                                   double* __restrict__ h, int ldh) {
   __shared__ double sh[4];
-  const int g = blockIdx.x;
+  const int g = blockIdx.x, k = blockIdx.y;
+  if (k > *dj) return;  // uniform per block
   const int t0 = seg_t0[g], t1 = seg_t0[g + 1];
-  const int j = *dj;
-  for (int k = 0; k <= j; ++k) {
-    double v = 0.0;
-    for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v += partial[(size_t)k * nt + t];
-    v = block_sum(v, sh);
-    if (threadIdx.x == 0) h[(size_t)g * ldh + k] = v;
-  }
+  double v = 0.0;
+  for (int t = t0 + threadIdx.x; t < t1; t += blockDim.x) v += partial[(size_t)k * nt + t];
+  v = block_sum(v, sh);
+  if (threadIdx.x == 0) h[(size_t)g * ldh + k] = v;
 }
 
 // w -= sum_k c[g][k] V_k over the tile; then partial dots V_k . w (second CGS
@@ -402,11 +401,22 @@ __global__ void k_givens_seg(int G, int m, const double* __restrict__ partial,
   __shared__ int sbrk[256];
   const int j = *dj;
   const int ld = m + 1;
+  // |w_g|^2 per segment: wave w sums the tiles of segments w, w + 4, ... across
+  // its lanes (fixed order), into segv row SV_HNEXT
+  {
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63, nw = blockDim.x >> 6;
+    for (int sg = wv; sg < G; sg += nw) {
+      double v = 0.0;
+      for (int t = seg_t0[sg] + ln; t < seg_t0[sg + 1]; t += 64) v += partial[t];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      if (ln == 0) segv[SV_HNEXT * ldsv + sg] = v;
+    }
+  }
+  __syncthreads();
   double rmax = 0.0;
   int brk = 0;
   for (int sg = threadIdx.x; sg < G; sg += blockDim.x) {
-    double v = 0.0;
-    for (int t = seg_t0[sg]; t < seg_t0[sg + 1]; ++t) v += partial[t];
+    const double v = segv[SV_HNEXT * ldsv + sg];
     const double* a1 = h1 + (size_t)sg * (m + 2);
     const double* a2 = h2 + (size_t)sg * (m + 2);
     double* col = H + ((size_t)sg * m + j) * ld;
@@ -838,13 +848,13 @@ struct Solver {
     }
     hipLaunchKernelGGL(k_dots_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), nt, n,
                        general() ? nullptr : W->dinv, W->w, W->V, W->dj, W->partial);
-    hipLaunchKernelGGL(k_finish_dots_seg, dim3(g), dim3(256), 0, W->stream, W->partial, nt,
-                       seg_t0(), W->dj, W->h, m + 2);
+    hipLaunchKernelGGL(k_finish_dots_seg, dim3(g, m + 1), dim3(256), 0, W->stream, W->partial,
+                       nt, seg_t0(), W->dj, W->h, m + 2);
     hipLaunchKernelGGL(k_update_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), nt, n,
                        W->w, W->V, W->dj, W->h, m + 2, 0, W->partial);
     // second Gram-Schmidt pass: coefficients h2
-    hipLaunchKernelGGL(k_finish_dots_seg, dim3(g), dim3(256), 0, W->stream, W->partial, nt,
-                       seg_t0(), W->dj, W->h2, m + 2);
+    hipLaunchKernelGGL(k_finish_dots_seg, dim3(g, m + 1), dim3(256), 0, W->stream, W->partial,
+                       nt, seg_t0(), W->dj, W->h2, m + 2);
     hipLaunchKernelGGL(k_update_seg, dim3(nt), dim3(kRedThreads), 0, W->stream, tiles(), nt, n,
                        W->w, W->V, W->dj, W->h2, m + 2, 1, W->partial);
     hipLaunchKernelGGL(k_givens_seg, dim3(1), dim3(256), 0, W->stream, g, m, W->partial, seg_t0(),
