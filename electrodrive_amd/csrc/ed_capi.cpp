@@ -241,9 +241,9 @@ int plan_upload(ed_plan* P) {
   ED_TRY(upload(&P->n_child, nch));
   ED_HIP(hipMalloc((void**)&P->M, sizeof(float) * (size_t)M * nc));
   ED_HIP(hipMalloc((void**)&P->L, sizeof(float) * (size_t)M * nc));
+  ED_HIP(hipMalloc((void**)&P->ym2m, sizeof(float) * (size_t)M * nc));
   ED_HIP(hipMemset(P->M, 0, sizeof(float) * (size_t)M * nc));
   ED_HIP(hipMemset(P->L, 0, sizeof(float) * (size_t)M * nc));
-  ED_HIP(hipMalloc((void**)&P->ym2m, sizeof(float) * (size_t)M * nc));
 
   // leaves and U lists
   std::vector<int32_t> leaves, uptr{0}, usrc;
@@ -348,7 +348,7 @@ int plan_upload(ed_plan* P) {
   for (int k = 0; k < nc; ++k) { af[k] = (float)asc[k]; bf[k] = (float)asc[k]; }
   ED_TRY(upload(&P->m2l_ascale, af));
   ED_TRY(upload(&P->m2l_binv, bf));
-  ED_TRY(upload(&P->m2l_mmax, std::vector<float>(256, 0.0f)));
+  ED_TRY(upload(&P->m2l_mmax, std::vector<unsigned>(64 * 32, 0u)));  // kMaxSlots x kMaxSlotStride
   std::vector<float> t8((size_t)8 * nc * nc), l8((size_t)8 * nc * nc);
   for (int o = 0; o < 8; ++o) {
     m2m_matrix(p, o, Td.data());

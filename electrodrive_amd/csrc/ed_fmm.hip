@@ -8,14 +8,15 @@
 //   p2m         one wave per leaf: lanes stream points, LDS column reduction
 //   m2m         per level: one block per child (T_oct * M_child), then one
 //               block per parent summing its children
-//   m2l         MFMA GEMM per offset table: Y[pair] = T_off * M[src] (fp32
-//               v_mfma_f32_32x32x2f32), then a deterministic per-target sum
+//   m2l         MFMA GEMM per offset table: Y[pair] = T_off * M[src] (FP32,
+//               split FP16 or 16-bit MFMA), then a deterministic per-target sum
 //   l2l         one block per node: L += T_oct * L_parent (level by level)
 //   p2p         one wave per work item (<= 64 targets x a piece of their
 //               leaf's U-list source stream), partial sums per item
 //   l2p_out     one thread per panel: L2P + P2P partials + near CSR + self
 //               term, K_E scaling, scatter to the output order
-// No float atomics anywhere: every sum has a fixed order.
+// No float sums by atomics: every sum has a fixed order.  (The one atomic is
+// an unsigned max, whose result does not depend on order.)
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -25,6 +26,13 @@
 namespace ed {
 
 constexpr int kLeafBlock = 64;
+// slots of the running multipole maximum (see fold_absmax)
+constexpr int kMaxSlots = 64, kMaxSlotStride = 32;  // 32 x 4 B = 128 B
+
+// The far-field passes raise their waves' issue priority: P2P runs beside
+// them on a side stream, and its VALU-bound waves would otherwise take most
+// issue cycles of every SIMD the two share, stretching the dependent chain.
+__device__ __forceinline__ void chain_prio() { __builtin_amdgcn_s_setprio(3); }
 
 // ---------------------------------------------------------------------------
 // Solid-harmonic streaming helpers (m-major recurrences; see ed_operators.cpp)
@@ -97,20 +105,44 @@ __device__ __forceinline__ float l2p_eval(float x, float y, float z, const float
 }
 
 // ---------------------------------------------------------------------------
-// gather: tree-order q, sigma, self term
+// gather: tree-order q, sigma, self term; resets the multipole maximum
 // ---------------------------------------------------------------------------
 template <typename TS>
 __global__ void k_gather(int n, const int32_t* __restrict__ perm, const TS* __restrict__ sigma,
                          int sigma_tree, const TS* __restrict__ selfi, float4* __restrict__ pts,
                          const float* __restrict__ area_t, float* __restrict__ sig_t,
-                         float* __restrict__ selfc_t) {
+                         float* __restrict__ selfc_t, unsigned* __restrict__ mmax_bits) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kMaxSlots && mmax_bits) mmax_bits[i * kMaxSlotStride] = 0u;  // P2M / M2M fold in
   if (i >= n) return;
   const int o = perm[i];
   const float s = (float)sigma[sigma_tree ? i : o];
   sig_t[i] = s;
   pts[i].w = s * area_t[i];
   selfc_t[i] = selfi ? (float)selfi[o] : 0.0f;
+}
+
+// Largest |M_k * ascale_k| of the multipoles a wave wrote, folded into one
+// of kMaxSlots running maxima of the matvec (reset by k_gather; the 16-bit
+// M2L takes the largest) as the bit pattern of a non-negative float:
+// unsigned order is float order there, so the result does not depend on the
+// order of the atomics.  Slots sit a cache line apart, so the atomics of
+// different waves do not queue on one address.  NaN entries are skipped, as
+// fmaxf would.  `bits` is NULL unless the M2L runs in a 16-bit form.
+__device__ __forceinline__ void fold_absmax(float v, unsigned* bits, int slot) {
+  v = v == v ? fabsf(v) : 0.0f;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0 && v > 0.0f)
+    atomicMax(bits + (slot & (kMaxSlots - 1)) * kMaxSlotStride, __float_as_uint(v));
+}
+
+// The largest of the slots (every lane of the calling wave gets it).
+__device__ __forceinline__ float slots_absmax(const unsigned* __restrict__ bits) {
+  float v = __uint_as_float(bits[(threadIdx.x & (kMaxSlots - 1)) * kMaxSlotStride]);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
 }
 
 // ---------------------------------------------------------------------------
@@ -121,7 +153,10 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
                                                     const int2* __restrict__ rng,
                                                     const float4* __restrict__ ctr,
                                                     const float4* __restrict__ pts,
-                                                    float* __restrict__ M) {
+                                                    float* __restrict__ M,
+                                                    const float* __restrict__ ascale,
+                                                    unsigned* __restrict__ mmax_bits) {
+  chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float red[kLeafBlock][NC + 1];
   const int node = leaves[blockIdx.x];
@@ -137,25 +172,29 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
 #pragma unroll
   for (int k = 0; k < NC; ++k) red[threadIdx.x][k] = acc[k];
   __syncthreads();
+  float mx = 0.0f;
   for (int k = threadIdx.x; k < NC; k += kLeafBlock) {
     float s = 0.0f;
     for (int t = 0; t < kLeafBlock; ++t) s += red[t][k];
     M[(size_t)node * NC + k] = s;
+    if (mmax_bits) mx = fmaxf(mx, fabsf(s * ascale[k]));
   }
+  if (mmax_bits) fold_absmax(mx, mmax_bits, blockIdx.x);  // kLeafBlock is one wave
 }
 
 // ---------------------------------------------------------------------------
-// M2M: one block per parent; M_parent = sum_c T_m2m[oct(c)] M_c
-// T stored column-major: T[oct][j][k] (input j, output k)
-// ---------------------------------------------------------------------------
 // M2M in two passes per level: one block per child applies its octant's
-// operator (Y_c = T_oct(c) M_c; 121 loads per thread instead of 8 x 121),
-// then one block per parent sums its children in a fixed order.
+// operator (Y_c = T_oct(c) M_c; T stored column-major, T[oct][j][k] = input
+// j, output k), then one block per parent sums its children in a fixed
+// order.  (Small blocks on purpose: the passes run beside P2P, and a block
+// of many waves waits until that many wave slots are free on one CU.)
+// ---------------------------------------------------------------------------
 template <int P>
 __global__ __launch_bounds__(128) void k_m2m_child(int child0, const int32_t* __restrict__ octant,
                                                    const float* __restrict__ T,
                                                    const float* __restrict__ M,
                                                    float* __restrict__ Y) {
+  chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float mc[NC];
   const int c = child0 + blockIdx.x;
@@ -175,43 +214,21 @@ __global__ __launch_bounds__(128) void k_m2m_sum(const int32_t* __restrict__ par
                                                  const int32_t* __restrict__ first_child,
                                                  const int32_t* __restrict__ n_child, int child0,
                                                  const float* __restrict__ Y,
-                                                 float* __restrict__ M) {
+                                                 float* __restrict__ M,
+                                                 const float* __restrict__ ascale,
+                                                 unsigned* __restrict__ mmax_bits) {
+  chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   const int par = parents[blockIdx.x];
   const int c0 = first_child[par] - child0, nch = n_child[par];
+  float mx = 0.0f;
   for (int k = threadIdx.x; k < NC; k += blockDim.x) {
     float s = 0.0f;
     for (int c = 0; c < nch; ++c) s += Y[(size_t)(c0 + c) * NC + k];
     M[(size_t)par * NC + k] = s;
+    if (mmax_bits) mx = fmaxf(mx, fabsf(s * ascale[k]));
   }
-}
-
-template <int P>
-__global__ __launch_bounds__(128) void k_m2m(const int32_t* __restrict__ parents,
-                                             const int32_t* __restrict__ first_child,
-                                             const int32_t* __restrict__ n_child,
-                                             const int32_t* __restrict__ octant,
-                                             const float* __restrict__ T,
-                                             float* __restrict__ M) {
-  constexpr int NC = (P + 1) * (P + 1);
-  __shared__ float mc[8][NC];
-  __shared__ int oc[8];
-  const int par = parents[blockIdx.x];
-  const int c0 = first_child[par], nch = n_child[par];
-  for (int t = threadIdx.x; t < nch * NC; t += blockDim.x) {
-    const int c = t / NC, j = t - c * NC;
-    mc[c][j] = M[(size_t)(c0 + c) * NC + j];
-  }
-  if (threadIdx.x < nch) oc[threadIdx.x] = octant[c0 + threadIdx.x];
-  __syncthreads();
-  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
-    float s = 0.0f;
-    for (int c = 0; c < nch; ++c) {
-      const float* Tc = T + (size_t)oc[c] * NC * NC + k;
-      for (int j = 0; j < NC; ++j) s += Tc[(size_t)j * NC] * mc[c][j];
-    }
-    M[(size_t)par * NC + k] = s;
-  }
+  if (mmax_bits) fold_absmax(mx, mmax_bits, blockIdx.x * 2 + (threadIdx.x >> 6));  // whole waves
 }
 
 // ---------------------------------------------------------------------------
@@ -226,6 +243,7 @@ __global__ __launch_bounds__(128) void k_m2l_valu(int node0, const int32_t* __re
                                                   const float* __restrict__ T,
                                                   const float* __restrict__ M,
                                                   float* __restrict__ L) {
+  chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float ms[NC];
   const int t = node0 + blockIdx.x;
@@ -287,27 +305,6 @@ struct M2LShapeH {
 // values are (the corrections of iterative refinement are orders of magnitude
 // below the first right-hand side) and the scale is undone exactly on Y.
 constexpr int kM2LScaleExp = 14;
-constexpr int kAbsMaxBlocks = 256;
-
-template <int P>
-__global__ __launch_bounds__(256) void k_m2l_absmax(int64_t total, const float* __restrict__ M,
-                                                   const float* __restrict__ ascale,
-                                                   float* __restrict__ part) {
-  constexpr int NC = (P + 1) * (P + 1);
-  __shared__ float red[256];
-  float m = 0.0f;
-  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-       e += (int64_t)gridDim.x * blockDim.x)
-    m = fmaxf(m, fabsf(M[e] * ascale[e % NC]));
-  red[threadIdx.x] = m;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
 template <int P>
 __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t* __restrict__ tile_tab,
                                                   const int32_t* __restrict__ tile_beg,
@@ -317,6 +314,7 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
                                                   const float* __restrict__ ascale,
                                                   const float* __restrict__ M,
                                                   float* __restrict__ Y) {
+  chain_prio();
   using S = M2LShape<P>;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
   constexpr int NT = S::THREADS, NW = NT / 64, KH = KP / 2;
@@ -420,25 +418,17 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
                                                     const T16* __restrict__ Tm,
                                                     const float* __restrict__ ascale,
                                                     const float* __restrict__ M,
-                                                    const float* __restrict__ mmax,
+                                                    const unsigned* __restrict__ mmax_bits,
                                                     float* __restrict__ Y) {
+  chain_prio();
   using S = M2LShapeH<P>;
   using V = typename Mfma16<T16>::vec;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
   __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
-  __shared__ float red[256];
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
-  // power-of-two input scale from the per-block maxima of |M * ascale|
-  float mx = 0.0f;
-  for (int t = threadIdx.x; t < kAbsMaxBlocks; t += blockDim.x) mx = fmaxf(mx, mmax[t]);
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
-    __syncthreads();
-  }
-  const float gmax = red[0];
+  // power-of-two input scale from the largest |M * ascale| of this matvec
+  const float gmax = slots_absmax(mmax_bits);
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
   const float inv_sc = 1.0f / sc;
   const int tile = blockIdx.x;
@@ -513,22 +503,15 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
                                                       const _Float16* __restrict__ Tm,
                                                       const float* __restrict__ ascale,
                                                       const float* __restrict__ M,
-                                                      const float* __restrict__ mmax,
+                                                      const unsigned* __restrict__ mmax_bits,
                                                       float* __restrict__ Y) {
+  chain_prio();
   using S = M2LShapeH<P>;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
   __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
-  __shared__ float red[256];
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
-  float mx = 0.0f;
-  for (int t = threadIdx.x; t < kAbsMaxBlocks; t += blockDim.x) mx = fmaxf(mx, mmax[t]);
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + o]);
-    __syncthreads();
-  }
-  const float gmax = red[0];
+  // power-of-two input scale from the largest |M * ascale| of this matvec
+  const float gmax = slots_absmax(mmax_bits);
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
   const float inv_sc = 1.0f / sc;
   const int tile = blockIdx.x;
@@ -602,6 +585,7 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
                                                     const float* __restrict__ binv,
                                                     const float* __restrict__ Y,
                                                     float* __restrict__ L) {
+  chain_prio();
   using S = M2LShape<P>;
   constexpr int NC = S::NC, NCP = S::NCP;
   const int t = node0 + blockIdx.x;
@@ -623,6 +607,7 @@ __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restric
                                              const int32_t* __restrict__ octant,
                                              const float* __restrict__ T,
                                              float* __restrict__ L) {
+  chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float lp[NC];
   const int c = node0 + blockIdx.x;
@@ -853,14 +838,19 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   const int n = (int)Pn->n;
   ED_MARK(0);
   const int gb = (n + 255) / 256;
+  // M2L form: -1 VALU, 0 FP32 MFMA, 1 / 2 / 3 BF16 / FP16 / split FP16 MFMA;
+  // the 16-bit forms scale their inputs by the largest |M * ascale|, which
+  // P2M and M2M fold into mmax as they write M
+  const int prec = Pn->n_m2l_tiles > 0 ? Pn->m2l_precision : -1;
+  unsigned* mmax = (prec >= 1 && prec <= 3) ? Pn->m2l_mmax : nullptr;
   if (sig64)
     hipLaunchKernelGGL(k_gather<double>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
                        (const double*)sigma, sig_tree, (const double*)selfi, Pn->pts, Pn->area_t,
-                       Pn->sig_t, Pn->selfc_t);
+                       Pn->sig_t, Pn->selfc_t, mmax);
   else
     hipLaunchKernelGGL(k_gather<float>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
                        (const float*)sigma, sig_tree, (const float*)selfi, Pn->pts, Pn->area_t,
-                       Pn->sig_t, Pn->selfc_t);
+                       Pn->sig_t, Pn->selfc_t, mmax);
   ED_LAUNCH_CHECK("gather");
   ED_MARK(1);
   // P2P depends only on the gathered points: off the profiling path it runs
@@ -878,7 +868,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   const int depth = H.depth;
   if (depth >= 2) {
     hipLaunchKernelGGL(k_p2m<P>, dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
-                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->M);
+                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->M, Pn->m2l_ascale, mmax);
     ED_LAUNCH_CHECK("p2m");
     ED_MARK(2);
     // upward pass: parents at level l (children at l+1), deepest first
@@ -890,7 +880,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
                            Pn->node_octant, Pn->t_m2m, Pn->M, Pn->ym2m);
         hipLaunchKernelGGL(k_m2m_sum<P>, dim3((unsigned)(pe - pb)), dim3(128), 0, st,
                            Pn->parents_list + pb, Pn->first_child, Pn->n_child, (int)cb,
-                           Pn->ym2m, Pn->M);
+                           Pn->ym2m, Pn->M, Pn->m2l_ascale, mmax);
         ED_LAUNCH_CHECK("m2m");
       }
     }
@@ -898,7 +888,6 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     // M2L for all nodes at levels >= 2
     const int node0 = (int)H.level_begin[2];
     const int nm = (int)(Pn->n_nodes - node0);
-    const int prec = Pn->n_m2l_tiles > 0 ? Pn->m2l_precision : -1;
     bool valu = prec < 0;
     if constexpr (!M2LShape<P>::ENABLED) valu = valu || prec == 0;
     if (valu) {
@@ -909,9 +898,6 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     } else {
       const dim3 tiles((unsigned)Pn->n_m2l_tiles);
       if (prec == 1 || prec == 2 || prec == 3) {
-        constexpr int NC = (P + 1) * (P + 1);
-        hipLaunchKernelGGL(k_m2l_absmax<P>, dim3(kAbsMaxBlocks), dim3(256), 0, st,
-                           (int64_t)Pn->n_nodes * NC, Pn->M, Pn->m2l_ascale, Pn->m2l_mmax);
         if (prec == 3)
           hipLaunchKernelGGL((k_m2l_mfma16x3<P>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
                              Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,

@@ -33,6 +33,7 @@ struct ed_plan {
   int32_t* n_child = nullptr;
   float* M = nullptr;  // n_nodes * nc
   float* L = nullptr;  // n_nodes * nc
+  float* ym2m = nullptr;  // n_nodes * nc, per-child M2M products
 
   // leaves (node ids) and their U lists (CSR over leaf ordinal)
   int32_t* pt_node = nullptr;   // n, leaf node of each tree-ordered panel
@@ -44,7 +45,6 @@ struct ed_plan {
   float* p2p_part = nullptr;
   int2* p2p_ref = nullptr;
   int64_t n_p2p_items = 0;
-  float* ym2m = nullptr;        // n_nodes x (p+1)^2, per-child M2M products
   int32_t* leaves = nullptr;
   int32_t* u_ptr = nullptr;
   int32_t* u_src = nullptr;  // node ids
@@ -80,7 +80,7 @@ struct ed_plan {
   uint16_t* t_m2l_h = nullptr;  // n_off * ncp * kp16, bf16/fp16 (scaled)
   float* m2l_ascale = nullptr;  // per input coefficient: sqrt((n+m)!(n-m)!)
   float* m2l_binv = nullptr;    // per output coefficient: sqrt((n+m)!(n-m)!)
-  float* m2l_mmax = nullptr;       // 256 per-block maxima of |M * ascale| (16-bit M2L scale)
+  unsigned* m2l_mmax = nullptr;    // 64 slots (128 B apart) of max |M * ascale| as float bits
   float* t_m2m = nullptr;       // 8 * nc * nc
   float* t_l2l = nullptr;       // 8 * nc * nc
 
