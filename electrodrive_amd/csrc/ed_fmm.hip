@@ -493,8 +493,11 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
 // A B ~ A_hi B_hi + A_hi B_lo + A_lo B_hi is three v_mfma_f32_32x32x16_f16
 // with FP32 accumulation (the dropped A_lo B_lo is ~2^-22 of the product).
 // Both table halves sit in LDS; each lane forms its column's B hi / lo in
-// registers from the gathered FP32 multipoles.  Four waves x 32 columns x all
-// rows, the same tile schedule and Y layout as the 16-bit kernel.
+// registers from the gathered FP32 multipoles.  Four waves x 32 columns x
+// all rows, the same tile schedule and Y layout as the 16-bit kernel.  Each
+// lane issues all loads of its multipole column (KP / 2 values) before the
+// table copy, so their latency hides behind it and the MFMA loop never
+// waits on global memory.
 template <int P>
 __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict__ tile_tab,
                                                       const int32_t* __restrict__ tile_beg,
@@ -507,15 +510,30 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
                                                       float* __restrict__ Y) {
   chain_prio();
   using S = M2LShapeH<P>;
-  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
+  __shared__ float as_l[KP];  // ascale x the power-of-two input scale, 0 past NC
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
+  const int tile = blockIdx.x;
+  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col0 = wave * 32;
+  const int li = lane & 31, lk = lane >> 5;
+  const int col = col0 + li;
+  const bool cv = col < cnt;
+  // this lane's multipole values: k = 16 sv + 8 lk + j (clamped addresses)
+  float mv[NSV][8];
+  if (col0 < cnt) {
+    const float* mrow = M + (size_t)pair_src[beg + min(col, cnt - 1)] * NC;
+#pragma unroll
+    for (int sv = 0; sv < NSV; ++sv)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, NC - 1)];
+  }
   // power-of-two input scale from the largest |M * ascale| of this matvec
   const float gmax = slots_absmax(mmax_bits);
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
   const float inv_sc = 1.0f / sc;
-  const int tile = blockIdx.x;
-  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
   // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
   // multiple of 16) elements long, so a chunk never crosses a row
   const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
@@ -526,29 +544,21 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
     const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
     *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
   }
+  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc : 0.0f;
   __syncthreads();
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int col0 = wave * 32;
   if (col0 >= cnt) return;  // no barrier below
-  const int li = lane & 31, lk = lane >> 5;
-  const int col = col0 + li;
-  const bool cv = col < cnt;
-  const float* mrow = M + (size_t)pair_src[beg + min(col, cnt - 1)] * NC;
   constexpr int NRB = NCP / 32;
   f32x16 acc[NRB];
 #pragma unroll
   for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
-#pragma unroll 2
-  for (int sv = 0; sv < KP / 16; ++sv) {
+#pragma unroll
+  for (int sv = 0; sv < NSV; ++sv) {
     f16x8 bh, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int k = 16 * sv + 8 * lk + j;
-      const int kk = min(k, NC - 1);
-      const float raw = mrow[kk] * ascale[kk] * sc;
-      const float v = (cv && k < NC) ? raw : 0.0f;
+      const float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] : 0.0f;
       const _Float16 h = (_Float16)v;
       bh[j] = h;
       bl[j] = (_Float16)(v - (float)h);
