@@ -73,13 +73,14 @@ void plan_free(ed_plan* P) {
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
                   P->leaves, P->pt_node, P->p2p_items, P->p2p_seg, P->p2p_part, P->p2p_ref, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
                   P->m2l_pair_src, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
-                  P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair, P->t_m2l,
-                  P->t_m2l_mfma, P->t_m2l_h, P->t_m2m, P->t_l2l, P->near_ptr, P->near_col,
-                  P->near_w, P->cent64_t, P->area64_t, P->near_w64, P->m2l_ascale,
-                  P->m2l_binv, P->m2l_mmax, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
+                  P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair,
+                  P->near_ptr, P->near_col, P->near_w, P->cent64_t, P->area64_t, P->near_w64,
+                  P->m2l_mmax, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
                   P->coarse_part, P->coarse_tmp, P->dense_items, P->dense_rns, P->dense_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
+  // t_m2l*, t_m2m, t_l2l, m2l_ascale / m2l_binv point into the shared
+  // per-device operator tables
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
   sai_free(P);
@@ -131,6 +132,94 @@ static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, 
       }
   }
   return cache.emplace(key, std::move(ot)).first->second;
+}
+
+// Device copies of the translation operators, shared by every plan with the
+// same (device, order, M2L form, ws): the M2L table of every lattice offset a
+// V list can hold (ws < max |o_i| <= 2 ws + 1, in slot order), the eight M2M
+// and L2L matrices and the balanced scaling a_n^m = sqrt((n+m)!(n-m)!).
+// Uploaded on first use and kept for the life of the process (20 MB for the
+// order-10 split-FP16 set), so a new plan uploads no operator.  Plans point
+// into it and never free it.
+struct DeviceTables {
+  float* valu = nullptr;        // n_tab x nc x nc (VALU path)
+  float* f32 = nullptr;         // n_tab x ncp x kp (FP32 MFMA)
+  uint16_t* h16 = nullptr;      // n_tab x hsz (16-bit and split-FP16 MFMA)
+  float* m2m = nullptr;         // 8 x nc x nc, column-major
+  float* l2l = nullptr;         // 8 x nc x nc, column-major
+  float* ascale = nullptr;      // nc
+  std::vector<int32_t> slot_tab;  // offset slot -> table index, -1 if none
+  int n_tab = 0;
+};
+
+static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
+                         const DeviceTables** out) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, int, int, int>, DeviceTables> cache;
+  std::lock_guard<std::mutex> lock(mu);
+  const auto key = std::make_tuple(device, p, kind, ws);
+  auto it = cache.find(key);
+  if (it != cache.end()) {
+    *out = &it->second;
+    return ED_OK;
+  }
+  const int nc = (p + 1) * (p + 1);
+  const int R = offset_radius(ws), W = 2 * R + 1;
+  DeviceTables D;
+  D.slot_tab.assign((size_t)W * W * W, -1);
+  for (int oz = -R; oz <= R; ++oz)
+    for (int oy = -R; oy <= R; ++oy)
+      for (int ox = -R; ox <= R; ++ox)
+        if (std::max(std::abs(ox), std::max(std::abs(oy), std::abs(oz))) > ws)
+          D.slot_tab[(size_t)offset_slot(ws, ox, oy, oz)] = D.n_tab++;
+  std::vector<double> asc(nc);
+  for (int n2 = 0; n2 <= p; ++n2)
+    for (int m2 = 0; m2 <= n2; ++m2) {
+      const double f = std::sqrt(std::tgamma(n2 + m2 + 1.0) * std::tgamma(n2 - m2 + 1.0));
+      asc[n2 * n2 + (m2 == 0 ? 0 : 2 * m2 - 1)] = f;
+      if (m2 > 0) asc[n2 * n2 + 2 * m2] = f;
+    }
+  const size_t hsz = (size_t)ncp * kp * (kind == 3 ? 2 : 1);
+  std::vector<float> tv(kind < 0 ? (size_t)D.n_tab * nc * nc : 0);
+  std::vector<float> tm(kind == 0 ? (size_t)D.n_tab * ncp * kp : 0);
+  std::vector<uint16_t> th(kind > 0 ? (size_t)D.n_tab * hsz : 0);
+  for (size_t slot = 0; slot < D.slot_tab.size(); ++slot) {
+    const int tab = D.slot_tab[slot];
+    if (tab < 0) continue;
+    const int ox = (int)(slot % W) - R, oy = (int)((slot / W) % W) - R, oz = (int)(slot / (W * W)) - R;
+    const OperatorTables& ot = cached_m2l_tables(p, kind, ox, oy, oz, ncp, kp, asc);
+    if (kind < 0) std::memcpy(tv.data() + (size_t)tab * nc * nc, ot.valu.data(), sizeof(float) * nc * nc);
+    else if (kind == 0)
+      std::memcpy(tm.data() + (size_t)tab * ncp * kp, ot.f32.data(), sizeof(float) * ncp * kp);
+    else
+      std::memcpy(th.data() + (size_t)tab * hsz, ot.h16.data(), sizeof(uint16_t) * hsz);
+  }
+  std::vector<double> Td((size_t)nc * nc);
+  std::vector<float> t8((size_t)8 * nc * nc), l8((size_t)8 * nc * nc);
+  for (int o = 0; o < 8; ++o) {
+    m2m_matrix(p, o, Td.data());
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < nc; ++j) t8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
+    l2l_matrix(p, o, Td.data());
+    for (int k = 0; k < nc; ++k)
+      for (int j = 0; j < nc; ++j) l8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
+  }
+  std::vector<float> af(nc);
+  for (int k = 0; k < nc; ++k) af[k] = (float)asc[k];
+  int rc = ED_OK;
+  if (!tv.empty()) rc = upload(&D.valu, tv);
+  if (rc == ED_OK && !tm.empty()) rc = upload(&D.f32, tm);
+  if (rc == ED_OK && !th.empty()) rc = upload(&D.h16, th);
+  if (rc == ED_OK) rc = upload(&D.m2m, t8);
+  if (rc == ED_OK) rc = upload(&D.l2l, l8);
+  if (rc == ED_OK) rc = upload(&D.ascale, af);
+  if (rc != ED_OK) {
+    for (void* q : {(void*)D.valu, (void*)D.f32, (void*)D.h16, (void*)D.m2m, (void*)D.l2l, (void*)D.ascale})
+      if (q) (void)hipFree(q);
+    return rc;
+  }
+  *out = &cache.emplace(key, std::move(D)).first->second;
+  return ED_OK;
 }
 
 // One side stream per device for the P2P branch of every plan on it, created
@@ -289,77 +378,42 @@ int plan_upload(ed_plan* P) {
   ED_TRY(upload(&P->parents_list, plist));
 
   // V lists and offset tables
-  std::map<int32_t, int32_t> slot_map;
-  for (int32_t s : H.v_off) slot_map.emplace(s, 0);
-  int32_t ntab = 0;
-  for (auto& kv : slot_map) kv.second = ntab++;
-  P->n_off = ntab;
-  std::vector<int32_t> vptr(M + 1), vsrc(H.v_src.size()), vtab(H.v_src.size());
-  for (int64_t i = 0; i <= M; ++i) vptr[i] = (int32_t)H.v_ptr[i];
-  for (size_t q = 0; q < H.v_src.size(); ++q) {
-    vsrc[q] = (int32_t)H.v_src[q];
-    vtab[q] = slot_map[H.v_off[q]];
-  }
-  P->n_v = (int64_t)vsrc.size();
-  ED_TRY(upload(&P->v_ptr, vptr));
-  ED_TRY(upload(&P->v_src, vsrc));
-  ED_TRY(upload(&P->v_tab, vtab));
-
-  const int R = offset_radius(P->ws), W = 2 * R + 1;
-  std::vector<double> Td((size_t)nc * nc);
   const int prec = P->m2l_precision;
   int ncp = 0, kp = 0, cols = 0, enabled = 0;
   ED_TRY(m2l_shape(p, prec < 0 ? 0 : prec, &ncp, &kp, &cols, &enabled));
   if (prec < 0) enabled = 0;
   P->ncp = ncp;
-  // Balanced scaling for the GEMM form: with a_n^m = sqrt((n+m)!(n-m)!) the
-  // table diag(1/a) T diag(1/a) has entries below 1 in magnitude (raw
-  // entries reach 1e17 at p = 10), so it survives fp16 / bf16 rounding.
-  std::vector<double> asc(nc);
-  for (int n2 = 0; n2 <= p; ++n2)
-    for (int m2 = 0; m2 <= n2; ++m2) {
-      const double f = std::sqrt(std::tgamma(n2 + m2 + 1.0) * std::tgamma(n2 - m2 + 1.0));
-      asc[n2 * n2 + (m2 == 0 ? 0 : 2 * m2 - 1)] = f;
-      if (m2 > 0) asc[n2 * n2 + 2 * m2] = f;
-    }
-  const size_t tsz = (size_t)std::max(ntab, 1) * ncp * kp;
   const bool need_valu = !enabled || prec < 0;
-  std::vector<float> tv(need_valu ? (size_t)std::max(ntab, 1) * nc * nc : 1, 0.0f);
-  std::vector<float> tm(enabled && prec == 0 ? tsz : 1, 0.0f);
-  const size_t hsz = (size_t)ncp * kp * (prec == 3 ? 2 : 1);  // per table
-  std::vector<uint16_t> th(enabled && prec > 0 ? (size_t)std::max(ntab, 1) * hsz : 1, 0);
-  for (auto& kv : slot_map) {
-    const int slot = kv.first, tab = kv.second;
-    const int ox = slot % W - R, oy = (slot / W) % W - R, oz = slot / (W * W) - R;
-    const OperatorTables& ot = cached_m2l_tables(p, need_valu ? -1 : prec, ox, oy, oz, ncp, kp, asc);
-    if (need_valu) std::memcpy(tv.data() + (size_t)tab * nc * nc, ot.valu.data(), sizeof(float) * nc * nc);
-    else if (prec == 0)
-      std::memcpy(tm.data() + (size_t)tab * ncp * kp, ot.f32.data(), sizeof(float) * ncp * kp);
-    else
-      std::memcpy(th.data() + (size_t)tab * hsz, ot.h16.data(), sizeof(uint16_t) * hsz);
+  const DeviceTables* DT = nullptr;
+  ED_TRY(device_tables(P->device, p, need_valu ? -1 : prec, P->ws, ncp, kp, &DT));
+  const int ntab = DT->n_tab;
+  std::vector<int32_t> vptr(M + 1), vsrc(H.v_src.size()), vtab(H.v_src.size());
+  std::vector<char> used((size_t)ntab, 0);
+  for (int64_t i = 0; i <= M; ++i) vptr[i] = (int32_t)H.v_ptr[i];
+  for (size_t q = 0; q < H.v_src.size(); ++q) {
+    vsrc[q] = (int32_t)H.v_src[q];
+    const int32_t s = H.v_off[q];
+    const int32_t tab = s >= 0 && (size_t)s < DT->slot_tab.size() ? DT->slot_tab[(size_t)s] : -1;
+    if (tab < 0) return fail(ED_ERR_INTERNAL, "plan: V-list offset outside the table set");
+    vtab[q] = tab;
+    used[(size_t)tab] = 1;
   }
-  ED_TRY(upload(&P->t_m2l, tv));
-  ED_TRY(upload(&P->t_m2l_mfma, tm));
-  ED_TRY(upload(&P->t_m2l_h, th));
+  P->n_off = (int)std::count(used.begin(), used.end(), (char)1);  // distinct offsets in use
+  P->n_v = (int64_t)vsrc.size();
+  ED_TRY(upload(&P->v_ptr, vptr));
+  ED_TRY(upload(&P->v_src, vsrc));
+  ED_TRY(upload(&P->v_tab, vtab));
+  P->t_m2l = DT->valu;
+  P->t_m2l_mfma = DT->f32;
+  P->t_m2l_h = DT->h16;
+  P->t_m2m = DT->m2m;
+  P->t_l2l = DT->l2l;
   // T = diag(a) T_s diag(a): the GEMM kernels multiply each gathered
   // multipole coefficient by a (ascale) and the reduce multiplies each summed
   // row by a (binv), so L = T M exactly up to rounding.
-  std::vector<float> af(nc), bf(nc);
-  for (int k = 0; k < nc; ++k) { af[k] = (float)asc[k]; bf[k] = (float)asc[k]; }
-  ED_TRY(upload(&P->m2l_ascale, af));
-  ED_TRY(upload(&P->m2l_binv, bf));
+  P->m2l_ascale = DT->ascale;
+  P->m2l_binv = DT->ascale;
   ED_TRY(upload(&P->m2l_mmax, std::vector<unsigned>(64 * 32, 0u)));  // kMaxSlots x kMaxSlotStride
-  std::vector<float> t8((size_t)8 * nc * nc), l8((size_t)8 * nc * nc);
-  for (int o = 0; o < 8; ++o) {
-    m2m_matrix(p, o, Td.data());
-    for (int k = 0; k < nc; ++k)
-      for (int j = 0; j < nc; ++j) t8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
-    l2l_matrix(p, o, Td.data());
-    for (int k = 0; k < nc; ++k)
-      for (int j = 0; j < nc; ++j) l8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
-  }
-  ED_TRY(upload(&P->t_m2m, t8));
-  ED_TRY(upload(&P->t_l2l, l8));
 
   // GEMM-form M2L schedule: pairs grouped by table, tiles of `cols` pairs
   P->n_m2l_tiles = 0;

@@ -148,58 +148,111 @@ __device__ __forceinline__ int near_cell_coord(double x, double lo, double inv_h
   return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
 }
 
-constexpr int kNearBlock = 128;
-__global__ __launch_bounds__(kNearBlock) void k_near_pairs(const double* __restrict__ C,
-                                                           const double* __restrict__ A,
-                                                           int64_t n, double df, NearGrid g,
-                                                           const int32_t* __restrict__ cell_start,
-                                                           const int32_t* __restrict__ cell_pts,
-                                                           int64_t* __restrict__ counts,
-                                                           const int64_t* __restrict__ offs,
-                                                           int64_t* __restrict__ pairs,
-                                                           int32_t* __restrict__ cols32,
-                                                           int64_t col_off) {
+// One wave per point i: the lanes share each candidate cell (64 candidates
+// per step), accepted partners are counted with a ballot and, when filling,
+// written at the row offset plus their rank among the accepted lanes.  The
+// row is then put in ascending j: rows of up to kNearSortCap entries are
+// collected in LDS and each lane places its entries by rank (j is unique in a
+// row); longer rows fall back to an insertion sort by one lane.
+constexpr int kNearWaves = 4;      // points per block
+constexpr int kNearSortCap = 256;  // row entries sorted in LDS
+__global__ __launch_bounds__(64 * kNearWaves) void k_near_pairs(const double* __restrict__ C,
+                                                               const double* __restrict__ A,
+                                                               int64_t n, double df, NearGrid g,
+                                                               const int32_t* __restrict__ cell_start,
+                                                               const int32_t* __restrict__ cell_pts,
+                                                               int64_t* __restrict__ counts,
+                                                               const int64_t* __restrict__ offs,
+                                                               int64_t* __restrict__ pairs,
+                                                               int32_t* __restrict__ cols32,
+                                                               int64_t col_off) {
 #pragma clang fp contract(off)
-  const int64_t i = (int64_t)blockIdx.x * kNearBlock + threadIdx.x;
-  if (i >= n) return;
-  const double xi = C[3 * i], yi = C[3 * i + 1], zi = C[3 * i + 2];
-  const double ri = near_radius(A[i]);
-  int64_t cnt = 0;
-  const int64_t w0 = offs ? offs[i] : 0;
-  if (isfinite(xi) && isfinite(yi) && isfinite(zi) && isfinite(ri)) {
-    const int cx = near_cell_coord(xi, g.lo[0], g.inv_h, g.dim[0]);
-    const int cy = near_cell_coord(yi, g.lo[1], g.inv_h, g.dim[1]);
-    const int cz = near_cell_coord(zi, g.lo[2], g.inv_h, g.dim[2]);
-    for (int dz = -1; dz <= 1; ++dz) {
-      const int z = cz + dz;
-      if (z < 0 || z >= g.dim[2]) continue;
-      for (int dy = -1; dy <= 1; ++dy) {
-        const int y = cy + dy;
-        if (y < 0 || y >= g.dim[1]) continue;
-        for (int dx = -1; dx <= 1; ++dx) {
-          const int x = cx + dx;
-          if (x < 0 || x >= g.dim[0]) continue;
-          const int64_t cell = ((int64_t)z * g.dim[1] + y) * g.dim[0] + x;
-          for (int32_t q = cell_start[cell]; q < cell_start[cell + 1]; ++q) {
-            const int64_t j = cell_pts[q];
-            const double ex = xi - C[3 * j], ey = yi - C[3 * j + 1], ez = zi - C[3 * j + 2];
-            const double d = sqrt(ex * ex + ey * ey + ez * ez);
-            const double th = df * (ri + near_radius(A[j]));
-            if (isfinite(d) && isfinite(th) && d > 0.0 && d <= th) {
-              if (pairs) {
-                pairs[2 * (w0 + cnt)] = i;
-                pairs[2 * (w0 + cnt) + 1] = j;
+  __shared__ int32_t srt[kNearWaves][kNearSortCap];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t i = (int64_t)blockIdx.x * kNearWaves + wv;
+  const bool valid = i < n;  // wave-uniform; every wave reaches the barrier
+  const bool fill = pairs != nullptr || cols32 != nullptr;
+  int64_t cnt = 0;  // wave-uniform
+  int64_t w0 = 0;
+  if (valid) {
+    const double xi = C[3 * i], yi = C[3 * i + 1], zi = C[3 * i + 2];
+    const double ri = near_radius(A[i]);
+    w0 = offs ? offs[i] : 0;
+    if (isfinite(xi) && isfinite(yi) && isfinite(zi) && isfinite(ri)) {
+      const int cx = near_cell_coord(xi, g.lo[0], g.inv_h, g.dim[0]);
+      const int cy = near_cell_coord(yi, g.lo[1], g.inv_h, g.dim[1]);
+      const int cz = near_cell_coord(zi, g.lo[2], g.inv_h, g.dim[2]);
+      for (int dz = -1; dz <= 1; ++dz) {
+        const int z = cz + dz;
+        if (z < 0 || z >= g.dim[2]) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+          const int y = cy + dy;
+          if (y < 0 || y >= g.dim[1]) continue;
+          for (int dx = -1; dx <= 1; ++dx) {
+            const int x = cx + dx;
+            if (x < 0 || x >= g.dim[0]) continue;
+            const int64_t cell = ((int64_t)z * g.dim[1] + y) * g.dim[0] + x;
+            const int32_t cs = cell_start[cell], ce = cell_start[cell + 1];
+            for (int32_t q0 = cs; q0 < ce; q0 += 64) {
+              const int32_t q = q0 + lane;
+              bool ok = false;
+              int64_t j = 0;
+              if (q < ce) {
+                j = cell_pts[q];
+                const double ex = xi - C[3 * j], ey = yi - C[3 * j + 1], ez = zi - C[3 * j + 2];
+                const double d = sqrt(ex * ex + ey * ey + ez * ez);
+                const double th = df * (ri + near_radius(A[j]));
+                ok = isfinite(d) && isfinite(th) && d > 0.0 && d <= th;
               }
-              if (cols32) cols32[w0 + cnt] = (int32_t)(j + col_off);
-              ++cnt;
+              const uint64_t m = __ballot(ok);
+              if (ok && fill) {
+                const int64_t pos = cnt + __popcll(m & ((1ull << lane) - 1));
+                if (pos < kNearSortCap) {
+                  srt[wv][pos] = (int32_t)j;
+                } else {
+                  if (pairs) {
+                    pairs[2 * (w0 + pos)] = i;
+                    pairs[2 * (w0 + pos) + 1] = j;
+                  }
+                  if (cols32) cols32[w0 + pos] = (int32_t)(j + col_off);
+                }
+              }
+              cnt += __popcll(m);
             }
           }
         }
       }
     }
+    if (counts && lane == 0) counts[i] = cnt;
   }
-  if (counts) counts[i] = cnt;
-  // insertion sort of the row by j (rows hold tens of entries)
+  if (!fill) return;  // uniform over the block
+  __syncthreads();
+  if (!valid || cnt == 0) return;
+  if (cnt <= kNearSortCap) {
+    for (int e = lane; e < cnt; e += 64) {
+      const int32_t v = srt[wv][e];
+      int r = 0;
+      for (int k = 0; k < cnt; ++k) r += srt[wv][k] < v ? 1 : 0;
+      if (pairs) {
+        pairs[2 * (w0 + r)] = i;
+        pairs[2 * (w0 + r) + 1] = v;
+      }
+      if (cols32) cols32[w0 + r] = (int32_t)(v + col_off);
+    }
+    return;
+  }
+  // long row: the LDS part joins the rest in global memory, then one lane
+  // sorts the whole row
+  for (int e = lane; e < kNearSortCap; e += 64) {
+    const int32_t v = srt[wv][e];
+    if (pairs) {
+      pairs[2 * (w0 + e)] = i;
+      pairs[2 * (w0 + e) + 1] = v;
+    }
+    if (cols32) cols32[w0 + e] = (int32_t)(v + col_off);
+  }
+  __threadfence_block();
+  if (lane != 0) return;
   if (pairs) {
     int64_t* row = pairs + 2 * w0;
     for (int64_t a = 1; a < cnt; ++a) {
@@ -497,8 +550,8 @@ int NearSearch::count(hipStream_t st, std::vector<int64_t>& ptr, double df_use) 
   int64_t* counts = nullptr;
   if (hipMalloc((void**)&counts, sizeof(int64_t) * n) != hipSuccess)
     return fail(ED_ERR_ALLOC, "near pairs: alloc");
-  const unsigned grid = (unsigned)((n + kNearBlock - 1) / kNearBlock);
-  hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, dfu, g,
+  const unsigned grid = (unsigned)((n + kNearWaves - 1) / kNearWaves);
+  hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(64 * kNearWaves), 0, st, C, A, n, dfu, g,
                      (const int32_t*)d_start, (const int32_t*)d_pts, counts,
                      (const int64_t*)nullptr, (int64_t*)nullptr, (int32_t*)nullptr, (int64_t)0);
   hipError_t e = hipGetLastError();
@@ -519,8 +572,8 @@ int NearSearch::fill(hipStream_t st, const std::vector<int64_t>& ptr, int64_t* p
     return fail(ED_ERR_ALLOC, "near pairs: alloc");
   hipError_t e = hipMemcpyAsync(offs, ptr.data(), sizeof(int64_t) * n, hipMemcpyHostToDevice, st);
   if (e == hipSuccess) {
-    const unsigned grid = (unsigned)((n + kNearBlock - 1) / kNearBlock);
-    hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(kNearBlock), 0, st, C, A, n, dfu, g,
+    const unsigned grid = (unsigned)((n + kNearWaves - 1) / kNearWaves);
+    hipLaunchKernelGGL(k_near_pairs, dim3(grid), dim3(64 * kNearWaves), 0, st, C, A, n, dfu, g,
                        (const int32_t*)d_start, (const int32_t*)d_pts, (int64_t*)nullptr,
                        (const int64_t*)offs, pairs, cols32, col_off);
     e = hipGetLastError();
