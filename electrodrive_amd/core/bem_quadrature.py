@@ -62,14 +62,19 @@ def build_near_pairs(centroids, areas, distance_factor: float = 1.5):
     cnt = ctypes.c_int64(0)
     st = nat.stream_ptr(dev)
     L = nat.lib()
-    nat.check(L.ed_near_pairs_f64(nat.ptr(C), nat.ptr(A), n, float(distance_factor), None, 0,
-                                  ctypes.byref(cnt), st), "ed_near_pairs_f64")
-    P = int(cnt.value)
-    pairs = torch.empty((max(P, 0), 2), dtype=torch.int64, device=dev)
-    if P > 0:
-        nat.check(L.ed_near_pairs_f64(nat.ptr(C), nat.ptr(A), n, float(distance_factor),
-                                      nat.ptr(pairs), P, ctypes.byref(cnt), st),
-                  "ed_near_pairs_f64")
+    # one search fills a buffer sized for ~16 partners a row (a 1.5 factor on
+    # a surface mesh gives ~10); a fuller result reports its size and the
+    # search runs again into an exact buffer
+    cap = 16 * max(n, 0) + 1024
+    buf = torch.empty((cap, 2), dtype=torch.int64, device=dev)
+    nat.check(L.ed_near_pairs_f64(nat.ptr(C), nat.ptr(A), n, float(distance_factor),
+                                  nat.ptr(buf), cap, ctypes.byref(cnt), st), "ed_near_pairs_f64")
+    P = max(int(cnt.value), 0)
+    if P <= cap:
+        return buf[:P].clone()
+    pairs = torch.empty((P, 2), dtype=torch.int64, device=dev)
+    nat.check(L.ed_near_pairs_f64(nat.ptr(C), nat.ptr(A), n, float(distance_factor),
+                                  nat.ptr(pairs), P, ctypes.byref(cnt), st), "ed_near_pairs_f64")
     return pairs
 
 

@@ -77,12 +77,16 @@ class BemOperator:
         self.n = int(self.geom_offsets[-1])
         cents = np.concatenate([m.centroids for m in meshes])
         areas = np.concatenate([m.areas for m in meshes])
-        pverts = np.concatenate([m.panel_vertices for m in meshes])
+        voff = np.cumsum([0] + [m.vertices.shape[0] for m in meshes])
+        verts = np.concatenate([m.vertices for m in meshes])
+        tris = np.concatenate([m.triangles + voff[k] for k, m in enumerate(meshes)])
         dev = self.device
         with torch.cuda.device(dev):
             self.C = torch.as_tensor(cents, dtype=torch.float64).to(dev).contiguous()
             self.A = torch.as_tensor(areas, dtype=torch.float64).to(dev).contiguous()
-            self.PV = torch.as_tensor(pverts, dtype=torch.float64).to(dev).contiguous()
+            # panel vertex coordinates (n, 3, 3), gathered on the device
+            vd = torch.as_tensor(verts, dtype=torch.float64).to(dev)
+            self.PV = vd[torch.as_tensor(tris, dtype=torch.int64).to(dev)].contiguous()
             if refined_self:
                 self.I_self, self.self_corr = bq.self_integrals(self.C, self.PV, self.A,
                                                                 depth=self_depth, order=near_order)

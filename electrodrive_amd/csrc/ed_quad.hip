@@ -614,8 +614,8 @@ int ed_near_pairs_f64(const double* C, const double* A, int64_t n, double df, in
   ED_TRY(ns.count(st, ptr));
   const int64_t total = ptr[n];
   *n_pairs_out = total;
-  if (!pairs || total == 0) return ED_OK;
-  if (capacity < total) return fail(ED_ERR_ARG, "near pairs: capacity too small");
+  // too small a buffer only reports the count (the caller retries)
+  if (!pairs || total == 0 || capacity < total) return ED_OK;
   return ns.fill(st, ptr, pairs, nullptr, 0);
 }
 

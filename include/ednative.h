@@ -166,8 +166,10 @@ int ed_self_integrals_f64(const double* panel_vertices, const double* centroids,
 
 /* Near pairs (core/bem.py:475-555): (i, j), i != j,
  * 0 < |c_i - c_j| <= factor (R_i + R_j), R = sqrt(A/pi), i-major, j ascending.
- * Two calls: counts (pairs_out NULL) then fill.  Device inputs, device
- * outputs; *n_pairs_out is a host int64. */
+ * *n_pairs_out (a host int64) receives the pair count; the pairs are written
+ * to pairs_out when it is non-NULL and capacity >= that count, else nothing
+ * is written (call again with a large enough buffer).  Device inputs, device
+ * outputs. */
 int ed_near_pairs_f64(const double* centroids, const double* areas, int64_t n,
                       double factor, int64_t* pairs_out, int64_t capacity,
                       int64_t* n_pairs_out, ed_stream stream);
