@@ -76,11 +76,16 @@ void plan_free(ed_plan* P) {
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair,
                   P->near_ptr, P->near_col, P->near_w, P->cent64_t, P->area64_t, P->near_w64,
                   P->m2l_mmax, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
-                  P->coarse_part, P->coarse_tmp, P->dense_items, P->dense_rns, P->dense_part};
+                  P->coarse_part, P->coarse_tmp};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   // t_m2l*, t_m2m, t_l2l, m2l_ascale / m2l_binv point into the shared
   // per-device operator tables
+  if (P->dense_sym) {
+    dense_sym_free(*P->dense_sym);
+    delete P->dense_sym;
+    P->dense_sym = nullptr;
+  }
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
   sai_free(P);

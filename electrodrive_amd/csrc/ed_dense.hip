@@ -1,14 +1,16 @@
 // Dense single-layer collocation matvec (core/bem_kernel.py:157-230, :715-810):
 //   V_i = sum_{j != i} K_E sigma_j A_j rsqrt(max(r_ij^2, 1e-24)) + K_ii sigma_i A_i
-// with K_ii = self_integrals[i] (per-area) or 0.  Two target rows per thread,
-// source panels staged through LDS in tiles of 256; up to 4 right-hand sides
-// per pass.  The FP64 kernel refines a hardware FP32 reciprocal square root
-// with one FP64 Newton step (relative error ~1e-14).
+// with K_ii = self_integrals[i] (per-area) or 0.  Split form: two target rows
+// per thread, source panels staged through LDS in tiles of 256, up to 4
+// right-hand sides per pass.  Symmetric form (FP64, one right-hand side): each
+// pair evaluated once for both rows.  The FP64 kernels refine the hardware
+// FP64 reciprocal square root with one Newton step.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <string>
 #include <vector>
+#include <type_traits>
 
 #include "ed_internal.h"
 #include "ed_plan.h"
@@ -22,11 +24,13 @@ constexpr int kDenseMaxSplit = 16;
 template <typename T>
 __device__ __forceinline__ T inv_sqrt(T r2);
 
-// FP32 hardware reciprocal square root (relative error ~1e-7) refined by one
-// FP64 Newton step: relative error ~1e-14, below what a 1e-8 residual needs.
+// Hardware FP64 reciprocal square root (v_rsq_f64, an approximation) refined
+// by one Newton step: measured max relative error of the dense matvec 1e-15
+// against the FP64 oracle, far below what a 1e-8 residual needs.  (One
+// instruction where an FP32 rsq needs two conversions around it.)
 template <>
 __device__ __forceinline__ double inv_sqrt<double>(double r2) {
-  double y = (double)__builtin_amdgcn_rsqf((float)r2);
+  const double y = __builtin_amdgcn_rsq(r2);
   return y * (1.5 - 0.5 * r2 * y * y);
 }
 
@@ -129,6 +133,212 @@ __global__ __launch_bounds__(kDenseBlock) void k_dense_part(const DenseItem* __r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric form (FP64, one right-hand side).  The kernel 1/r is symmetric,
+// so a tile of 512 rows a x 512 sources b of one geometry with b > a adds
+// k_ij q_j to row i and k_ij q_i to row j: each pair is evaluated once, not
+// twice.  Thread t holds rows t and t + 256 of the tile and at step s meets
+// source (t + s) mod 512, so the 64 lanes of a wave touch 64 different
+// sources per step; each wave adds its column contributions into its own LDS
+// array (a plain read-modify-write, one lane per address per step), and the
+// four arrays are added in a fixed order at the end.  Diagonal tiles (a = b)
+// take row contributions only, skipping i = j.  Row and column partials go
+// to one slot per tile; k_dense_sym_finish adds a row's slots in a fixed
+// order (column slots of tiles c < a, then row slots of tiles b >= a).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sym_k(double xi, double yi, double zi, double x, double y,
+                                        double z) {
+  const double dx = xi - x, dy = yi - y, dz = zi - z;
+  double r2 = dx * dx + dy * dy + dz * dz;
+  r2 = r2 > 1e-24 ? r2 : 1e-24;
+  return inv_sqrt<double>(r2);
+}
+
+__global__ __launch_bounds__(kDenseBlock) void k_dense_sym(const DenseSymItem* __restrict__ items,
+                                                           const double* __restrict__ C,
+                                                           const double* __restrict__ A,
+                                                           const double* __restrict__ sigma,
+                                                           double* __restrict__ rowpart,
+                                                           double* __restrict__ colpart) {
+  static_assert(kDenseSymTile == 2 * kDenseBlock, "two rows per thread");
+  constexpr int NW = kDenseBlock / 64, TM = kDenseSymTile - 1;
+  // sources as separate arrays: consecutive lanes read consecutive doubles
+  __shared__ double sx[kDenseSymTile], sy[kDenseSymTile], sz[kDenseSymTile], sq[kDenseSymTile];
+  __shared__ double cacc[NW][kDenseSymTile];
+  const DenseSymItem it = items[blockIdx.x];
+  const int t = threadIdx.x, w = t >> 6;
+  for (int k = t; k < kDenseSymTile; k += kDenseBlock) {
+    if (k < it.ns) {
+      const int64_t j = it.s0 + k;
+      sx[k] = C[3 * j]; sy[k] = C[3 * j + 1]; sz[k] = C[3 * j + 2];
+      sq[k] = sigma[j] * A[j];
+    } else {  // padding: far away, no charge
+      sx[k] = sy[k] = sz[k] = 1e30;
+      sq[k] = 0.0;
+    }
+#pragma unroll
+    for (int ww = 0; ww < NW; ++ww) cacc[ww][k] = 0.0;
+  }
+  double xi[2], yi[2], zi[2], qi[2], acc[2] = {0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int r = t + u * kDenseBlock;
+    if (r < it.nr) {
+      const int64_t i = it.r0 + r;
+      xi[u] = C[3 * i]; yi[u] = C[3 * i + 1]; zi[u] = C[3 * i + 2];
+      qi[u] = sigma[i] * A[i];
+    } else {  // padding row: far from every source and from the padding sources
+      xi[u] = yi[u] = zi[u] = -1e30;
+      qi[u] = 0.0;
+    }
+  }
+  __syncthreads();
+  if (it.diag) {
+    for (int s = 0; s < kDenseSymTile; ++s) {
+      const int k = (t + s) & TM;
+      const double x = sx[k], y = sy[k], z = sz[k], q = sq[k];
+#pragma unroll
+      for (int u = 0; u < 2; ++u) {
+        double kv = sym_k(xi[u], yi[u], zi[u], x, y, z);
+        if (k == t + u * kDenseBlock) kv = 0.0;
+        acc[u] += kv * q;
+      }
+    }
+  } else {
+    double* cw = cacc[w];
+    int k = t;
+    double x = sx[k], y = sy[k], z = sz[k], q = sq[k];
+    for (int s = 0; s < kDenseSymTile; ++s) {
+      // the next step's source (read-only) is fetched a step ahead
+      const int kn = (k + 1) & TM;
+      const double xn = sx[kn], yn = sy[kn], zn = sz[kn], qn = sq[kn];
+      const double c = cw[k];
+      const double k0 = sym_k(xi[0], yi[0], zi[0], x, y, z);
+      const double k1 = sym_k(xi[1], yi[1], zi[1], x, y, z);
+      acc[0] += k0 * q;
+      acc[1] += k1 * q;
+      cw[k] = c + (k0 * qi[0] + k1 * qi[1]);
+      // the next step reads the column address another lane of this wave
+      // wrote in this one: keep the accesses to cw in program order (the
+      // wave's LDS operations execute in order)
+      asm volatile("" ::: "memory");
+      k = kn;
+      x = xn; y = yn; z = zn; q = qn;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < 2; ++u)
+    rowpart[(size_t)blockIdx.x * kDenseSymTile + t + u * kDenseBlock] = acc[u];
+  if (!it.diag)
+    for (int k = t; k < kDenseSymTile; k += kDenseBlock) {
+      double v = cacc[0][k];
+#pragma unroll
+      for (int ww = 1; ww < NW; ++ww) v += cacc[ww][k];
+      colpart[(size_t)blockIdx.x * kDenseSymTile + k] = v;
+    }
+}
+
+__global__ __launch_bounds__(kDenseBlock) void k_dense_sym_finish(
+    const int64_t* __restrict__ trow0, const int32_t* __restrict__ tnr,
+    const int32_t* __restrict__ tptr, const int32_t* __restrict__ tent,
+    const double* __restrict__ rowpart, const double* __restrict__ colpart,
+    const double* __restrict__ sigma, const double* __restrict__ selfi,
+    const double* __restrict__ A, double ke, double* __restrict__ out) {
+  const int rt = blockIdx.x;
+  const int e0 = tptr[rt], e1 = tptr[rt + 1];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const int r = threadIdx.x + u * kDenseBlock;
+    if (r >= tnr[rt]) continue;
+    const int64_t i = trow0[rt] + r;
+    double acc = 0.0;
+    for (int e = e0; e < e1; ++e) {
+      const int32_t ent = tent[e];
+      const double* part = (ent & 1) ? colpart : rowpart;
+      acc += part[(size_t)(ent >> 1) * kDenseSymTile + r];
+    }
+    const double diag = selfi ? selfi[i] * A[i] : 0.0;
+    double v = ke * acc + diag * sigma[i];
+    if (!isfinite(v)) v = 0.0;  // core/bem_kernel.py:375-381
+    out[i] = v;
+  }
+}
+
+// Tiles of the symmetric form for geometry segments [off[g], off[g+1]).
+void dense_sym_schedule(const std::vector<int64_t>& off, DenseSymSchedule& S) {
+  S = DenseSymSchedule();
+  S.tptr.push_back(0);
+  for (size_t g = 0; g + 1 < off.size(); ++g) {
+    const int64_t b = off[g], e = off[g + 1], ng = e - b;
+    if (ng <= 0) continue;
+    const int64_t T = (ng + kDenseSymTile - 1) / kDenseSymTile;
+    const int64_t base = (int64_t)S.items.size();
+    auto nrows = [&](int64_t a) { return (int32_t)std::min<int64_t>(kDenseSymTile, ng - a * kDenseSymTile); };
+    auto idx = [&](int64_t a, int64_t c) { return base + a * T - a * (a - 1) / 2 + (c - a); };
+    for (int64_t a = 0; a < T; ++a)
+      for (int64_t c = a; c < T; ++c) {
+        DenseSymItem it;
+        it.r0 = b + a * kDenseSymTile;
+        it.s0 = b + c * kDenseSymTile;
+        it.nr = nrows(a);
+        it.ns = nrows(c);
+        it.diag = a == c ? 1 : 0;
+        it.pad = 0;
+        S.items.push_back(it);
+      }
+    for (int64_t a = 0; a < T; ++a) {
+      S.trow0.push_back(b + a * kDenseSymTile);
+      S.tnr.push_back(nrows(a));
+      for (int64_t c = 0; c < a; ++c) S.tent.push_back((int32_t)(idx(c, a) * 2 + 1));
+      for (int64_t c = a; c < T; ++c) S.tent.push_back((int32_t)(idx(a, c) * 2));
+      S.tptr.push_back((int32_t)S.tent.size());
+    }
+  }
+}
+
+int dense_sym_upload(const DenseSymSchedule& S, DenseSymDevice& D) {
+  D = DenseSymDevice();
+  D.n_items = (int64_t)S.items.size();
+  D.n_tiles = (int64_t)S.trow0.size();
+  if (D.n_items == 0) return ED_OK;
+  auto up = [](void** dst, const void* src, size_t bytes) {
+    if (hipMalloc(dst, bytes) != hipSuccess) return false;
+    return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+  };
+  if (!up((void**)&D.items, S.items.data(), sizeof(DenseSymItem) * S.items.size()) ||
+      !up((void**)&D.trow0, S.trow0.data(), sizeof(int64_t) * S.trow0.size()) ||
+      !up((void**)&D.tnr, S.tnr.data(), sizeof(int32_t) * S.tnr.size()) ||
+      !up((void**)&D.tptr, S.tptr.data(), sizeof(int32_t) * S.tptr.size()) ||
+      !up((void**)&D.tent, S.tent.data(), sizeof(int32_t) * S.tent.size()) ||
+      hipMalloc((void**)&D.part, sizeof(double) * 2 * kDenseSymTile * (size_t)D.n_items) != hipSuccess) {
+    dense_sym_free(D);
+    return fail(ED_ERR_ALLOC, "dense matvec: alloc");
+  }
+  return ED_OK;
+}
+
+void dense_sym_free(DenseSymDevice& D) {
+  for (void* q : {(void*)D.items, (void*)D.trow0, (void*)D.tnr, (void*)D.tptr, (void*)D.tent, (void*)D.part})
+    if (q) (void)hipFree(q);
+  D = DenseSymDevice();
+}
+
+int dense_sym_run(const DenseSymDevice& D, const double* C, const double* A, const double* sigma,
+                  const double* selfi, double* out, hipStream_t st) {
+  if (D.n_items == 0) return ED_OK;
+  double* rowpart = D.part;
+  double* colpart = D.part + (size_t)kDenseSymTile * D.n_items;
+  hipLaunchKernelGGL(k_dense_sym, dim3((unsigned)D.n_items), dim3(kDenseBlock), 0, st, D.items, C,
+                     A, sigma, rowpart, colpart);
+  hipLaunchKernelGGL(k_dense_sym_finish, dim3((unsigned)D.n_tiles), dim3(kDenseBlock), 0, st,
+                     D.trow0, D.tnr, D.tptr, D.tent, (const double*)rowpart,
+                     (const double*)colpart, sigma, selfi, A, (double)kKE, out);
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ED_ERR_LAUNCH, std::string("dense matvec: ") + hipGetErrorString(e));
+  return ED_OK;
+}
+
 template <typename T, int NR>
 __global__ void k_dense_finish(int64_t n, const int32_t* __restrict__ row_nsplit,
                                const T* __restrict__ partial, const T* __restrict__ sigma,
@@ -217,6 +427,20 @@ static int dense_launch(const T* C, const T* A, const T* sigma, const T* selfi, 
   if (n < 0 || nrhs < 1) return fail(ED_ERR_ARG, "dense matvec: bad sizes");
   if (n == 0) return ED_OK;
   if (!C || !A || !sigma || !out) return fail(ED_ERR_ARG, "dense matvec: NULL pointer");
+  if constexpr (std::is_same<T, double>::value) {
+    if (nrhs == 1) {  // symmetric form: every pair once
+      DenseSymSchedule S;
+      dense_sym_schedule({0, n}, S);
+      DenseSymDevice D;
+      int rc = dense_sym_upload(S, D);
+      if (rc == ED_OK) rc = dense_sym_run(D, C, A, sigma, selfi, out, st);
+      hipError_t e = hipSuccess;
+      if (rc == ED_OK && (e = hipStreamSynchronize(st)) != hipSuccess)
+        rc = fail(ED_ERR_DEVICE, std::string("dense matvec: ") + hipGetErrorString(e));
+      dense_sym_free(D);
+      return rc;
+    }
+  }
   std::vector<DenseItem> items;
   std::vector<int32_t> rns;
   int smax = 1;

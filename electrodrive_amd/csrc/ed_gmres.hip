@@ -747,23 +747,21 @@ struct Solver {
     return fmm_launch(P, in_tree, 1, 1, selfi_orig, out_tree, 1, 1, W->stream);
   }
   int accurate_op(const double* in_tree, double* out_tree) {
-    // block diagonal over the geometries of a batched plan: one split-form
-    // launch covers every geometry (schedule cached on the plan)
-    if (!P->dense_items) {
-      std::vector<DenseItem> items;
-      std::vector<int32_t> rns;
-      int smax = 1;
-      dense_schedule(P->host.geom_pt_off, items, rns, &smax);
-      ED_G(hipMalloc((void**)&P->dense_items, sizeof(DenseItem) * std::max<size_t>(items.size(), 1)));
-      ED_G(hipMalloc((void**)&P->dense_rns, sizeof(int32_t) * n));
-      ED_G(hipMalloc((void**)&P->dense_part, sizeof(double) * (size_t)smax * n));
-      ED_G(hipMemcpy(P->dense_items, items.data(), sizeof(DenseItem) * items.size(), hipMemcpyHostToDevice));
-      ED_G(hipMemcpy(P->dense_rns, rns.data(), sizeof(int32_t) * n, hipMemcpyHostToDevice));
-      P->dense_nitems = (int64_t)items.size();
+    // block diagonal over the geometries of a batched plan: one symmetric-
+    // form launch covers every geometry (schedule cached on the plan)
+    if (!P->dense_sym) {
+      DenseSymSchedule S;
+      dense_sym_schedule(P->host.geom_pt_off, S);
+      auto* D = new DenseSymDevice();
+      const int rc = dense_sym_upload(S, *D);
+      if (rc != ED_OK) {
+        delete D;
+        return rc;
+      }
+      P->dense_sym = D;
     }
-    ED_T(dense_run<double>(P->dense_items, P->dense_nitems, P->dense_rns, 0, P->cent64_t,
-                           P->area64_t, in_tree, W->selfi_t, out_tree, n, 1, P->dense_part,
-                           W->stream));
+    ED_T(dense_sym_run(*P->dense_sym, P->cent64_t, P->area64_t, in_tree, W->selfi_t, out_tree,
+                       W->stream));
     if (P->n_near > 0) {
       hipLaunchKernelGGL(k_near_csr, dim3(gb), dim3(256), 0, W->stream, n, P->near_ptr,
                          P->near_col, P->near_w64, in_tree, out_tree);

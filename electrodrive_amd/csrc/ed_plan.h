@@ -8,6 +8,7 @@
 
 namespace ed {
 struct DenseItem;
+struct DenseSymDevice;
 }
 
 struct ed_plan {
@@ -104,11 +105,8 @@ struct ed_plan {
   double* coarse_cy = nullptr;
   double* coarse_part = nullptr;
   double* coarse_tmp = nullptr;   // n, coarse solve staging
-  // split-form dense FP64 operator of the refinement (built on first use)
-  ed::DenseItem* dense_items = nullptr;
-  int64_t dense_nitems = 0;
-  int32_t* dense_rns = nullptr;   // n, source splits of each row
-  double* dense_part = nullptr;   // max_split x n partial sums
+  // symmetric dense FP64 operator of the refinement (built on first use)
+  ed::DenseSymDevice* dense_sym = nullptr;
   // sparse approximate inverse smoother (column-major ELL, 32 slots per row)
   int32_t* sai_col = nullptr;
   double* sai_val = nullptr;
@@ -145,6 +143,32 @@ struct DenseItem {
 };
 void dense_schedule(const std::vector<int64_t>& off, std::vector<DenseItem>& items,
                     std::vector<int32_t>& row_nsplit, int* max_split);
+// Symmetric FP64 dense operator (one right-hand side): tiles of 512 rows a x
+// 512 sources b >= a of one geometry, one partial slot per tile for the rows
+// and one for the columns, and per row tile the list of slots to add
+// (slot * 2 + 1 for a column slot).
+constexpr int kDenseSymTile = 512;
+struct DenseSymItem {
+  int64_t r0, s0;
+  int32_t nr, ns, diag, pad;
+};
+struct DenseSymSchedule {
+  std::vector<DenseSymItem> items;
+  std::vector<int64_t> trow0;
+  std::vector<int32_t> tnr, tptr, tent;
+};
+struct DenseSymDevice {
+  DenseSymItem* items = nullptr;
+  int64_t* trow0 = nullptr;
+  int32_t *tnr = nullptr, *tptr = nullptr, *tent = nullptr;
+  double* part = nullptr;  // 2 x n_items x kDenseSymTile: row slots, then column slots
+  int64_t n_items = 0, n_tiles = 0;
+};
+void dense_sym_schedule(const std::vector<int64_t>& off, DenseSymSchedule& S);
+int dense_sym_upload(const DenseSymSchedule& S, DenseSymDevice& D);
+void dense_sym_free(DenseSymDevice& D);
+int dense_sym_run(const DenseSymDevice& D, const double* C, const double* A, const double* sigma,
+                  const double* selfi, double* out, hipStream_t st);
 template <typename T>
 int dense_run(const DenseItem* items, int64_t n_items, const int32_t* row_nsplit, int max_split,
               const T* C, const T* A, const T* sigma, const T* selfi, T* out, int64_t n,
