@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "ed_internal.h"
 
@@ -257,54 +258,86 @@ int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
     return true;
   };
 
-  t.v_ptr.assign(M + 1, 0);
-  t.u_ptr.assign(M + 1, 0);
-  std::vector<int64_t> col, col2, stack, us;
-  std::vector<std::pair<int64_t, int32_t>> vs;
-  for (int64_t nidx = 0; nidx < M; ++nidx) {
-    vs.clear();
-    const int64_t P = t.parent[nidx];
-    if (P >= 0) {
-      colleagues(P, col);
-      for (int64_t c : col) {
-        if (t.is_leaf[c]) continue;
-        for (int64_t s = t.first_child[c]; s < t.first_child[c] + t.n_child[c]; ++s) {
-          if (adjacent(s, nidx)) continue;
-          const int ox = t.coord[3 * nidx] - t.coord[3 * s];
-          const int oy = t.coord[3 * nidx + 1] - t.coord[3 * s + 1];
-          const int oz = t.coord[3 * nidx + 2] - t.coord[3 * s + 2];
-          vs.emplace_back(s, offset_slot(ws, ox, oy, oz));
+  // V and U lists: nodes are independent, so contiguous node ranges go to
+  // worker threads (read-only access to the tree and the hash); each range
+  // fills its own lists and the ranges are joined in node order, so the
+  // result does not depend on the thread count.
+  struct Lists {
+    std::vector<int64_t> v_src, u_src, v_cnt, u_cnt;
+    std::vector<int32_t> v_off;
+  };
+  auto build_lists = [&](int64_t n0, int64_t n1, Lists& out) {
+    std::vector<int64_t> col, col2, stack, us;
+    std::vector<std::pair<int64_t, int32_t>> vs;
+    out.v_cnt.assign((size_t)(n1 - n0), 0);
+    out.u_cnt.assign((size_t)(n1 - n0), 0);
+    for (int64_t nidx = n0; nidx < n1; ++nidx) {
+      vs.clear();
+      const int64_t P = t.parent[nidx];
+      if (P >= 0) {
+        colleagues(P, col);
+        for (int64_t c : col) {
+          if (t.is_leaf[c]) continue;
+          for (int64_t s = t.first_child[c]; s < t.first_child[c] + t.n_child[c]; ++s) {
+            if (adjacent(s, nidx)) continue;
+            const int ox = t.coord[3 * nidx] - t.coord[3 * s];
+            const int oy = t.coord[3 * nidx + 1] - t.coord[3 * s + 1];
+            const int oz = t.coord[3 * nidx + 2] - t.coord[3 * s + 2];
+            vs.emplace_back(s, offset_slot(ws, ox, oy, oz));
+          }
         }
+        std::sort(vs.begin(), vs.end());
       }
-      std::sort(vs.begin(), vs.end());
-    }
-    for (auto& e : vs) { t.v_src.push_back(e.first); t.v_off.push_back(e.second); }
-    t.v_ptr[nidx + 1] = (int64_t)t.v_src.size();
+      for (auto& e : vs) { out.v_src.push_back(e.first); out.v_off.push_back(e.second); }
+      out.v_cnt[(size_t)(nidx - n0)] = (int64_t)vs.size();
 
-    us.clear();
-    if (t.is_leaf[nidx]) {
-      for (int64_t a = nidx; a >= 0; a = t.parent[a]) {
-        colleagues(a, col2);
-        for (int64_t c : col2) {
-          if (t.is_leaf[c]) {
-            us.push_back(c);
-          } else if (a == nidx) {
-            stack.assign(1, c);
-            while (!stack.empty()) {
-              const int64_t d = stack.back();
-              stack.pop_back();
-              if (t.is_leaf[d]) { us.push_back(d); continue; }
-              for (int64_t s = t.first_child[d]; s < t.first_child[d] + t.n_child[d]; ++s)
-                stack.push_back(s);
+      us.clear();
+      if (t.is_leaf[nidx]) {
+        for (int64_t a = nidx; a >= 0; a = t.parent[a]) {
+          colleagues(a, col2);
+          for (int64_t c : col2) {
+            if (t.is_leaf[c]) {
+              us.push_back(c);
+            } else if (a == nidx) {
+              stack.assign(1, c);
+              while (!stack.empty()) {
+                const int64_t d = stack.back();
+                stack.pop_back();
+                if (t.is_leaf[d]) { us.push_back(d); continue; }
+                for (int64_t s = t.first_child[d]; s < t.first_child[d] + t.n_child[d]; ++s)
+                  stack.push_back(s);
+              }
             }
           }
         }
+        std::sort(us.begin(), us.end());
+        us.erase(std::unique(us.begin(), us.end()), us.end());
       }
-      std::sort(us.begin(), us.end());
-      us.erase(std::unique(us.begin(), us.end()), us.end());
+      out.u_src.insert(out.u_src.end(), us.begin(), us.end());
+      out.u_cnt[(size_t)(nidx - n0)] = (int64_t)us.size();
     }
-    t.u_src.insert(t.u_src.end(), us.begin(), us.end());
-    t.u_ptr[nidx + 1] = (int64_t)t.u_src.size();
+  };
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int nthr = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(hw, 8), M / 128));
+  std::vector<Lists> parts((size_t)nthr);
+  {
+    std::vector<std::thread> pool;
+    for (int k = 1; k < nthr; ++k)
+      pool.emplace_back(build_lists, M * k / nthr, M * (k + 1) / nthr, std::ref(parts[(size_t)k]));
+    build_lists(0, M / nthr, parts[0]);
+    for (auto& th : pool) th.join();
+  }
+  t.v_ptr.assign(M + 1, 0);
+  t.u_ptr.assign(M + 1, 0);
+  int64_t nidx = 0;
+  for (auto& L : parts) {
+    t.v_src.insert(t.v_src.end(), L.v_src.begin(), L.v_src.end());
+    t.v_off.insert(t.v_off.end(), L.v_off.begin(), L.v_off.end());
+    t.u_src.insert(t.u_src.end(), L.u_src.begin(), L.u_src.end());
+    for (size_t q = 0; q < L.v_cnt.size(); ++q, ++nidx) {
+      t.v_ptr[nidx + 1] = t.v_ptr[nidx] + L.v_cnt[q];
+      t.u_ptr[nidx + 1] = t.u_ptr[nidx] + L.u_cnt[q];
+    }
   }
   return ED_OK;
 }
