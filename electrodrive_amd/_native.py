@@ -5,6 +5,7 @@ fallback for any compute entry point.
 """
 import ctypes
 import os
+import threading
 
 import torch
 
@@ -102,12 +103,17 @@ _SIGS = {
 }
 
 _lib = None
+_lib_lock = threading.Lock()
 
 
 def lib():
-    """Load the native library (once).  Raises if it is not built."""
+    """Load the native library (once, thread-safe).  Raises if it is not built."""
     global _lib
-    if _lib is None:
+    if _lib is not None:
+        return _lib
+    with _lib_lock:
+        if _lib is not None:
+            return _lib
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 f"native library {LIB_PATH} is missing; build it with "

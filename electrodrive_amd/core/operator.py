@@ -14,6 +14,7 @@ with the FP64 dense kernel plus the FP64 near correction (the residual operator
 of the outer iterative refinement).
 """
 import math
+import threading
 
 import numpy as np
 import torch
@@ -81,6 +82,23 @@ class BemOperator:
         verts = np.concatenate([m.vertices for m in meshes])
         tris = np.concatenate([m.triangles + voff[k] for k, m in enumerate(meshes)])
         dev = self.device
+        # The plan (host tree, interaction lists, schedules) needs only the
+        # host geometry: it is built on a worker thread (the C call releases
+        # the GIL) while this thread runs the device quadrature below.
+        plan_box = {}
+
+        def make_plan():
+            try:
+                with torch.cuda.device(dev):
+                    plan_box["plan"] = FmmPlan(
+                        cents, areas, leaf_size=leaf_size, order=order, ws=ws,
+                        max_level=max_level, m2l_precision=m2l_precision,
+                        geom_offsets=None if len(meshes) == 1 else self.geom_offsets)
+            except BaseException as exc:  # re-raised on the calling thread
+                plan_box["error"] = exc
+
+        worker = threading.Thread(target=make_plan, name="fmm-plan", daemon=True)
+        worker.start()
         with torch.cuda.device(dev):
             self.C = torch.as_tensor(cents, dtype=torch.float64).to(dev).contiguous()
             self.A = torch.as_tensor(areas, dtype=torch.float64).to(dev).contiguous()
@@ -107,9 +125,10 @@ class BemOperator:
                                    else torch.zeros((0, 2), dtype=torch.int64, device=dev))
                 self.near_w = bq.near_correction_weights(self.C, self.A, self.PV, self.near_pairs,
                                                          quad_order=near_order, max_depth=near_depth)
-            self.plan = FmmPlan(cents, areas, leaf_size=leaf_size, order=order, ws=ws,
-                                max_level=max_level, m2l_precision=m2l_precision,
-                                geom_offsets=None if len(meshes) == 1 else self.geom_offsets)
+            worker.join()
+            if "error" in plan_box:
+                raise plan_box["error"]
+            self.plan = plan_box["plan"]
             if self.near_pairs is not None and self.near_pairs.shape[0] > 0:
                 self.plan.attach_near_device(self.near_pairs, self.near_w)
         self.fmm = FmmOperator(self.plan, self.self_corr)
