@@ -81,6 +81,7 @@ void plan_free(ed_plan* P) {
     if (p) (void)hipFree(p);
   // t_m2l*, t_m2m, t_l2l, m2l_ascale / m2l_binv point into the shared
   // per-device operator tables
+  multi_free(P);
   if (P->dense_sym) {
     dense_sym_free(*P->dense_sym);
     delete P->dense_sym;
@@ -510,6 +511,46 @@ int plan_upload(ed_plan* P) {
 
 }  // namespace ed
 
+namespace ed {
+void multi_free(ed_plan* P) {
+  auto& m = P->multi;
+  for (void* q : {(void*)m.qT, (void*)m.sig, (void*)m.M, (void*)m.L, (void*)m.ym2m, (void*)m.ypair,
+                  (void*)m.part, (void*)m.mmax})
+    if (q) (void)hipFree(q);
+  m = ed_plan::Multi();
+}
+
+int multi_reserve(ed_plan* P, int nrhs) {
+  if (P->multi.cap >= nrhs) return ED_OK;
+  multi_free(P);
+  auto& m = P->multi;
+  const size_t R = (size_t)nrhs, n = (size_t)P->n, mn = (size_t)P->n_nodes * P->nc;
+  const size_t pairs = (size_t)std::max<int64_t>(P->n_v, 1) * (size_t)std::max(P->ncp, 1);
+  const size_t part = 64 * (size_t)std::max<int64_t>(P->n_p2p_items, 1);
+  const size_t slots = 64 * 32;  // kMaxSlots x kMaxSlotStride
+  bool ok = hipMalloc((void**)&m.qT, sizeof(float) * n * R) == hipSuccess &&
+            hipMalloc((void**)&m.sig, sizeof(float) * n * R) == hipSuccess &&
+            hipMalloc((void**)&m.M, sizeof(float) * mn * R) == hipSuccess &&
+            hipMalloc((void**)&m.L, sizeof(float) * mn * R) == hipSuccess &&
+            hipMalloc((void**)&m.ym2m, sizeof(float) * mn * R) == hipSuccess &&
+            hipMalloc((void**)&m.ypair, sizeof(float) * pairs * R) == hipSuccess &&
+            hipMalloc((void**)&m.part, sizeof(float) * part * R) == hipSuccess &&
+            hipMalloc((void**)&m.mmax, sizeof(unsigned) * slots * R) == hipSuccess;
+  // nodes that no pass writes (levels 0 and 1) read as zero, as in the
+  // one-RHS buffers
+  ok = ok && hipMemset(m.M, 0, sizeof(float) * mn * R) == hipSuccess &&
+       hipMemset(m.L, 0, sizeof(float) * mn * R) == hipSuccess &&
+       hipMemset(m.mmax, 0, sizeof(unsigned) * slots * R) == hipSuccess;
+  if (!ok) {
+    multi_free(P);
+    return fail(ED_ERR_ALLOC, "multi-RHS matvec: alloc");
+  }
+  m.cap = nrhs;
+  m.pair_stride = (int64_t)pairs;
+  return ED_OK;
+}
+}  // namespace ed
+
 using namespace ed;
 
 extern "C" {
@@ -674,6 +715,15 @@ int ed_fmm_matvec_f64(ed_plan* P, const double* sigma, const double* selfi, doub
                       ed_stream stream) {
   if (!P || !sigma || !out) return fail(ED_ERR_ARG, "ed_fmm_matvec_f64: NULL argument");
   return fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, (hipStream_t)stream);
+}
+
+int ed_fmm_matvec_multi_f64(ed_plan* P, const double* sigma, const double* selfi, double* out,
+                            int32_t nrhs, ed_stream stream) {
+  if (!P || !sigma || !out) return fail(ED_ERR_ARG, "ed_fmm_matvec_multi_f64: NULL argument");
+  if (nrhs < 1 || nrhs > 8) return fail(ED_ERR_ARG, "ed_fmm_matvec_multi_f64: nrhs must be 1..8");
+  if (nrhs == 1) return fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, (hipStream_t)stream);
+  ED_TRY(multi_reserve(P, nrhs));
+  return fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, (hipStream_t)stream, nrhs);
 }
 
 int ed_plan_attach_near(ed_plan* P, const int64_t* pairs, const double* w, int64_t np) {

@@ -34,6 +34,20 @@ constexpr int kMaxSlots = 64, kMaxSlotStride = 32;  // 32 x 4 B = 128 B
 // issue cycles of every SIMD the two share, stretching the dependent chain.
 __device__ __forceinline__ void chain_prio() { __builtin_amdgcn_s_setprio(3); }
 
+// Several right-hand sides in one matvec: blockIdx.y is the RHS of every
+// far-field launch and each per-RHS array is offset by these strides (all
+// zero, R = 1 and qT = NULL for one right-hand side, whose charges sit in
+// pts[i].w).
+struct RhsStride {
+  int64_t node = 0;  // M, L, ym2m: n_nodes * nc
+  int64_t pair = 0;  // ypair: n_pairs * ncp
+  int64_t slot = 0;  // multipole-maximum slots
+  int64_t part = 0;  // P2P partial rows
+  int64_t n = 0;     // tree-order sigma and outputs
+  const float* qT = nullptr;  // n x R charges, point-major
+  int R = 1;
+};
+
 // ---------------------------------------------------------------------------
 // Solid-harmonic streaming helpers (m-major recurrences; see ed_operators.cpp)
 // ---------------------------------------------------------------------------
@@ -155,10 +169,13 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
                                                     const float4* __restrict__ pts,
                                                     float* __restrict__ M,
                                                     const float* __restrict__ ascale,
-                                                    unsigned* __restrict__ mmax_bits) {
+                                                    unsigned* __restrict__ mmax_bits, RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float red[kLeafBlock][NC + 1];
+  const int ry = blockIdx.y;
+  M += ry * rs.node;
+  if (mmax_bits) mmax_bits += ry * rs.slot;
   const int node = leaves[blockIdx.x];
   const int2 r = rng[node];
   const float4 c = ctr[node];
@@ -167,7 +184,8 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
   for (int k = 0; k < NC; ++k) acc[k] = 0.0f;
   for (int i = r.x + threadIdx.x; i < r.y; i += kLeafBlock) {
     const float4 pq = pts[i];
-    p2m_accum<P>((pq.x - c.x) * c.w, (pq.y - c.y) * c.w, (pq.z - c.z) * c.w, pq.w, acc);
+    const float q = rs.qT ? rs.qT[(size_t)i * rs.R + ry] : pq.w;
+    p2m_accum<P>((pq.x - c.x) * c.w, (pq.y - c.y) * c.w, (pq.z - c.z) * c.w, q, acc);
   }
 #pragma unroll
   for (int k = 0; k < NC; ++k) red[threadIdx.x][k] = acc[k];
@@ -193,10 +211,12 @@ template <int P>
 __global__ __launch_bounds__(128) void k_m2m_child(int child0, const int32_t* __restrict__ octant,
                                                    const float* __restrict__ T,
                                                    const float* __restrict__ M,
-                                                   float* __restrict__ Y) {
+                                                   float* __restrict__ Y, RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float mc[NC];
+  M += blockIdx.y * rs.node;
+  Y += blockIdx.y * rs.node;
   const int c = child0 + blockIdx.x;
   for (int j = threadIdx.x; j < NC; j += blockDim.x) mc[j] = M[(size_t)c * NC + j];
   __syncthreads();
@@ -216,9 +236,12 @@ __global__ __launch_bounds__(128) void k_m2m_sum(const int32_t* __restrict__ par
                                                  const float* __restrict__ Y,
                                                  float* __restrict__ M,
                                                  const float* __restrict__ ascale,
-                                                 unsigned* __restrict__ mmax_bits) {
+                                                 unsigned* __restrict__ mmax_bits, RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
+  Y += blockIdx.y * rs.node;
+  M += blockIdx.y * rs.node;
+  if (mmax_bits) mmax_bits += blockIdx.y * rs.slot;
   const int par = parents[blockIdx.x];
   const int c0 = first_child[par] - child0, nch = n_child[par];
   float mx = 0.0f;
@@ -242,10 +265,12 @@ __global__ __launch_bounds__(128) void k_m2l_valu(int node0, const int32_t* __re
                                                   const float4* __restrict__ ctr,
                                                   const float* __restrict__ T,
                                                   const float* __restrict__ M,
-                                                  float* __restrict__ L) {
+                                                  float* __restrict__ L, RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float ms[NC];
+  M += blockIdx.y * rs.node;
+  L += blockIdx.y * rs.node;
   const int t = node0 + blockIdx.x;
   const int b = v_ptr[t], e = v_ptr[t + 1];
   float acc0 = 0.0f, acc1 = 0.0f;
@@ -313,12 +338,14 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
                                                   const float* __restrict__ Tm,
                                                   const float* __restrict__ ascale,
                                                   const float* __restrict__ M,
-                                                  float* __restrict__ Y) {
+                                                  float* __restrict__ Y, RhsStride rs) {
   chain_prio();
   using S = M2LShape<P>;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
   constexpr int NT = S::THREADS, NW = NT / 64, KH = KP / 2;
   __shared__ float ta[NCP * KS];
+  M += blockIdx.y * rs.node;
+  Y += blockIdx.y * rs.pair;
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
   if (cnt == 0) return;  // padding tile of the XCD-aware order
@@ -419,10 +446,13 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
                                                     const float* __restrict__ ascale,
                                                     const float* __restrict__ M,
                                                     const unsigned* __restrict__ mmax_bits,
-                                                    float* __restrict__ Y) {
+                                                    float* __restrict__ Y, RhsStride rs) {
   chain_prio();
   using S = M2LShapeH<P>;
   using V = typename Mfma16<T16>::vec;
+  M += blockIdx.y * rs.node;
+  Y += blockIdx.y * rs.pair;
+  mmax_bits += blockIdx.y * rs.slot;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
   __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
@@ -496,69 +526,47 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
 // registers from the gathered FP32 multipoles.  Four waves x 32 columns x
 // all rows, the same tile schedule and Y layout as the 16-bit kernel.  Each
 // lane issues all loads of its multipole column (KP / 2 values) before the
-// table copy, so their latency hides behind it and the MFMA loop never
-// waits on global memory.
+// table copy, so their latency hides behind it.  With several right-hand
+// sides (rs.R > 1) one block runs the tile for every RHS in turn against the
+// table it loaded once.
+// The KP / 2 multipole values of one lane's column: k = 16 sv + 8 lk + j
+// (addresses clamped to the row).
 template <int P>
-__global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict__ tile_tab,
-                                                      const int32_t* __restrict__ tile_beg,
-                                                      const int32_t* __restrict__ tile_cnt,
-                                                      const int32_t* __restrict__ pair_src,
-                                                      const _Float16* __restrict__ Tm,
-                                                      const float* __restrict__ ascale,
-                                                      const float* __restrict__ M,
-                                                      const unsigned* __restrict__ mmax_bits,
-                                                      float* __restrict__ Y) {
-  chain_prio();
+__device__ __forceinline__ void m2l16x3_load(const float* __restrict__ mrow, int lk,
+                                             float (&mv)[M2LShapeH<P>::KP / 16][8]) {
   using S = M2LShapeH<P>;
-  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
-  __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
-  __shared__ float as_l[KP];  // ascale x the power-of-two input scale, 0 past NC
-  if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
-  const int tile = blockIdx.x;
-  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int col0 = wave * 32;
-  const int li = lane & 31, lk = lane >> 5;
-  const int col = col0 + li;
-  const bool cv = col < cnt;
-  // this lane's multipole values: k = 16 sv + 8 lk + j (clamped addresses)
-  float mv[NSV][8];
-  if (col0 < cnt) {
-    const float* mrow = M + (size_t)pair_src[beg + min(col, cnt - 1)] * NC;
 #pragma unroll
-    for (int sv = 0; sv < NSV; ++sv)
+  for (int sv = 0; sv < S::KP / 16; ++sv)
 #pragma unroll
-      for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, NC - 1)];
-  }
-  // power-of-two input scale from the largest |M * ascale| of this matvec
-  const float gmax = slots_absmax(mmax_bits);
+    for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, S::NC - 1)];
+}
+
+// One wave's 32 columns x all rows of the split-FP16 product for one RHS,
+// scaled by that RHS's power of two, stored to its Y rows.
+template <int P>
+__device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
+                                               const float* __restrict__ as_l,
+                                               const float (&mv)[M2LShapeH<P>::KP / 16][8],
+                                               const unsigned* __restrict__ slots, bool cv,
+                                               int li, int lk, float* __restrict__ yrow) {
+  using S = M2LShapeH<P>;
+  constexpr int NCP = S::NCP, KS = S::KS, NSV = S::KP / 16, NRB = NCP / 32;
+  // power-of-two input scale from the largest |M * ascale| of this RHS
+  const float gmax = slots_absmax(slots);
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
   const float inv_sc = 1.0f / sc;
-  // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
-  // multiple of 16) elements long, so a chunk never crosses a row
-  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
-  constexpr int N8 = NCP * KP / 8;
-#pragma unroll 4
-  for (int t = threadIdx.x; t < 2 * N8; t += 256) {
-    const int hh = t >= N8 ? 1 : 0;
-    const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
-    *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
-  }
-  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc : 0.0f;
-  __syncthreads();
-  if (col0 >= cnt) return;  // no barrier below
-  constexpr int NRB = NCP / 32;
   f32x16 acc[NRB];
 #pragma unroll
   for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
+    for (int q = 0; q < 16; ++q) acc[rb][q] = 0.0f;
 #pragma unroll
   for (int sv = 0; sv < NSV; ++sv) {
     f16x8 bh, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] : 0.0f;
+      // sc is a power of two: (m a) sc rounds like m (a sc)
+      const float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] * sc : 0.0f;
       const _Float16 h = (_Float16)v;
       bh[j] = h;
       bl[j] = (_Float16)(v - (float)h);
@@ -573,17 +581,67 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
       acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[rb], 0, 0, 0);
     }
   }
-  if (cv) {
-    float* yrow = Y + (size_t)(beg + col) * NCP;
+  if (!cv) return;
 #pragma unroll
-    for (int rb = 0; rb < NRB; ++rb)
+  for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = rb * 32 + q * 8 + lk * 4;
-        float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
-                               acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
-        *reinterpret_cast<float4*>(yrow + row) = v;
-      }
+    for (int q = 0; q < 4; ++q) {
+      const int row = rb * 32 + q * 8 + lk * 4;
+      float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
+                             acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
+      *reinterpret_cast<float4*>(yrow + row) = v;
+    }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict__ tile_tab,
+                                                      const int32_t* __restrict__ tile_beg,
+                                                      const int32_t* __restrict__ tile_cnt,
+                                                      const int32_t* __restrict__ pair_src,
+                                                      const _Float16* __restrict__ Tm,
+                                                      const float* __restrict__ ascale,
+                                                      const float* __restrict__ M,
+                                                      const unsigned* __restrict__ mmax_bits,
+                                                      float* __restrict__ Y, RhsStride rs) {
+  chain_prio();
+  using S = M2LShapeH<P>;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
+  __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
+  __shared__ float as_l[KP];  // ascale, 0 past NC
+  if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
+  const int tile = blockIdx.x;
+  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col0 = wave * 32;
+  const int li = lane & 31, lk = lane >> 5;
+  const int col = col0 + li;
+  const bool cv = col < cnt;
+  const int src = pair_src[beg + min(col, cnt - 1)];
+  // this lane's multipole values of RHS r: k = 16 sv + 8 lk + j (clamped)
+  float mv[NSV][8];
+  if (col0 < cnt) m2l16x3_load<P>(M + (size_t)src * NC, lk, mv);
+  // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
+  // multiple of 16) elements long, so a chunk never crosses a row
+  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
+  constexpr int N8 = NCP * KP / 8;
+#pragma unroll 4
+  for (int t = threadIdx.x; t < 2 * N8; t += 256) {
+    const int hh = t >= N8 ? 1 : 0;
+    const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
+    *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
+  }
+  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] : 0.0f;
+  __syncthreads();
+  if (col0 >= cnt) return;  // no barrier below
+  m2l16x3_column<P>(ta, as_l, mv, mmax_bits, cv, li, lk, Y + (size_t)(beg + col) * NCP);
+#pragma unroll 1
+  for (int r = 1; r < rs.R; ++r) {
+    // keeps the table reads inside the loop: hoisted out of it they would
+    // hold every A fragment of the tile in registers
+    asm volatile("" ::: "memory");
+    m2l16x3_load<P>(M + r * rs.node + (size_t)src * NC, lk, mv);
+    m2l16x3_column<P>(ta, as_l, mv, mmax_bits + r * rs.slot, cv, li, lk,
+                      Y + r * rs.pair + (size_t)(beg + col) * NCP);
   }
 }
 
@@ -594,9 +652,11 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
                                                     const float4* __restrict__ ctr,
                                                     const float* __restrict__ binv,
                                                     const float* __restrict__ Y,
-                                                    float* __restrict__ L) {
+                                                    float* __restrict__ L, RhsStride rs) {
   chain_prio();
   using S = M2LShape<P>;
+  Y += blockIdx.y * rs.pair;
+  L += blockIdx.y * rs.node;
   constexpr int NC = S::NC, NCP = S::NCP;
   const int t = node0 + blockIdx.x;
   const int b = tptr[t], e = tptr[t + 1];
@@ -616,10 +676,11 @@ template <int P>
 __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restrict__ parent,
                                              const int32_t* __restrict__ octant,
                                              const float* __restrict__ T,
-                                             float* __restrict__ L) {
+                                             float* __restrict__ L, RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float lp[NC];
+  L += blockIdx.y * rs.node;
   const int c = node0 + blockIdx.x;
   const int par = parent[c];
   for (int j = threadIdx.x; j < NC; j += blockDim.x) lp[j] = L[(size_t)par * NC + j];
@@ -754,6 +815,134 @@ __global__ __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(
   if (lane < ntc) part[it.out + lane] = acc;
 }
 
+// P2P for R right-hand sides: the same items and schedule; r^2 and its
+// rsqrt are computed once per interaction and weight the R charges of the
+// source (qT, point-major, read at wave-uniform addresses with the point).
+// Per RHS the sums run in the same order as the one-RHS kernel, so each
+// RHS's partials equal that kernel's bit for bit.  Partial rows of RHS r sit
+// at part + r * part_stride.
+template <bool SELF, int R>
+__device__ __forceinline__ void p2p_uniform_multi(const float4* __restrict__ pts,
+                                                  const float* __restrict__ qT, int j, int end,
+                                                  float4 xi, int me, float (&acc)[R]) {
+  constexpr int U = 4;
+  for (; j + U <= end; j += U) {
+    float4 s[U];
+    float q[U][R];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      s[u] = pts[j + u];
+#pragma unroll
+      for (int r = 0; r < R; ++r) q[u][r] = qT[(size_t)(j + u) * R + r];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      float r2 = p2p_r2(xi, s[u]);
+      if (SELF) r2 = (j + u == me) ? INFINITY : r2;
+      const float k = __builtin_amdgcn_rsqf(r2);
+#pragma unroll
+      for (int r = 0; r < R; ++r) acc[r] = fmaf(q[u][r], k, acc[r]);
+    }
+  }
+  for (; j < end; ++j) {
+    const float4 s = pts[j];
+    float r2 = p2p_r2(xi, s);
+    if (SELF) r2 = (j == me) ? INFINITY : r2;
+    const float k = __builtin_amdgcn_rsqf(r2);
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = fmaf(qT[(size_t)j * R + r], k, acc[r]);
+  }
+}
+
+template <bool SELF, int R>
+__device__ __forceinline__ void p2p_sliced_multi(const float4* __restrict__ pts,
+                                                 const float* __restrict__ qT, int j0, int end,
+                                                 int sl, int S, float4 xi, int me,
+                                                 float (&acc)[R]) {
+  const int nsteps = (end - j0 + S - 1) / S;
+  const int jb = j0 + sl;
+  for (int t = 0; t < nsteps; ++t) {
+    const int j = jb + t * S;
+    const int jc = min(j, end - 1);
+    const float4 s = pts[jc];
+    float r2 = p2p_r2(xi, s);
+    r2 = (j >= end || (SELF && j == me)) ? INFINITY : r2;
+    const float k = __builtin_amdgcn_rsqf(r2);
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = fmaf(qT[(size_t)jc * R + r], k, acc[r]);
+  }
+}
+
+template <int R>
+__global__ __launch_bounds__(64 * kP2PWaves) void k_p2p_multi(const ed::P2PItem* __restrict__ items,
+                                                              int n_items,
+                                                              const int2* __restrict__ seg,
+                                                              const float4* __restrict__ pts,
+                                                              const float* __restrict__ qT,
+                                                              float* __restrict__ part,
+                                                              int64_t part_stride) {
+  const int item = blockIdx.x * kP2PWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (item >= n_items) return;  // whole waves only
+  const ed::P2PItem it = items[item];
+  const int lane = threadIdx.x & 63;
+  const int ntc = it.ntc, S = it.slices;
+  const int tl = lane % ntc, sl = lane / ntc;
+  const bool act = sl < S;
+  const int me = it.tgt0 + tl;
+  const float4 xi = pts[me];
+  float acc[R];
+#pragma unroll
+  for (int r = 0; r < R; ++r) acc[r] = 0.0f;
+  int k = it.seg, off = it.off, left = it.cnt;
+  while (left > 0) {
+    const int2 rg = seg[k];
+    const int j0 = rg.x + off;
+    const int n = min(rg.y - j0, left);
+    const bool self = rg.x <= it.tgt0 && it.tgt0 < rg.y;
+    if (S == 1) {
+      if (self) p2p_uniform_multi<true, R>(pts, qT, j0, j0 + n, xi, me, acc);
+      else p2p_uniform_multi<false, R>(pts, qT, j0, j0 + n, xi, me, acc);
+    } else {
+      if (self) p2p_sliced_multi<true, R>(pts, qT, j0, j0 + n, act ? sl : 0, S, xi, me, acc);
+      else p2p_sliced_multi<false, R>(pts, qT, j0, j0 + n, act ? sl : 0, S, xi, me, acc);
+    }
+    left -= n;
+    ++k;
+    off = 0;
+  }
+#pragma unroll
+  for (int r = 0; r < R; ++r) {
+    float a = acc[r];
+    if (S > 1) {  // slices of target tl sit in lanes tl + q n_t, q < S
+      float v = 0.0f;
+      for (int q = 0; q < S; ++q) v += __shfl(a, tl + q * ntc, 64);
+      a = v;
+    }
+    if (lane < ntc) part[r * part_stride + it.out + lane] = a;
+  }
+}
+
+// gather for R right-hand sides (sigma R x n, original or tree order): sigma
+// of each RHS into sig_m (R x n), the charges point-major into qT (n x R), the
+// self term once, and every RHS's multipole-maximum slots reset.
+template <typename TS>
+__global__ void k_gather_multi(int n, int R, const int32_t* __restrict__ perm,
+                               const TS* __restrict__ sigma, int sigma_tree,
+                               const TS* __restrict__ selfi, const float* __restrict__ area_t,
+                               float* __restrict__ sig_m, float* __restrict__ qT,
+                               float* __restrict__ selfc_t, unsigned* __restrict__ mmax_bits,
+                               int64_t slot_stride) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = blockIdx.y;
+  if (i < kMaxSlots && mmax_bits) mmax_bits[r * slot_stride + i * kMaxSlotStride] = 0u;
+  if (i >= n) return;
+  const int o = perm[i];
+  const float s = (float)sigma[(size_t)r * n + (sigma_tree ? i : o)];
+  sig_m[(size_t)r * n + i] = s;
+  qT[(size_t)i * R + r] = s * area_t[i];
+  if (r == 0) selfc_t[i] = selfi ? (float)selfi[o] : 0.0f;
+}
+
 // Per panel (tree order): far field by L2P from its leaf's local expansion,
 // the P2P sum, the near-field correction (CSR row), the self term; output
 // in tree or original order.  Non-finite results become 0
@@ -766,8 +955,13 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     const int32_t* __restrict__ near_ptr, const int32_t* __restrict__ near_col,
     const float* __restrict__ near_w, const float* __restrict__ sig_t,
     const float* __restrict__ selfc_t, const int32_t* __restrict__ perm, int out_tree, float ke,
-    TO* __restrict__ out) {
+    TO* __restrict__ out, RhsStride rs) {
   constexpr int NC = (P + 1) * (P + 1);
+  const int ry = blockIdx.y;
+  L += ry * rs.node;
+  p2p_part += ry * rs.part;
+  sig_t += ry * rs.n;
+  out += ry * rs.n;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int node = pt_node[i];
@@ -806,7 +1000,8 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     for (int u = 0; u < 4; ++u) near += a[u];
   }
   for (; k < pr.y; ++k) near += p2p_part[pr.x + 64 * k];
-  float v = ke * (far + near) + selfc_t[i] * xi.w + corr;
+  const float qi = rs.qT ? rs.qT[(size_t)i * rs.R + ry] : xi.w;
+  float v = ke * (far + near) + selfc_t[i] * qi + corr;
   if (!isfinite(v)) v = 0.0f;
   out[out_tree ? i : perm[i]] = (TO)v;
 }
@@ -834,33 +1029,79 @@ __global__ __launch_bounds__(256) void k_l2p_out(
                                                          hipGetErrorString(e_)); \
   } while (0)
 
-static void p2p_launch(ed_plan* Pn, hipStream_t s) {
+static void p2p_launch(ed_plan* Pn, int R, const RhsStride& rs, hipStream_t s) {
   const unsigned nb = (unsigned)((Pn->n_p2p_items + kP2PWaves - 1) / kP2PWaves);
-  hipLaunchKernelGGL(k_p2p, dim3(nb), dim3(64 * kP2PWaves), 0, s,
-                     (const ed::P2PItem*)Pn->p2p_items, (int)Pn->n_p2p_items, Pn->p2p_seg,
-                     Pn->pts, Pn->p2p_part);
+  if (R == 1) {
+    hipLaunchKernelGGL(k_p2p, dim3(nb), dim3(64 * kP2PWaves), 0, s,
+                       (const ed::P2PItem*)Pn->p2p_items, (int)Pn->n_p2p_items, Pn->p2p_seg,
+                       Pn->pts, Pn->p2p_part);
+    return;
+  }
+  const auto* items = (const ed::P2PItem*)Pn->p2p_items;
+  const int ni = (int)Pn->n_p2p_items;
+  float* part = Pn->multi.part;
+  switch (R) {
+#define ED_P2P_R(RR)                                                                        \
+  case RR:                                                                                  \
+    hipLaunchKernelGGL(k_p2p_multi<RR>, dim3(nb), dim3(64 * kP2PWaves), 0, s, items, ni,    \
+                       Pn->p2p_seg, Pn->pts, rs.qT, part, rs.part);                         \
+    break;
+    ED_P2P_R(2) ED_P2P_R(3) ED_P2P_R(4) ED_P2P_R(5) ED_P2P_R(6) ED_P2P_R(7) ED_P2P_R(8)
+#undef ED_P2P_R
+    default: break;  // R is checked by the caller
+  }
 }
 
+// One matvec for R right-hand sides (R = 1: the plan's own buffers; R > 1:
+// the multi-RHS buffers, blockIdx.y = RHS in every far-field launch).
 template <int P>
 static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
-                   void* out, int out64, int out_tree, hipStream_t st) {
+                   void* out, int out64, int out_tree, int R, hipStream_t st) {
   constexpr int NC = (P + 1) * (P + 1);
   const int n = (int)Pn->n;
+  const bool multi = R > 1;
+  RhsStride rs;
+  if (multi) {
+    rs.node = Pn->n_nodes * NC;
+    rs.pair = Pn->multi.pair_stride;
+    rs.slot = (int64_t)kMaxSlots * kMaxSlotStride;
+    rs.part = 64 * std::max<int64_t>(Pn->n_p2p_items, 1);
+    rs.n = n;
+    rs.qT = Pn->multi.qT;
+    rs.R = R;
+  }
+  float* Mb = multi ? Pn->multi.M : Pn->M;
+  float* Lb = multi ? Pn->multi.L : Pn->L;
+  float* Yb = multi ? Pn->multi.ym2m : Pn->ym2m;
+  float* Yp = multi ? Pn->multi.ypair : Pn->ypair;
+  float* sigb = multi ? Pn->multi.sig : Pn->sig_t;
+  float* partb = multi ? Pn->multi.part : Pn->p2p_part;
+  const unsigned RY = (unsigned)R;
   ED_MARK(0);
   const int gb = (n + 255) / 256;
   // M2L form: -1 VALU, 0 FP32 MFMA, 1 / 2 / 3 BF16 / FP16 / split FP16 MFMA;
   // the 16-bit forms scale their inputs by the largest |M * ascale|, which
   // P2M and M2M fold into mmax as they write M
   const int prec = Pn->n_m2l_tiles > 0 ? Pn->m2l_precision : -1;
-  unsigned* mmax = (prec >= 1 && prec <= 3) ? Pn->m2l_mmax : nullptr;
-  if (sig64)
+  unsigned* mmax = (prec >= 1 && prec <= 3) ? (multi ? Pn->multi.mmax : Pn->m2l_mmax) : nullptr;
+  if (multi) {
+    if (sig64)
+      hipLaunchKernelGGL(k_gather_multi<double>, dim3(gb, RY), dim3(256), 0, st, n, R, Pn->perm,
+                         (const double*)sigma, sig_tree, (const double*)selfi, Pn->area_t, sigb,
+                         Pn->multi.qT, Pn->selfc_t, mmax, rs.slot);
+    else
+      hipLaunchKernelGGL(k_gather_multi<float>, dim3(gb, RY), dim3(256), 0, st, n, R, Pn->perm,
+                         (const float*)sigma, sig_tree, (const float*)selfi, Pn->area_t, sigb,
+                         Pn->multi.qT, Pn->selfc_t, mmax, rs.slot);
+  } else if (sig64) {
     hipLaunchKernelGGL(k_gather<double>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
                        (const double*)sigma, sig_tree, (const double*)selfi, Pn->pts, Pn->area_t,
                        Pn->sig_t, Pn->selfc_t, mmax);
-  else
+  } else {
     hipLaunchKernelGGL(k_gather<float>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
                        (const float*)sigma, sig_tree, (const float*)selfi, Pn->pts, Pn->area_t,
                        Pn->sig_t, Pn->selfc_t, mmax);
+  }
   ED_LAUNCH_CHECK("gather");
   ED_MARK(1);
   // P2P depends only on the gathered points: off the profiling path it runs
@@ -869,7 +1110,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   if (fork) {
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_fork, st), "p2p fork");
     ED_HIP_LAUNCH(hipStreamWaitEvent(Pn->side, Pn->ev_fork, 0), "p2p fork");
-    p2p_launch(Pn, Pn->side);
+    p2p_launch(Pn, R, rs, Pn->side);
     ED_LAUNCH_CHECK("p2p");
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_join, Pn->side), "p2p join");
   }
@@ -877,8 +1118,8 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   const auto& H = Pn->host;
   const int depth = H.depth;
   if (depth >= 2) {
-    hipLaunchKernelGGL(k_p2m<P>, dim3(nleaf), dim3(kLeafBlock), 0, st, Pn->leaves,
-                       Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->M, Pn->m2l_ascale, mmax);
+    hipLaunchKernelGGL(k_p2m<P>, dim3(nleaf, RY), dim3(kLeafBlock), 0, st, Pn->leaves,
+                       Pn->node_rng, Pn->node_ctr, Pn->pts, Mb, Pn->m2l_ascale, mmax, rs);
     ED_LAUNCH_CHECK("p2m");
     ED_MARK(2);
     // upward pass: parents at level l (children at l+1), deepest first
@@ -886,11 +1127,11 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
       const int64_t pb = Pn->parents_begin[l], pe = Pn->parents_begin[l + 1];
       if (pe > pb) {
         const int64_t cb = H.level_begin[l + 1], ce = H.level_begin[l + 2];
-        hipLaunchKernelGGL(k_m2m_child<P>, dim3((unsigned)(ce - cb)), dim3(128), 0, st, (int)cb,
-                           Pn->node_octant, Pn->t_m2m, Pn->M, Pn->ym2m);
-        hipLaunchKernelGGL(k_m2m_sum<P>, dim3((unsigned)(pe - pb)), dim3(128), 0, st,
-                           Pn->parents_list + pb, Pn->first_child, Pn->n_child, (int)cb,
-                           Pn->ym2m, Pn->M, Pn->m2l_ascale, mmax);
+        hipLaunchKernelGGL(k_m2m_child<P>, dim3((unsigned)(ce - cb), RY), dim3(128), 0, st,
+                           (int)cb, Pn->node_octant, Pn->t_m2m, Mb, Yb, rs);
+        hipLaunchKernelGGL(k_m2m_sum<P>, dim3((unsigned)(pe - pb), RY), dim3(128), 0, st,
+                           Pn->parents_list + pb, Pn->first_child, Pn->n_child, (int)cb, Yb, Mb,
+                           Pn->m2l_ascale, mmax, rs);
         ED_LAUNCH_CHECK("m2m");
       }
     }
@@ -901,45 +1142,42 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     bool valu = prec < 0;
     if constexpr (!M2LShape<P>::ENABLED) valu = valu || prec == 0;
     if (valu) {
-      hipLaunchKernelGGL(k_m2l_valu<P>, dim3(nm), dim3(128), 0, st, node0, Pn->v_ptr,
-                         Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Pn->M, Pn->L);
+      hipLaunchKernelGGL(k_m2l_valu<P>, dim3(nm, RY), dim3(128), 0, st, node0, Pn->v_ptr,
+                         Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Mb, Lb, rs);
       ED_LAUNCH_CHECK("m2l_valu");
       ED_MARK(4);
     } else {
-      const dim3 tiles((unsigned)Pn->n_m2l_tiles);
-      if (prec == 1 || prec == 2 || prec == 3) {
-        if (prec == 3)
-          hipLaunchKernelGGL((k_m2l_mfma16x3<P>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
-                             Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
-                             (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M, Pn->m2l_mmax,
-                             Pn->ypair);
-        else if (prec == 1)
-          hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16>), tiles, dim3(256), 0, st,
-                             Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
-                             Pn->m2l_pair_src, (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Pn->M,
-                             Pn->m2l_mmax, Pn->ypair);
-        else
-          hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16>), tiles, dim3(256), 0, st,
-                             Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
-                             Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale,
-                             Pn->M, Pn->m2l_mmax, Pn->ypair);
-      } else if constexpr (M2LShape<P>::ENABLED) {
+      const dim3 tiles((unsigned)Pn->n_m2l_tiles, RY);
+      if (prec == 3)  // one block per tile runs every RHS (table loaded once)
+        hipLaunchKernelGGL((k_m2l_mfma16x3<P>), dim3((unsigned)Pn->n_m2l_tiles), dim3(256), 0, st,
+                           Pn->m2l_tile_tab,
+                           Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
+                           (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
+      else if (prec == 1)
+        hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
+                           Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
+                           (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
+      else if (prec == 2)
+        hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16>), tiles, dim3(256), 0, st,
+                           Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
+                           Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb,
+                           mmax, Yp, rs);
+      else if constexpr (M2LShape<P>::ENABLED)
         hipLaunchKernelGGL(k_m2l_mfma<P>, tiles, dim3(M2LShape<P>::THREADS), 0, st,
                            Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
-                           Pn->m2l_pair_src, Pn->t_m2l_mfma, Pn->m2l_ascale, Pn->M, Pn->ypair);
-      }
+                           Pn->m2l_pair_src, Pn->t_m2l_mfma, Pn->m2l_ascale, Mb, Yp, rs);
       ED_LAUNCH_CHECK("m2l_mfma");
       ED_MARK(4);
-      hipLaunchKernelGGL(k_m2l_reduce<P>, dim3(nm), dim3(128), 0, st, node0, Pn->m2l_tgt_ptr,
-                         Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Pn->ypair, Pn->L);
+      hipLaunchKernelGGL(k_m2l_reduce<P>, dim3(nm, RY), dim3(128), 0, st, node0, Pn->m2l_tgt_ptr,
+                         Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Yp, Lb, rs);
       ED_LAUNCH_CHECK("m2l_reduce");
     }
     ED_MARK(5);
     // downward pass: levels 3..depth (levels 0..2 have no inherited local)
     for (int l = 3; l <= depth; ++l) {
       const int64_t b = H.level_begin[l], e = H.level_begin[l + 1];
-      hipLaunchKernelGGL(k_l2l<P>, dim3((unsigned)(e - b)), dim3(128), 0, st, (int)b,
-                         Pn->node_parent, Pn->node_octant, Pn->t_l2l, Pn->L);
+      hipLaunchKernelGGL(k_l2l<P>, dim3((unsigned)(e - b), RY), dim3(128), 0, st, (int)b,
+                         Pn->node_parent, Pn->node_octant, Pn->t_l2l, Lb, rs);
       ED_LAUNCH_CHECK("l2l");
     }
   } else {
@@ -949,19 +1187,19 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   if (fork) {
     ED_HIP_LAUNCH(hipStreamWaitEvent(st, Pn->ev_join, 0), "p2p join");
   } else if (Pn->n_p2p_items > 0) {
-    p2p_launch(Pn, st);
+    p2p_launch(Pn, R, rs, st);
     ED_LAUNCH_CHECK("p2p");
   }
   const float ke = (float)kKE;
   ED_MARK(7);
   if (out64)
-    hipLaunchKernelGGL((k_l2p_out<P, double>), dim3(gb), dim3(256), 0, st, n, Pn->pt_node,
-                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p_ref, Pn->p2p_part, Pn->near_ptr, Pn->near_col,
-                       Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm, out_tree, ke, (double*)out);
+    hipLaunchKernelGGL((k_l2p_out<P, double>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
+                       Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->near_ptr, Pn->near_col,
+                       Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (double*)out, rs);
   else
-    hipLaunchKernelGGL((k_l2p_out<P, float>), dim3(gb), dim3(256), 0, st, n, Pn->pt_node,
-                       Pn->node_ctr, Pn->pts, Pn->L, Pn->p2p_ref, Pn->p2p_part, Pn->near_ptr, Pn->near_col,
-                       Pn->near_w, Pn->sig_t, Pn->selfc_t, Pn->perm, out_tree, ke, (float*)out);
+    hipLaunchKernelGGL((k_l2p_out<P, float>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
+                       Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->near_ptr, Pn->near_col,
+                       Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (float*)out, rs);
   ED_LAUNCH_CHECK("l2p_out");
   ED_MARK(8);
   (void)NC;
@@ -969,13 +1207,13 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
 }
 
 int fmm_launch(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
-               void* out, int out64, int out_tree, hipStream_t st) {
+               void* out, int out64, int out_tree, hipStream_t st, int nrhs) {
   switch (Pn->p) {
-    case 4: return fmm_run<4>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
-    case 6: return fmm_run<6>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
-    case 8: return fmm_run<8>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
-    case 10: return fmm_run<10>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
-    case 12: return fmm_run<12>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, st);
+    case 4: return fmm_run<4>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
+    case 6: return fmm_run<6>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
+    case 8: return fmm_run<8>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
+    case 10: return fmm_run<10>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
+    case 12: return fmm_run<12>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
     default: return fail(ED_ERR_ARG, "expansion order must be one of 4, 6, 8, 10, 12");
   }
 }

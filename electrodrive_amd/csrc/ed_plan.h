@@ -122,6 +122,21 @@ struct ed_plan {
 
   // GMRES workspace (allocated lazily by the solver, owned by the plan)
   void* gmres_ws = nullptr;
+
+  // buffers of the multi-right-hand-side matvec (allocated on first use for
+  // up to `cap` right-hand sides)
+  struct Multi {
+    int cap = 0;
+    float* qT = nullptr;       // n x R charges, point-major (tree order)
+    float* sig = nullptr;      // cap x n sigma (tree order)
+    float* M = nullptr;        // cap x n_nodes x nc
+    float* L = nullptr;        // cap x n_nodes x nc
+    float* ym2m = nullptr;     // cap x n_nodes x nc
+    float* ypair = nullptr;    // cap x pair_stride
+    int64_t pair_stride = 0;   // n_v x ncp
+    float* part = nullptr;     // cap x 64 x n_p2p_items
+    unsigned* mmax = nullptr;  // cap x 64 x 32 multipole-maximum slots
+  } multi;
 };
 
 namespace ed {
@@ -174,12 +189,14 @@ int dense_run(const DenseItem* items, int64_t n_items, const int32_t* row_nsplit
               const T* C, const T* A, const T* sigma, const T* selfi, T* out, int64_t n,
               int64_t nrhs, T* partial, hipStream_t st);
 int plan_upload(ed_plan* P);
+int multi_reserve(ed_plan* P, int nrhs);
+void multi_free(ed_plan* P);
 void plan_free(ed_plan* P);
 // FMM matvec with tree-order q/sigma/self already in place; writes the result
 // either in original order (out_orig) or tree order (out_tree), FP32 or FP64.
 int fmm_launch(ed_plan* P, const void* sigma, int sigma_is_f64, int sigma_tree_order,
                const void* self_integrals, void* out, int out_is_f64,
-               int out_tree_order, hipStream_t stream);
+               int out_tree_order, hipStream_t stream, int nrhs = 1);
 void gmres_free(ed_plan* P);
 void gmres_invalidate_graph(ed_plan* P);
 // z = M_sai u (+ Z cy when k > 0), tree order

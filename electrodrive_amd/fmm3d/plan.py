@@ -136,6 +136,30 @@ class FmmPlan:
                   "ed_fmm_matvec")
         return out
 
+    def matvec_multi(self, sigma, self_integrals=None, out=None, stream=None):
+        """One matvec for R = sigma.shape[0] (1..8) right-hand sides: sigma
+        (R, n) FP64 device tensor; returns (R, n).  Each row equals
+        ``matvec`` of that row alone."""
+        if (not sigma.is_cuda or sigma.dim() != 2 or sigma.shape[1] != self.n
+                or sigma.dtype != torch.float64):
+            raise ValueError("sigma must be a (R, n) float64 device tensor")
+        R = int(sigma.shape[0])
+        if not 1 <= R <= 8:
+            raise ValueError("1..8 right-hand sides per call")
+        sigma = sigma.contiguous()
+        if self_integrals is not None:
+            if (not self_integrals.is_cuda or self_integrals.dtype != torch.float64
+                    or self_integrals.shape != (self.n,)):
+                raise ValueError("self_integrals must be a float64 device tensor of length n")
+            self_integrals = self_integrals.contiguous()
+        if out is None:
+            out = torch.empty_like(sigma)
+        st = stream if stream is not None else nat.stream_ptr(sigma.device)
+        nat.check(nat.lib().ed_fmm_matvec_multi_f64(self.handle, nat.ptr(sigma),
+                                                    nat.ptr(self_integrals), nat.ptr(out), R,
+                                                    st), "ed_fmm_matvec_multi_f64")
+        return out
+
     STAGES = ("gather", "p2m", "m2m", "m2l", "m2l_reduce", "l2l", "p2p", "l2p_out")
 
     def profile(self, sigma, self_integrals=None, reps=10):

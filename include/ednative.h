@@ -93,6 +93,16 @@ int ed_fmm_matvec_f32(ed_plan* plan, const float* sigma, const float* self_integ
                       float* out, ed_stream stream);
 int ed_fmm_matvec_f64(ed_plan* plan, const double* sigma, const double* self_integrals,
                       double* out, ed_stream stream);
+/* The same matvec for nrhs (1..8) right-hand sides at once: sigma and out are
+ * (nrhs, n) row-major FP64 device arrays in original panel order,
+ * self_integrals (n) is shared.  Extends LaplaceFmm3D.matvec
+ * (fmm3d/bem_fmm.py:760-885), which takes one vector: the P2P evaluates each
+ * r_ij once for all right-hand sides and every far-field launch covers all of
+ * them.  Each right-hand side's result equals ed_fmm_matvec_f64 on it alone
+ * bit for bit.  Buffers for the extra right-hand sides are allocated on the
+ * first call with a larger nrhs and kept with the plan. */
+int ed_fmm_matvec_multi_f64(ed_plan* plan, const double* sigma, const double* self_integrals,
+                            double* out, int32_t nrhs, ed_stream stream);
 
 /* Per-stage device time of the FMM matvec (no reference counterpart: the
  * reference logs MultipoleOpStats counters, fmm3d/multipole_operators.py:
