@@ -38,8 +38,8 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # and the VALU-busy fraction of k_p2p from SQ counters; summary and commands
 # in profiles/pmc_traffic_icosphere81920_p10_leaf256.md
 PMC_SOURCE = "profiles/pmc_traffic_icosphere81920_p10_leaf256.md"
-TRAFFIC_BYTES = {("p2p", 81920, 10, 256): 2.188e7,
-                 ("m2l", 81920, 10, 256): 5.635e7}
+TRAFFIC_BYTES = {("p2p", 81920, 10, 256): 2.189e7,
+                 ("m2l", 81920, 10, 256): 5.689e7}
 P2P_VALU_BUSY = {(81920, 10, 256): 0.69}
 
 
