@@ -84,7 +84,11 @@ __global__ __launch_bounds__(kSaiThreads) void k_sai_build(
     double* __restrict__ out_val, int32_t* __restrict__ out_cnt) {
   __shared__ int Jl[kSaiJ];
   __shared__ int Il[kSaiI];
-  __shared__ double B[kSaiI][kSaiJ + 1];
+  // B holds the fitting rows of the row (at most ni_cap of them): sized at
+  // launch from the longest fitting-row list, so short lists leave LDS for
+  // more blocks per CU
+  extern __shared__ double Bdyn[];
+  auto B = reinterpret_cast<double(*)[kSaiJ + 1]>(Bdyn);
   __shared__ double G[kSaiJ][kSaiJ + 1];
   __shared__ double h[kSaiJ];
   __shared__ double dsh[kSaiCand];
@@ -309,8 +313,14 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
     if (rc == ED_OK) {
       hipError_t e = hipMemcpyAsync(dpj, pj.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
       if (e == hipSuccess) e = hipMemcpyAsync(dpi, pi.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
+      // the longest fitting-row list of the segment sizes B (row i + at most
+      // kSaiI - 1 candidates)
+      int64_t maxc = 0;
+      for (int64_t k = 0; k < ng; ++k) maxc = std::max(maxc, pi[k + 1] - pi[k]);
+      const int ni_cap = 1 + (int)std::min<int64_t>(maxc, kSaiI - 1);
+      const size_t lds_b = sizeof(double) * (kSaiJ + 1) * (size_t)ni_cap;
       if (e == hipSuccess) {
-        hipLaunchKernelGGL(k_sai_build, dim3((unsigned)ng), dim3(kSaiThreads), 0, st, n, b,
+        hipLaunchKernelGGL(k_sai_build, dim3((unsigned)ng), dim3(kSaiThreads), lds_b, st, n, b,
                            (const int64_t*)dpj, (const int32_t*)cj, (const int64_t*)dpi,
                            (const int32_t*)ci, P->cent64_t, P->area64_t, (const double*)selfi_t,
                            P->n_near > 0 ? P->near_ptr : nullptr, P->near_col, P->near_w64,
