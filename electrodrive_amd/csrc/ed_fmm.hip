@@ -15,6 +15,8 @@
 //               leaf's U-list source stream), partial sums per item
 //   l2p_out     one thread per panel: L2P + P2P partials + near CSR + self
 //               term, K_E scaling, scatter to the output order
+// Several right-hand sides (up to 8) run through the same launches with the
+// RHS as blockIdx.y (RhsStride); P2P then evaluates each r_ij once for all.
 // No float sums by atomics: every sum has a fixed order.  (The one atomic is
 // an unsigned max, whose result does not depend on order.)
 #include <hip/hip_runtime.h>
