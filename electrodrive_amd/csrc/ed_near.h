@@ -8,6 +8,26 @@
 
 namespace ed {
 
+// R = sqrt(clip(A, 0, inf) / pi): a NaN area stays NaN (no partner at all)
+__device__ __forceinline__ double near_radius(double a) {
+#pragma clang fp contract(off)
+  a = a < 0.0 ? 0.0 : a;
+  return sqrt(a / 3.141592653589793);
+}
+
+// The near-pair acceptance test of core/bem.py:475-555 for panels i and j:
+// 0 < |c_i - c_j| <= df (R_i + R_j), with FP contraction off so that it
+// takes the same decisions as a brute-force scan in FP64.
+__device__ __forceinline__ bool near_accept(double xi, double yi, double zi, double ri,
+                                            double xj, double yj, double zj, double aj,
+                                            double df) {
+#pragma clang fp contract(off)
+  const double ex = xi - xj, ey = yi - yj, ez = zi - zj;
+  const double d = sqrt(ex * ex + ey * ey + ez * ez);
+  const double th = df * (ri + near_radius(aj));
+  return isfinite(d) && isfinite(th) && d > 0.0 && d <= th;
+}
+
 struct NearGrid {
   double lo[3];
   double inv_h;

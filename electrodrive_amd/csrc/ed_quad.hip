@@ -136,13 +136,6 @@ __global__ void k_self_integrals(const double* __restrict__ PV, const double* __
 // test itself is the reference's, evaluated with the same operations as a
 // brute-force scan.  Phase 0 counts per row, phase 1 fills at the row offsets
 // and sorts each row by j, so the output equals the i-major, j-ascending scan.
-// R = sqrt(clip(A, 0, inf) / pi): a NaN area stays NaN (no partner at all)
-__device__ __forceinline__ double near_radius(double a) {
-#pragma clang fp contract(off)
-  a = a < 0.0 ? 0.0 : a;
-  return sqrt(a / 3.141592653589793);
-}
-
 __device__ __forceinline__ int near_cell_coord(double x, double lo, double inv_h, int dim) {
   int c = (int)floor((x - lo) * inv_h);
   return c < 0 ? 0 : (c >= dim ? dim - 1 : c);
@@ -199,10 +192,7 @@ __global__ __launch_bounds__(64 * kNearWaves) void k_near_pairs(const double* __
               int64_t j = 0;
               if (q < ce) {
                 j = cell_pts[q];
-                const double ex = xi - C[3 * j], ey = yi - C[3 * j + 1], ez = zi - C[3 * j + 2];
-                const double d = sqrt(ex * ex + ey * ey + ez * ez);
-                const double th = df * (ri + near_radius(A[j]));
-                ok = isfinite(d) && isfinite(th) && d > 0.0 && d <= th;
+                ok = near_accept(xi, yi, zi, ri, C[3 * j], C[3 * j + 1], C[3 * j + 2], A[j], df);
               }
               const uint64_t m = __ballot(ok);
               if (ok && fill) {

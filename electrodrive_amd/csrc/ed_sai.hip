@@ -76,7 +76,7 @@ __device__ __forceinline__ int sai_hash(int key) {
 
 // One 128-thread block per row (rows row0 + blockIdx.x of a geometry segment).
 __global__ __launch_bounds__(kSaiThreads) void k_sai_build(
-    int64_t n, int64_t row0, const int64_t* __restrict__ jptr, const int32_t* __restrict__ jcol,
+    int64_t n, int64_t row0, double factor_pattern,
     const int64_t* __restrict__ iptr, const int32_t* __restrict__ icol,
     const double* __restrict__ C, const double* __restrict__ A, const double* __restrict__ selfi,
     const int32_t* __restrict__ near_ptr, const int32_t* __restrict__ near_col,
@@ -95,11 +95,44 @@ __global__ __launch_bounds__(kSaiThreads) void k_sai_build(
   __shared__ int csh[kSaiCand];
   __shared__ int hk[kSaiHash], hv[kSaiHash];
   __shared__ int pre[kSaiJ + 1];
+  __shared__ int jc[kSaiCand];  // pattern candidates, ascending
+  __shared__ int wcnt[kSaiThreads / 64];
   __shared__ int nJ, nI;
   const int tid = threadIdx.x;
   const int64_t loc = blockIdx.x;
   const int64_t i = row0 + loc;
-  sai_select(i, jcol, jptr[loc], jptr[loc + 1], C, kSaiJ, Jl, &nJ, dsh, csh);
+  // Pattern candidates: the fitting-row candidates (factor_rows >= the
+  // pattern factor) that pass the pattern's near-pair test, kept in the
+  // row's ascending order by a ballot compaction (the first kSaiCand, as the
+  // pattern's own search would give).
+  {
+    const int64_t ib = iptr[loc], ie = iptr[loc + 1];
+    const double xi = C[3 * i], yi = C[3 * i + 1], zi = C[3 * i + 2];
+    const double ri = near_radius(A[i]);
+    const int lane = tid & 63, w = tid >> 6;
+    int base = 0;
+    for (int64_t q0 = ib; q0 < ie; q0 += kSaiThreads) {
+      const int64_t q = q0 + tid;
+      int j = 0;
+      bool ok = false;
+      if (q < ie) {
+        j = icol[q];
+        ok = near_accept(xi, yi, zi, ri, C[3 * (int64_t)j], C[3 * (int64_t)j + 1],
+                         C[3 * (int64_t)j + 2], A[j], factor_pattern);
+      }
+      const uint64_t m = __ballot(ok);
+      if (lane == 0) wcnt[w] = __popcll(m);
+      __syncthreads();
+      int pos = base + __popcll(m & ((1ull << lane) - 1));
+      for (int u = 0; u < w; ++u) pos += wcnt[u];
+      if (ok && pos < kSaiCand) jc[pos] = j;
+      for (int u = 0; u < kSaiThreads / 64; ++u) base += wcnt[u];
+      __syncthreads();
+    }
+    if (tid == 0) nJ = base;
+    __syncthreads();
+  }
+  sai_select(i, jc, 0, std::min(nJ, kSaiCand), C, kSaiJ, Jl, &nJ, dsh, csh);
   sai_select(i, icol, iptr[loc], iptr[loc + 1], C, kSaiI, Il, &nI, dsh, csh);
   const int nj = nJ, ni = nI;
   // B[l][k] = A[J_k][I_l] (zero for k >= nj)
@@ -294,25 +327,21 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
   for (size_t gi = 0; gi + 1 < go.size() && rc == ED_OK; ++gi) {
     const int64_t b = go[gi], ng = go[gi + 1] - go[gi];
     if (ng <= 0) continue;
-    // one cell grid (sized for the larger factor) serves both searches
+    // one search with the fitting-row factor; the kernel takes the pattern
+    // candidates out of those rows
     NearSearch si;
-    std::vector<int64_t> pj, pi;
-    int32_t *cj = nullptr, *ci = nullptr;
-    int64_t *dpj = nullptr, *dpi = nullptr;
+    std::vector<int64_t> pi;
+    int32_t* ci = nullptr;
+    int64_t* dpi = nullptr;
     rc = si.build(P->cent64_t + 3 * b, P->area64_t + b, ng, factor_rows, st);
-    if (rc == ED_OK) rc = si.count(st, pj, factor_pattern);
     if (rc == ED_OK) rc = si.count(st, pi);
     if (rc == ED_OK &&
-        (hipMalloc((void**)&cj, sizeof(int32_t) * std::max<int64_t>(pj[ng], 1)) != hipSuccess ||
-         hipMalloc((void**)&ci, sizeof(int32_t) * std::max<int64_t>(pi[ng], 1)) != hipSuccess ||
-         hipMalloc((void**)&dpj, sizeof(int64_t) * (ng + 1)) != hipSuccess ||
+        (hipMalloc((void**)&ci, sizeof(int32_t) * std::max<int64_t>(pi[ng], 1)) != hipSuccess ||
          hipMalloc((void**)&dpi, sizeof(int64_t) * (ng + 1)) != hipSuccess))
       rc = fail(ED_ERR_ALLOC, "ed_plan_build_sai: alloc");
-    if (rc == ED_OK) rc = si.fill(st, pj, nullptr, cj, b, factor_pattern);
     if (rc == ED_OK) rc = si.fill(st, pi, nullptr, ci, b);
     if (rc == ED_OK) {
-      hipError_t e = hipMemcpyAsync(dpj, pj.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
-      if (e == hipSuccess) e = hipMemcpyAsync(dpi, pi.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
+      hipError_t e = hipMemcpyAsync(dpi, pi.data(), sizeof(int64_t) * (ng + 1), hipMemcpyHostToDevice, st);
       // the longest fitting-row list of the segment sizes B (row i + at most
       // kSaiI - 1 candidates)
       int64_t maxc = 0;
@@ -321,8 +350,8 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
       const size_t lds_b = sizeof(double) * (kSaiJ + 1) * (size_t)ni_cap;
       if (e == hipSuccess) {
         hipLaunchKernelGGL(k_sai_build, dim3((unsigned)ng), dim3(kSaiThreads), lds_b, st, n, b,
-                           (const int64_t*)dpj, (const int32_t*)cj, (const int64_t*)dpi,
-                           (const int32_t*)ci, P->cent64_t, P->area64_t, (const double*)selfi_t,
+                           factor_pattern, (const int64_t*)dpi, (const int32_t*)ci, P->cent64_t,
+                           P->area64_t, (const double*)selfi_t,
                            P->n_near > 0 ? P->near_ptr : nullptr, P->near_col, P->near_w64,
                            1e-12, P->sai_col, P->sai_val, P->sai_cnt);
         e = hipGetLastError();
@@ -330,9 +359,7 @@ extern "C" int ed_plan_build_sai(ed_plan* P, const double* self_integrals, doubl
       if (e == hipSuccess) e = hipStreamSynchronize(st);
       if (e != hipSuccess) rc = fail(ED_ERR_LAUNCH, std::string("ed_plan_build_sai: ") + hipGetErrorString(e));
     }
-    if (cj) (void)hipFree(cj);
     if (ci) (void)hipFree(ci);
-    if (dpj) (void)hipFree(dpj);
     if (dpi) (void)hipFree(dpi);
   }
   (void)hipFree(selfi_t);
