@@ -164,7 +164,7 @@ __device__ __forceinline__ float slots_absmax(const unsigned* __restrict__ bits)
 // ---------------------------------------------------------------------------
 // P2M: one 64-lane block per leaf
 // ---------------------------------------------------------------------------
-template <int P>
+template <int P, bool MULTI>
 __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ leaves,
                                                     const int2* __restrict__ rng,
                                                     const float4* __restrict__ ctr,
@@ -175,9 +175,11 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float red[kLeafBlock][NC + 1];
-  const int ry = blockIdx.y;
-  M += ry * rs.node;
-  if (mmax_bits) mmax_bits += ry * rs.slot;
+  const int ry = MULTI ? blockIdx.y : 0;
+  if constexpr (MULTI) {
+    M += ry * rs.node;
+    if (mmax_bits) mmax_bits += ry * rs.slot;
+  }
   const int node = leaves[blockIdx.x];
   const int2 r = rng[node];
   const float4 c = ctr[node];
@@ -186,7 +188,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
   for (int k = 0; k < NC; ++k) acc[k] = 0.0f;
   for (int i = r.x + threadIdx.x; i < r.y; i += kLeafBlock) {
     const float4 pq = pts[i];
-    const float q = rs.qT ? rs.qT[(size_t)i * rs.R + ry] : pq.w;
+    const float q = MULTI ? rs.qT[(size_t)i * rs.R + ry] : pq.w;
     p2m_accum<P>((pq.x - c.x) * c.w, (pq.y - c.y) * c.w, (pq.z - c.z) * c.w, q, acc);
   }
 #pragma unroll
@@ -209,7 +211,7 @@ __global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ 
 // order.  (Small blocks on purpose: the passes run beside P2P, and a block
 // of many waves waits until that many wave slots are free on one CU.)
 // ---------------------------------------------------------------------------
-template <int P>
+template <int P, bool MULTI>
 __global__ __launch_bounds__(128) void k_m2m_child(int child0, const int32_t* __restrict__ octant,
                                                    const float* __restrict__ T,
                                                    const float* __restrict__ M,
@@ -217,8 +219,10 @@ __global__ __launch_bounds__(128) void k_m2m_child(int child0, const int32_t* __
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float mc[NC];
-  M += blockIdx.y * rs.node;
-  Y += blockIdx.y * rs.node;
+  if constexpr (MULTI) {
+    M += blockIdx.y * rs.node;
+    Y += blockIdx.y * rs.node;
+  }
   const int c = child0 + blockIdx.x;
   for (int j = threadIdx.x; j < NC; j += blockDim.x) mc[j] = M[(size_t)c * NC + j];
   __syncthreads();
@@ -231,7 +235,7 @@ __global__ __launch_bounds__(128) void k_m2m_child(int child0, const int32_t* __
   }
 }
 
-template <int P>
+template <int P, bool MULTI>
 __global__ __launch_bounds__(128) void k_m2m_sum(const int32_t* __restrict__ parents,
                                                  const int32_t* __restrict__ first_child,
                                                  const int32_t* __restrict__ n_child, int child0,
@@ -241,9 +245,11 @@ __global__ __launch_bounds__(128) void k_m2m_sum(const int32_t* __restrict__ par
                                                  unsigned* __restrict__ mmax_bits, RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
-  Y += blockIdx.y * rs.node;
-  M += blockIdx.y * rs.node;
-  if (mmax_bits) mmax_bits += blockIdx.y * rs.slot;
+  if constexpr (MULTI) {
+    Y += blockIdx.y * rs.node;
+    M += blockIdx.y * rs.node;
+    if (mmax_bits) mmax_bits += blockIdx.y * rs.slot;
+  }
   const int par = parents[blockIdx.x];
   const int c0 = first_child[par] - child0, nch = n_child[par];
   float mx = 0.0f;
@@ -260,7 +266,7 @@ __global__ __launch_bounds__(128) void k_m2m_sum(const int32_t* __restrict__ par
 // M2L (VALU, CSR by target): L[t] = inv_s * sum_{s in V(t)} T_off M[s].
 // Kept as the plain reference path of the GEMM form below.
 // ---------------------------------------------------------------------------
-template <int P>
+template <int P, bool MULTI>
 __global__ __launch_bounds__(128) void k_m2l_valu(int node0, const int32_t* __restrict__ v_ptr,
                                                   const int32_t* __restrict__ v_src,
                                                   const int32_t* __restrict__ v_tab,
@@ -271,8 +277,10 @@ __global__ __launch_bounds__(128) void k_m2l_valu(int node0, const int32_t* __re
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float ms[NC];
-  M += blockIdx.y * rs.node;
-  L += blockIdx.y * rs.node;
+  if constexpr (MULTI) {
+    M += blockIdx.y * rs.node;
+    L += blockIdx.y * rs.node;
+  }
   const int t = node0 + blockIdx.x;
   const int b = v_ptr[t], e = v_ptr[t + 1];
   float acc0 = 0.0f, acc1 = 0.0f;
@@ -332,7 +340,7 @@ struct M2LShapeH {
 // values are (the corrections of iterative refinement are orders of magnitude
 // below the first right-hand side) and the scale is undone exactly on Y.
 constexpr int kM2LScaleExp = 14;
-template <int P>
+template <int P, bool MULTI>
 __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t* __restrict__ tile_tab,
                                                   const int32_t* __restrict__ tile_beg,
                                                   const int32_t* __restrict__ tile_cnt,
@@ -346,8 +354,10 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
   constexpr int NT = S::THREADS, NW = NT / 64, KH = KP / 2;
   __shared__ float ta[NCP * KS];
-  M += blockIdx.y * rs.node;
-  Y += blockIdx.y * rs.pair;
+  if constexpr (MULTI) {
+    M += blockIdx.y * rs.node;
+    Y += blockIdx.y * rs.pair;
+  }
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
   if (cnt == 0) return;  // padding tile of the XCD-aware order
@@ -439,7 +449,7 @@ template <> struct Mfma16<_Float16> {
   }
 };
 
-template <int P, typename T16>
+template <int P, typename T16, bool MULTI>
 __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ tile_tab,
                                                     const int32_t* __restrict__ tile_beg,
                                                     const int32_t* __restrict__ tile_cnt,
@@ -452,9 +462,11 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   chain_prio();
   using S = M2LShapeH<P>;
   using V = typename Mfma16<T16>::vec;
-  M += blockIdx.y * rs.node;
-  Y += blockIdx.y * rs.pair;
-  mmax_bits += blockIdx.y * rs.slot;
+  if constexpr (MULTI) {
+    M += blockIdx.y * rs.node;
+    Y += blockIdx.y * rs.pair;
+    mmax_bits += blockIdx.y * rs.slot;
+  }
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
   __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
@@ -528,9 +540,9 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
 // registers from the gathered FP32 multipoles.  Four waves x 32 columns x
 // all rows, the same tile schedule and Y layout as the 16-bit kernel.  Each
 // lane issues all loads of its multipole column (KP / 2 values) before the
-// table copy, so their latency hides behind it.  With several right-hand
-// sides (rs.R > 1) one block runs the tile for every RHS in turn against the
-// table it loaded once.
+// table copy, so their latency hides behind it.  (k_m2l_mfma16x3 for one
+// right-hand side, k_m2l_mfma16x3_multi for several.)
+
 // The KP / 2 multipole values of one lane's column: k = 16 sv + 8 lk + j
 // (addresses clamped to the row).
 template <int P>
@@ -543,20 +555,25 @@ __device__ __forceinline__ void m2l16x3_load(const float* __restrict__ mrow, int
     for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, S::NC - 1)];
 }
 
+// Power-of-two input scale of the 16-bit M2L from the largest |M * ascale|
+// of one RHS.
+__device__ __forceinline__ float m2l_scale(const unsigned* __restrict__ slots) {
+  const float gmax = slots_absmax(slots);
+  return gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
+}
+
 // One wave's 32 columns x all rows of the split-FP16 product for one RHS,
-// scaled by that RHS's power of two, stored to its Y rows.
-template <int P>
+// stored to its Y rows.  as_l holds ascale x sc0; for another RHS with scale
+// sc the inputs take the power of two rel = sc / sc0 on top (RESCALE), which
+// rounds exactly like ascale x sc.
+template <int P, bool RESCALE>
 __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
                                                const float* __restrict__ as_l,
                                                const float (&mv)[M2LShapeH<P>::KP / 16][8],
-                                               const unsigned* __restrict__ slots, bool cv,
-                                               int li, int lk, float* __restrict__ yrow) {
+                                               float rel, float inv_sc, bool cv, int li, int lk,
+                                               float* __restrict__ yrow) {
   using S = M2LShapeH<P>;
   constexpr int NCP = S::NCP, KS = S::KS, NSV = S::KP / 16, NRB = NCP / 32;
-  // power-of-two input scale from the largest |M * ascale| of this RHS
-  const float gmax = slots_absmax(slots);
-  const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
-  const float inv_sc = 1.0f / sc;
   f32x16 acc[NRB];
 #pragma unroll
   for (int rb = 0; rb < NRB; ++rb)
@@ -567,8 +584,8 @@ __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
     f16x8 bh, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      // sc is a power of two: (m a) sc rounds like m (a sc)
-      const float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] * sc : 0.0f;
+      float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] : 0.0f;
+      if (RESCALE) v *= rel;
       const _Float16 h = (_Float16)v;
       bh[j] = h;
       bl[j] = (_Float16)(v - (float)h);
@@ -604,12 +621,12 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
                                                       const float* __restrict__ ascale,
                                                       const float* __restrict__ M,
                                                       const unsigned* __restrict__ mmax_bits,
-                                                      float* __restrict__ Y, RhsStride rs) {
+                                                      float* __restrict__ Y) {
   chain_prio();
   using S = M2LShapeH<P>;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
-  __shared__ float as_l[KP];  // ascale, 0 past NC
+  __shared__ float as_l[KP];  // ascale x the power-of-two input scale, 0 past NC
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
@@ -618,10 +635,19 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
   const int li = lane & 31, lk = lane >> 5;
   const int col = col0 + li;
   const bool cv = col < cnt;
-  const int src = pair_src[beg + min(col, cnt - 1)];
-  // this lane's multipole values of RHS r: k = 16 sv + 8 lk + j (clamped)
+  // this lane's multipole values: k = 16 sv + 8 lk + j (clamped addresses)
   float mv[NSV][8];
-  if (col0 < cnt) m2l16x3_load<P>(M + (size_t)src * NC, lk, mv);
+  if (col0 < cnt) {
+    const float* mrow = M + (size_t)pair_src[beg + min(col, cnt - 1)] * NC;
+#pragma unroll
+    for (int sv = 0; sv < NSV; ++sv)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, NC - 1)];
+  }
+  // power-of-two input scale from the largest |M * ascale| of this matvec
+  const float gmax = slots_absmax(mmax_bits);
+  const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
+  const float inv_sc = 1.0f / sc;
   // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
   // multiple of 16) elements long, so a chunk never crosses a row
   const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
@@ -632,23 +658,114 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
     const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
     *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
   }
-  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] : 0.0f;
+  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc : 0.0f;
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
-  m2l16x3_column<P>(ta, as_l, mv, mmax_bits, cv, li, lk, Y + (size_t)(beg + col) * NCP);
+  constexpr int NRB = NCP / 32;
+  f32x16 acc[NRB];
+#pragma unroll
+  for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
+#pragma unroll
+  for (int sv = 0; sv < NSV; ++sv) {
+    f16x8 bh, bl;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] : 0.0f;
+      const _Float16 h = (_Float16)v;
+      bh[j] = h;
+      bl[j] = (_Float16)(v - (float)h);
+    }
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb) {
+      const int at = (rb * 32 + li) * KS + 16 * sv + 8 * lk;
+      const f16x8 ah = *reinterpret_cast<const f16x8*>(ta + at);
+      const f16x8 al = *reinterpret_cast<const f16x8*>(ta + NCP * KS + at);
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[rb], 0, 0, 0);
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[rb], 0, 0, 0);
+      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[rb], 0, 0, 0);
+    }
+  }
+  if (cv) {
+    float* yrow = Y + (size_t)(beg + col) * NCP;
+#pragma unroll
+    for (int rb = 0; rb < NRB; ++rb)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = rb * 32 + q * 8 + lk * 4;
+        float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
+                               acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
+        *reinterpret_cast<float4*>(yrow + row) = v;
+      }
+  }
+}
+
+// Several right-hand sides: one block per tile runs the tile for every RHS
+// against the table it loaded once (the first RHS's scale folded into the
+// LDS coefficients, the others' as an exact power of two on top).  The
+// one-RHS kernel above keeps its own body: this helper-based form measured
+// 1-2 us slower at 82k when it served one RHS too.
+template <int P>
+__global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __restrict__ tile_tab,
+                                                      const int32_t* __restrict__ tile_beg,
+                                                      const int32_t* __restrict__ tile_cnt,
+                                                      const int32_t* __restrict__ pair_src,
+                                                      const _Float16* __restrict__ Tm,
+                                                      const float* __restrict__ ascale,
+                                                      const float* __restrict__ M,
+                                                      const unsigned* __restrict__ mmax_bits,
+                                                      float* __restrict__ Y, RhsStride rs) {
+  chain_prio();
+  using S = M2LShapeH<P>;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
+  __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
+  __shared__ float as_l[KP];  // ascale x the first RHS's scale, 0 past NC
+  if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
+  const int tile = blockIdx.x;
+  const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int col0 = wave * 32;
+  const int li = lane & 31, lk = lane >> 5;
+  const int col = col0 + li;
+  const bool cv = col < cnt;
+  // this lane's multipole values of RHS r: k = 16 sv + 8 lk + j (clamped)
+  float mv[NSV][8];
+  int src = 0;
+  if (col0 < cnt) {
+    src = pair_src[beg + min(col, cnt - 1)];
+    m2l16x3_load<P>(M + (size_t)src * NC, lk, mv);
+  }
+  const float sc0 = m2l_scale(mmax_bits);
+  // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
+  // multiple of 16) elements long, so a chunk never crosses a row
+  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
+  constexpr int N8 = NCP * KP / 8;
+#pragma unroll 4
+  for (int t = threadIdx.x; t < 2 * N8; t += 256) {
+    const int hh = t >= N8 ? 1 : 0;
+    const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
+    *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
+  }
+  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc0 : 0.0f;
+  __syncthreads();
+  if (col0 >= cnt) return;  // no barrier below
+  m2l16x3_column<P, false>(ta, as_l, mv, 1.0f, 1.0f / sc0, cv, li, lk,
+                           Y + (size_t)(beg + col) * NCP);
 #pragma unroll 1
   for (int r = 1; r < rs.R; ++r) {
     // keeps the table reads inside the loop: hoisted out of it they would
     // hold every A fragment of the tile in registers
     asm volatile("" ::: "memory");
     m2l16x3_load<P>(M + r * rs.node + (size_t)src * NC, lk, mv);
-    m2l16x3_column<P>(ta, as_l, mv, mmax_bits + r * rs.slot, cv, li, lk,
-                      Y + r * rs.pair + (size_t)(beg + col) * NCP);
+    const float sc = m2l_scale(mmax_bits + r * rs.slot);
+    m2l16x3_column<P, true>(ta, as_l, mv, sc / sc0, 1.0f / sc, cv, li, lk,
+                            Y + r * rs.pair + (size_t)(beg + col) * NCP);
   }
 }
 
 // Deterministic per-target sum of the GEMM rows: L[t] = inv_s * sum Y[row].
-template <int P>
+template <int P, bool MULTI>
 __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __restrict__ tptr,
                                                     const int32_t* __restrict__ trow,
                                                     const float4* __restrict__ ctr,
@@ -657,8 +774,10 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
                                                     float* __restrict__ L, RhsStride rs) {
   chain_prio();
   using S = M2LShape<P>;
-  Y += blockIdx.y * rs.pair;
-  L += blockIdx.y * rs.node;
+  if constexpr (MULTI) {
+    Y += blockIdx.y * rs.pair;
+    L += blockIdx.y * rs.node;
+  }
   constexpr int NC = S::NC, NCP = S::NCP;
   const int t = node0 + blockIdx.x;
   const int b = tptr[t], e = tptr[t + 1];
@@ -674,7 +793,7 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
 // ---------------------------------------------------------------------------
 // L2L: one block per node of a level: L[c] += T_l2l[oct(c)] L[parent(c)]
 // ---------------------------------------------------------------------------
-template <int P>
+template <int P, bool MULTI>
 __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restrict__ parent,
                                              const int32_t* __restrict__ octant,
                                              const float* __restrict__ T,
@@ -682,7 +801,7 @@ __global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restric
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float lp[NC];
-  L += blockIdx.y * rs.node;
+  if constexpr (MULTI) L += blockIdx.y * rs.node;
   const int c = node0 + blockIdx.x;
   const int par = parent[c];
   for (int j = threadIdx.x; j < NC; j += blockDim.x) lp[j] = L[(size_t)par * NC + j];
@@ -949,7 +1068,7 @@ __global__ void k_gather_multi(int n, int R, const int32_t* __restrict__ perm,
 // the P2P sum, the near-field correction (CSR row), the self term; output
 // in tree or original order.  Non-finite results become 0
 // (fmm3d/bem_fmm.py:741-749).
-template <int P, typename TO>
+template <int P, typename TO, bool MULTI>
 __global__ __launch_bounds__(256) void k_l2p_out(
     int n, const int32_t* __restrict__ pt_node, const float4* __restrict__ ctr,
     const float4* __restrict__ pts, const float* __restrict__ L,
@@ -959,11 +1078,13 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     const float* __restrict__ selfc_t, const int32_t* __restrict__ perm, int out_tree, float ke,
     TO* __restrict__ out, RhsStride rs) {
   constexpr int NC = (P + 1) * (P + 1);
-  const int ry = blockIdx.y;
-  L += ry * rs.node;
-  p2p_part += ry * rs.part;
-  sig_t += ry * rs.n;
-  out += ry * rs.n;
+  const int ry = MULTI ? blockIdx.y : 0;
+  if constexpr (MULTI) {
+    L += ry * rs.node;
+    p2p_part += ry * rs.part;
+    sig_t += ry * rs.n;
+    out += ry * rs.n;
+  }
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int node = pt_node[i];
@@ -1002,7 +1123,7 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     for (int u = 0; u < 4; ++u) near += a[u];
   }
   for (; k < pr.y; ++k) near += p2p_part[pr.x + 64 * k];
-  const float qi = rs.qT ? rs.qT[(size_t)i * rs.R + ry] : xi.w;
+  const float qi = MULTI ? rs.qT[(size_t)i * rs.R + ry] : xi.w;
   float v = ke * (far + near) + selfc_t[i] * qi + corr;
   if (!isfinite(v)) v = 0.0f;
   out[out_tree ? i : perm[i]] = (TO)v;
@@ -1056,12 +1177,12 @@ static void p2p_launch(ed_plan* Pn, int R, const RhsStride& rs, hipStream_t s) {
 
 // One matvec for R right-hand sides (R = 1: the plan's own buffers; R > 1:
 // the multi-RHS buffers, blockIdx.y = RHS in every far-field launch).
-template <int P>
+template <int P, bool MULTI>
 static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
                    void* out, int out64, int out_tree, int R, hipStream_t st) {
   constexpr int NC = (P + 1) * (P + 1);
   const int n = (int)Pn->n;
-  const bool multi = R > 1;
+  constexpr bool multi = MULTI;
   RhsStride rs;
   if (multi) {
     rs.node = Pn->n_nodes * NC;
@@ -1120,7 +1241,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   const auto& H = Pn->host;
   const int depth = H.depth;
   if (depth >= 2) {
-    hipLaunchKernelGGL(k_p2m<P>, dim3(nleaf, RY), dim3(kLeafBlock), 0, st, Pn->leaves,
+    hipLaunchKernelGGL((k_p2m<P, MULTI>), dim3(nleaf, RY), dim3(kLeafBlock), 0, st, Pn->leaves,
                        Pn->node_rng, Pn->node_ctr, Pn->pts, Mb, Pn->m2l_ascale, mmax, rs);
     ED_LAUNCH_CHECK("p2m");
     ED_MARK(2);
@@ -1129,9 +1250,9 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
       const int64_t pb = Pn->parents_begin[l], pe = Pn->parents_begin[l + 1];
       if (pe > pb) {
         const int64_t cb = H.level_begin[l + 1], ce = H.level_begin[l + 2];
-        hipLaunchKernelGGL(k_m2m_child<P>, dim3((unsigned)(ce - cb), RY), dim3(128), 0, st,
+        hipLaunchKernelGGL((k_m2m_child<P, MULTI>), dim3((unsigned)(ce - cb), RY), dim3(128), 0, st,
                            (int)cb, Pn->node_octant, Pn->t_m2m, Mb, Yb, rs);
-        hipLaunchKernelGGL(k_m2m_sum<P>, dim3((unsigned)(pe - pb), RY), dim3(128), 0, st,
+        hipLaunchKernelGGL((k_m2m_sum<P, MULTI>), dim3((unsigned)(pe - pb), RY), dim3(128), 0, st,
                            Pn->parents_list + pb, Pn->first_child, Pn->n_child, (int)cb, Yb, Mb,
                            Pn->m2l_ascale, mmax, rs);
         ED_LAUNCH_CHECK("m2m");
@@ -1144,33 +1265,40 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     bool valu = prec < 0;
     if constexpr (!M2LShape<P>::ENABLED) valu = valu || prec == 0;
     if (valu) {
-      hipLaunchKernelGGL(k_m2l_valu<P>, dim3(nm, RY), dim3(128), 0, st, node0, Pn->v_ptr,
+      hipLaunchKernelGGL((k_m2l_valu<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0, Pn->v_ptr,
                          Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Mb, Lb, rs);
       ED_LAUNCH_CHECK("m2l_valu");
       ED_MARK(4);
     } else {
       const dim3 tiles((unsigned)Pn->n_m2l_tiles, RY);
-      if (prec == 3)  // one block per tile runs every RHS (table loaded once)
-        hipLaunchKernelGGL((k_m2l_mfma16x3<P>), dim3((unsigned)Pn->n_m2l_tiles), dim3(256), 0, st,
-                           Pn->m2l_tile_tab,
-                           Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
-                           (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
+      if (prec == 3) {
+        if constexpr (MULTI)  // one block per tile runs every RHS (table loaded once)
+          hipLaunchKernelGGL((k_m2l_mfma16x3_multi<P>), dim3((unsigned)Pn->n_m2l_tiles),
+                             dim3(256), 0, st, Pn->m2l_tile_tab, Pn->m2l_tile_beg,
+                             Pn->m2l_tile_cnt, Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h,
+                             Pn->m2l_ascale, Mb, mmax, Yp, rs);
+        else
+          hipLaunchKernelGGL((k_m2l_mfma16x3<P>), dim3((unsigned)Pn->n_m2l_tiles), dim3(256), 0,
+                             st, Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
+                             Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb,
+                             mmax, Yp);
+      }
       else if (prec == 1)
-        hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
+        hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16, MULTI>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
                            Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
                            (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
       else if (prec == 2)
-        hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16>), tiles, dim3(256), 0, st,
+        hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16, MULTI>), tiles, dim3(256), 0, st,
                            Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
                            Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb,
                            mmax, Yp, rs);
       else if constexpr (M2LShape<P>::ENABLED)
-        hipLaunchKernelGGL(k_m2l_mfma<P>, tiles, dim3(M2LShape<P>::THREADS), 0, st,
+        hipLaunchKernelGGL((k_m2l_mfma<P, MULTI>), tiles, dim3(M2LShape<P>::THREADS), 0, st,
                            Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
                            Pn->m2l_pair_src, Pn->t_m2l_mfma, Pn->m2l_ascale, Mb, Yp, rs);
       ED_LAUNCH_CHECK("m2l_mfma");
       ED_MARK(4);
-      hipLaunchKernelGGL(k_m2l_reduce<P>, dim3(nm, RY), dim3(128), 0, st, node0, Pn->m2l_tgt_ptr,
+      hipLaunchKernelGGL((k_m2l_reduce<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0, Pn->m2l_tgt_ptr,
                          Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Yp, Lb, rs);
       ED_LAUNCH_CHECK("m2l_reduce");
     }
@@ -1178,7 +1306,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     // downward pass: levels 3..depth (levels 0..2 have no inherited local)
     for (int l = 3; l <= depth; ++l) {
       const int64_t b = H.level_begin[l], e = H.level_begin[l + 1];
-      hipLaunchKernelGGL(k_l2l<P>, dim3((unsigned)(e - b), RY), dim3(128), 0, st, (int)b,
+      hipLaunchKernelGGL((k_l2l<P, MULTI>), dim3((unsigned)(e - b), RY), dim3(128), 0, st, (int)b,
                          Pn->node_parent, Pn->node_octant, Pn->t_l2l, Lb, rs);
       ED_LAUNCH_CHECK("l2l");
     }
@@ -1195,11 +1323,11 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   const float ke = (float)kKE;
   ED_MARK(7);
   if (out64)
-    hipLaunchKernelGGL((k_l2p_out<P, double>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
+    hipLaunchKernelGGL((k_l2p_out<P, double, MULTI>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
                        Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->near_ptr, Pn->near_col,
                        Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (double*)out, rs);
   else
-    hipLaunchKernelGGL((k_l2p_out<P, float>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
+    hipLaunchKernelGGL((k_l2p_out<P, float, MULTI>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
                        Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->near_ptr, Pn->near_col,
                        Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (float*)out, rs);
   ED_LAUNCH_CHECK("l2p_out");
@@ -1210,14 +1338,20 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
 
 int fmm_launch(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const void* selfi,
                void* out, int out64, int out_tree, hipStream_t st, int nrhs) {
+#define ED_RUN(PP)                                                                             \
+  return nrhs > 1 ? fmm_run<PP, true>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, \
+                                      nrhs, st)                                                \
+                  : fmm_run<PP, false>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, \
+                                       1, st);
   switch (Pn->p) {
-    case 4: return fmm_run<4>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
-    case 6: return fmm_run<6>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
-    case 8: return fmm_run<8>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
-    case 10: return fmm_run<10>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
-    case 12: return fmm_run<12>(Pn, sigma, sig64, sig_tree, selfi, out, out64, out_tree, nrhs, st);
+    case 4: ED_RUN(4)
+    case 6: ED_RUN(6)
+    case 8: ED_RUN(8)
+    case 10: ED_RUN(10)
+    case 12: ED_RUN(12)
     default: return fail(ED_ERR_ARG, "expansion order must be one of 4, 6, 8, 10, 12");
   }
+#undef ED_RUN
 }
 
 int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled) {
