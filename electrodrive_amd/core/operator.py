@@ -180,7 +180,9 @@ class BemOperator:
         """FP64 GMRES on the FMM operator with FP64 iterative refinement against
         the accurate operator (``refine=True``); returns (sigma, info).
         ``precond``: True / "jacobi" (core/bem.py:2141-2158), "sai" (sparse
-        approximate inverse, built on first use) or False / "none"."""
+        approximate inverse, built on first use) or False / "none".
+        ``b`` of shape (R, n), R <= 8, solves R right-hand sides together (one
+        bem_solve per charge set in the reference); sigma has b's shape."""
         if x0 is None:
             x0 = (b / self.diag).contiguous()  # core/bem.py:2060
         # The inner solves run on the FMM operator, whose rounding (16-bit M2L:
@@ -196,42 +198,60 @@ class BemOperator:
             self.enable_sai()
         use_diag = precond not in (False, None, "none")
         G = len(self.meshes)
-        if G > 1:
-            # batches: the operator is block diagonal, so each geometry's block
-            # is scaled to a unit right-hand side.  Without a coarse space the
-            # native solver runs one Krylov space per geometry and stops each
-            # at tol |b_g|; a coarse space couples the geometries into one
-            # system, solved to tol / sqrt(G) so that every block still meets
-            # tol relative to its own |b_g|
-            scale = torch.ones_like(b)
-            for g in range(G):
-                lo, hi = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
-                nb = float(torch.linalg.norm(b[lo:hi]))
-                scale[lo:hi] = 1.0 / nb if nb > 0.0 else 1.0
-            x, info = self.plan.solve((b * scale).contiguous(), (x0 * scale).contiguous(),
-                                      self.diag if use_diag else None, self.self_corr,
-                                      restart=restart, maxiter=maxiter,
-                                      tol=tol if self.plan.coarse_k == 0 else tol / math.sqrt(G),
-                                      precond=precond, refine=refine, max_refine=max_refine,
-                                      inner_tol=inner_tol, use_graph=use_graph,
-                                      check_every=check_every)
-            x = x / scale
-            r = b - self.accurate(x) if refine else None
-            if r is not None:
-                worst = 0.0
-                for g in range(G):
-                    lo, hi = int(self.geom_offsets[g]), int(self.geom_offsets[g + 1])
-                    nb = float(torch.linalg.norm(b[lo:hi]))
-                    if nb > 0.0:
-                        worst = max(worst, float(torch.linalg.norm(r[lo:hi])) / nb)
-                info["max_geometry_rel_resid"] = worst
-                info["resid"] = float(torch.linalg.norm(r))
-                info["rel_resid"] = info["resid"] / max(float(torch.linalg.norm(b)), 1e-300)
-            return x, info
-        return self.plan.solve(b, x0, self.diag if use_diag else None, self.self_corr,
-                               restart=restart, maxiter=maxiter, tol=tol, precond=precond,
-                               refine=refine, max_refine=max_refine, inner_tol=inner_tol,
-                               use_graph=use_graph, check_every=check_every)
+        multi = b.dim() == 2
+        kw = dict(restart=restart, maxiter=maxiter, precond=precond, refine=refine,
+                  max_refine=max_refine, inner_tol=inner_tol, use_graph=use_graph,
+                  check_every=check_every)
+        if G == 1 and not multi:
+            return self.plan.solve(b, x0, self.diag if use_diag else None, self.self_corr,
+                                   tol=tol, **kw)
+        # batches: the operator is block diagonal, so each (right-hand side,
+        # geometry) block is scaled to a unit right-hand side.  Without a
+        # coarse space the native solver runs one Krylov space per block and
+        # stops each at tol |b_block|; a coarse space couples the geometries of
+        # one right-hand side into one system, solved to tol / sqrt(G) so that
+        # every block still meets tol relative to its own |b_g|
+        B = b if multi else b[None]
+        gid = torch.as_tensor(np.repeat(np.arange(G), np.diff(self.geom_offsets)),
+                              device=B.device)
+        off = [int(v) for v in self.geom_offsets]
+
+        def block_norms(V):
+            # one reduction per geometry (fixed order; an index_add_ would sum
+            # with atomics and change the scaling in its last bits)
+            return torch.stack([torch.linalg.norm(V[:, off[g]:off[g + 1]], dim=1)
+                                for g in range(G)], 1)
+        nb = block_norms(B)
+        scale = torch.where(nb > 0.0, 1.0 / nb, torch.ones_like(nb))[:, gid]
+        Bs, Xs = B * scale, (x0 if multi else x0[None]) * scale
+        if not multi:
+            Bs, Xs = Bs[0], Xs[0]
+        x, info = self.plan.solve(Bs.contiguous(), Xs.contiguous(),
+                                  self.diag if use_diag else None, self.self_corr,
+                                  tol=tol if self.plan.coarse_k == 0 else tol / math.sqrt(G),
+                                  **kw)
+        X = (x if multi else x[None]) / scale
+        if refine:
+            nbh = nb.cpu().numpy()
+            rn, bn, per_geometry = self.plan.block_residuals()
+            if per_geometry:
+                # the solver's own FP64 residual per block (scaled units):
+                # relative residuals are scale-free, absolute ones are
+                # rn_block * |b_block|
+                rn, bn = rn.reshape(nbh.shape), bn.reshape(nbh.shape)
+                rel = np.where(nbh > 0.0, rn / np.where(bn > 0.0, bn, 1.0), 0.0)
+                res = np.where(nbh > 0.0, rn * nbh, rn)
+            else:
+                Rr = torch.stack([B[r] - self.accurate(X[r]) for r in range(B.shape[0])])
+                res = block_norms(Rr).cpu().numpy()
+                rel = np.where(nbh > 0.0, res / np.where(nbh > 0.0, nbh, 1.0), 0.0)
+            info["max_geometry_rel_resid"] = float(rel.max())
+            info["resid"] = float(np.sqrt((res ** 2).sum()))
+            info["rel_resid"] = info["resid"] / max(float(np.sqrt((nbh ** 2).sum())), 1e-300)
+            if multi:
+                info["rhs_rel_resid"] = (np.sqrt((res ** 2).sum(1))
+                                         / np.maximum(np.sqrt((nbh ** 2).sum(1)), 1e-300)).tolist()
+        return (X if multi else X[0]), info
 
     # -- post-solve evaluation (core/bem.py:1441-1522) -------------------------
     def free_space(self, points, charges, field=False):

@@ -522,6 +522,7 @@ void multi_free(ed_plan* P) {
 
 int multi_reserve(ed_plan* P, int nrhs) {
   if (P->multi.cap >= nrhs) return ED_OK;
+  gmres_invalidate_graph(P);  // a captured Arnoldi step refers to the old buffers
   multi_free(P);
   auto& m = P->multi;
   const size_t R = (size_t)nrhs, n = (size_t)P->n, mn = (size_t)P->n_nodes * P->nc;

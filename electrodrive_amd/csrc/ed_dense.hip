@@ -1,9 +1,10 @@
 // Dense single-layer collocation matvec (core/bem_kernel.py:157-230, :715-810):
 //   V_i = sum_{j != i} K_E sigma_j A_j rsqrt(max(r_ij^2, 1e-24)) + K_ii sigma_i A_i
 // with K_ii = self_integrals[i] (per-area) or 0.  Split form: two target rows
-// per thread, source panels staged through LDS in tiles of 256, up to 4
-// right-hand sides per pass.  Symmetric form (FP64, one right-hand side): each
-// pair evaluated once for both rows.  The FP64 kernels refine the hardware
+// per thread, source panels staged through LDS in tiles of 256, up to 8
+// right-hand sides per pass (one r^2 and rsqrt per pair for all of them).
+// Symmetric form (FP64, one right-hand side): each pair evaluated once for
+// both rows.  The FP64 kernels refine the hardware
 // FP64 reciprocal square root with one Newton step.
 #include <hip/hip_runtime.h>
 
@@ -18,7 +19,7 @@
 namespace ed {
 
 constexpr int kDenseBlock = 256;
-constexpr int kDenseRhs = 4;
+constexpr int kDenseRhs = 8;
 constexpr int kDenseMaxSplit = 16;
 
 template <typename T>
@@ -406,7 +407,11 @@ int dense_run(const DenseItem* items, int64_t n_items, const int32_t* row_nsplit
       ED_DENSE_NR(1)
       ED_DENSE_NR(2)
       ED_DENSE_NR(3)
-      default: ED_DENSE_NR(4)
+      ED_DENSE_NR(4)
+      ED_DENSE_NR(5)
+      ED_DENSE_NR(6)
+      ED_DENSE_NR(7)
+      default: ED_DENSE_NR(8)
     }
 #undef ED_DENSE_NR
     hipError_t e = hipGetLastError();

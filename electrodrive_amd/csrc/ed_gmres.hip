@@ -31,7 +31,9 @@ constexpr int kRedBlocks = 240;  // partial-sum blocks for multi-dots
 constexpr int kRedThreads = 256;
 
 struct GmresWS {
-  int64_t n = 0;
+  int64_t n = 0;   // vector length: R stacked right-hand sides of n1 panels
+  int64_t n1 = 0;  // panels of the plan
+  int R = 1;
   int m = 0;
   double *V = nullptr, *w = nullptr, *vin = nullptr, *x = nullptr, *b = nullptr, *r = nullptr,
          *t = nullptr, *dinv = nullptr, *selfi_t = nullptr, *d = nullptr, *rhs = nullptr,
@@ -44,9 +46,10 @@ struct GmresWS {
   double* hscal = nullptr;    // pinned host mirror
   int* dj = nullptr;          // device Arnoldi index
   // Segments: the Arnoldi process runs independently on every geometry of a
-  // batched plan (the operator is block diagonal), sharing each launch.
-  // Tiling k = 0: one segment per geometry; k = 1: the whole vector as one
-  // segment (single geometries, or a coarse space that couples geometries).
+  // batched plan (the operator is block diagonal) and every right-hand side,
+  // sharing each launch.  Tiling k = 0: one segment per (right-hand side,
+  // geometry); k = 1: one segment per right-hand side (single geometries, or
+  // a coarse space that couples geometries).
   int Gmax = 1;
   int G_k[2] = {1, 1}, NT_k[2] = {0, 0};
   int4* tiles_k[2] = {nullptr, nullptr};  // (segment, lo, hi, 0)
@@ -58,6 +61,18 @@ struct GmresWS {
   hipGraph_t graph = nullptr;
   const double* graph_selfi = nullptr;
   int graph_mode = -1;  // preconditioner the captured graph was recorded with
+  // per-segment residual and right-hand-side norms of the last solve (the
+  // accurate operator's when it refined) and whether its segments were
+  // (right-hand side, geometry) blocks (tiling 0) or whole right-hand sides
+  std::vector<double> last_rn, last_bn;
+  int last_tiling = -1;
+  // split dense FP64 operator for R > 1 right-hand sides (built on first use):
+  // work items over the plan's geometries, splits per row, partial sums
+  DenseItem* dn_items = nullptr;
+  int32_t* dn_rns = nullptr;
+  double* dn_part = nullptr;
+  int64_t dn_n_items = 0;
+  int dn_smax = 0;
 };
 
 // device scalar slots
@@ -72,7 +87,8 @@ void gmres_free(ed_plan* P) {
   if (W->graph) (void)hipGraphDestroy(W->graph);
   void* ptrs[] = {W->V, W->w, W->vin, W->x, W->b, W->r, W->t, W->dinv, W->selfi_t, W->d, W->rhs,
                   W->u2, W->H, W->g, W->cs, W->sn, W->h, W->h2, W->partial, W->scal, W->dj,
-                  W->tiles_k[0], W->tiles_k[1], W->seg_t0_k[0], W->seg_t0_k[1], W->segv};
+                  W->tiles_k[0], W->tiles_k[1], W->seg_t0_k[0], W->seg_t0_k[1], W->segv,
+                  W->dn_items, W->dn_rns, W->dn_part};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   if (W->hscal) (void)hipHostFree(W->hscal);
@@ -126,18 +142,27 @@ static void make_tiles(const std::vector<int64_t>& off, int64_t ts, std::vector<
   }
 }
 
-static int ws_get(ed_plan* P, int m, GmresWS** out) {
+static int ws_get(ed_plan* P, int m, int R, GmresWS** out) {
   GmresWS* W = (GmresWS*)P->gmres_ws;
-  if (W && W->m == m && W->n == P->n) { *out = W; return ED_OK; }
+  if (W && W->m == m && W->n1 == P->n && W->R == R) { *out = W; return ED_OK; }
+  if ((int64_t)R * P->n >= ((int64_t)1 << 31))
+    return fail(ED_ERR_ARG, "GMRES: right-hand sides x panels must stay below 2^31");
   gmres_free(P);
   W = new GmresWS();
   P->gmres_ws = W;
-  W->n = P->n;
+  W->n1 = P->n;
+  W->R = R;
+  W->n = (int64_t)R * P->n;
   W->m = m;
-  const size_t n = (size_t)P->n;
-  std::vector<int64_t> geo = P->host.geom_pt_off;
-  if (geo.size() < 2) geo = {0, (int64_t)n};
-  const std::vector<int64_t> whole = {0, (int64_t)n};
+  const size_t n = (size_t)W->n;
+  std::vector<int64_t> geo1 = P->host.geom_pt_off;
+  if (geo1.size() < 2) geo1 = {0, P->n};
+  // segment boundaries of the R stacked vectors
+  std::vector<int64_t> geo = {0}, whole = {0};
+  for (int r = 0; r < R; ++r) {
+    for (size_t g = 1; g < geo1.size(); ++g) geo.push_back((int64_t)r * P->n + geo1[g]);
+    whole.push_back((int64_t)(r + 1) * P->n);
+  }
   // ~512 tiles over the whole vector, 256..4096 entries each
   const int64_t ts = std::min<int64_t>(4096, std::max<int64_t>(256, ((int64_t)n / 512 + 63) / 64 * 64));
   int ntmax = 1;
@@ -739,14 +764,19 @@ struct Solver {
   GmresWS* W;
   const ed_gmres_opts* o;
   const double* selfi_orig;
-  int64_t n;
+  int64_t n;   // stacked length R x n1
   unsigned gb;
+  int64_t n1;  // panels
+  int R;
+  unsigned gn1;  // one block per 256 panels
   int total_iters = 0, cycles = 0, first_below = -1;
 
+  // the FMM operator on all R right-hand sides (one multi-RHS matvec)
   int approx_op(const double* in_tree, double* out_tree) {
-    return fmm_launch(P, in_tree, 1, 1, selfi_orig, out_tree, 1, 1, W->stream);
+    return fmm_launch(P, in_tree, 1, 1, selfi_orig, out_tree, 1, 1, W->stream, R);
   }
   int accurate_op(const double* in_tree, double* out_tree) {
+    if (R > 1) return accurate_op_multi(in_tree, out_tree);
     // block diagonal over the geometries of a batched plan: one symmetric-
     // form launch covers every geometry (schedule cached on the plan)
     if (!P->dense_sym) {
@@ -760,43 +790,89 @@ struct Solver {
       }
       P->dense_sym = D;
     }
-    ED_T(dense_sym_run(*P->dense_sym, P->cent64_t, P->area64_t, in_tree, W->selfi_t, out_tree,
-                       W->stream));
-    if (P->n_near > 0) {
-      hipLaunchKernelGGL(k_near_csr, dim3(gb), dim3(256), 0, W->stream, n, P->near_ptr,
-                         P->near_col, P->near_w64, in_tree, out_tree);
-      ED_KCHECK("near csr");
+    for (int r = 0; r < R; ++r) {
+      const size_t o1 = (size_t)r * n1;
+      ED_T(dense_sym_run(*P->dense_sym, P->cent64_t, P->area64_t, in_tree + o1, W->selfi_t,
+                         out_tree + o1, W->stream));
+      if (P->n_near > 0)
+        hipLaunchKernelGGL(k_near_csr, dim3(gn1), dim3(256), 0, W->stream, n1, P->near_ptr,
+                           P->near_col, P->near_w64, in_tree + o1, out_tree + o1);
     }
+    ED_KCHECK("near csr");
+    return ED_OK;
+  }
+  // R right-hand sides: the split form evaluates each pair once for all of
+  // them (one r^2 and rsqrt, R FMAs), where the symmetric form would run R
+  // times; then the FP64 near correction per right-hand side
+  int accurate_op_multi(const double* in_tree, double* out_tree) {
+    if (!W->dn_items) {
+      std::vector<int64_t> geo = P->host.geom_pt_off;
+      if (geo.size() < 2) geo = {0, n1};
+      std::vector<DenseItem> items;
+      std::vector<int32_t> rns;
+      int smax = 1;
+      dense_schedule(geo, items, rns, &smax);
+      ED_G(hipMalloc((void**)&W->dn_items, sizeof(DenseItem) * std::max<size_t>(items.size(), 1)));
+      ED_G(hipMalloc((void**)&W->dn_rns, sizeof(int32_t) * n1));
+      ED_G(hipMalloc((void**)&W->dn_part, sizeof(double) * (size_t)smax * R * n1));
+      if (!items.empty())
+        ED_G(hipMemcpy(W->dn_items, items.data(), sizeof(DenseItem) * items.size(),
+                       hipMemcpyHostToDevice));
+      ED_G(hipMemcpy(W->dn_rns, rns.data(), sizeof(int32_t) * n1, hipMemcpyHostToDevice));
+      W->dn_n_items = (int64_t)items.size();
+      W->dn_smax = smax;
+    }
+    ED_T(dense_run<double>(W->dn_items, W->dn_n_items, W->dn_rns, W->dn_smax, P->cent64_t,
+                           P->area64_t, in_tree, W->selfi_t, out_tree, n1, R, W->dn_part,
+                           W->stream));
+    if (P->n_near > 0)
+      for (int r = 0; r < R; ++r) {
+        const size_t o1 = (size_t)r * n1;
+        hipLaunchKernelGGL(k_near_csr, dim3(gn1), dim3(256), 0, W->stream, n1, P->near_ptr,
+                           P->near_col, P->near_w64, in_tree + o1, out_tree + o1);
+      }
+    ED_KCHECK("near csr");
     return ED_OK;
   }
   bool use_sai() const { return o->precond == 2; }
   bool general() const { return P->coarse_k > 0 || use_sai(); }
   int mode() const { return (use_sai() ? 4 : 0) + (o->precond ? 2 : 0) + (P->coarse_k > 0 ? 1 : 0); }
-  // z = M^-1 u in place (coarse-corrected Jacobi); u may alias z
+  // z = M^-1 u in place (coarse-corrected Jacobi), per right-hand side;
+  // u may alias z
   void coarse_apply(const double* u, double* z) {
     const int k = P->coarse_k;
-    hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, k,
-                       P->coarse_z, u, P->coarse_part);
-    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), 0, W->stream, k, P->coarse_part,
-                       P->coarse_einv, P->coarse_cy);
-    hipLaunchKernelGGL(k_coarse_apply, dim3(gb), dim3(256), 0, W->stream, n, k, u, W->dinv,
-                       P->coarse_z, P->coarse_az, P->coarse_cy, z);
-  }
-  // z = M^-1 u with the SAI smoother, M^-1 u = S (u - AZ c) + Z c; u is
-  // overwritten (in-place subtraction) and must not alias z
-  void sai_precond(double* u, double* z) {
-    const int k = P->coarse_k;
-    if (k > 0) {
-      hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n, k,
-                         P->coarse_z, u, P->coarse_part);
+    const unsigned gb1 = (unsigned)std::min<int64_t>((n1 + 255) / 256, 4096);
+    for (int r = 0; r < R; ++r) {
+      const size_t o1 = (size_t)r * n1;
+      hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n1, k,
+                         P->coarse_z, u + o1, P->coarse_part);
       hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), 0, W->stream, k, P->coarse_part,
                          P->coarse_einv, P->coarse_cy);
-      hipLaunchKernelGGL(k_coarse_sub, dim3(gb), dim3(256), 0, W->stream, n, k, P->coarse_az,
-                         P->coarse_cy, u);
+      hipLaunchKernelGGL(k_coarse_apply, dim3(gb1), dim3(256), 0, W->stream, n1, k, u + o1,
+                         W->dinv + o1, P->coarse_z, P->coarse_az, P->coarse_cy, z + o1);
     }
-    sai_apply_launch(P, u, k, P->coarse_z, P->coarse_cy, z, W->stream);
   }
-  // active tiling: per geometry unless a coarse space couples the geometries
+  // z = M^-1 u with the SAI smoother, M^-1 u = S (u - AZ c) + Z c, per
+  // right-hand side; u is overwritten (in-place subtraction) and must not
+  // alias z
+  void sai_precond(double* u, double* z) {
+    const int k = P->coarse_k;
+    const unsigned gb1 = (unsigned)std::min<int64_t>((n1 + 255) / 256, 4096);
+    for (int r = 0; r < R; ++r) {
+      const size_t o1 = (size_t)r * n1;
+      if (k > 0) {
+        hipLaunchKernelGGL(k_coarse_dots, dim3(kRedBlocks), dim3(kRedThreads), 0, W->stream, n1,
+                           k, P->coarse_z, u + o1, P->coarse_part);
+        hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), 0, W->stream, k, P->coarse_part,
+                           P->coarse_einv, P->coarse_cy);
+        hipLaunchKernelGGL(k_coarse_sub, dim3(gb1), dim3(256), 0, W->stream, n1, k, P->coarse_az,
+                           P->coarse_cy, u + o1);
+      }
+      sai_apply_launch(P, u + o1, k, P->coarse_z, P->coarse_cy, z + o1, W->stream);
+    }
+  }
+  // active tiling: per (right-hand side, geometry) unless a coarse space
+  // couples the geometries
   int tk() const { return P->coarse_k > 0 ? 1 : 0; }
   int G() const { return W->G_k[tk()]; }
   int NT() const { return W->NT_k[tk()]; }
@@ -987,10 +1063,12 @@ extern "C" void ed_gmres_opts_default(ed_gmres_opts* o) {
   o->check_every = 4;
 }
 
-extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const double* diag,
-                              const double* selfi, const ed_gmres_opts* opts,
-                              ed_gmres_info* info, ed_stream stream) {
-  if (!P || !b || !x || !info) return fail(ED_ERR_ARG, "ed_gmres_solve: NULL argument");
+// R right-hand sides stacked (R x n, original order); every (right-hand
+// side, geometry) segment runs its own Krylov space and stops at tol times its
+// own |b| block, and all of them share each operator launch.
+static int gmres_solve(ed_plan* P, const double* b, double* x, int R, const double* diag,
+                       const double* selfi, const ed_gmres_opts* opts, ed_gmres_info* info,
+                       ed_stream stream) {
   ed_gmres_opts o;
   ed_gmres_opts_default(&o);
   if (opts) o = *opts;
@@ -1001,27 +1079,32 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
     return fail(ED_ERR_ARG, "precond 2 needs the sparse approximate inverse (ed_plan_build_sai)");
   std::memset(info, 0, sizeof(*info));
   info->first_iter_below_tol = -1;
+  if (R > 1) ED_T(multi_reserve(P, R));
   GmresWS* W = nullptr;
-  ED_T(ws_get(P, o.restart, &W));
-  const int64_t n = P->n;
+  ED_T(ws_get(P, o.restart, R, &W));
+  W->last_tiling = -1;
+  const int64_t n1 = P->n, n = W->n;
   const unsigned gb = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
-  const unsigned gn = (unsigned)((n + 255) / 256);
+  const unsigned gn = (unsigned)((n + 255) / 256), gn1 = (unsigned)((n1 + 255) / 256);
   hipStream_t caller = (hipStream_t)stream;
   // order the solver's stream after the caller's work
   hipEvent_t ev;
   ED_G(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
   ED_G(hipEventRecord(ev, caller));
   ED_G(hipStreamWaitEvent(W->stream, ev, 0));
-  hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, W->stream, n, P->perm, b, W->b);
-  hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, W->stream, n, P->perm, x, W->x);
-  hipLaunchKernelGGL(k_dinv, dim3(gn), dim3(256), 0, W->stream, n, P->perm,
-                     o.precond ? diag : nullptr, W->dinv);
+  for (int r = 0; r < R; ++r) {
+    const size_t o1 = (size_t)r * n1;
+    hipLaunchKernelGGL(k_perm_in, dim3(gn1), dim3(256), 0, W->stream, n1, P->perm, b + o1, W->b + o1);
+    hipLaunchKernelGGL(k_perm_in, dim3(gn1), dim3(256), 0, W->stream, n1, P->perm, x + o1, W->x + o1);
+    hipLaunchKernelGGL(k_dinv, dim3(gn1), dim3(256), 0, W->stream, n1, P->perm,
+                       o.precond ? diag : nullptr, W->dinv + o1);
+  }
   if (selfi)
-    hipLaunchKernelGGL(k_perm_in, dim3(gn), dim3(256), 0, W->stream, n, P->perm, selfi, W->selfi_t);
+    hipLaunchKernelGGL(k_perm_in, dim3(gn1), dim3(256), 0, W->stream, n1, P->perm, selfi, W->selfi_t);
   else
-    ED_G(hipMemsetAsync(W->selfi_t, 0, sizeof(double) * n, W->stream));
+    ED_G(hipMemsetAsync(W->selfi_t, 0, sizeof(double) * n1, W->stream));
   ED_KCHECK("solver setup");
-  Solver S{P, W, &o, selfi, n, gb};
+  Solver S{P, W, &o, selfi, n, gb, n1, R, gn1};
   const int G = S.G();
   int code = 1;
   std::vector<double> bn, dummy;
@@ -1037,6 +1120,7 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
     ED_T(S.gmres(W->b, W->x, std::vector<double>(G, o.tol), o.maxiter, &code, &r0, &r1));
     info->resid0 = l2(r0);
     info->resid = l2(r1);
+    W->last_rn = r1;
     info->tol_abs = o.tol * (info->resid0 > 1e-300 ? info->resid0 : 1.0);
   } else {
     // outer FP64 iterative refinement with the accurate operator; segment g
@@ -1104,19 +1188,56 @@ extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const doub
       ED_KCHECK("refine update");
     }
     info->resid = l2(rn);
+    W->last_rn = rn;
   }
+  W->last_bn = bn;
+  W->last_tiling = S.tk();
   info->code = code;
   info->iters = S.total_iters;
   info->cycles = S.cycles;
   info->first_iter_below_tol = S.first_below;
   info->rel_resid = info->resid / (bn_all > 1e-300 ? bn_all : 1.0);
-  hipLaunchKernelGGL(k_perm_out, dim3(gn), dim3(256), 0, W->stream, n, P->perm, W->x, x);
+  for (int r = 0; r < R; ++r)
+    hipLaunchKernelGGL(k_perm_out, dim3(gn1), dim3(256), 0, W->stream, n1, P->perm,
+                       W->x + (size_t)r * n1, x + (size_t)r * n1);
   ED_KCHECK("solver output");
   ED_G(hipEventRecord(ev, W->stream));
   ED_G(hipStreamWaitEvent(caller, ev, 0));
   ED_G(hipStreamSynchronize(W->stream));
   (void)hipEventDestroy(ev);
   return ED_OK;
+}
+
+extern "C" int ed_gmres_solve(ed_plan* P, const double* b, double* x, const double* diag,
+                              const double* selfi, const ed_gmres_opts* opts,
+                              ed_gmres_info* info, ed_stream stream) {
+  if (!P || !b || !x || !info) return fail(ED_ERR_ARG, "ed_gmres_solve: NULL argument");
+  return gmres_solve(P, b, x, 1, diag, selfi, opts, info, stream);
+}
+
+extern "C" int ed_gmres_block_resid(ed_plan* P, double* resid, double* bnorm, int32_t cap,
+                                    int32_t* n_blocks, int32_t* per_geometry) {
+  if (!P || !n_blocks) return fail(ED_ERR_ARG, "ed_gmres_block_resid: NULL argument");
+  const GmresWS* W = (const GmresWS*)P->gmres_ws;
+  if (!W || W->last_tiling < 0 || W->last_rn.size() != W->last_bn.size())
+    return fail(ED_ERR_ARG, "ed_gmres_block_resid: no solve has run on this plan");
+  const int nb = (int)W->last_rn.size();
+  *n_blocks = nb;
+  if (per_geometry) *per_geometry = W->last_tiling == 0 ? 1 : 0;
+  for (int k = 0; k < nb && k < cap; ++k) {
+    if (resid) resid[k] = W->last_rn[k];
+    if (bnorm) bnorm[k] = W->last_bn[k];
+  }
+  return ED_OK;
+}
+
+extern "C" int ed_gmres_solve_multi(ed_plan* P, const double* b, double* x, int32_t nrhs,
+                                    const double* diag, const double* selfi,
+                                    const ed_gmres_opts* opts, ed_gmres_info* info,
+                                    ed_stream stream) {
+  if (!P || !b || !x || !info) return fail(ED_ERR_ARG, "ed_gmres_solve_multi: NULL argument");
+  if (nrhs < 1 || nrhs > 8) return fail(ED_ERR_ARG, "ed_gmres_solve_multi: nrhs must be in [1, 8]");
+  return gmres_solve(P, b, x, nrhs, diag, selfi, opts, info, stream);
 }
 
 // Coarse space for the native solver (no reference counterpart).  Z holds k

@@ -218,18 +218,44 @@ class FmmPlan:
                                                  nat.stream_ptr(u.device)), "ed_plan_coarse_solve")
         return out
 
+    def block_residuals(self):
+        """Per-block residual and right-hand-side norms of the last ``solve``
+        as the native solver measured them (the FP64 operator when it
+        refined): (resid, bnorm, per_geometry) with blocks (right-hand side,
+        geometry), right-hand side major, when per_geometry, else whole
+        right-hand sides (a coarse space couples the geometries)."""
+        L = nat.lib()
+        nb, pg = ctypes.c_int32(0), ctypes.c_int32(0)
+        nat.check(L.ed_gmres_block_resid(self.handle, None, None, 0, ctypes.byref(nb),
+                                         ctypes.byref(pg)), "ed_gmres_block_resid")
+        r = np.zeros(nb.value)
+        bn = np.zeros(nb.value)
+        nat.check(L.ed_gmres_block_resid(self.handle, r.ctypes.data, bn.ctypes.data, nb.value,
+                                         ctypes.byref(nb), ctypes.byref(pg)),
+                  "ed_gmres_block_resid")
+        return r, bn, bool(pg.value)
+
     def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
               tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,
               use_graph=True, check_every=4):
-        """Native FP64 GMRES (+ iterative refinement); returns (x, info dict)."""
-        def f64(t):
+        """Native FP64 GMRES (+ iterative refinement); returns (x, info dict).
+        ``b`` of shape (n,) or (R, n) with R <= 8: the R right-hand sides are
+        solved together (one multi-RHS FMM matvec per Arnoldi step), each
+        (right-hand side, geometry) block to tol relative to its own |b|."""
+        def f64(t, rows=False):
             if t is None:
                 return None
-            if not t.is_cuda or t.dtype != torch.float64 or t.shape != (self.n,):
-                raise ValueError("solver vectors must be float64 device tensors of length n")
+            ok = t.is_cuda and t.dtype == torch.float64 and (
+                t.shape == (self.n,) or (rows and t.dim() == 2 and t.shape[1] == self.n
+                                         and 1 <= t.shape[0] <= 8))
+            if not ok:
+                raise ValueError("solver vectors must be float64 device tensors of length n "
+                                 "(right-hand sides: (n,) or (R, n), R <= 8)")
             return t.contiguous()
-        b = f64(b)
-        x = torch.zeros_like(b) if x0 is None else f64(x0).clone()
+        b = f64(b, rows=True)
+        x = torch.zeros_like(b) if x0 is None else f64(x0, rows=True).clone()
+        if x.shape != b.shape:
+            raise ValueError("x0 must have the shape of b")
         diag, self_integrals = f64(diag), f64(self_integrals)
         L = nat.lib()
         o = nat.GmresOpts()
@@ -242,9 +268,16 @@ class FmmPlan:
         o.precond, o.refine, o.max_refine = pc, int(bool(refine)), int(max_refine)
         o.inner_tol, o.use_graph, o.check_every = float(inner_tol), int(bool(use_graph)), int(check_every)
         info = nat.GmresInfo()
-        nat.check(L.ed_gmres_solve(self.handle, nat.ptr(b), nat.ptr(x), nat.ptr(diag),
-                                   nat.ptr(self_integrals), ctypes.byref(o), ctypes.byref(info),
-                                   nat.stream_ptr(b.device)), "ed_gmres_solve")
+        if b.dim() == 2:
+            nat.check(L.ed_gmres_solve_multi(self.handle, nat.ptr(b), nat.ptr(x), int(b.shape[0]),
+                                             nat.ptr(diag), nat.ptr(self_integrals),
+                                             ctypes.byref(o), ctypes.byref(info),
+                                             nat.stream_ptr(b.device)), "ed_gmres_solve_multi")
+        else:
+            nat.check(L.ed_gmres_solve(self.handle, nat.ptr(b), nat.ptr(x), nat.ptr(diag),
+                                       nat.ptr(self_integrals), ctypes.byref(o),
+                                       ctypes.byref(info), nat.stream_ptr(b.device)),
+                      "ed_gmres_solve")
         d = {k: getattr(info, k) for k, _ in nat.GmresInfo._fields_}
         d["success"] = d["code"] == 0
         return x, d

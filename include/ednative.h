@@ -257,6 +257,30 @@ int ed_gmres_solve(ed_plan* plan, const double* b, double* x, const double* diag
                    const double* self_integrals, const ed_gmres_opts* opts,
                    ed_gmres_info* info, ed_stream stream);
 
+/* ed_gmres_solve for nrhs <= 8 right-hand sides at once: b, x are nrhs x n
+ * (row r = right-hand side r, original order, device FP64); diag and
+ * self_integrals as above, shared.  Replaces one bem_solve call per
+ * right-hand side (core/bem.py:1369-2700 solves one charge set per call).
+ * Every (right-hand side, geometry) block runs its own Krylov space and stops
+ * at ||r|| <= tol ||b|| of that block; all blocks share each launch (one
+ * multi-right-hand-side FMM matvec per Arnoldi step, as
+ * ed_fmm_matvec_multi_f64).  `info` covers the whole solve: iters counts the
+ * shared Arnoldi steps, resid / rel_resid are over all nrhs x n entries. */
+int ed_gmres_solve_multi(ed_plan* plan, const double* b, double* x, int32_t nrhs,
+                         const double* diag, const double* self_integrals,
+                         const ed_gmres_opts* opts, ed_gmres_info* info, ed_stream stream);
+
+/* Per-block norms of the last ed_gmres_solve / ed_gmres_solve_multi on this
+ * plan (no reference counterpart; the reference reports one residual per
+ * solve, core/bem_solver.py:703-730): resid[k] = ||b_k - A x_k|| as the solver
+ * last measured it (the accurate FP64 operator when it refined, tree-order
+ * sums), bnorm[k] = ||b_k||, host arrays of `cap` entries.  Blocks are
+ * (right-hand side, geometry) pairs, right-hand side major
+ * (*per_geometry = 1), or whole right-hand sides when a coarse space couples
+ * the geometries (*per_geometry = 0); *n_blocks is their number. */
+int ed_gmres_block_resid(ed_plan* plan, double* resid, double* bnorm, int32_t cap,
+                         int32_t* n_blocks, int32_t* per_geometry);
+
 /* Sparse approximate inverse smoother for ed_gmres_solve (precond = 2; no
  * reference counterpart, whose setup_preconditioner offers Jacobi and mass
  * scaling, core/bem_solver.py:36-117).  Row i of M minimises

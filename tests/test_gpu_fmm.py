@@ -50,7 +50,7 @@ def test_dense_f32_and_batched():
     from electrodrive_amd.core.bem_kernel import bem_matvec_batched
     m = _mesh(3)
     rng = np.random.default_rng(7)
-    sig = rng.normal(size=(5, m.n_panels))
+    sig = rng.normal(size=(11, m.n_panels))  # passes of 8 and 3 right-hand sides
     ref = odense.dense_matvec_batched(m.centroids, m.areas, sig)
     out64 = bem_matvec_batched(torch.tensor(sig).cuda(), torch.tensor(m.centroids),
                                torch.tensor(m.areas)).cpu().numpy()
