@@ -403,6 +403,23 @@ def run_batch(a):
     sizes = np.array([m.n_panels for m in meshes])
     N = int(sizes.sum())
 
+    tune = None
+    if not a.no_autotune:
+        # (order, leaf) of the fastest multi-RHS batched matvec within the
+        # error bound on every geometry
+        from electrodrive_amd.fmm3d import FmmConfig, autotune_for_problem
+        base = FmmConfig(expansion_order=a.order, leaf_size=a.leaf,
+                         mac_theta=0.8 if a.ws == 1 else 0.5, m2l_precision=a.m2l)
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        tr, tune_s = timed(lambda: autotune_for_problem(
+            base, N, dry_run=False, target_rel_error=a.tune_target,
+            centroids=np.concatenate([m.centroids for m in meshes]),
+            areas=np.concatenate([m.areas for m in meshes]), use_cache=False,
+            geom_offsets=off, nrhs=a.rhs))
+        a.order, a.leaf = tr.config.native_order, tr.config.leaf_size
+        tune = dict(order=a.order, leaf=a.leaf, seconds=tune_s, target_rel_error=a.tune_target,
+                    **{k: v for k, v in tr.metrics.items() if k in ("ms", "rel_err", "trials")})
+
     def build():
         op = BemOperator(meshes, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
                          near=True, self_depth=3, near_depth=3)
@@ -462,11 +479,14 @@ def run_batch(a):
                     f"plates; {int(sizes.min())}-{int(sizes.max())} panels, {N} total) x {a.rhs} "
                     f"point-charge RHS; one batched launch per FMM stage covering all RHS "
                     f"(multi-RHS matvec); FMM p={a.order} "
-                    f"ws={a.ws} leaf<={a.leaf} M2L {a.m2l}; GMRES {a.precond} + fp64 refinement",
+                    f"ws={a.ws} leaf<={a.leaf} M2L {a.m2l}"
+                    + (" (order and leaf from the autotuner at rel err <= "
+                       f"{a.tune_target:g} vs FP64 dense on every geometry)" if tune else "")
+                    + f"; GMRES {a.precond} + fp64 refinement",
         "geometries": a.n_geoms, "rhs": a.rhs, "panels_total": N, "order": a.order,
         "leaf": a.leaf, "m2l_precision": a.m2l})
     line.update({"roofline": mm["roofline"], "roofline_extra": mm["extra"], "cpu_baseline": cpu,
-                 "stage_ms": mm["stages"], "plan_stats": mm["stats"],
+                 "stage_ms": mm["stages"], "plan_stats": mm["stats"], "autotune": tune,
                  "fmm_vs_fp64_dense_max_rel_err": max(errs),
                  "solve": dict(time_to_solution_s=setup_s + t_solve, setup_s=setup_s,
                                solve_all_rhs_s=t_solve, all_rhs=multi_solve,

@@ -109,22 +109,31 @@ def _synthetic_geometry(n):
     return m.centroids, m.areas
 
 
-def _measure(cfg, C, A, sigma, ref, reps, dev):
+def _measure(cfg, C, A, sigma, ref, reps, dev, geom_offsets=None):
+    """Mean matvec time and relative L2 error against the FP64 dense ``ref``
+    (the worst geometry's when ``geom_offsets`` is given).  ``sigma`` of shape
+    (R, n) times the multi-right-hand-side matvec; row 0 is checked."""
     from .plan import FmmPlan
     plan = FmmPlan(C, A, leaf_size=cfg.leaf_size, order=cfg.native_order, ws=cfg.ws,
-                   max_level=cfg.max_level, m2l_precision=cfg.m2l_precision)
+                   max_level=cfg.max_level, m2l_precision=cfg.m2l_precision,
+                   geom_offsets=geom_offsets)
     out = torch.empty_like(sigma)
+    mv = (lambda: plan.matvec_multi(sigma, out=out)) if sigma.dim() == 2 else \
+        (lambda: plan.matvec(sigma, out=out))
     for _ in range(2):
-        plan.matvec(sigma, out=out)
+        mv()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        plan.matvec(sigma, out=out)
+        mv()
     e1.record()
     torch.cuda.synchronize(dev)
     ms = e0.elapsed_time(e1) / reps
-    err = float(torch.linalg.norm(out - ref) / torch.linalg.norm(ref))
+    o = out[0] if sigma.dim() == 2 else out
+    off = [0, o.shape[0]] if geom_offsets is None else [int(v) for v in geom_offsets]
+    err = max(float(torch.linalg.norm(o[b:e] - ref[b:e]) / torch.linalg.norm(ref[b:e]))
+              for b, e in zip(off[:-1], off[1:]) if e > b)
     st = plan.stats()
     plan.close()
     return dict(ms=ms, rel_err=err, n_leaves=float(st["n_leaves"]),
@@ -135,13 +144,20 @@ def _measure(cfg, C, A, sigma, ref, reps, dev):
 def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: bool = True,
                          target_rel_error: float = 1e-2, max_trials: int = 24, device=None,
                          use_cache: bool = True, centroids=None, areas=None,
-                         reps: int = 5) -> TuningResult:
+                         reps: int = 5, geom_offsets=None, nrhs: int = 1) -> TuningResult:
+    """Picks (order, leaf) with the fastest matvec whose relative L2 error
+    against the FP64 dense operator is at most ``target_rel_error``.  With
+    ``geom_offsets`` the plans are batched over those geometries and the error
+    is the worst geometry's; with ``nrhs`` > 1 the multi-right-hand-side
+    matvec over that many vectors is timed."""
     if problem_size <= 0:
         raise ValueError("problem_size must be positive")
     base_cfg.validate()
     if dry_run:
         return TuningResult(config=base_cfg,
                             metrics={"mode": 0.0, "problem_size": float(problem_size)})
+    # cache records describe one geometry and one right-hand side
+    use_cache = use_cache and geom_offsets is None and nrhs == 1
     nat.require_gpu()
     dev = torch.device(device or "cuda")
     if dev.type != "cuda":
@@ -167,7 +183,16 @@ def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: boo
     rng = np.random.default_rng(0)
     sigma = torch.as_tensor(rng.normal(size=n), dtype=torch.float64, device=dev)
     from ..core.bem_kernel import dense_matvec
-    ref = dense_matvec(sigma, torch.as_tensor(C, device=dev), torch.as_tensor(A, device=dev))
+    Ct, At = torch.as_tensor(C, device=dev), torch.as_tensor(A, device=dev)
+    if geom_offsets is None:
+        ref = dense_matvec(sigma, Ct, At)
+    else:
+        off = [int(v) for v in geom_offsets]
+        ref = torch.cat([dense_matvec(sigma[b:e], Ct[b:e], At[b:e])
+                         for b, e in zip(off[:-1], off[1:])])
+    if nrhs > 1:
+        sigma = torch.cat([sigma[None], torch.as_tensor(rng.normal(size=(nrhs - 1, n)),
+                                                        dtype=torch.float64, device=dev)])
 
     cands: List[FmmConfig] = []
     for p in (o for o in SUPPORTED_ORDERS if o >= 6):
@@ -177,7 +202,7 @@ def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: boo
     trials = []
     for cfg in cands:
         try:
-            m = _measure(cfg, C, A, sigma, ref, reps, dev)
+            m = _measure(cfg, C, A, sigma, ref, reps, dev, geom_offsets)
         except RuntimeError:
             continue
         trials.append((cfg, m))
