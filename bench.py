@@ -519,6 +519,19 @@ def run_torus(a):
     n = mesh.n_panels
     ch = torus_charge(rng)
     sig = torch.as_tensor(rng.normal(size=n), dtype=torch.float64, device="cuda")
+    tune = None
+    if not a.no_autotune:
+        # one (order, leaf) for all four M2L forms: the fastest split-FP16
+        # matvec within the error bound (the 16-bit forms run on the same tree)
+        from electrodrive_amd.fmm3d import FmmConfig, autotune_for_problem
+        base = FmmConfig(expansion_order=a.order, leaf_size=a.leaf,
+                         mac_theta=0.8 if a.ws == 1 else 0.5, m2l_precision="fp16x3")
+        tr, tune_s = timed(lambda: autotune_for_problem(
+            base, n, dry_run=False, target_rel_error=a.tune_target, centroids=mesh.centroids,
+            areas=mesh.areas, use_cache=False))
+        a.order, a.leaf = tr.config.native_order, tr.config.leaf_size
+        tune = dict(order=a.order, leaf=a.leaf, seconds=tune_s, target_rel_error=a.tune_target,
+                    **{k: v for k, v in tr.metrics.items() if k in ("ms", "rel_err", "trials")})
     res = {}
     ref_x = None
     dense = None
@@ -557,12 +570,15 @@ def run_torus(a):
                    kind="port", sample=info["sample"])
     line = base_line(a, head["value"], head["ms"], "bf16", {
         "workload": f"torus R=1 r=0.3 ({n} panels), point charge ~U[-2,2]^3 outside the "
-                    f"surface (seed {a.seed}); FMM p={a.order} ws={a.ws} leaf<={a.leaf}; M2L "
+                    f"surface (seed {a.seed}); FMM p={a.order} ws={a.ws} leaf<={a.leaf}"
+                    + (" (order and leaf from the autotuner at rel err <= "
+                       f"{a.tune_target:g} with split-FP16 M2L)" if tune else "") + "; M2L "
                     f"BF16/FP16 MFMA (FP32 accumulation), FP64 GMRES (SAI) + FP64 refinement",
         "panels": n, "order": a.order, "leaf": a.leaf, "m2l_precision": "bf16"})
     line.update({"roofline": head["roofline"], "roofline_extra": head["extra"],
                  "cpu_baseline": cpu, "stage_ms": head["stages"], "plan_stats": head["stats"],
                  "precisions": res, "time_to_1e-8_s": res["bf16"]["time_to_solution_s"],
+                 "autotune": tune,
                  "timed_wall_s": head["wall"]})
     return line
 
