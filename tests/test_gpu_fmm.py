@@ -176,6 +176,36 @@ def test_autotuner_meets_target_and_caches(tmp_path, monkeypatch):
                                                                 r2.config.leaf_size)
 
 
+def test_autotuner_batched_multi_rhs(tmp_path, monkeypatch):
+    """Batched, multi-RHS tuning: the chosen plan meets the bound on every
+    geometry (checked here against the FP64 dense oracle per geometry), and
+    nothing is written to the one-geometry cache."""
+    from electrodrive_amd.core import bem_mesh
+    from electrodrive_amd.fmm3d import FmmConfig, autotune_for_problem
+    from electrodrive_amd.fmm3d import autotune as at
+    from electrodrive_amd.fmm3d.plan import FmmPlan
+    cache = tmp_path / "tune.json"
+    monkeypatch.setenv(at.AUTOTUNE_CACHE_ENV_VAR, str(cache))
+    ms = [bem_mesh.icosphere(3, 0.8), bem_mesh.torus(1.0, 0.3, 40, 14), bem_mesh.square_plate(1.0, 20)]
+    C = np.concatenate([m.centroids for m in ms])
+    A = np.concatenate([m.areas for m in ms])
+    off = np.concatenate([[0], np.cumsum([m.n_panels for m in ms])])
+    cfg = FmmConfig(expansion_order=8, leaf_size=64, mac_theta=0.8)
+    r = autotune_for_problem(cfg, C.shape[0], dry_run=False, target_rel_error=1e-5,
+                             centroids=C, areas=A, geom_offsets=off, nrhs=3, max_trials=8)
+    print("tuned batch", r.config.expansion_order, r.config.leaf_size, r.metrics)
+    assert r.metrics["met_target"] == 1.0 and r.metrics["rel_err"] <= 1e-5
+    assert not cache.exists()
+    plan = FmmPlan(C, A, leaf_size=r.config.leaf_size, order=r.config.native_order, ws=1,
+                   geom_offsets=off)
+    sig = np.random.default_rng(5).normal(size=C.shape[0])
+    out = plan.matvec(torch.tensor(sig).cuda()).cpu().numpy()
+    for g, m in enumerate(ms):
+        b, e = off[g], off[g + 1]
+        ref = odense.dense_matvec(m.centroids, m.areas, sig[b:e])
+        assert np.linalg.norm(out[b:e] - ref) / np.linalg.norm(ref) <= 2e-5
+
+
 @pytest.mark.parametrize("prec", ["fp16x3", "fp32", "bf16", "valu"])
 def test_matvec_multi_equals_single(prec):
     """ed_fmm_matvec_multi_f64: each right-hand side of a multi-RHS matvec
