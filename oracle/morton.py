@@ -10,8 +10,9 @@ clamped to [0, 1-1e-9] in that dtype (:696-703), floor(x*2^level) clamped to
 When a bounding box (center, half) is given, mins/maxs are center -/+ half
 (geometry.py:261-268), as the reference does for ``global_bbox``.
 
-Parity: pinned by hand-computed keys (tests/test_oracle_morton.py) and by the
-ordering property (keys sort points into nested octants).
+Parity: pinned by hand-computed keys and the FP32 clamp case
+(tests/test_cpu_oracle.py::test_morton_hand_keys) and by the ordering property
+(keys sort points into nested octants).
 """
 import numpy as np
 

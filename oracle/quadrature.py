@@ -12,9 +12,13 @@ Test infrastructure only (see oracle/__init__.py).  numpy, FP64.
 - ``near_weights``                   core/bem.py:558-667
 - ``near_apply``                     core/bem.py:675-734
 
-Parity: the rules are pinned by sum(w) = area, degree-2 exactness and the
-closed-form potential of a flat triangle at an in-plane point (tests); the
-near-pair set by a brute-force double loop; the apply by a dense reference.
+Parity: the order-2 rule's nodes by sum(w) = area, degree-2 exactness and its
+closed-form degree-3 error +A/540 (which the edge-midpoint rule, also exact to
+degree 2, misses by -A/60); the refined self rule by the closed form of its
+self-similar recursion at the centroid; ``near_weights`` by scipy's adaptive
+2-D integral of panel j seen from C_i on panels 22x apart in area (a swap of
+i and j fails by ~99 %); the near-pair set by a brute-force double loop; the
+apply by an explicit scatter (tests/test_cpu_oracle.py).
 """
 import math
 

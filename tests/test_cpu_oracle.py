@@ -120,6 +120,36 @@ def test_standard_rule_exactness():
     assert abs(wr.sum() - 0.5) < 1e-15 and pr.shape == (3, 2)
 
 
+def test_order2_rule_nodes_by_degree3_moments():
+    """Symmetric 3-point rules (a, b, b) exact to degree 2 are exactly two:
+    a = 2/3 (interior, the reference's rule, core/bem_quadrature.py:111-150)
+    and a = 0 (edge midpoints).  Degree-3 moments tell them apart: from the
+    barycentric integral int l1^i l2^j l3^k = 2A i! j! k! / (i+j+k+2)!,
+    int l1^3 = A/10 and int l1 l2 l3 = A/60; the (2/3, 1/6, 1/6) rule gives
+    11A/108 and A/54, i.e. an error of +A/540 on both, where the midpoint rule
+    is off by -A/60 on both."""
+    rng = np.random.default_rng(31)
+    for _ in range(6):
+        v = rng.normal(size=(3, 3))
+        p, w = oq.standard_triangle_quadrature(v, 2)
+        a = 0.5 * np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0]))
+        # barycentric coordinates of the nodes (least squares on the 3x3 system
+        # [v0 v1 v2]^T lam = p with sum(lam) = 1)
+        G = np.vstack([v.T, np.ones(3)])
+        lam = np.linalg.lstsq(G, np.vstack([p.T, np.ones(p.shape[0])]), rcond=None)[0].T
+        assert np.allclose(np.sort(lam, axis=1), [[1 / 6, 1 / 6, 2 / 3]] * 3, atol=1e-12)
+        for k in range(3):
+            e3 = w @ lam[:, k] ** 3 - a / 10.0
+            assert abs(e3 - a / 540.0) < 1e-12 * a, (e3, a / 540.0)
+        e111 = w @ (lam[:, 0] * lam[:, 1] * lam[:, 2]) - a / 60.0
+        assert abs(e111 - a / 540.0) < 1e-12 * a
+    # the edge-midpoint rule passes the degree-2 checks but fails these
+    mid = np.array([[0.5, 0.5, 0.0], [0.0, 0.5, 0.5], [0.5, 0.0, 0.5]])
+    wm = np.full(3, 1.0 / 3.0)
+    assert abs(wm @ mid[:, 0] ** 2 - 2 * 2 / 24) < 1e-15  # degree 2 exact (A = 1)
+    assert abs((wm @ mid[:, 0] ** 3 - 0.1) - 1.0 / 540.0) > 1e-2
+
+
 def test_near_singular_rule_self_similarity():
     """At the centroid the reference refinement always splits the central
     sub-triangle, which is the panel scaled by -1/2 about the centroid; the
@@ -198,6 +228,38 @@ def test_near_weights_limits():
     for (i, j), w in zip(pairs, w0):
         ref[i] += w
     assert np.allclose(V, ref)
+
+
+def test_near_weights_orientation_vs_exact_integral():
+    """delta_ij is panel j's integral seen from the centroid of panel i minus
+    its lumped term (core/bem.py:558-667), pinned against scipy's adaptive
+    2-D quadrature of that integral at depth 5.  The panels differ 22x in
+    area, so swapping the roles of i and j (panel i seen from C_j) moves the
+    value by ~99 % and fails."""
+    from scipy.integrate import dblquad
+    vj = np.array([[0.0, 0, 0], [2.0, 0, 0], [0.0, 2.0, 0]])
+    vi = np.array([[0.55, 0.45, 0.35], [0.95, 0.55, 0.35], [0.6, 0.9, 0.45]])
+
+    def area(v):
+        return 0.5 * np.linalg.norm(np.cross(v[1] - v[0], v[2] - v[0]))
+
+    def panel_integral(v, P):
+        e1, e2 = v[1] - v[0], v[2] - v[0]
+        val = dblquad(lambda t, u: 1.0 / np.linalg.norm(v[0] + u * e1 + t * e2 - P),
+                      0, 1, lambda u: 0, lambda u: 1 - u, epsabs=1e-13, epsrel=1e-12)[0]
+        return 2 * area(v) * val
+
+    Ai, Aj = area(vi), area(vj)
+    assert Aj / Ai > 4
+    Ci, Cj = vi.mean(0), vj.mean(0)
+    d = np.linalg.norm(Ci - Cj)
+    exact_ij = K_E * (panel_integral(vj, Ci) - Aj / d)
+    exact_ji = K_E * (panel_integral(vi, Cj) - Ai / d)
+    w = oq.near_weights(np.stack([Ci, Cj]), np.array([Ai, Aj]), np.stack([vi, vj]),
+                        np.array([[0, 1], [1, 0]]), 2, 5)
+    assert abs(w[0] - exact_ij) <= 1e-3 * abs(exact_ij)
+    assert abs(w[1] - exact_ji) <= 1e-3 * abs(exact_ji)
+    assert abs(exact_ji - exact_ij) > 0.1 * abs(exact_ij)  # a swap would fail
 
 
 # -------------------------------------------------------------------- gmres
