@@ -94,7 +94,6 @@ __global__ __launch_bounds__(kSaiThreads) void k_sai_build(
   __shared__ double dsh[kSaiCand];
   __shared__ int csh[kSaiCand];
   __shared__ int hk[kSaiHash], hv[kSaiHash];
-  __shared__ int pre[kSaiJ + 1];
   __shared__ int jc[kSaiCand];  // pattern candidates, ascending
   __shared__ int wcnt[kSaiThreads / 64];
   __shared__ int nJ, nI;
@@ -132,7 +131,11 @@ __global__ __launch_bounds__(kSaiThreads) void k_sai_build(
     if (tid == 0) nJ = base;
     __syncthreads();
   }
-  sai_select(i, jc, 0, std::min(nJ, kSaiCand), C, kSaiJ, Jl, &nJ, dsh, csh);
+  // every thread reads the candidate count before sai_select's thread 0
+  // overwrites nJ with the pattern size
+  const int n_cand = std::min(nJ, kSaiCand);
+  __syncthreads();
+  sai_select(i, jc, 0, n_cand, C, kSaiJ, Jl, &nJ, dsh, csh);
   sai_select(i, icol, iptr[loc], iptr[loc + 1], C, kSaiI, Il, &nI, dsh, csh);
   const int nj = nJ, ni = nI;
   // B[l][k] = A[J_k][I_l] (zero for k >= nj)
@@ -151,14 +154,11 @@ __global__ __launch_bounds__(kSaiThreads) void k_sai_build(
     B[l][k] = v;
   }
   // near-field corrections A[J_k][col] += w for the near entries of the rows
-  // J_k: the fitting rows go into an LDS hash (panel -> position), and the
-  // near entries of all pattern rows are spread over the threads
+  // J_k: the fitting rows go into an LDS hash (panel -> position); thread k
+  // adds the whole near row of J_k into column k of B, so entries that share
+  // a (row, column) (duplicate pairs) are added by one thread in CSR order
   if (near_ptr) {
     for (int e = tid; e < kSaiHash; e += blockDim.x) hk[e] = -1;
-    if (tid == 0) {
-      pre[0] = 0;
-      for (int k = 0; k < nj; ++k) pre[k + 1] = pre[k] + (near_ptr[Jl[k] + 1] - near_ptr[Jl[k]]);
-    }
     __syncthreads();
     for (int l = tid; l < ni; l += blockDim.x) {
       const int key = Il[l];
@@ -170,19 +170,14 @@ __global__ __launch_bounds__(kSaiThreads) void k_sai_build(
       }
     }
     __syncthreads();
-    const int tot = pre[nj];
-    for (int e = tid; e < tot; e += blockDim.x) {
-      int lo = 0, hi = nj - 1;  // last k with pre[k] <= e
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (pre[mid] <= e) lo = mid; else hi = mid - 1;
+    for (int k = tid; k < nj; k += blockDim.x) {
+      const int q1 = near_ptr[Jl[k] + 1];
+      for (int q = near_ptr[Jl[k]]; q < q1; ++q) {
+        const int b = near_col[q];
+        int sl = sai_hash(b);
+        while (hk[sl] != -1 && hk[sl] != b) sl = (sl + 1) & (kSaiHash - 1);
+        if (hk[sl] == b) B[hv[sl]][k] += near_w[q];
       }
-      const int k = lo;
-      const int q = near_ptr[Jl[k]] + (e - pre[k]);
-      const int b = near_col[q];
-      int sl = sai_hash(b);
-      while (hk[sl] != -1 && hk[sl] != b) sl = (sl + 1) & (kSaiHash - 1);
-      if (hk[sl] == b) B[hv[sl]][k] += near_w[q];
     }
   }
   __syncthreads();

@@ -2,9 +2,9 @@
 
 Oracle: oracle.dense.dense_matvec (restatement of core/bem_kernel.py:157-230,
 FP64).  Tolerances: FP64 dense kernel 1e-12 relative (max-norm); FP32 dense
-kernel 1e-5; FMM matvec (FP32, p = 8 or 10, ws = 1) relative L2 error against
-the FP64 dense result 1e-5 for the p/ws pairs the solver uses, reported for
-the others.
+kernel 1e-5; FMM matvec relative L2 error against the FP64 dense result
+1e-5 for the (p, ws) pairs the solver uses (p = 10 / ws = 1, p = 8 / ws = 2),
+looser bounds for the others.
 """
 import numpy as np
 import pytest
@@ -81,7 +81,9 @@ def test_fmm_matvec_vs_dense(s, order, ws):
     plan_v = FmmPlan(m.centroids, m.areas, leaf_size=64, order=order, ws=ws, m2l_precision="valu")
     out_v = plan_v.matvec(torch.tensor(sig).cuda(), torch.tensor(si).cuda()).cpu().numpy()
     assert _rel(out_v, out) < 2e-6
-    bound = {(8, 1): 2e-4, (10, 1): 5e-5, (6, 2): 1e-4, (8, 2): 2e-5}[(order, ws)]
+    # 1e-5 for the (p, ws) pairs the solver runs (p = 10, ws = 1 from the
+    # autotuner; p = 8, ws = 2 for the reference's theta = 0.5)
+    bound = {(8, 1): 2e-4, (10, 1): 1e-5, (6, 2): 1e-4, (8, 2): 1e-5}[(order, ws)]
     assert err < bound
 
 
@@ -149,7 +151,7 @@ def test_m2l_precisions(prec, bound, order):
     if prec in ("fp32", "fp16x3"):
         pv = FmmPlan(m.centroids, m.areas, leaf_size=32, order=order, ws=1, m2l_precision="valu")
         assert _rel(pv.matvec(torch.tensor(sig).cuda()).cpu().numpy(), out) < 2e-6
-        bound = 1e-4 if order == 8 else 2e-5
+        bound = 1e-4 if order == 8 else 1e-5
     assert err < bound
 
 
