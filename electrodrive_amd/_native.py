@@ -62,6 +62,7 @@ _SIGS = {
     "ed_fmm_profile": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
     "ed_dense_matvec_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     "ed_dense_matvec_f32": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
+    "ed_dense_matvec_f64_noalloc": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     "ed_self_integrals_f64": (c_i32, [c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),
     "ed_near_pairs_f64": (c_i32, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_i64p, c_vp]),
     "ed_near_weights_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),

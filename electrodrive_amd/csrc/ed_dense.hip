@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 #include <type_traits>
@@ -75,15 +76,12 @@ __device__ __forceinline__ void dense_tile(const T (&xi)[kDenseRows], const T (&
 // launch covers every geometry of a batched plan.
 // ---------------------------------------------------------------------------
 template <typename T, int NR>
-__global__ __launch_bounds__(kDenseBlock) void k_dense_part(const DenseItem* __restrict__ items,
-                                                            const T* __restrict__ C,
-                                                            const T* __restrict__ A,
-                                                            const T* __restrict__ sigma,
-                                                            int64_t n, int64_t sig_ld, int r0,
-                                                            T* __restrict__ partial) {
+__device__ __forceinline__ void dense_part_item(const DenseItem& it, const T* __restrict__ C,
+                                                const T* __restrict__ A,
+                                                const T* __restrict__ sigma, int64_t n,
+                                                int64_t sig_ld, int r0, T* __restrict__ partial) {
   __shared__ T sx[kDenseBlock], sy[kDenseBlock], sz[kDenseBlock];
   __shared__ T sq[kDenseRhs][kDenseBlock];
-  const DenseItem it = items[blockIdx.x];
   const int64_t rend = it.row0 + it.nrows;
   int64_t ii[kDenseRows];
   T xi[kDenseRows], yi[kDenseRows], zi[kDenseRows];
@@ -132,6 +130,36 @@ __global__ __launch_bounds__(kDenseBlock) void k_dense_part(const DenseItem* __r
 #pragma unroll
     for (int r = 0; r < NR; ++r) partial[((int64_t)it.split * NR + r) * n + ii[u]] = acc[u][r];
   }
+}
+
+template <typename T, int NR>
+__global__ __launch_bounds__(kDenseBlock) void k_dense_part(const DenseItem* __restrict__ items,
+                                                            const T* __restrict__ C,
+                                                            const T* __restrict__ A,
+                                                            const T* __restrict__ sigma,
+                                                            int64_t n, int64_t sig_ld, int r0,
+                                                            T* __restrict__ partial) {
+  dense_part_item<T, NR>(items[blockIdx.x], C, A, sigma, n, sig_ld, r0, partial);
+}
+
+// The same work items of one geometry [0, n) computed from the block index
+// (item k = row block k / S, split k % S, as dense_schedule orders them):
+// no schedule in device memory.
+template <typename T, int NR>
+__global__ __launch_bounds__(kDenseBlock) void k_dense_part_grid(int split, int64_t chunk,
+                                                                 const T* __restrict__ C,
+                                                                 const T* __restrict__ A,
+                                                                 const T* __restrict__ sigma,
+                                                                 int64_t n, int64_t sig_ld,
+                                                                 int r0, T* __restrict__ partial) {
+  const int64_t per_block = (int64_t)kDenseBlock * kDenseRows;
+  DenseItem it;
+  it.row0 = (int64_t)(blockIdx.x / split) * per_block;
+  it.nrows = (int32_t)(n - it.row0 < per_block ? n - it.row0 : per_block);
+  it.split = (int32_t)(blockIdx.x % split);
+  it.src0 = (int64_t)it.split * chunk < n ? (int64_t)it.split * chunk : n;
+  it.src1 = (int64_t)(it.split + 1) * chunk < n ? (int64_t)(it.split + 1) * chunk : n;
+  dense_part_item<T, NR>(it, C, A, sigma, n, sig_ld, r0, partial);
 }
 
 // ---------------------------------------------------------------------------
@@ -340,15 +368,16 @@ int dense_sym_run(const DenseSymDevice& D, const double* C, const double* A, con
   return ED_OK;
 }
 
+// row_nsplit NULL: every row has `split` partials (one geometry)
 template <typename T, int NR>
-__global__ void k_dense_finish(int64_t n, const int32_t* __restrict__ row_nsplit,
+__global__ void k_dense_finish(int64_t n, const int32_t* __restrict__ row_nsplit, int split,
                                const T* __restrict__ partial, const T* __restrict__ sigma,
                                int64_t sig_ld, const T* __restrict__ selfi,
                                const T* __restrict__ A, int r0, T ke, T* __restrict__ out,
                                int64_t out_ld) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int S = row_nsplit[i];
+  const int S = row_nsplit ? row_nsplit[i] : split;
   const T diag = selfi ? selfi[i] * A[i] : (T)0;
 #pragma unroll
   for (int r = 0; r < NR; ++r) {
@@ -400,7 +429,7 @@ int dense_run(const DenseItem* items, int64_t n_items, const int32_t* row_nsplit
   case NRV:                                                                                     \
     hipLaunchKernelGGL((k_dense_part<T, NRV>), dim3((unsigned)n_items), dim3(kDenseBlock), 0,  \
                        st, items, C, A, sigma, n, n, (int)r0, partial);                        \
-    hipLaunchKernelGGL((k_dense_finish<T, NRV>), dim3(gn), dim3(256), 0, st, n, row_nsplit,    \
+    hipLaunchKernelGGL((k_dense_finish<T, NRV>), dim3(gn), dim3(256), 0, st, n, row_nsplit, 1, \
                        (const T*)partial, sigma, n, selfi, A, (int)r0, (T)kKE, out, n);        \
     break;
     switch (nr) {
@@ -482,6 +511,88 @@ int dense_launch_f64(const double* C, const double* A, const double* sigma, cons
 }
 
 }  // namespace ed
+
+// ---------------------------------------------------------------------------
+// No-allocation dense matvec (ed_dense_matvec_f64_noalloc): the split form's
+// partial sums live in `scratch`, one copy of which each GPU holds until the
+// process ends; the work items come from the block index.  Host code never
+// passes `scratch` itself (a host-side stand-in): the device address of each
+// GPU's copy is looked up once per device with that device current.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int64_t kNoAllocDoubles = int64_t(1) << 21;  // 16 MiB per GPU
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+}  // namespace
+
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static int noalloc_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return ed::fail(ED_ERR_DEVICE, std::string("dense noalloc: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) return ed::fail(ED_ERR_DEVICE, "dense noalloc: device index");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return ed::fail(ED_ERR_DEVICE, std::string("dense noalloc: scratch address: ") + hipGetErrorString(e));
+    g_noalloc_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return ED_OK;
+}
+
+extern "C" int ed_dense_matvec_f64_noalloc(const double* C, const double* A, const double* sigma,
+                                           const double* selfi, double* out, int64_t n,
+                                           int64_t nrhs, ed_stream stream) {
+  using namespace ed;
+  if (n < 0 || nrhs < 1) return fail(ED_ERR_ARG, "dense noalloc: bad sizes");
+  if (n == 0) return ED_OK;
+  if (!C || !A || !sigma || !out) return fail(ED_ERR_ARG, "dense noalloc: NULL pointer");
+  // the split of dense_schedule for one geometry
+  const int S = (int)std::max<int64_t>(1, std::min<int64_t>(kDenseMaxSplit, (n + 2047) / 2048));
+  const int64_t chunk = (n + S - 1) / S;
+  if ((int64_t)S * n > kNoAllocDoubles)
+    return fail(ED_ERR_ARG, "dense noalloc: n = " + std::to_string(n) + " exceeds the scratch limit "
+                            "(split x n <= 2^21 doubles: n <= 131072)");
+  const int64_t width = std::min<int64_t>(std::min<int64_t>(nrhs, kDenseRhs),
+                                          kNoAllocDoubles / ((int64_t)S * n));
+  double* part = nullptr;
+  int rc = noalloc_scratch(&part);
+  if (rc != ED_OK) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t per_block = (int64_t)kDenseBlock * kDenseRows;
+  const unsigned nb = (unsigned)(((n + per_block - 1) / per_block) * S);
+  const unsigned gn = (unsigned)((n + 255) / 256);
+  for (int64_t r0 = 0; r0 < nrhs; r0 += width) {
+    const int nr = (int)std::min<int64_t>(width, nrhs - r0);
+#define ED_NOALLOC_NR(NRV)                                                                     \
+  case NRV:                                                                                     \
+    hipLaunchKernelGGL((k_dense_part_grid<double, NRV>), dim3(nb), dim3(kDenseBlock), 0, st, S, \
+                       chunk, C, A, sigma, n, n, (int)r0, part);                                \
+    hipLaunchKernelGGL((k_dense_finish<double, NRV>), dim3(gn), dim3(256), 0, st, n,            \
+                       (const int32_t*)nullptr, S, (const double*)part, sigma, n, selfi, A,     \
+                       (int)r0, (double)kKE, out, n);                                           \
+    break;
+    switch (nr) {
+      ED_NOALLOC_NR(1)
+      ED_NOALLOC_NR(2)
+      ED_NOALLOC_NR(3)
+      ED_NOALLOC_NR(4)
+      ED_NOALLOC_NR(5)
+      ED_NOALLOC_NR(6)
+      ED_NOALLOC_NR(7)
+      default: ED_NOALLOC_NR(8)
+    }
+#undef ED_NOALLOC_NR
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(ED_ERR_LAUNCH, std::string("dense noalloc: ") + hipGetErrorString(e));
+  }
+  return ED_OK;
+}
 
 extern "C" int ed_dense_matvec_f64(const double* C, const double* A, const double* sigma,
                                    const double* selfi, double* out, int64_t n, int64_t nrhs,

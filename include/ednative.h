@@ -141,6 +141,19 @@ int ed_dense_matvec_f64(const double* centroids, const double* areas,
 int ed_dense_matvec_f32(const float* centroids, const float* areas,
                         const float* sigma, const float* self_integrals,
                         float* out, int64_t n, int64_t nrhs, ed_stream stream);
+/* No reference counterpart: ed_dense_matvec_f64 for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same inputs and outputs (one geometry,
+ * nrhs >= 1), the split form's kernels; it allocates nothing, copies nothing
+ * and does not synchronise the stream.  Its partial sums use a fixed 16 MiB
+ * device array that each GPU the library runs on holds until the process
+ * ends: n is limited to 131072 panels (split x n <= 2^21 doubles; larger n ->
+ * ED_ERR_ARG), right-hand sides run in passes that fit.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU (other streams,
+ * other threads) race and must be serialised by the caller. */
+int ed_dense_matvec_f64_noalloc(const double* centroids, const double* areas,
+                                const double* sigma, const double* self_integrals,
+                                double* out, int64_t n, int64_t nrhs, ed_stream stream);
 
 /* ------------------------------------------------------------------------ */
 /* Quadrature                                                                */
