@@ -6,7 +6,8 @@ runs in hand-written HIP kernels (``libednative.so``, C ABI in
 include/ednative.h).  Python keeps the reference call signatures:
 
 - ``electrodrive_amd.fmm3d``: FmmConfig, LaplaceFmm3D, make_laplace_fmm_backend,
-  morton_keys_from_points
+  morton_keys_from_points, BoundingBox, build_fmm_tree / FmmTree (the
+  reference's adaptive octree), build_interaction_lists / InteractionLists
 - ``electrodrive_amd.core.bem_kernel``: bem_matvec_gpu, bem_matvec_batched
 - ``electrodrive_amd.core.bem_solver``: gmres_restart, setup_preconditioner
 - ``electrodrive_amd.core.bem_quadrature``: self integrals, near pairs,

@@ -106,6 +106,14 @@ _SIGS = {
                                    c_vp, c_vp, c_vp, c_vp]),
     "ed_host_operator": (c_i32, [c_i32, c_i32, c_i32, c_i32, c_i32, c_i32, c_vp]),
     "ed_host_morton_keys_f64": (c_i32, [c_vp, c_i64, c_i32, c_vp, c_vp, c_vp]),
+    "ed_ref_tree_create": (c_i32, [c_vp, c_i32, c_i64, c_i32, c_i32, ctypes.POINTER(c_vp)]),
+    "ed_ref_tree_sizes": (c_i32, [c_vp, c_vp]),
+    "ed_ref_tree_export": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp]),
+    "ed_ref_tree_destroy": (c_i32, [c_vp]),
+    "ed_ref_lists_create": (c_i32, [c_vp, c_vp, c_dbl, c_i64, c_i32, ctypes.POINTER(c_vp)]),
+    "ed_ref_lists_sizes": (c_i32, [c_vp, c_vp]),
+    "ed_ref_lists_export": (c_i32, [c_vp, c_vp, c_vp]),
+    "ed_ref_lists_destroy": (c_i32, [c_vp]),
 }
 
 _lib = None

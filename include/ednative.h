@@ -370,6 +370,53 @@ int ed_proj_loss_f32(const float* Y, int32_t ky, const float* X, int32_t kx, con
                      ed_stream stream);
 
 /* ------------------------------------------------------------------------ */
+/* Reference-mode octree and interaction lists (host; bit-exact indexing).   */
+/* The FMM runs on the lattice octree of ed_plan_create; these reproduce the */
+/* reference's own adaptive tree and pair lists for callers of its tree and  */
+/* list API.  Points are host arrays, FP64 (is_f32 = 0) or FP32.             */
+/* ------------------------------------------------------------------------ */
+typedef struct ed_ref_tree ed_ref_tree;
+typedef struct ed_ref_lists ed_ref_lists;
+
+/* Replaces build_fmm_tree / _build_adaptive_octree (fmm3d/tree.py:801-1050):
+ * tight per-node boxes, split at the box centre with a strict '>', octant
+ * x + 2y + 4z, stable partition, node ids in depth-first pre-order, leaf iff
+ * n <= leaf_size, level >= max_depth or max half extent < 1e-7.  points (n,3)
+ * must be finite; leaf_size > 0; 1 <= max_depth <= 64 (tree.py:1018-1031). */
+int ed_ref_tree_create(const void* points, int32_t is_f32, int64_t n, int32_t leaf_size,
+                       int32_t max_depth, ed_ref_tree** out);
+/* out[0..3] = n_points, n_nodes, n_leaves, deepest level (-1 when empty). */
+int ed_ref_tree_sizes(const ed_ref_tree* tree, int64_t* out);
+/* The FmmTree SoA (fmm3d/tree.py:286-339); any pointer may be NULL:
+ * index_map (n) tree position -> original index; centers, half_extents
+ * (n_nodes,3) and radii (n_nodes) in the points' dtype; levels (n_nodes)
+ * int32; parents (n_nodes, -1 for the root); children (n_nodes,8), -1 for a
+ * missing octant; ranges (n_nodes,2) [start, end) in tree order; is_leaf
+ * (n_nodes) bytes. */
+int ed_ref_tree_export(const ed_ref_tree* tree, int64_t* index_map, void* centers,
+                       void* half_extents, void* radii, int32_t* levels, int64_t* parents,
+                       int64_t* children, int64_t* ranges, uint8_t* is_leaf);
+int ed_ref_tree_destroy(ed_ref_tree* tree);
+
+/* Replaces build_interaction_lists / _build_interaction_lists
+ * (fmm3d/interaction_lists.py:374-577): dual-tree traversal with the MAC
+ * max(2 r_s, 2 r_t) / dist < theta (:273-297) and the refinement rule of
+ * :306-337.  target NULL (or == source) is the one-tree case (a node with
+ * itself is never admissible).  0 < theta < 1.  max_pairs < 0: no cap;
+ * otherwise the traversal stops pushing once that many pairs were emitted and
+ * reports truncated = 1 (the reference then raises).  sort_pairs != 0 sorts
+ * pairs by (tgt, src), else by (src, tgt). */
+int ed_ref_lists_create(const ed_ref_tree* source, const ed_ref_tree* target, double theta,
+                        int64_t max_pairs, int32_t sort_pairs, ed_ref_lists** out);
+/* out[0..7] = n_p2p_pairs, n_m2l_pairs, mac_accept, mac_reject, leaf_leaf,
+ * stack_pops, max_stack_size, truncated (InteractionStats,
+ * interaction_lists.py:59-88). */
+int ed_ref_lists_sizes(const ed_ref_lists* lists, int64_t* out);
+/* Pairs as (src, tgt) rows: p2p (n_p2p,2), m2l (n_m2l,2); NULL skips. */
+int ed_ref_lists_export(const ed_ref_lists* lists, int64_t* p2p_pairs, int64_t* m2l_pairs);
+int ed_ref_lists_destroy(ed_ref_lists* lists);
+
+/* ------------------------------------------------------------------------ */
 /* Host-only helpers (no device needed): expose the plan builder's octree,   */
 /* lists and operator tables so they can be checked on a CPU-only machine.   */
 /* ------------------------------------------------------------------------ */

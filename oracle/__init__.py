@@ -24,6 +24,9 @@ to the Electrodrive repository root) in plain numpy (FP64) or C.  Modules:
                   summation, not against a reference file)
 - ``lattice_tree`` restatement of the native lattice octree and its U/V lists,
                   with a brute-force partition check
+- ``ref_tree``    the reference's adaptive tight-bbox octree and dual-tree MAC
+                  interaction lists (fmm3d/tree.py:801-960,
+                  fmm3d/interaction_lists.py:273-528)
 - ``mlp``         FP64 forward/backward of the learned-preconditioner MLP and
                   its projection loss (no reference counterpart)
 - ``sai``         sparse approximate inverse smoother (no reference counterpart)
