@@ -98,6 +98,8 @@ def stage_roofline(name, ms, st, n, order, m2l_peak=MFMA_FP32_PEAK_TFLOPS):
         return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     algorithmic=byts)
     if name == "m2l_reduce":
+        # the GEMM row of every M2L pair and its index, the node's own part
+        # written
         byts = 4.0 * 96 * st["n_m2l_pairs"] + 8.0 * st["n_m2l_pairs"] + 4.0 * nc * st["n_nodes"]
         return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     algorithmic=byts)
@@ -106,21 +108,49 @@ def stage_roofline(name, ms, st, n, order, m2l_peak=MFMA_FP32_PEAK_TFLOPS):
                 algorithmic=byts)
 
 
-def cpu_baseline(mesh, sigma_np, self_corr_np, budget_s):
+def host_cpu_info():
+    """lscpu model, sockets, cores and threads of this host (the cores the CPU
+    baseline runs on)."""
+    import subprocess
+    info = {"usable_cpus": len(os.sched_getaffinity(0))}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+    except Exception:
+        out = ""
+    keys = {"Model name": "model", "Socket(s)": "sockets", "Core(s) per socket": "cores_per_socket",
+            "Thread(s) per core": "threads_per_core", "CPU(s)": "cpus", "NUMA node(s)": "numa_nodes"}
+    for ln in out.splitlines():
+        k, _, v = ln.partition(":")
+        if k.strip() in keys:
+            info[keys[k.strip()]] = v.strip()
+    return info
+
+
+def cpu_baseline(mesh, sigma_np, self_corr_np, budget_s, repeats=3):
+    """The oracle's C port of the dense FP64 reference matvec on this host's
+    cores: a 64-row warm-up (which also sizes the sample), then ``repeats``
+    timed passes over the same bounded row sample; value = median rate."""
+    import statistics
     from oracle import cdense  # CPU baseline only
     n = mesh.n_panels
     threads = cdense.max_threads()
     t0 = time.perf_counter()
     cdense.dense_rows(mesh.centroids, mesh.areas, sigma_np, self_corr_np, 0, 64)
     t_probe = max(time.perf_counter() - t0, 1e-6)
-    rows = int(min(n, max(64, 64 * budget_s / t_probe)))
-    t0 = time.perf_counter()
-    cdense.dense_rows(mesh.centroids, mesh.areas, sigma_np, self_corr_np, 0, rows)
-    t = time.perf_counter() - t0
+    rows = int(min(n, max(64, 64 * budget_s / repeats / t_probe)))
+    times = []
+    for _ in range(repeats):
+        t0 = time.perf_counter()
+        cdense.dense_rows(mesh.centroids, mesh.areas, sigma_np, self_corr_np, 0, rows)
+        times.append(time.perf_counter() - t0)
+    t = statistics.median(times)
+    host = host_cpu_info()
     return dict(value=rows / t, unit="panels/s", cores=threads, kind="port",
                 sample=f"{rows} of {n} rows of the FP64 dense matvec (oracle/csrc/dense_ref.c, "
-                       f"OpenMP {threads} threads), {t:.1f} s; full matvec extrapolated "
-                       f"{n / (rows / t):.1f} s"), n / (rows / t)
+                       f"OpenMP {threads} threads), median of {repeats} timed passes after a "
+                       f"64-row warm-up ({', '.join(f'{x:.2f}' for x in times)} s); full matvec "
+                       f"extrapolated {n / (rows / t):.1f} s",
+                host=host, repeats_s=times), n / (rows / t)
 
 
 def timed(fn):
@@ -172,7 +202,11 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
         ms_m = e2.elapsed_time(e3) / a.steps
         multi = dict(ms=ms_m, value=n_units / (ms_m * 1e-3), wall=time.perf_counter() - t1)
     stages = op.plan.profile(sig[0], op.self_corr, reps=max(5, a.steps // 2))
-    dom = max(stages, key=stages.get)
+    # P2P runs on a side stream beside the far-field chain, so the step is set
+    # by the chain (gather .. l2p_out); the roofline is that of the chain's
+    # largest kernel, the M2L GEMM (the upward stage is three launches)
+    chain = [k for k in stages if k != "p2p"]
+    dom = "m2l" if "m2l" in stages else max(chain, key=stages.get)
     # 16-bit MFMA peak for the 16-bit forms; the split form executes three
     # FP16 products per FP32-equivalent one
     prec = op.plan.m2l_precision
@@ -199,8 +233,52 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
                           frac_valu=p2p["valu_tflops"] / VALU_FP32_PEAK_TFLOPS,
                           valu_busy_pmc=P2P_VALU_BUSY.get(key),
                           launch_ms=stages["p2p"]))
+    extra["critical_path"] = dict(chain_stages_ms=sum(stages[k] for k in chain),
+                                  p2p_side_stream_ms=stages["p2p"], ms_per_matvec=ms,
+                                  note="profile-mode stage times include their event pairs")
     return dict(ms=ms, wall=wall, stages=stages, roofline=roofline, extra=extra, stats=st,
                 value=n_units / (ms * 1e-3), multi=multi)
+
+
+# arithmetic of the matvec: FP32 P2P / expansions / quadrature apply, M2L on
+# MFMA in the named form (FP32 accumulation), FP64 Krylov and refinement
+DTYPE_LABEL = {"fp16x3": "fp32 (M2L split-fp16 MFMA)", "fp32": "fp32",
+               "bf16": "fp32 (M2L bf16 MFMA)", "fp16": "fp32 (M2L fp16 MFMA)",
+               "valu": "fp32"}
+
+
+def matvec_accuracy(mesh, op, a, rng):
+    """Relative L2 error of the FMM matvec (no near correction) against the FP64
+    dense operator for the configuration measured and for the one the workload
+    names (p = 8, leaf <= 64, theta = 0.5 -> ws = 2), with that one's time."""
+    import numpy as np
+    import torch
+    from electrodrive_amd.core.bem_kernel import dense_matvec
+    from electrodrive_amd.fmm3d.plan import FmmPlan
+    sig = torch.as_tensor(np.random.default_rng(a.seed + 1).normal(size=op.n),
+                          dtype=torch.float64, device=op.device)
+    ref = dense_matvec(sig, op.C, op.A, op.self_corr)
+    out = {}
+    for name, (p, ws, leaf) in (("measured", (a.order, a.ws, a.leaf)),
+                                ("workload_p8_leaf64_theta0.5", (8, 2, 64))):
+        plan = FmmPlan(mesh.centroids, mesh.areas, leaf_size=leaf, order=p, ws=ws,
+                       m2l_precision=a.m2l)
+        v = plan.matvec(sig, op.self_corr)
+        y = torch.empty_like(v)
+        for _ in range(3):
+            plan.matvec(sig, op.self_corr, out=y)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(20):
+            plan.matvec(sig, op.self_corr, out=y)
+        e1.record()
+        torch.cuda.synchronize()
+        out[name] = dict(order=p, ws=ws, leaf=leaf, m2l=a.m2l,
+                         rel_err_vs_fp64_dense=float(torch.linalg.norm(v - ref)
+                                                     / torch.linalg.norm(ref)),
+                         ms_per_matvec=e0.elapsed_time(e1) / 20)
+        del plan
+    return out
 
 
 def base_line(a, value, ms, dtype, config):
@@ -322,6 +400,7 @@ def run_sphere(a):
 
     mm = matvec_measure(op, sigma, a, n)
     ms, stages, st = mm["ms"], mm["stages"], mm["stats"]
+    accuracy = matvec_accuracy(mesh, op, a, rng)
 
     cpu = None
     cpu_full = None
@@ -336,7 +415,7 @@ def run_sphere(a):
             cpu["time_to_solution_estimate"] = (f"{its} Jacobi GMRES iterations x the full "
                                                 f"dense matvec time")
 
-    line = base_line(a, mm["value"], ms, "fp32", {
+    line = base_line(a, mm["value"], ms, DTYPE_LABEL.get(a.m2l, "fp32"), {
         "workload": f"grounded icosphere s={a.subdiv} ({n} panels), point charge q=1 "
                     f"at rho={rho:.4f} (seed {a.seed}); FMM p={a.order} ws={a.ws} "
                     f"leaf<={a.leaf} M2L {a.m2l} (order and leaf from the autotuner at "
@@ -345,9 +424,12 @@ def run_sphere(a):
         "m2l_precision": a.m2l, "krylov": "fp64 GMRES(50) + fp64 refinement"})
     line.update({"roofline": mm["roofline"], "roofline_extra": mm["extra"], "cpu_baseline": cpu,
                  "stage_ms": stages, "plan_stats": st, "solve": solve, "autotune": tune,
-                 "timed_wall_s": mm["wall"]})
+                 "matvec_accuracy": accuracy, "timed_wall_s": mm["wall"]})
     if solve is not None:
+        # tuning is paid once per size bucket (cached on disk); both figures
         line["time_to_1e-8_s"] = solve["time_to_solution_s"]
+        if tune is not None:
+            line["time_to_1e-8_incl_autotune_s"] = solve["time_to_solution_s"] + tune["seconds"]
         if "learned_vs_plain" in solve:
             line["gmres_iters_plain_vs_learned"] = [
                 solve["learned_vs_plain"]["plain_jacobi"]["iters"],
@@ -467,6 +549,7 @@ def run_batch(a):
         rate, info = cpu_pair_rate(meshes[int(np.argmax(sizes))], a.cpu_seconds)
         pairs = float((sizes.astype(np.float64) ** 2).sum())
         cpu = dict(value=N / (pairs / rate), unit="panels/s", cores=info["cores"], kind="port",
+                   host=info["host"],
                    sample=f"dense FP64 pair rate {rate:.3e}/s of the oracle's C port measured on "
                           f"the largest geometry ({info['sample']}); batch matvec = sum N_g^2 "
                           f"pairs")
@@ -474,7 +557,7 @@ def run_batch(a):
     # multi-RHS matvec); the R one-RHS matvecs in a row are reported beside it
     mv = mm["multi"] if mm["multi"] is not None else dict(ms=mm["ms"], value=mm["value"],
                                                           wall=mm["wall"])
-    line = base_line(a, mv["value"], mv["ms"], "fp32", {
+    line = base_line(a, mv["value"], mv["ms"], DTYPE_LABEL.get(a.m2l, "fp32"), {
         "workload": f"uneven batch: {a.n_geoms} seeded geometries (spheres, tori, sphere dimers, "
                     f"plates; {int(sizes.min())}-{int(sizes.max())} panels, {N} total) x {a.rhs} "
                     f"point-charge RHS; one batched launch per FMM stage covering all RHS "
@@ -491,7 +574,10 @@ def run_batch(a):
                  "solve": dict(time_to_solution_s=setup_s + t_solve, setup_s=setup_s,
                                solve_all_rhs_s=t_solve, all_rhs=multi_solve,
                                one_rhs_at_a_time_s=t_one, per_rhs=solves),
-                 "time_to_1e-8_s": setup_s + t_solve, "timed_wall_s": mv["wall"],
+                 "time_to_1e-8_s": setup_s + t_solve,
+                 "time_to_1e-8_incl_autotune_s": setup_s + t_solve + (tune["seconds"] if tune
+                                                                      else 0.0),
+                 "timed_wall_s": mv["wall"],
                  "rhs_one_at_a_time": dict(ms_per_step=mm["ms"], value=mm["value"])})
     return line
 
@@ -567,8 +653,9 @@ def run_torus(a):
     if not a.no_cpu:
         rate, info = cpu_pair_rate(mesh, a.cpu_seconds)
         cpu = dict(value=n / (float(n) ** 2 / rate), unit="panels/s", cores=info["cores"],
+                   host=info["host"],
                    kind="port", sample=info["sample"])
-    line = base_line(a, head["value"], head["ms"], "bf16", {
+    line = base_line(a, head["value"], head["ms"], DTYPE_LABEL["bf16"], {
         "workload": f"torus R=1 r=0.3 ({n} panels), point charge ~U[-2,2]^3 outside the "
                     f"surface (seed {a.seed}); FMM p={a.order} ws={a.ws} leaf<={a.leaf}"
                     + (" (order and leaf from the autotuner at rel err <= "
@@ -578,6 +665,8 @@ def run_torus(a):
     line.update({"roofline": head["roofline"], "roofline_extra": head["extra"],
                  "cpu_baseline": cpu, "stage_ms": head["stages"], "plan_stats": head["stats"],
                  "precisions": res, "time_to_1e-8_s": res["bf16"]["time_to_solution_s"],
+                 "time_to_1e-8_incl_autotune_s": res["bf16"]["time_to_solution_s"]
+                 + (tune["seconds"] if tune else 0.0),
                  "autotune": tune,
                  "timed_wall_s": head["wall"]})
     return line

@@ -72,7 +72,7 @@ void plan_free(ed_plan* P) {
   void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->perm, P->node_ctr, P->node_rng,
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
                   P->leaves, P->pt_node, P->p2p_items, P->p2p_seg, P->p2p_part, P->p2p_ref, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
-                  P->m2l_pair_src, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
+                  P->m2l_pair_src, P->m2l_pair_sgn, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair,
                   P->near_ptr, P->near_col, P->near_w, P->cent64_t, P->area64_t, P->near_w64,
                   P->m2l_mmax, P->coarse_z, P->coarse_az, P->coarse_einv, P->coarse_cy,
@@ -148,6 +148,11 @@ static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, 
 // order-10 split-FP16 set), so a new plan uploads no operator.  Plans point
 // into it and never free it.
 struct DeviceTables {
+  // For the GEMM forms only offsets with non-negative components get a
+  // table: a reflection R of the lattice axes maps T(o) to T(R o) =
+  // D T(o) D with D = diag(+-1) (the real solid-harmonic coefficients are
+  // even or odd under each axis reflection), so 56 tables (ws = 1) serve all
+  // 316 offsets and each pair carries the sign pattern of its reflection.
   float* valu = nullptr;        // n_tab x nc x nc (VALU path)
   float* f32 = nullptr;         // n_tab x ncp x kp (FP32 MFMA)
   uint16_t* h16 = nullptr;      // n_tab x hsz (16-bit and split-FP16 MFMA)
@@ -155,8 +160,30 @@ struct DeviceTables {
   float* l2l = nullptr;         // 8 x nc x nc, column-major
   float* ascale = nullptr;      // nc
   std::vector<int32_t> slot_tab;  // offset slot -> table index, -1 if none
+  std::vector<uint8_t> slot_sgn;  // offset slot -> reflection (bit a: axis a negated)
+  uint8_t* flip = nullptr;        // per coefficient (ncp): bit a set if it changes sign
+                                  // under the reflection of axis a
   int n_tab = 0;
 };
+
+// Sign pattern of the packed real coefficients under axis reflections
+// (x -> -x: cos part (-1)^m, sin part (-1)^(m+1); y -> -y: sin part -1;
+// z -> -z: (-1)^(n+m)), bit 0/1/2 for x/y/z.  Both multipoles (packed
+// conj(R)) and local expansions transform with this pattern.
+static std::vector<uint8_t> reflection_flips(int p, int len) {
+  std::vector<uint8_t> f((size_t)len, 0);
+  for (int n = 0; n <= p; ++n)
+    for (int m = 0; m <= n; ++m) {
+      const uint8_t z = (uint8_t)(((n + m) & 1) << 2);
+      if (m == 0) {
+        f[(size_t)n * n] = z;
+      } else {
+        f[(size_t)(n * n + 2 * m - 1)] = (uint8_t)(((m & 1) ? 1 : 0) | z);
+        f[(size_t)(n * n + 2 * m)] = (uint8_t)(((m & 1) ? 0 : 1) | 2 | z);
+      }
+    }
+  return f;
+}
 
 static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
                          const DeviceTables** out) {
@@ -173,11 +200,23 @@ static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
   const int R = offset_radius(ws), W = 2 * R + 1;
   DeviceTables D;
   D.slot_tab.assign((size_t)W * W * W, -1);
+  D.slot_sgn.assign((size_t)W * W * W, 0);
+  const bool reflect = kind >= 0;  // GEMM forms: canonical tables + signs
+  std::vector<int32_t> canon_tab((size_t)W * W * W, -1);
   for (int oz = -R; oz <= R; ++oz)
     for (int oy = -R; oy <= R; ++oy)
       for (int ox = -R; ox <= R; ++ox)
-        if (std::max(std::abs(ox), std::max(std::abs(oy), std::abs(oz))) > ws)
-          D.slot_tab[(size_t)offset_slot(ws, ox, oy, oz)] = D.n_tab++;
+        if (std::max(std::abs(ox), std::max(std::abs(oy), std::abs(oz))) > ws) {
+          const size_t slot = (size_t)offset_slot(ws, ox, oy, oz);
+          if (!reflect) {
+            D.slot_tab[slot] = D.n_tab++;
+            continue;
+          }
+          const size_t cs = (size_t)offset_slot(ws, std::abs(ox), std::abs(oy), std::abs(oz));
+          if (canon_tab[cs] < 0) canon_tab[cs] = D.n_tab++;
+          D.slot_tab[slot] = canon_tab[cs];
+          D.slot_sgn[slot] = (uint8_t)((ox < 0 ? 1 : 0) | (oy < 0 ? 2 : 0) | (oz < 0 ? 4 : 0));
+        }
   std::vector<double> asc(nc);
   for (int n2 = 0; n2 <= p; ++n2)
     for (int m2 = 0; m2 <= n2; ++m2) {
@@ -191,7 +230,7 @@ static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
   std::vector<uint16_t> th(kind > 0 ? (size_t)D.n_tab * hsz : 0);
   for (size_t slot = 0; slot < D.slot_tab.size(); ++slot) {
     const int tab = D.slot_tab[slot];
-    if (tab < 0) continue;
+    if (tab < 0 || D.slot_sgn[slot] != 0) continue;  // each table from its canonical offset
     const int ox = (int)(slot % W) - R, oy = (int)((slot / W) % W) - R, oz = (int)(slot / (W * W)) - R;
     const OperatorTables& ot = cached_m2l_tables(p, kind, ox, oy, oz, ncp, kp, asc);
     if (kind < 0) std::memcpy(tv.data() + (size_t)tab * nc * nc, ot.valu.data(), sizeof(float) * nc * nc);
@@ -219,8 +258,10 @@ static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
   if (rc == ED_OK) rc = upload(&D.m2m, t8);
   if (rc == ED_OK) rc = upload(&D.l2l, l8);
   if (rc == ED_OK) rc = upload(&D.ascale, af);
+  if (rc == ED_OK) rc = upload(&D.flip, reflection_flips(p, std::max(ncp, kp)));
   if (rc != ED_OK) {
-    for (void* q : {(void*)D.valu, (void*)D.f32, (void*)D.h16, (void*)D.m2m, (void*)D.l2l, (void*)D.ascale})
+    for (void* q : {(void*)D.valu, (void*)D.f32, (void*)D.h16, (void*)D.m2m, (void*)D.l2l,
+                    (void*)D.ascale, (void*)D.flip})
       if (q) (void)hipFree(q);
     return rc;
   }
@@ -394,7 +435,8 @@ int plan_upload(ed_plan* P) {
   ED_TRY(device_tables(P->device, p, need_valu ? -1 : prec, P->ws, ncp, kp, &DT));
   const int ntab = DT->n_tab;
   std::vector<int32_t> vptr(M + 1), vsrc(H.v_src.size()), vtab(H.v_src.size());
-  std::vector<char> used((size_t)ntab, 0);
+  std::vector<uint8_t> vsgn(H.v_src.size());
+  std::vector<char> used(DT->slot_tab.size(), 0);
   for (int64_t i = 0; i <= M; ++i) vptr[i] = (int32_t)H.v_ptr[i];
   for (size_t q = 0; q < H.v_src.size(); ++q) {
     vsrc[q] = (int32_t)H.v_src[q];
@@ -402,7 +444,8 @@ int plan_upload(ed_plan* P) {
     const int32_t tab = s >= 0 && (size_t)s < DT->slot_tab.size() ? DT->slot_tab[(size_t)s] : -1;
     if (tab < 0) return fail(ED_ERR_INTERNAL, "plan: V-list offset outside the table set");
     vtab[q] = tab;
-    used[(size_t)tab] = 1;
+    vsgn[q] = DT->slot_sgn[(size_t)s];
+    used[(size_t)s] = 1;
   }
   P->n_off = (int)std::count(used.begin(), used.end(), (char)1);  // distinct offsets in use
   P->n_v = (int64_t)vsrc.size();
@@ -419,16 +462,21 @@ int plan_upload(ed_plan* P) {
   // row by a (binv), so L = T M exactly up to rounding.
   P->m2l_ascale = DT->ascale;
   P->m2l_binv = DT->ascale;
+  P->m2l_flip = DT->flip;
   ED_TRY(upload(&P->m2l_mmax, std::vector<unsigned>(64 * 32, 0u)));  // kMaxSlots x kMaxSlotStride
 
   // GEMM-form M2L schedule: pairs grouped by table, tiles of `cols` pairs
   P->n_m2l_tiles = 0;
   if (enabled && P->n_v > 0) {
     std::vector<std::vector<std::pair<int32_t, int32_t>>> by_tab(ntab);
+    std::vector<std::vector<uint8_t>> sgn_tab(ntab);
     for (int64_t t = 0; t < M; ++t)
-      for (int64_t q = H.v_ptr[t]; q < H.v_ptr[t + 1]; ++q)
+      for (int64_t q = H.v_ptr[t]; q < H.v_ptr[t + 1]; ++q) {
         by_tab[vtab[q]].emplace_back((int32_t)t, vsrc[q]);
+        sgn_tab[vtab[q]].push_back(vsgn[q]);
+      }
     std::vector<int32_t> psrc, ptgt, ttab, tbeg, tcnt;
+    std::vector<uint8_t> psgn;
     for (int tab = 0; tab < ntab; ++tab) {
       auto& v = by_tab[tab];
       for (size_t b = 0; b < v.size(); b += (size_t)cols) {
@@ -438,6 +486,7 @@ int plan_upload(ed_plan* P) {
         for (size_t c = b; c < std::min(v.size(), b + (size_t)cols); ++c) {
           ptgt.push_back(v[c].first);
           psrc.push_back(v[c].second);
+          psgn.push_back(sgn_tab[tab][c]);
         }
       }
     }
@@ -489,6 +538,7 @@ int plan_upload(ed_plan* P) {
       tcnt.swap(xc);
     }
     ED_TRY(upload(&P->m2l_pair_src, psrc));
+    ED_TRY(upload(&P->m2l_pair_sgn, psgn));
     ED_TRY(upload(&P->m2l_pair_tgt, ptgt));
     ED_TRY(upload(&P->m2l_tile_tab, ttab));
     ED_TRY(upload(&P->m2l_tile_beg, tbeg));

@@ -3,14 +3,15 @@
 // Data layout (tree order, Morton-sorted):
 //   pts[i]       float4 (x, y, z, q) relative to the root centre, q = sigma A
 //   M, L         n_nodes x nc (real-packed solid-harmonic coefficients)
-// Stages of one matvec (one launch each, level loops for M2M / L2L):
+// Stages of one matvec (one launch each, a level loop for M2M):
 //   gather      q, sigma, self term into tree order
 //   p2m         one wave per leaf: lanes stream points, LDS column reduction
-//   m2m         per level: one block per child (T_oct * M_child), then one
-//               block per parent summing its children
+//   m2m         per level: one block per parent, sum of T_oct * M_child
 //   m2l         MFMA GEMM per offset table: Y[pair] = T_off * M[src] (FP32,
 //               split FP16 or 16-bit MFMA), then a deterministic per-target sum
-//   l2l         one block per node: L += T_oct * L_parent (level by level)
+//               into each node's own part of its local expansion
+//   down        one block per node: the L2L recursion over its own chain of
+//               ancestors (all levels in one launch)
 //   p2p         one wave per work item (<= 64 targets x a piece of their
 //               leaf's U-list source stream), partial sums per item
 //   l2p_out     one thread per panel: L2P + P2P partials + near CSR + self
@@ -41,7 +42,7 @@ __device__ __forceinline__ void chain_prio() { __builtin_amdgcn_s_setprio(3); }
 // zero, R = 1 and qT = NULL for one right-hand side, whose charges sit in
 // pts[i].w).
 struct RhsStride {
-  int64_t node = 0;  // M, L, ym2m: n_nodes * nc
+  int64_t node = 0;  // M, L: n_nodes * nc
   int64_t pair = 0;  // ypair: n_pairs * ncp
   int64_t slot = 0;  // multipole-maximum slots
   int64_t part = 0;  // P2P partial rows
@@ -161,110 +162,167 @@ __device__ __forceinline__ float slots_absmax(const unsigned* __restrict__ bits)
   return v;
 }
 
+// y = T x for the NC outputs owned by this thread (k = t + W u), T column
+// major (T[j NC + k]: input j, output k), x in LDS.  Each global round trip
+// costs about half a microsecond here, so the table column is loaded in as
+// few batches as the registers allow (one for NC <= 128 with one output per
+// thread): every load of a batch is issued before the first use.  Indices
+// are clamped instead of branched on.
+template <int NC, int W>
+__device__ __forceinline__ void op_apply(const float* __restrict__ T, const float* x, int t,
+                                         float (&y)[(NC + W - 1) / W]) {
+  constexpr int KO = (NC + W - 1) / W;
+  constexpr int B = (128 / KO) < NC ? (128 / KO) : NC;  // rows per batch
+  int kk[KO];
+#pragma unroll
+  for (int u = 0; u < KO; ++u) {
+    kk[u] = min(t + W * u, NC - 1);
+    y[u] = 0.0f;
+  }
+#pragma unroll 1
+  for (int j0 = 0; j0 < NC; j0 += B) {
+    float tv[B][KO];
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const int j = min(j0 + b, NC - 1);
+#pragma unroll
+      for (int u = 0; u < KO; ++u) tv[b][u] = T[(size_t)j * NC + kk[u]];
+    }
+#pragma unroll
+    for (int b = 0; b < B; ++b) {
+      const float xv = j0 + b < NC ? x[min(j0 + b, NC - 1)] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < KO; ++u) y[u] = fmaf(tv[b][u], xv, y[u]);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
-// P2M: one 64-lane block per leaf
+// P2M: one 2-wave block per leaf at level >= 2 (lanes stream the points, the
+// two waves' sums meet in LDS, then a column reduction in a fixed order).
 // ---------------------------------------------------------------------------
+constexpr int kP2MThreads = 128;
 template <int P, bool MULTI>
-__global__ __launch_bounds__(kLeafBlock) void k_p2m(const int32_t* __restrict__ leaves,
-                                                    const int2* __restrict__ rng,
-                                                    const float4* __restrict__ ctr,
-                                                    const float4* __restrict__ pts,
-                                                    float* __restrict__ M,
-                                                    const float* __restrict__ ascale,
-                                                    unsigned* __restrict__ mmax_bits, RhsStride rs) {
+__global__ __launch_bounds__(kP2MThreads) void k_p2m(const int32_t* __restrict__ leaves,
+                                                     const int2* __restrict__ rng,
+                                                     const float4* __restrict__ ctr,
+                                                     const float4* __restrict__ pts, int node2,
+                                                     float* __restrict__ M,
+                                                     const float* __restrict__ ascale,
+                                                     unsigned* __restrict__ mmax_bits,
+                                                     RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
-  __shared__ float red[kLeafBlock][NC + 1];
+  constexpr int W = kP2MThreads;
+  __shared__ float red[64][NC + 1];
   const int ry = MULTI ? blockIdx.y : 0;
   if constexpr (MULTI) {
     M += ry * rs.node;
     if (mmax_bits) mmax_bits += ry * rs.slot;
   }
-  const int node = leaves[blockIdx.x];
-  const int2 r = rng[node];
-  const float4 c = ctr[node];
-  float acc[NC];
+  const int leaf = leaves[blockIdx.x];
+  if (leaf < node2) return;  // no multipole needed above level 2
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  {
+    const int2 r = rng[leaf];
+    const float4 c = ctr[leaf];
+    float acc[NC];
 #pragma unroll
-  for (int k = 0; k < NC; ++k) acc[k] = 0.0f;
-  for (int i = r.x + threadIdx.x; i < r.y; i += kLeafBlock) {
-    const float4 pq = pts[i];
-    const float q = MULTI ? rs.qT[(size_t)i * rs.R + ry] : pq.w;
-    p2m_accum<P>((pq.x - c.x) * c.w, (pq.y - c.y) * c.w, (pq.z - c.z) * c.w, q, acc);
+    for (int k = 0; k < NC; ++k) acc[k] = 0.0f;
+    for (int i = r.x + tid; i < r.y; i += W) {
+      const float4 pq = pts[i];
+      const float q = MULTI ? rs.qT[(size_t)i * rs.R + ry] : pq.w;
+      p2m_accum<P>((pq.x - c.x) * c.w, (pq.y - c.y) * c.w, (pq.z - c.z) * c.w, q, acc);
+    }
+    // the two waves' sums per lane: wave 1 writes, wave 0 adds its own
+    if (wave == 1) {
+#pragma unroll
+      for (int k = 0; k < NC; ++k) red[lane][k] = acc[k];
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int k = 0; k < NC; ++k) red[lane][k] = acc[k] + red[lane][k];
+    }
   }
-#pragma unroll
-  for (int k = 0; k < NC; ++k) red[threadIdx.x][k] = acc[k];
   __syncthreads();
   float mx = 0.0f;
-  for (int k = threadIdx.x; k < NC; k += kLeafBlock) {
+  for (int k = tid; k < NC; k += W) {
+    // 64 lane sums in a fixed order, read 16 at a time (independent loads)
     float s = 0.0f;
-    for (int t = 0; t < kLeafBlock; ++t) s += red[t][k];
-    M[(size_t)node * NC + k] = s;
+#pragma unroll
+    for (int t0 = 0; t0 < 64; t0 += 16) {
+      float v[16];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) v[t] = red[t0 + t][k];
+#pragma unroll
+      for (int t = 0; t < 16; ++t) s += v[t];
+    }
+    M[(size_t)leaf * NC + k] = s;
     if (mmax_bits) mx = fmaxf(mx, fabsf(s * ascale[k]));
   }
-  if (mmax_bits) fold_absmax(mx, mmax_bits, blockIdx.x);  // kLeafBlock is one wave
+  if (mmax_bits) fold_absmax(mx, mmax_bits, 2 * blockIdx.x + wave);
 }
 
 // ---------------------------------------------------------------------------
-// M2M in two passes per level: one block per child applies its octant's
-// operator (Y_c = T_oct(c) M_c; T stored column-major, T[oct][j][k] = input
-// j, output k), then one block per parent sums its children in a fixed
-// order.  (Small blocks on purpose: the passes run beside P2P, and a block
-// of many waves waits until that many wave slots are free on one CU.)
+// M2M, one launch per level: one 4-wave block per parent.  The children's
+// multipoles are staged in LDS; the two 128-thread halves apply the octant
+// operators of children h, h + 2, ... (M_p += T_oct(c) M_c, one output per
+// thread), and the two half sums are added in a fixed order.
 // ---------------------------------------------------------------------------
 template <int P, bool MULTI>
-__global__ __launch_bounds__(128) void k_m2m_child(int child0, const int32_t* __restrict__ octant,
-                                                   const float* __restrict__ T,
-                                                   const float* __restrict__ M,
-                                                   float* __restrict__ Y, RhsStride rs) {
+__global__ __launch_bounds__(256) void k_m2m(const int32_t* __restrict__ parents,
+                                             const int32_t* __restrict__ first_child,
+                                             const int32_t* __restrict__ n_child,
+                                             const int32_t* __restrict__ octant,
+                                             const float* __restrict__ T, float* __restrict__ M,
+                                             const float* __restrict__ ascale,
+                                             unsigned* __restrict__ mmax_bits, RhsStride rs) {
   chain_prio();
   constexpr int NC = (P + 1) * (P + 1);
-  __shared__ float mc[NC];
+  constexpr int W = 128, KO = (NC + W - 1) / W;
+  __shared__ float mc[8][NC];
+  __shared__ float part[2][NC];
+  __shared__ int octs[8];
   if constexpr (MULTI) {
-    M += blockIdx.y * rs.node;
-    Y += blockIdx.y * rs.node;
-  }
-  const int c = child0 + blockIdx.x;
-  for (int j = threadIdx.x; j < NC; j += blockDim.x) mc[j] = M[(size_t)c * NC + j];
-  __syncthreads();
-  const float* Tc = T + (size_t)octant[c] * NC * NC;
-  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
-    float s = 0.0f;
-#pragma unroll 16
-    for (int j = 0; j < NC; ++j) s += Tc[(size_t)j * NC + k] * mc[j];
-    Y[(size_t)blockIdx.x * NC + k] = s;
-  }
-}
-
-template <int P, bool MULTI>
-__global__ __launch_bounds__(128) void k_m2m_sum(const int32_t* __restrict__ parents,
-                                                 const int32_t* __restrict__ first_child,
-                                                 const int32_t* __restrict__ n_child, int child0,
-                                                 const float* __restrict__ Y,
-                                                 float* __restrict__ M,
-                                                 const float* __restrict__ ascale,
-                                                 unsigned* __restrict__ mmax_bits, RhsStride rs) {
-  chain_prio();
-  constexpr int NC = (P + 1) * (P + 1);
-  if constexpr (MULTI) {
-    Y += blockIdx.y * rs.node;
     M += blockIdx.y * rs.node;
     if (mmax_bits) mmax_bits += blockIdx.y * rs.slot;
   }
   const int par = parents[blockIdx.x];
-  const int c0 = first_child[par] - child0, nch = n_child[par];
+  const int c0 = first_child[par], nch = n_child[par];
+  for (int e = threadIdx.x; e < nch * NC; e += blockDim.x) {
+    const int c = e / NC;
+    mc[c][e - c * NC] = M[(size_t)(c0 + c) * NC + (e - c * NC)];
+  }
+  if (threadIdx.x < nch) octs[threadIdx.x] = octant[c0 + threadIdx.x];
+  __syncthreads();
+  const int h = threadIdx.x >> 7, t = threadIdx.x & (W - 1);
+  float acc[KO];
+#pragma unroll
+  for (int u = 0; u < KO; ++u) acc[u] = 0.0f;
+  for (int c = h; c < nch; c += 2) {
+    float y[KO];
+    op_apply<NC, W>(T + (size_t)octs[c] * NC * NC, mc[c], t, y);
+#pragma unroll
+    for (int u = 0; u < KO; ++u) acc[u] += y[u];
+  }
+#pragma unroll
+  for (int u = 0; u < KO; ++u)
+    if (t + W * u < NC) part[h][t + W * u] = acc[u];
+  __syncthreads();
   float mx = 0.0f;
   for (int k = threadIdx.x; k < NC; k += blockDim.x) {
-    float s = 0.0f;
-    for (int c = 0; c < nch; ++c) s += Y[(size_t)(c0 + c) * NC + k];
-    M[(size_t)par * NC + k] = s;
-    if (mmax_bits) mx = fmaxf(mx, fabsf(s * ascale[k]));
+    const float v = part[0][k] + part[1][k];
+    M[(size_t)par * NC + k] = v;
+    if (mmax_bits) mx = fmaxf(mx, fabsf(v * ascale[k]));
   }
-  if (mmax_bits) fold_absmax(mx, mmax_bits, blockIdx.x * 2 + (threadIdx.x >> 6));  // whole waves
+  if (mmax_bits) fold_absmax(mx, mmax_bits, blockIdx.x * 4 + (threadIdx.x >> 6));  // whole waves
 }
 
 // ---------------------------------------------------------------------------
-// M2L (VALU, CSR by target): L[t] = inv_s * sum_{s in V(t)} T_off M[s].
-// Kept as the plain reference path of the GEMM form below.
+// M2L (VALU, CSR by target): L_own[t] = inv_s * sum_{s in V(t)} T_off M[s]
+// (the node's own part; k_m2l_down adds the ancestors').  Kept as the plain
+// reference path of the GEMM form below.
 // ---------------------------------------------------------------------------
 template <int P, bool MULTI>
 __global__ __launch_bounds__(128) void k_m2l_valu(int node0, const int32_t* __restrict__ v_ptr,
@@ -340,11 +398,22 @@ struct M2LShapeH {
 // values are (the corrections of iterative refinement are orders of magnitude
 // below the first right-hand side) and the scale is undone exactly on Y.
 constexpr int kM2LScaleExp = 14;
+
+// Tables exist for offsets with non-negative components only: a pair whose
+// offset is the reflection R_s o of a table's offset o (bit a of s: axis a
+// negated) uses D T(o) D, D = diag(+-1), with coefficient k flipped when
+// popcount(s & flip[k]) is odd.  Applied to the gathered multipole column
+// and to the output column; a sign flip is exact.
+__device__ __forceinline__ float refl(float v, int s, int fk) {
+  return __int_as_float(__float_as_int(v) ^ ((__popc((unsigned)(s & fk)) & 1) << 31));
+}
 template <int P, bool MULTI>
 __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t* __restrict__ tile_tab,
                                                   const int32_t* __restrict__ tile_beg,
                                                   const int32_t* __restrict__ tile_cnt,
                                                   const int32_t* __restrict__ pair_src,
+                                                  const uint8_t* __restrict__ pair_sgn,
+                                                  const uint8_t* __restrict__ flip,
                                                   const float* __restrict__ Tm,
                                                   const float* __restrict__ ascale,
                                                   const float* __restrict__ M,
@@ -354,6 +423,7 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS;
   constexpr int NT = S::THREADS, NW = NT / 64, KH = KP / 2;
   __shared__ float ta[NCP * KS];
+  __shared__ uint8_t fl[NCP];
   if constexpr (MULTI) {
     M += blockIdx.y * rs.node;
     Y += blockIdx.y * rs.pair;
@@ -376,6 +446,7 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
       ta[i * KS + k] = vv[q];
     }
   }
+  for (int k = threadIdx.x; k < NCP; k += NT) fl[k] = flip[k];
   __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int li = lane & 31, lk = lane >> 5;
@@ -391,13 +462,14 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
     // so the compiler issues all of them before the first wait.
     const bool cv = col < cnt;
     const int srow = cv ? pair_src[beg + min(col, cnt - 1)] : pair_src[beg];
+    const int sg = pair_sgn[beg + min(col, cnt - 1)];
     const float* mrow = M + (size_t)srow * NC;
     float bv[KH];
 #pragma unroll
     for (int s = 0; s < KH; ++s) {
       const int k = 2 * s + lk;
       const int kk = min(k, NC - 1);
-      const float v = mrow[kk] * ascale[kk];
+      const float v = refl(mrow[kk] * ascale[kk], sg, fl[kk]);
       bv[s] = (cv && k < NC) ? v : 0.0f;
     }
     f32x16 acc[NRB];
@@ -420,8 +492,10 @@ __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
           const int row = (rb0 + rb) * 32 + q * 8 + lk * 4;
-          float4 v = make_float4(acc[rb][4 * q + 0], acc[rb][4 * q + 1], acc[rb][4 * q + 2],
-                                 acc[rb][4 * q + 3]);
+          float4 v = make_float4(refl(acc[rb][4 * q + 0], sg, fl[row]),
+                                 refl(acc[rb][4 * q + 1], sg, fl[row + 1]),
+                                 refl(acc[rb][4 * q + 2], sg, fl[row + 2]),
+                                 refl(acc[rb][4 * q + 3], sg, fl[row + 3]));
           *reinterpret_cast<float4*>(yrow + row) = v;
         }
     }
@@ -454,6 +528,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
                                                     const int32_t* __restrict__ tile_beg,
                                                     const int32_t* __restrict__ tile_cnt,
                                                     const int32_t* __restrict__ pair_src,
+                                                    const uint8_t* __restrict__ pair_sgn,
+                                                    const uint8_t* __restrict__ flip,
                                                     const T16* __restrict__ Tm,
                                                     const float* __restrict__ ascale,
                                                     const float* __restrict__ M,
@@ -470,7 +546,9 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
   __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
+  __shared__ uint8_t fl[NCP];
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
+  for (int k = threadIdx.x; k < NCP; k += 256) fl[k] = flip[k];
   // power-of-two input scale from the largest |M * ascale| of this matvec
   const float gmax = slots_absmax(mmax_bits);
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
@@ -489,11 +567,13 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   // gathered, scaled multipoles: consecutive threads read consecutive
   // coefficients of one source row
   // (unconditional clamped loads, masked after: no serialising branches)
+  __syncthreads();  // fl
 #pragma unroll 8
   for (int t = threadIdx.x; t < COLS * KP; t += 256) {
     const int c = t / KP, k = t - c * KP;
     const int cc = min(c, cnt - 1), kk = min(k, NC - 1);
-    const float v = M[(size_t)pair_src[beg + cc] * NC + kk] * ascale[kk] * sc;
+    const float v = refl(M[(size_t)pair_src[beg + cc] * NC + kk] * ascale[kk] * sc,
+                         pair_sgn[beg + cc], fl[kk]);
     bm[c * KS + k] = (T16)((c < cnt && k < NC) ? v : 0.0f);
   }
   __syncthreads();
@@ -519,14 +599,17 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   }
   const int col = col0 + li;
   if (col < cnt) {
+    const int sg = pair_sgn[beg + col];
     float* yrow = Y + (size_t)(beg + col) * NCP;
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = rb * 32 + q * 8 + lk * 4;
-        float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
-                               acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
+        float4 v = make_float4(refl(acc[rb][4 * q + 0] * inv_sc, sg, fl[row]),
+                               refl(acc[rb][4 * q + 1] * inv_sc, sg, fl[row + 1]),
+                               refl(acc[rb][4 * q + 2] * inv_sc, sg, fl[row + 2]),
+                               refl(acc[rb][4 * q + 3] * inv_sc, sg, fl[row + 3]));
         *reinterpret_cast<float4*>(yrow + row) = v;
       }
   }
@@ -569,6 +652,7 @@ __device__ __forceinline__ float m2l_scale(const unsigned* __restrict__ slots) {
 template <int P, bool RESCALE>
 __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
                                                const float* __restrict__ as_l,
+                                               const uint8_t* __restrict__ fl, int sg,
                                                const float (&mv)[M2LShapeH<P>::KP / 16][8],
                                                float rel, float inv_sc, bool cv, int li, int lk,
                                                float* __restrict__ yrow) {
@@ -584,7 +668,8 @@ __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
     f16x8 bh, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] : 0.0f;
+      const int k = 16 * sv + 8 * lk + j;
+      float v = cv ? refl(mv[sv][j] * as_l[k], sg, fl[k]) : 0.0f;
       if (RESCALE) v *= rel;
       const _Float16 h = (_Float16)v;
       bh[j] = h;
@@ -606,8 +691,10 @@ __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int row = rb * 32 + q * 8 + lk * 4;
-      float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
-                             acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
+      float4 v = make_float4(refl(acc[rb][4 * q + 0] * inv_sc, sg, fl[row]),
+                             refl(acc[rb][4 * q + 1] * inv_sc, sg, fl[row + 1]),
+                             refl(acc[rb][4 * q + 2] * inv_sc, sg, fl[row + 2]),
+                             refl(acc[rb][4 * q + 3] * inv_sc, sg, fl[row + 3]));
       *reinterpret_cast<float4*>(yrow + row) = v;
     }
 }
@@ -617,6 +704,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
                                                       const int32_t* __restrict__ tile_beg,
                                                       const int32_t* __restrict__ tile_cnt,
                                                       const int32_t* __restrict__ pair_src,
+                                                      const uint8_t* __restrict__ pair_sgn,
+                                                      const uint8_t* __restrict__ flip,
                                                       const _Float16* __restrict__ Tm,
                                                       const float* __restrict__ ascale,
                                                       const float* __restrict__ M,
@@ -627,6 +716,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
   __shared__ float as_l[KP];  // ascale x the power-of-two input scale, 0 past NC
+  __shared__ uint8_t fl[NCP];
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
@@ -637,6 +727,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
   const bool cv = col < cnt;
   // this lane's multipole values: k = 16 sv + 8 lk + j (clamped addresses)
   float mv[NSV][8];
+  const int sg = pair_sgn[beg + min(col, cnt - 1)];
   if (col0 < cnt) {
     const float* mrow = M + (size_t)pair_src[beg + min(col, cnt - 1)] * NC;
 #pragma unroll
@@ -659,6 +750,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
     *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
   }
   for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc : 0.0f;
+  for (int k = threadIdx.x; k < NCP; k += 256) fl[k] = flip[k];
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
   constexpr int NRB = NCP / 32;
@@ -672,7 +764,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
     f16x8 bh, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const float v = cv ? mv[sv][j] * as_l[16 * sv + 8 * lk + j] : 0.0f;
+      const int k = 16 * sv + 8 * lk + j;
+      const float v = cv ? refl(mv[sv][j] * as_l[k], sg, fl[k]) : 0.0f;
       const _Float16 h = (_Float16)v;
       bh[j] = h;
       bl[j] = (_Float16)(v - (float)h);
@@ -694,8 +787,10 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = rb * 32 + q * 8 + lk * 4;
-        float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc, acc[rb][4 * q + 1] * inv_sc,
-                               acc[rb][4 * q + 2] * inv_sc, acc[rb][4 * q + 3] * inv_sc);
+        float4 v = make_float4(refl(acc[rb][4 * q + 0] * inv_sc, sg, fl[row]),
+                               refl(acc[rb][4 * q + 1] * inv_sc, sg, fl[row + 1]),
+                               refl(acc[rb][4 * q + 2] * inv_sc, sg, fl[row + 2]),
+                               refl(acc[rb][4 * q + 3] * inv_sc, sg, fl[row + 3]));
         *reinterpret_cast<float4*>(yrow + row) = v;
       }
   }
@@ -711,6 +806,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
                                                       const int32_t* __restrict__ tile_beg,
                                                       const int32_t* __restrict__ tile_cnt,
                                                       const int32_t* __restrict__ pair_src,
+                                                      const uint8_t* __restrict__ pair_sgn,
+                                                      const uint8_t* __restrict__ flip,
                                                       const _Float16* __restrict__ Tm,
                                                       const float* __restrict__ ascale,
                                                       const float* __restrict__ M,
@@ -721,6 +818,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
   __shared__ float as_l[KP];  // ascale x the first RHS's scale, 0 past NC
+  __shared__ uint8_t fl[NCP];
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
@@ -732,6 +830,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
   // this lane's multipole values of RHS r: k = 16 sv + 8 lk + j (clamped)
   float mv[NSV][8];
   int src = 0;
+  const int sg = pair_sgn[beg + min(col, cnt - 1)];
   if (col0 < cnt) {
     src = pair_src[beg + min(col, cnt - 1)];
     m2l16x3_load<P>(M + (size_t)src * NC, lk, mv);
@@ -748,9 +847,10 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
     *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
   }
   for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc0 : 0.0f;
+  for (int k = threadIdx.x; k < NCP; k += 256) fl[k] = flip[k];
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
-  m2l16x3_column<P, false>(ta, as_l, mv, 1.0f, 1.0f / sc0, cv, li, lk,
+  m2l16x3_column<P, false>(ta, as_l, fl, sg, mv, 1.0f, 1.0f / sc0, cv, li, lk,
                            Y + (size_t)(beg + col) * NCP);
 #pragma unroll 1
   for (int r = 1; r < rs.R; ++r) {
@@ -759,59 +859,118 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
     asm volatile("" ::: "memory");
     m2l16x3_load<P>(M + r * rs.node + (size_t)src * NC, lk, mv);
     const float sc = m2l_scale(mmax_bits + r * rs.slot);
-    m2l16x3_column<P, true>(ta, as_l, mv, sc / sc0, 1.0f / sc, cv, li, lk,
+    m2l16x3_column<P, true>(ta, as_l, fl, sg, mv, sc / sc0, 1.0f / sc, cv, li, lk,
                             Y + r * rs.pair + (size_t)(beg + col) * NCP);
   }
 }
 
-// Deterministic per-target sum of the GEMM rows: L[t] = inv_s * sum Y[row].
+// Own part of each node's local expansion from the GEMM rows:
+// Lown[t] = inv_s(t) binv * (sum of t's rows Y in a fixed order).  The row
+// indices of a batch are loaded first (uniform, scalar), then the rows.
 template <int P, bool MULTI>
 __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __restrict__ tptr,
                                                     const int32_t* __restrict__ trow,
                                                     const float4* __restrict__ ctr,
                                                     const float* __restrict__ binv,
                                                     const float* __restrict__ Y,
-                                                    float* __restrict__ L, RhsStride rs) {
+                                                    float* __restrict__ Lown, RhsStride rs) {
   chain_prio();
   using S = M2LShape<P>;
+  constexpr int NC = S::NC, NCP = S::NCP, W = 128, KO = (NC + W - 1) / W, RB = 16;
   if constexpr (MULTI) {
     Y += blockIdx.y * rs.pair;
-    L += blockIdx.y * rs.node;
+    Lown += blockIdx.y * rs.node;
   }
-  constexpr int NC = S::NC, NCP = S::NCP;
   const int t = node0 + blockIdx.x;
   const int b = tptr[t], e = tptr[t + 1];
   const float is = ctr[t].w;
-  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
-    float s = 0.0f;
-#pragma unroll 8
-    for (int q = b; q < e; ++q) s += Y[(size_t)trow[q] * NCP + k];
-    L[(size_t)t * NC + k] = s * (is * binv[k]);
+  float sum[KO];
+#pragma unroll
+  for (int u = 0; u < KO; ++u) sum[u] = 0.0f;
+  for (int r0 = b; r0 < e; r0 += RB) {
+    int rows[RB];
+#pragma unroll
+    for (int v = 0; v < RB; ++v) rows[v] = trow[min(r0 + v, e - 1)];
+    float yv[RB][KO];
+#pragma unroll
+    for (int v = 0; v < RB; ++v)
+#pragma unroll
+      for (int u = 0; u < KO; ++u)
+        yv[v][u] = Y[(size_t)rows[v] * NCP + min((int)threadIdx.x + W * u, NC - 1)];
+#pragma unroll
+    for (int v = 0; v < RB; ++v)
+#pragma unroll
+      for (int u = 0; u < KO; ++u)
+        if (r0 + v < e) sum[u] += yv[v][u];
+  }
+#pragma unroll
+  for (int u = 0; u < KO; ++u) {
+    const int k = threadIdx.x + W * u;
+    if (k < NC) Lown[(size_t)t * NC + k] = sum[u] * (is * binv[k]);
   }
 }
 
-// ---------------------------------------------------------------------------
-// L2L: one block per node of a level: L[c] += T_l2l[oct(c)] L[parent(c)]
-// ---------------------------------------------------------------------------
+// Complete local expansions in one launch: one block per node t at level >=
+// 2 runs the L2L recursion down its own chain of ancestors (level 2 to t),
+// x = Lown(a_top), then x = T_oct(a) x + Lown(a), so that no block waits for
+// another (a node's chain is at most the tree depth long).
+constexpr int kMaxChain = 24;
 template <int P, bool MULTI>
-__global__ __launch_bounds__(128) void k_l2l(int node0, const int32_t* __restrict__ parent,
-                                             const int32_t* __restrict__ octant,
-                                             const float* __restrict__ T,
-                                             float* __restrict__ L, RhsStride rs) {
+__global__ __launch_bounds__(128) void k_m2l_down(int node0, const int32_t* __restrict__ parent,
+                                                  const int32_t* __restrict__ octant,
+                                                  const float* __restrict__ Lown,
+                                                  const float* __restrict__ T,
+                                                  float* __restrict__ L, RhsStride rs) {
   chain_prio();
-  constexpr int NC = (P + 1) * (P + 1);
-  __shared__ float lp[NC];
-  if constexpr (MULTI) L += blockIdx.y * rs.node;
-  const int c = node0 + blockIdx.x;
-  const int par = parent[c];
-  for (int j = threadIdx.x; j < NC; j += blockDim.x) lp[j] = L[(size_t)par * NC + j];
+  constexpr int NC = (P + 1) * (P + 1), W = 128, KO = (NC + W - 1) / W;
+  __shared__ float xs[2][NC];
+  __shared__ int chain[kMaxChain], octs[kMaxChain];
+  __shared__ int nchain;
+  if constexpr (MULTI) {
+    Lown += blockIdx.y * rs.node;
+    L += blockIdx.y * rs.node;
+  }
+  const int t = node0 + blockIdx.x;
+  if (threadIdx.x == 0) {
+    int d = 0;
+    for (int a = t; a >= node0 && d < kMaxChain; a = parent[a]) {
+      chain[d] = a;
+      octs[d] = octant[a];
+      ++d;
+    }
+    nchain = d;
+  }
   __syncthreads();
-  const float* Tc = T + (size_t)octant[c] * NC * NC;
-  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
-    float s = 0.0f;
-#pragma unroll 16
-    for (int j = 0; j < NC; ++j) s += Tc[(size_t)j * NC + k] * lp[j];
-    L[(size_t)c * NC + k] += s;
+  const int nq = nchain;
+  if (nq == 1) {  // level 2: nothing to inherit
+    for (int k = threadIdx.x; k < NC; k += W) L[(size_t)t * NC + k] = Lown[(size_t)t * NC + k];
+    return;
+  }
+  int buf = 0;
+  for (int k = threadIdx.x; k < NC; k += W) xs[0][k] = Lown[(size_t)chain[nq - 1] * NC + k];
+  __syncthreads();
+  for (int q = nq - 2; q >= 0; --q) {
+    const int a = chain[q];
+    float own[KO];
+#pragma unroll
+    for (int u = 0; u < KO; ++u) own[u] = Lown[(size_t)a * NC + min((int)threadIdx.x + W * u, NC - 1)];
+    float tx[KO];
+    op_apply<NC, W>(T + (size_t)octs[q] * NC * NC, xs[buf], threadIdx.x, tx);
+    if (q == 0) {
+#pragma unroll
+      for (int u = 0; u < KO; ++u) {
+        const int k = threadIdx.x + W * u;
+        if (k < NC) L[(size_t)t * NC + k] = own[u] + tx[u];
+      }
+      return;
+    }
+#pragma unroll
+    for (int u = 0; u < KO; ++u) {
+      const int k = threadIdx.x + W * u;
+      if (k < NC) xs[buf ^ 1][k] = own[u] + tx[u];
+    }
+    __syncthreads();
+    buf ^= 1;
   }
 }
 
@@ -1088,7 +1247,6 @@ __global__ __launch_bounds__(256) void k_l2p_out(
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int node = pt_node[i];
-  const float4 c = ctr[node];
   const float4 xi = pts[i];
   const int nb = near_ptr[i], ne = near_ptr[i + 1];
   float corr = 0.0f;
@@ -1110,6 +1268,7 @@ __global__ __launch_bounds__(256) void k_l2p_out(
 #pragma unroll
     for (int u = 0; u < 4; ++u) corr += wq[u] * sv[u];
   }
+  const float4 c = ctr[node];
   const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w,
                                 L + (size_t)node * NC);
   const int2 pr = p2p_ref[i];
@@ -1195,7 +1354,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   }
   float* Mb = multi ? Pn->multi.M : Pn->M;
   float* Lb = multi ? Pn->multi.L : Pn->L;
-  float* Yb = multi ? Pn->multi.ym2m : Pn->ym2m;
+  float* Yb = multi ? Pn->multi.ym2m : Pn->ym2m;   // own parts of the local expansions
   float* Yp = multi ? Pn->multi.ypair : Pn->ypair;
   float* sigb = multi ? Pn->multi.sig : Pn->sig_t;
   float* partb = multi ? Pn->multi.part : Pn->p2p_part;
@@ -1228,92 +1387,98 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   ED_LAUNCH_CHECK("gather");
   ED_MARK(1);
   // P2P depends only on the gathered points: off the profiling path it runs
-  // on the side stream while this stream does the far-field passes
+  // on the side stream beside the far-field passes.  (Forked after P2M
+  // instead, P2P became the longer branch: 0.125 vs 0.114 ms per matvec at
+  // 82k panels.)
   const bool fork = Pn->prof_events == nullptr && Pn->n_p2p_items > 0;
-  if (fork) {
+  auto fork_p2p = [&]() -> int {
+    if (!fork) return ED_OK;
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_fork, st), "p2p fork");
     ED_HIP_LAUNCH(hipStreamWaitEvent(Pn->side, Pn->ev_fork, 0), "p2p fork");
     p2p_launch(Pn, R, rs, Pn->side);
     ED_LAUNCH_CHECK("p2p");
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_join, Pn->side), "p2p join");
-  }
+    return ED_OK;
+  };
+  if (int rc = fork_p2p(); rc != ED_OK) return rc;
   const int nleaf = (int)Pn->n_leaves;
   const auto& H = Pn->host;
   const int depth = H.depth;
   if (depth >= 2) {
-    hipLaunchKernelGGL((k_p2m<P, MULTI>), dim3(nleaf, RY), dim3(kLeafBlock), 0, st, Pn->leaves,
-                       Pn->node_rng, Pn->node_ctr, Pn->pts, Mb, Pn->m2l_ascale, mmax, rs);
+    const int node2 = (int)H.level_begin[2];
+    hipLaunchKernelGGL((k_p2m<P, MULTI>), dim3(nleaf, RY), dim3(kP2MThreads), 0, st, Pn->leaves,
+                       Pn->node_rng, Pn->node_ctr, Pn->pts, node2, Mb, Pn->m2l_ascale, mmax, rs);
     ED_LAUNCH_CHECK("p2m");
-    ED_MARK(2);
     // upward pass: parents at level l (children at l+1), deepest first
     for (int l = depth - 1; l >= 2; --l) {
       const int64_t pb = Pn->parents_begin[l], pe = Pn->parents_begin[l + 1];
       if (pe > pb) {
-        const int64_t cb = H.level_begin[l + 1], ce = H.level_begin[l + 2];
-        hipLaunchKernelGGL((k_m2m_child<P, MULTI>), dim3((unsigned)(ce - cb), RY), dim3(128), 0, st,
-                           (int)cb, Pn->node_octant, Pn->t_m2m, Mb, Yb, rs);
-        hipLaunchKernelGGL((k_m2m_sum<P, MULTI>), dim3((unsigned)(pe - pb), RY), dim3(128), 0, st,
-                           Pn->parents_list + pb, Pn->first_child, Pn->n_child, (int)cb, Yb, Mb,
-                           Pn->m2l_ascale, mmax, rs);
+        hipLaunchKernelGGL((k_m2m<P, MULTI>), dim3((unsigned)(pe - pb), RY), dim3(256), 0, st,
+                           Pn->parents_list + pb, Pn->first_child, Pn->n_child, Pn->node_octant,
+                           Pn->t_m2m, Mb, Pn->m2l_ascale, mmax, rs);
         ED_LAUNCH_CHECK("m2m");
       }
     }
-    ED_MARK(3);
+    ED_MARK(2);
     // M2L for all nodes at levels >= 2
-    const int node0 = (int)H.level_begin[2];
+    const int node0 = node2;
     const int nm = (int)(Pn->n_nodes - node0);
     bool valu = prec < 0;
     if constexpr (!M2LShape<P>::ENABLED) valu = valu || prec == 0;
     if (valu) {
+      // each node's own part into Yb
       hipLaunchKernelGGL((k_m2l_valu<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0, Pn->v_ptr,
-                         Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Mb, Lb, rs);
+                         Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Mb, Yb, rs);
       ED_LAUNCH_CHECK("m2l_valu");
-      ED_MARK(4);
     } else {
       const dim3 tiles((unsigned)Pn->n_m2l_tiles, RY);
       if (prec == 3) {
         if constexpr (MULTI)  // one block per tile runs every RHS (table loaded once)
           hipLaunchKernelGGL((k_m2l_mfma16x3_multi<P>), dim3((unsigned)Pn->n_m2l_tiles),
                              dim3(256), 0, st, Pn->m2l_tile_tab, Pn->m2l_tile_beg,
-                             Pn->m2l_tile_cnt, Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h,
-                             Pn->m2l_ascale, Mb, mmax, Yp, rs);
+                             Pn->m2l_tile_cnt, Pn->m2l_pair_src, Pn->m2l_pair_sgn, Pn->m2l_flip,
+                             (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
         else
           hipLaunchKernelGGL((k_m2l_mfma16x3<P>), dim3((unsigned)Pn->n_m2l_tiles), dim3(256), 0,
                              st, Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
-                             Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb,
-                             mmax, Yp);
+                             Pn->m2l_pair_src, Pn->m2l_pair_sgn, Pn->m2l_flip,
+                             (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp);
       }
       else if (prec == 1)
         hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16, MULTI>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,
                            Pn->m2l_tile_beg, Pn->m2l_tile_cnt, Pn->m2l_pair_src,
-                           (const __bf16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
+                           Pn->m2l_pair_sgn, Pn->m2l_flip, (const __bf16*)Pn->t_m2l_h,
+                           Pn->m2l_ascale, Mb, mmax, Yp, rs);
       else if (prec == 2)
         hipLaunchKernelGGL((k_m2l_mfma16<P, _Float16, MULTI>), tiles, dim3(256), 0, st,
                            Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
-                           Pn->m2l_pair_src, (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb,
-                           mmax, Yp, rs);
+                           Pn->m2l_pair_src, Pn->m2l_pair_sgn, Pn->m2l_flip,
+                           (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
       else if constexpr (M2LShape<P>::ENABLED)
         hipLaunchKernelGGL((k_m2l_mfma<P, MULTI>), tiles, dim3(M2LShape<P>::THREADS), 0, st,
                            Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
-                           Pn->m2l_pair_src, Pn->t_m2l_mfma, Pn->m2l_ascale, Mb, Yp, rs);
+                           Pn->m2l_pair_src, Pn->m2l_pair_sgn, Pn->m2l_flip, Pn->t_m2l_mfma,
+                           Pn->m2l_ascale, Mb, Yp, rs);
       ED_LAUNCH_CHECK("m2l_mfma");
-      ED_MARK(4);
-      hipLaunchKernelGGL((k_m2l_reduce<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0, Pn->m2l_tgt_ptr,
-                         Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Yp, Lb, rs);
+    }
+    ED_MARK(3);
+    if (!valu) {
+      hipLaunchKernelGGL((k_m2l_reduce<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0,
+                         Pn->m2l_tgt_ptr, Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Yp, Yb, rs);
       ED_LAUNCH_CHECK("m2l_reduce");
     }
+    ED_MARK(4);
+    // complete local expansions (own part + the ancestors', translated)
+    hipLaunchKernelGGL((k_m2l_down<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0,
+                       Pn->node_parent, Pn->node_octant, Yb, Pn->t_l2l, Lb, rs);
+    ED_LAUNCH_CHECK("m2l_down");
     ED_MARK(5);
-    // downward pass: levels 3..depth (levels 0..2 have no inherited local)
-    for (int l = 3; l <= depth; ++l) {
-      const int64_t b = H.level_begin[l], e = H.level_begin[l + 1];
-      hipLaunchKernelGGL((k_l2l<P, MULTI>), dim3((unsigned)(e - b), RY), dim3(128), 0, st, (int)b,
-                         Pn->node_parent, Pn->node_octant, Pn->t_l2l, Lb, rs);
-      ED_LAUNCH_CHECK("l2l");
-    }
   } else {
-    for (int k = 2; k <= 5; ++k) ED_MARK(k);
+    ED_MARK(2);
+    ED_MARK(3);
+    ED_MARK(4);
+    ED_MARK(5);
   }
-  ED_MARK(6);
   if (fork) {
     ED_HIP_LAUNCH(hipStreamWaitEvent(st, Pn->ev_join, 0), "p2p join");
   } else if (Pn->n_p2p_items > 0) {
@@ -1376,32 +1541,38 @@ int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled) {
 }  // namespace ed
 
 // Per-stage device time of the FMM matvec (FP64 in/out, original order),
-// averaged over `reps` launches: stage_ms[0..7] = gather, p2m, m2m, m2l (GEMM
-// or VALU), m2l reduce, l2l, p2p, l2p_out (L2P + near + self + output).
+// averaged over `reps` launches, between the marks the launcher records
+// (0 start, 1 after gather, 2 after the upward pass, 3 after the M2L GEMM,
+// 4 after the M2L sums, 5 after the downward pass, 7 after P2P, 8 end):
+// stage_ms[0..7] = gather, upward (P2M + M2M), M2L, M2L sums, downward
+// (L2L), 0, P2P (inline here), l2p_out (L2P + near + self + output).
 extern "C" int ed_fmm_profile(ed_plan* P, const double* sigma, const double* selfi, double* out,
                               int32_t reps, double* stage_ms, int32_t n_stages, ed_stream stream) {
   using namespace ed;
   if (!P || !sigma || !out || !stage_ms || reps < 1) return fail(ED_ERR_ARG, "ed_fmm_profile: bad argument");
   hipStream_t st = (hipStream_t)stream;
-  constexpr int NE = 9;  // stage boundaries
+  constexpr int NE = 9;  // mark slots
+  constexpr int from[8] = {0, 1, 2, 3, 4, -1, 5, 7};
+  constexpr int to[8] = {1, 2, 3, 4, 5, -1, 7, 8};
   hipEvent_t ev[NE];
   for (int k = 0; k < NE; ++k)
     if (hipEventCreate(&ev[k]) != hipSuccess) return fail(ED_ERR_DEVICE, "ed_fmm_profile: event");
-  double acc[NE - 1] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   int rc = ED_OK;
   P->prof_events = ev;
   for (int r = 0; r < reps && rc == ED_OK; ++r) {
     rc = fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, st);
     if (rc != ED_OK) break;
     if (hipEventSynchronize(ev[NE - 1]) != hipSuccess) { rc = fail(ED_ERR_DEVICE, "ed_fmm_profile: sync"); break; }
-    for (int k = 0; k < NE - 1; ++k) {
+    for (int k = 0; k < 8; ++k) {
+      if (from[k] < 0) continue;
       float ms = 0.0f;
-      (void)hipEventElapsedTime(&ms, ev[k], ev[k + 1]);
+      (void)hipEventElapsedTime(&ms, ev[from[k]], ev[to[k]]);
       acc[k] += ms;
     }
   }
   P->prof_events = nullptr;
   for (int k = 0; k < NE; ++k) (void)hipEventDestroy(ev[k]);
-  for (int k = 0; k < n_stages && k < NE - 1; ++k) stage_ms[k] = acc[k] / reps;
+  for (int k = 0; k < n_stages && k < 8; ++k) stage_ms[k] = acc[k] / reps;
   return rc;
 }

@@ -34,7 +34,8 @@ struct ed_plan {
   int32_t* n_child = nullptr;
   float* M = nullptr;  // n_nodes * nc
   float* L = nullptr;  // n_nodes * nc
-  float* ym2m = nullptr;  // n_nodes * nc, per-child M2M products
+  float* ym2m = nullptr;   // n_nodes * nc: each node's own part of its local
+                           // expansion (M2L sums, before the ancestors' part)
 
   // leaves (node ids) and their U lists (CSR over leaf ordinal)
   int32_t* pt_node = nullptr;   // n, leaf node of each tree-ordered panel
@@ -65,6 +66,7 @@ struct ed_plan {
 
   // M2L in GEMM form: pairs grouped by offset table, tiles of kM2LTile pairs
   int32_t* m2l_pair_src = nullptr;   // n_v, grouped by table
+  uint8_t* m2l_pair_sgn = nullptr;   // n_v: reflection of the pair's offset onto its table
   int32_t* m2l_pair_tgt = nullptr;
   int32_t* m2l_tile_tab = nullptr;   // per tile: table id
   int32_t* m2l_tile_beg = nullptr;   // per tile: first pair
@@ -81,6 +83,7 @@ struct ed_plan {
   uint16_t* t_m2l_h = nullptr;  // n_off * ncp * kp16, bf16/fp16 (scaled)
   float* m2l_ascale = nullptr;  // per input coefficient: sqrt((n+m)!(n-m)!)
   float* m2l_binv = nullptr;    // per output coefficient: sqrt((n+m)!(n-m)!)
+  const uint8_t* m2l_flip = nullptr;  // per coefficient: reflection sign bits (shared)
   unsigned* m2l_mmax = nullptr;    // 64 slots (128 B apart) of max |M * ascale| as float bits
   float* t_m2m = nullptr;       // 8 * nc * nc
   float* t_l2l = nullptr;       // 8 * nc * nc

@@ -50,11 +50,26 @@ def _bucket_problem_size(n: int) -> int:
     return 1 << max(0, int(math.ceil(math.log2(max(1, n)))))
 
 
+_BUILD_ID = None
+
+
+def _build_id() -> str:
+    """Content hash of the loaded native library: a rebuilt kernel set gets
+    fresh cache entries."""
+    global _BUILD_ID
+    if _BUILD_ID is None:
+        import hashlib
+        with open(nat.LIB_PATH, "rb") as f:
+            _BUILD_ID = hashlib.sha1(f.read()).hexdigest()[:12]
+    return _BUILD_ID
+
+
 def _hardware_fingerprint(device=None) -> str:
     dev = torch.device(device or "cuda")
     props = torch.cuda.get_device_properties(dev)
     arch = getattr(props, "gcnArchName", "") or ""
-    return f"{props.name}|{arch}|cu{props.multi_processor_count}|lib{nat.lib().ed_abi_version()}"
+    return (f"{props.name}|{arch}|cu{props.multi_processor_count}|lib{nat.lib().ed_abi_version()}"
+            f"|build{_build_id()}")
 
 
 def _get_cache_path() -> Path:
@@ -156,8 +171,9 @@ def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: boo
     if dry_run:
         return TuningResult(config=base_cfg,
                             metrics={"mode": 0.0, "problem_size": float(problem_size)})
-    # cache records describe one geometry and one right-hand side
-    use_cache = use_cache and geom_offsets is None and nrhs == 1
+    # cache records describe the synthetic geometry of a size bucket with one
+    # right-hand side: a caller's own geometry is tuned but never cached
+    use_cache = use_cache and geom_offsets is None and nrhs == 1 and centroids is None
     nat.require_gpu()
     dev = torch.device(device or "cuda")
     if dev.type != "cuda":
@@ -165,7 +181,7 @@ def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: boo
     fp = _hardware_fingerprint(dev)
     bucket = _bucket_problem_size(problem_size)
     key = _cache_key(fp, bucket, base_cfg, target_rel_error)
-    if use_cache and centroids is None:
+    if use_cache:
         rec = _load_best_from_cache(key)
         if rec is not None:
             cfg = dataclasses.replace(base_cfg, expansion_order=int(rec["expansion_order"]),

@@ -160,10 +160,13 @@ class FmmPlan:
                                                     st), "ed_fmm_matvec_multi_f64")
         return out
 
-    STAGES = ("gather", "p2m", "m2m", "m2l", "m2l_reduce", "l2l", "p2p", "l2p_out")
+    # stage slots of ed_fmm_profile ("" = a slot the launcher leaves empty)
+    STAGES = ("gather", "upward", "m2l", "m2l_reduce", "downward", "", "p2p", "l2p_out")
 
     def profile(self, sigma, self_integrals=None, reps=10):
-        """Average device milliseconds per matvec stage (HIP events)."""
+        """Average device milliseconds per matvec stage (HIP events): gather,
+        upward (P2M + M2M), m2l (GEMM), m2l_reduce (per-node sums), downward
+        (L2L chains), p2p (inline), l2p_out."""
         if not sigma.is_cuda or sigma.dtype != torch.float64:
             raise ValueError("profile takes a float64 device sigma")
         out = torch.empty_like(sigma)
@@ -171,7 +174,7 @@ class FmmPlan:
         nat.check(nat.lib().ed_fmm_profile(self.handle, nat.ptr(sigma), nat.ptr(self_integrals),
                                            nat.ptr(out), int(reps), ms, len(self.STAGES),
                                            nat.stream_ptr(sigma.device)), "ed_fmm_profile")
-        return dict(zip(self.STAGES, list(ms)))
+        return {k: v for k, v in zip(self.STAGES, list(ms)) if k}
 
     def set_coarse(self, Z, self_integrals=None):
         """Coarse space (n, k) float64 device tensor (k <= 32) for the

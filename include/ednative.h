@@ -108,9 +108,9 @@ int ed_fmm_matvec_multi_f64(ed_plan* plan, const double* sigma, const double* se
  * reference logs MultipoleOpStats counters, fmm3d/multipole_operators.py:
  * 984-1041, not times).  Runs the f64 matvec `reps` times with HIP events
  * between stages on `stream` (P2P inline, not on the side stream);
- * stage_ms[0..7] = gather, P2M, M2M, M2L, M2L sum, L2L, P2P, output
- * (L2P + P2P partials + near + self), averaged; every stage time includes
- * the cost of its event pair (a few microseconds). */
+ * stage_ms[0..7] = gather, upward (P2M + M2M), M2L, M2L sums, downward
+ * (L2L), 0, P2P, output (L2P + P2P partials + near + self), averaged; every
+ * stage time includes the cost of its event pair (a few microseconds). */
 int ed_fmm_profile(ed_plan* plan, const double* sigma, const double* self_integrals,
                    double* out, int32_t reps, double* stage_ms, int32_t n_stages,
                    ed_stream stream);

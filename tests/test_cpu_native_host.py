@@ -112,6 +112,58 @@ def test_operator_tables_equal_oracle(p):
     assert lib.ed_host_operator(p, 0, 1, 0, 0, 0, out.ctypes.data_as(vp)) != 0
 
 
+def _reflection_signs(p, s):
+    """D(s) on the packed real coefficients: index n^2 (m = 0), n^2 + 2m - 1
+    (Re), n^2 + 2m (Im); the expansions are polynomials in (x + iy)^m z^(n-m)
+    times r^2 powers, so x -> -x conjugates and multiplies by (-1)^m,
+    y -> -y conjugates, z -> -z multiplies by (-1)^(n+m); multipoles store
+    conj(R), whose packed (Re, -Im) pattern changes the same way."""
+    d = np.ones((p + 1) ** 2)
+    for n in range(p + 1):
+        for m in range(n + 1):
+            for part, k in ((0, n * n + (2 * m - 1 if m else 0)), (1, n * n + 2 * m)):
+                if m == 0 and part == 1:
+                    continue
+                f = 1.0
+                if s & 1:
+                    f *= (-1) ** m * (1 if part == 0 else -1)
+                if s & 2 and part == 1:
+                    f *= -1
+                if s & 4:
+                    f *= (-1) ** (n + m)
+                d[k] = f
+    return d
+
+
+def test_m2l_reflection_symmetry():
+    """The GEMM M2L stores tables for offsets with non-negative components
+    only and uses T(R_s o) = D(s) T(o) D(s) for the reflected ones: checked
+    for every lattice offset of ws = 1 and a sample of ws = 2 on the host
+    operators (pinned above against the oracle's direct sums)."""
+    lib = _lib()
+    vp = ctypes.c_void_p
+    for p in (6, 10):
+        nc = (p + 1) ** 2
+        T = np.zeros((nc, nc))
+        Tc = np.zeros((nc, nc))
+        offs = [(x, y, z) for x in range(-3, 4) for y in range(-3, 4) for z in range(-3, 4)
+                if max(abs(x), abs(y), abs(z)) >= 2]
+        offs += [(-5, 2, -4), (4, -5, 0), (-2, -3, 5)]
+        for o in offs:
+            s = (1 if o[0] < 0 else 0) | (2 if o[1] < 0 else 0) | (4 if o[2] < 0 else 0)
+            assert lib.ed_host_operator(p, 0, *o, 0, T.ctypes.data_as(vp)) == 0
+            assert lib.ed_host_operator(p, 0, *(abs(c) for c in o), 0,
+                                        Tc.ctypes.data_as(vp)) == 0
+            d = _reflection_signs(p, s)
+            assert np.allclose(T, d[:, None] * Tc * d[None, :], rtol=0,
+                               atol=1e-13 * np.abs(Tc).max()), (p, o)
+        # a wrong pattern (x reflection ignored) fails
+        o = (-3, 2, 1)
+        assert lib.ed_host_operator(p, 0, *o, 0, T.ctypes.data_as(vp)) == 0
+        assert lib.ed_host_operator(p, 0, 3, 2, 1, 0, Tc.ctypes.data_as(vp)) == 0
+        assert not np.allclose(T, Tc, rtol=0, atol=1e-6 * np.abs(Tc).max())
+
+
 def test_host_morton_keys_bit_exact():
     lib = _lib()
     rng = np.random.default_rng(3)

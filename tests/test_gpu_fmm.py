@@ -170,6 +170,7 @@ def test_autotuner_meets_target_and_caches(tmp_path, monkeypatch):
     print("tuned", r.config.expansion_order, r.config.leaf_size, r.metrics)
     assert r.metrics["met_target"] == 1.0 and r.metrics["rel_err"] <= 1e-5
     assert r.config.leaf_size in at.LEAF_SIZES and r.config.expansion_order >= 6
+    assert not (tmp_path / "tune.json").exists()  # a caller's geometry is never cached
     r2 = autotune_for_problem(cfg, m.n_panels, dry_run=False, target_rel_error=1e-3)
     assert r2.metrics["mode"] == 2.0
     r3 = autotune_for_problem(cfg, m.n_panels, dry_run=False, target_rel_error=1e-3)
@@ -232,3 +233,38 @@ def test_matvec_multi_equals_single(prec):
             err = float(torch.linalg.norm(multi[0] - ref) / torch.linalg.norm(ref))
             print(prec, "multi row 0 vs dense", err)
             assert err < (2e-3 if prec == "bf16" else 1e-4)
+
+
+@pytest.mark.parametrize("prec", ["fp16x3", "valu"])
+def test_matvec_repeatable_bit_for_bit(prec):
+    """The upward pass hands multipoles between blocks (the last child of a
+    parent to finish sums its siblings' products after a device fence): 30
+    matvecs of one vector, interleaved with other vectors, all give the same
+    bits, on one geometry (depth 4-5 trees) and on a batched plan, with one
+    right-hand side and with eight; the first also matches the oracle."""
+    from electrodrive_amd.core import bem_mesh
+    from electrodrive_amd.fmm3d.plan import FmmPlan
+    rng = np.random.default_rng(99)
+    for meshes, leaf in (([bem_mesh.icosphere(5)], 32),
+                         ([bem_mesh.icosphere(4), bem_mesh.torus(1.0, 0.3, 60, 20)], 16)):
+        C = np.concatenate([m.centroids for m in meshes])
+        A = np.concatenate([m.areas for m in meshes])
+        off = np.concatenate([[0], np.cumsum([m.n_panels for m in meshes])])
+        plan = FmmPlan(C, A, leaf_size=leaf, order=8, ws=1, m2l_precision=prec,
+                       geom_offsets=None if len(meshes) == 1 else off)
+        print("stats", plan.stats())
+        s0 = torch.as_tensor(rng.normal(size=C.shape[0]), device="cuda")
+        other = torch.as_tensor(rng.normal(size=C.shape[0]), device="cuda")
+        first = plan.matvec(s0)
+        for k in range(30):
+            if k % 3 == 0:
+                plan.matvec(other)
+            assert torch.equal(plan.matvec(s0), first), k
+        S = torch.as_tensor(rng.normal(size=(8, C.shape[0])), device="cuda")
+        m1 = plan.matvec_multi(S)
+        for _ in range(5):
+            assert torch.equal(plan.matvec_multi(S), m1)
+        for g in range(len(meshes)):
+            b, e = off[g], off[g + 1]
+            ref = odense.dense_matvec(meshes[g].centroids, meshes[g].areas, s0[b:e].cpu().numpy())
+            assert _rel(first[b:e].cpu().numpy(), ref) < (2e-4 if prec == "fp16x3" else 2e-4)

@@ -109,20 +109,30 @@ def test_near_weights_and_apply():
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 def test_morton_keys_bit_exact(dtype):
-    from electrodrive_amd.fmm3d import morton_keys_from_points
+    from electrodrive_amd.fmm3d import BoundingBox, morton_keys_from_points
     rng = np.random.default_rng(42)
     for pts in (rng.uniform(-3, 5, size=(5000, 3)), rng.normal(size=(777, 3)) * [1, 1e-3, 0],
                 np.zeros((1, 3))):
         p = pts.astype(dtype)
         for level in (1, 10, 16, 21):
             ref = omorton.morton_keys(p, level)
-            got, _ = morton_keys_from_points(torch.tensor(p).cuda(), level=level)
+            got, bb = morton_keys_from_points(torch.tensor(p).cuda(), level=level)
             assert np.array_equal(got.cpu().numpy(), ref), (level, p.shape)
+        # the returned box is BoundingBox.from_min_max of the padded tight box
+        # (fmm3d/geometry.py:790-793)
+        mn, mx = omorton.global_bbox(p)
+        assert isinstance(bb, BoundingBox) and bb.center.dtype == torch.tensor(p).dtype
+        assert np.array_equal(bb.center.cpu().numpy(), (dtype(0.5) * (mn + mx)).astype(dtype))
+        assert np.array_equal(bb.half_extent.cpu().numpy(),
+                              np.maximum(dtype(0.5) * (mx - mn), 0).astype(dtype))
         c = np.array([0.5, 0.5, 0.5], dtype=dtype)
         h = np.array([4.0, 4.0, 4.0], dtype=dtype)
         ref = omorton.morton_keys(p, 16, c, h)
         got, _ = morton_keys_from_points(torch.tensor(p).cuda(), level=16, global_bbox=(c, h))
         assert np.array_equal(got.cpu().numpy(), ref)
+        box = BoundingBox(torch.tensor(c), torch.tensor(h))
+        got, bb2 = morton_keys_from_points(torch.tensor(p).cuda(), level=16, global_bbox=box)
+        assert np.array_equal(got.cpu().numpy(), ref) and bb2 is box
 
 
 def test_attach_near_device_equals_host_path():
