@@ -77,6 +77,7 @@ _SIGS = {
                                      ctypes.POINTER(GmresOpts), ctypes.POINTER(GmresInfo), c_vp]),
     "ed_gmres_block_resid": (c_i32, [c_vp, c_vp, c_vp, c_i32, ctypes.POINTER(c_i32),
                                      ctypes.POINTER(c_i32)]),
+    "ed_gmres_trace": (c_i32, [c_vp, c_vp, c_i32, ctypes.POINTER(c_i32)]),
     "ed_plan_set_coarse": (c_i32, [c_vp, c_vp, c_i32, c_vp, c_vp]),
     "ed_plan_coarse_solve": (c_i32, [c_vp, c_vp, c_vp, c_vp]),
     "ed_plan_build_sai": (c_i32, [c_vp, c_vp, ctypes.c_double, ctypes.c_double, c_vp]),

@@ -24,9 +24,13 @@ What is the reference's and what is this build's:
   the autotuner, or the configured ones) with FP64 iterative refinement
   against the FP64 dense operator, to gmres_tol times ||b - A x0|| (the
   reference's tolerance convention, core/bem_solver.py:404-407), with the SAI
-  smoother unless ``precond`` says otherwise.  The reference's Jacobi
-  fall-back solve has no counterpart: a solve that does not converge is an
-  error result.
+  smoother unless ``precond`` says otherwise;
+- failure recovery follows core/bem.py:2170-2303: a preconditioned solve
+  that fails (or has a non-finite residual) is retried once without the
+  preconditioner, then handed to the damped Jacobi fall-back solve
+  (core/bem.py:1156-1251: x += omega D^-1 (b - A x) from x = 0 until
+  ||r|| <= tol max(||r_0||, 1)) when allow_jacobi_fallback is set; if that
+  fails too the result is the reference's error dict.
 
 Out of scope (reference control plane): run manifests, control.json polling,
 the BEM intercept recorder, the near_alpha neighbour blending (off by
@@ -204,6 +208,99 @@ class BEMSolution:
         return V, E
 
 
+def _jacobi_fallback_solve(A, b, diag, logger, *, maxiter: int = 1024, tol: float = 1e-6,
+                           omega: float = 0.8):
+    """Damped Jacobi for A x = b (core/bem.py:1156-1251): x_0 = 0,
+    x += omega D^-1 r, r = b - A x, until ||r|| <= tol max(||r_0||, 1);
+    returns (x, info) with iters, resid, success, solver = "jacobi_fallback"."""
+    def log(level, msg, **kw):
+        if logger is not None:
+            getattr(logger, level)(msg, **kw)
+
+    x = torch.zeros_like(b)
+    dinv = 1.0 / diag.clamp_min(1e-18)
+    with torch.no_grad():
+        r = b - A(x)
+        rn = float(torch.linalg.norm(r))
+        if not math.isfinite(rn):
+            log("error", "Jacobi fallback: initial residual is non-finite.", r0=rn)
+            return x, {"iters": 0, "resid": rn, "success": False, "solver": "jacobi_fallback"}
+        target = tol * max(rn, 1.0)
+        log("warning", "Jacobi fallback: starting iterative solve.", r0=rn, tol=tol,
+            target=target, maxiter=maxiter)
+        it = 0
+        for k in range(maxiter):
+            it = k + 1
+            x = x + omega * dinv * r
+            r = b - A(x)
+            rn = float(torch.linalg.norm(r))
+            if not math.isfinite(rn):
+                log("error", "Jacobi fallback: residual became non-finite.", iter=it, resid=rn)
+                return x, {"iters": it, "resid": rn, "success": False,
+                           "solver": "jacobi_fallback"}
+            if k == 0 or (k + 1) % 25 == 0:
+                log("info", "Jacobi fallback iter.", iter=it, resid=rn)
+            if rn <= target:
+                log("info", "Jacobi fallback converged.", iters=it, resid=rn, target=target)
+                return x, {"iters": it, "resid": rn, "success": True,
+                           "solver": "jacobi_fallback"}
+        log("warning", "Jacobi fallback reached maxiter without convergence.", iters=it,
+            resid=rn, target=target)
+        return x, {"iters": it, "resid": rn, "success": False, "solver": "jacobi_fallback"}
+
+
+def _failed(info) -> Tuple[bool, float, int]:
+    try:
+        resid = float(info.get("resid", math.inf))
+    except (TypeError, ValueError):
+        resid = math.inf
+    return (not bool(info.get("success", True))) or (not math.isfinite(resid)), resid, \
+        int(info.get("code", 0) or 0)
+
+
+def _solve_with_recovery(solve, jacobi, precond, allow_jacobi, logger):
+    """The reference's recovery chain around one linear solve
+    (core/bem.py:2170-2303).  ``solve(precond)`` -> (sigma, info);
+    ``jacobi()`` -> (sigma, info).  Returns (sigma, info, None) or
+    (None, None, (message, extra keys)) for the error result."""
+    def log(level, msg, **kw):
+        if logger is not None:
+            getattr(logger, level)(msg, **kw)
+
+    sigma, info = solve(precond)
+    bad, resid, code = _failed(info)
+    if precond != "none" and bad:
+        log("warning", "GMRES reported failure or non-finite residual with preconditioner; "
+            "retrying without preconditioner.", gmres_resid=resid, gmres_code=code)
+        sigma, info = solve("none")
+        info = dict(info)
+        info["retried_without_precond"] = True
+        bad, resid, code = _failed(info)
+    if not bad:
+        return sigma, info, None
+    if not allow_jacobi:
+        log("error", "BEM linear solve failed; GMRES reported failure and Jacobi fallback is "
+            "disabled.", gmres_resid=resid, gmres_code=code)
+        return None, None, (f"BEM linear solve failed: GMRES(code={code}, resid={resid})",
+                            dict(gmres_resid=resid, gmres_code=code))
+    log("warning", "GMRES reported failure; attempting Jacobi fallback.", gmres_resid=resid,
+        gmres_code=code)
+    sigma_j, info_j = jacobi()
+    try:
+        resid_j = float(info_j.get("resid", math.inf))
+    except (TypeError, ValueError):
+        resid_j = math.inf
+    if bool(info_j.get("success", False)) and math.isfinite(resid_j):
+        log("info", "Jacobi fallback succeeded.", resid=resid_j, iters=info_j.get("iters"))
+        return sigma_j, dict(info_j), None
+    log("error", "Jacobi fallback failed after GMRES failure.", gmres_resid=resid,
+        gmres_code=code, jacobi_resid=resid_j, jacobi_iters=info_j.get("iters"))
+    return None, None, (f"BEM linear solve failed: GMRES(code={code}, resid={resid}) "
+                        f"and Jacobi(resid={resid_j})",
+                        dict(gmres_resid=resid, gmres_code=code, jacobi_resid=resid_j,
+                             jacobi_iters=info_j.get("iters")))
+
+
 def _near_depths(spec) -> Tuple[int, int]:
     """(self / evaluation depth, near-matvec depth) (core/bem.py:1762-1774)."""
     kinds = [c.get("type") for c in spec.conductors]
@@ -324,12 +421,21 @@ def bem_solve(spec: CanonicalSpec, cfg, logger, differentiable: bool = False) ->
             r0 = float(torch.linalg.norm(b - op.approx(x0)))
             bn = float(torch.linalg.norm(b))
             tol_b = gtol * (r0 / bn if bn > 0.0 else 1.0)
-            sigma, info = op.solve(b, x0, tol=tol_b, restart=restart, maxiter=maxiter,
-                                   precond=precond)
+            sigma, info, err = _solve_with_recovery(
+                lambda pc: op.solve(b, x0, tol=tol_b, restart=restart, maxiter=maxiter,
+                                    precond=pc),
+                lambda: _jacobi_fallback_solve(op.approx, b, op.diag, logger,
+                                               maxiter=int(getattr(cfg, "jacobi_maxiter", 256)),
+                                               tol=float(getattr(cfg, "jacobi_tol", 1e-5)),
+                                               omega=float(getattr(cfg, "jacobi_omega", 0.8))),
+                precond, bool(getattr(cfg, "allow_jacobi_fallback", True)), logger)
+            if err is not None:
+                return error_result(err[0], **err[1])
             mv_final = op.accurate
         info = dict(info)
         info["tol"] = gtol
-        info["used_preconditioner"] = precond != "none"
+        info["used_preconditioner"] = precond != "none" and not info.get(
+            "retried_without_precond", False) and info.get("solver") != "jacobi_fallback"
         info["fmm_order"], info["fmm_leaf_size"] = order, leaf
         log("info", "BEM linear solve done.", iters=info.get("iters"), resid=info.get("resid"),
             success=info.get("success"))

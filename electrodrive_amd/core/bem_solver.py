@@ -137,6 +137,18 @@ def gmres_restart(A: Callable, b, M: Optional[Callable] = None, restart: int = 8
         diag = None if M_local is None else M_local.diag
         x, info = native(b, x0=x0, diag=diag, restart=restart, tol=tol, maxiter=maxiter)
         info["used_preconditioner"] = M_local is not None
+        if telemetry is not None:
+            # one SOLVER_TRACE row per host poll of the native solver
+            for row in A.plan.trace():
+                try:
+                    telemetry.append_row(dict(ts=row["ts"], pass_idx=int(pass_index),
+                                              cycle=row["cycle"], iter=row["iter"],
+                                              resid_true_l2=row["resid_true_l2"],
+                                              resid_precond_l2=row["resid_precond_l2"],
+                                              arnoldi_dim=row["arnoldi_dim"],
+                                              phase=row["phase"]))
+                except Exception:
+                    pass
         return x.to(orig_device, orig_dtype), info
 
     # ---- generic path: reference control flow on native vector kernels ------

@@ -629,8 +629,9 @@ static int plan_create_impl(const double* centroids, const double* areas, int64_
   ed_plan_opts o;
   ed_plan_opts_default(&o);
   if (opts) o = *opts;
-  if (o.order != 4 && o.order != 6 && o.order != 8 && o.order != 10 && o.order != 12)
-    return fail(ED_ERR_ARG, "ed_plan_create: order must be one of 4, 6, 8, 10, 12");
+  if (o.order != 4 && o.order != 6 && o.order != 8 && o.order != 10 && o.order != 12 &&
+      o.order != 14 && o.order != 16)
+    return fail(ED_ERR_ARG, "ed_plan_create: order must be one of 4, 6, 8, 10, 12, 14, 16");
   if (o.m2l_precision < -1 || o.m2l_precision > 3)
     return fail(ED_ERR_ARG, "ed_plan_create: m2l_precision must be -1, 0, 1, 2 or 3");
   if (n_geoms < 1) return fail(ED_ERR_ARG, "ed_plan_create: n_geoms must be >= 1");

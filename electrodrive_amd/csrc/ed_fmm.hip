@@ -341,22 +341,29 @@ __global__ __launch_bounds__(128) void k_m2l_valu(int node0, const int32_t* __re
   }
   const int t = node0 + blockIdx.x;
   const int b = v_ptr[t], e = v_ptr[t + 1];
-  float acc0 = 0.0f, acc1 = 0.0f;
-  const int k0 = threadIdx.x, k1 = threadIdx.x + 128;
+  constexpr int KO = (NC + 127) / 128;  // outputs per thread
+  float acc[KO];
+#pragma unroll
+  for (int u = 0; u < KO; ++u) acc[u] = 0.0f;
   for (int q = b; q < e; ++q) {
     const int s = v_src[q];
     __syncthreads();
     for (int j = threadIdx.x; j < NC; j += blockDim.x) ms[j] = M[(size_t)s * NC + j];
     __syncthreads();
     const float* Tq = T + (size_t)v_tab[q] * NC * NC;
-    if (k0 < NC)
-      for (int j = 0; j < NC; ++j) acc0 += Tq[(size_t)j * NC + k0] * ms[j];
-    if (k1 < NC)
-      for (int j = 0; j < NC; ++j) acc1 += Tq[(size_t)j * NC + k1] * ms[j];
+#pragma unroll
+    for (int u = 0; u < KO; ++u) {
+      const int k = threadIdx.x + 128 * u;
+      if (k < NC)
+        for (int j = 0; j < NC; ++j) acc[u] += Tq[(size_t)j * NC + k] * ms[j];
+    }
   }
   const float is = ctr[t].w;
-  if (k0 < NC) L[(size_t)t * NC + k0] = acc0 * is;
-  if (k1 < NC) L[(size_t)t * NC + k1] = acc1 * is;
+#pragma unroll
+  for (int u = 0; u < KO; ++u) {
+    const int k = threadIdx.x + 128 * u;
+    if (k < NC) L[(size_t)t * NC + k] = acc[u] * is;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1424,13 +1431,13 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     const int node0 = node2;
     const int nm = (int)(Pn->n_nodes - node0);
     bool valu = prec < 0;
-    if constexpr (!M2LShape<P>::ENABLED) valu = valu || prec == 0;
+    if constexpr (!M2LShape<P>::ENABLED) valu = true;  // tables beyond LDS: VALU form only
     if (valu) {
       // each node's own part into Yb
       hipLaunchKernelGGL((k_m2l_valu<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0, Pn->v_ptr,
                          Pn->v_src, Pn->v_tab, Pn->node_ctr, Pn->t_m2l, Mb, Yb, rs);
       ED_LAUNCH_CHECK("m2l_valu");
-    } else {
+    } else if constexpr (M2LShape<P>::ENABLED) {
       const dim3 tiles((unsigned)Pn->n_m2l_tiles, RY);
       if (prec == 3) {
         if constexpr (MULTI)  // one block per tile runs every RHS (table loaded once)
@@ -1462,10 +1469,13 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
       ED_LAUNCH_CHECK("m2l_mfma");
     }
     ED_MARK(3);
-    if (!valu) {
-      hipLaunchKernelGGL((k_m2l_reduce<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0,
-                         Pn->m2l_tgt_ptr, Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Yp, Yb, rs);
-      ED_LAUNCH_CHECK("m2l_reduce");
+    if constexpr (M2LShape<P>::ENABLED) {
+      if (!valu) {
+        hipLaunchKernelGGL((k_m2l_reduce<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0,
+                           Pn->m2l_tgt_ptr, Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Yp, Yb,
+                           rs);
+        ED_LAUNCH_CHECK("m2l_reduce");
+      }
     }
     ED_MARK(4);
     // complete local expansions (own part + the ancestors', translated)
@@ -1514,7 +1524,9 @@ int fmm_launch(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, const vo
     case 8: ED_RUN(8)
     case 10: ED_RUN(10)
     case 12: ED_RUN(12)
-    default: return fail(ED_ERR_ARG, "expansion order must be one of 4, 6, 8, 10, 12");
+    case 14: ED_RUN(14)
+    case 16: ED_RUN(16)
+    default: return fail(ED_ERR_ARG, "expansion order must be one of 4, 6, 8, 10, 12, 14, 16");
   }
 #undef ED_RUN
 }
@@ -1534,6 +1546,13 @@ int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled) {
     return ED_OK;
     ED_SHAPE(4) ED_SHAPE(6) ED_SHAPE(8) ED_SHAPE(10) ED_SHAPE(12)
 #undef ED_SHAPE
+    case 14:
+    case 16:  // M2L tables beyond the LDS: VALU form only
+      *ncp = (p + 1) * (p + 1);
+      *kp = *ncp;
+      *cols = 0;
+      *enabled = 0;
+      return ED_OK;
     default: return fail(ED_ERR_ARG, "unsupported order");
   }
 }

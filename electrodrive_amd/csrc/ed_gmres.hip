@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -66,6 +67,9 @@ struct GmresWS {
   // (right-hand side, geometry) blocks (tiling 0) or whole right-hand sides
   std::vector<double> last_rn, last_bn;
   int last_tiling = -1;
+  // per-poll trace of the last solve (kTraceCols doubles a row, see
+  // ed_gmres_trace)
+  std::vector<double> trace;
   // split dense FP64 operator for R > 1 right-hand sides (built on first use):
   // work items over the plan's geometries, splits per row, partial sums
   DenseItem* dn_items = nullptr;
@@ -759,6 +763,14 @@ __global__ void k_coarse_einv(int k, const double* __restrict__ partial, double*
 int dense_launch_f64(const double* C, const double* A, const double* sigma, const double* selfi,
                      double* out, int64_t n, int64_t nrhs, hipStream_t st);
 
+// Trace rows: host time (s since the epoch), outer refinement step, cycle,
+// Arnoldi steps so far, true residual norm (NaN when not measured),
+// preconditioned residual norm or estimate, Krylov dimension, phase
+// (0 initial residual, 1 Arnoldi poll, 2 cycle end, 3 refinement residual
+// with the accurate operator).
+constexpr int kTraceCols = 8;
+enum TracePhase { kTraceInitial = 0, kTracePoll = 1, kTraceCycleEnd = 2, kTraceRefine = 3 };
+
 struct Solver {
   ed_plan* P;
   GmresWS* W;
@@ -770,6 +782,20 @@ struct Solver {
   int R;
   unsigned gn1;  // one block per 256 panels
   int total_iters = 0, cycles = 0, first_below = -1;
+  int refine_step = 0;
+
+  void trace(int phase, int dim, double rtrue, double rprec) {
+    const double ts = std::chrono::duration<double>(
+                          std::chrono::system_clock::now().time_since_epoch()).count();
+    const double row[kTraceCols] = {ts, (double)refine_step, (double)cycles, (double)total_iters,
+                                    rtrue, rprec, (double)dim, (double)phase};
+    W->trace.insert(W->trace.end(), row, row + kTraceCols);
+  }
+  static double l2v(const std::vector<double>& v) {
+    double q = 0.0;
+    for (double e : v) q += e * e;
+    return std::sqrt(q);
+  }
 
   // the FMM operator on all R right-hand sides (one multi-RHS matvec)
   int approx_op(const double* in_tree, double* out_tree) {
@@ -950,6 +976,7 @@ struct Solver {
     std::vector<double> nr, nz;
     ED_T(approx_op(x, W->t));
     ED_T(resid_norms(rhs, W->t, W->r, W->w, nr, nz));
+    trace(kTraceInitial, 0, l2v(nr), l2v(nz));
     if (r0_out) *r0_out = nr;
     auto finish = [&](int c) {
       *code = c;
@@ -1017,6 +1044,9 @@ struct Solver {
         ED_G(hipMemcpyAsync(W->hscal, W->scal, sizeof(double) * S_NSCAL, hipMemcpyDeviceToHost, W->stream));
         ED_G(hipStreamSynchronize(W->stream));
         const double ratio = W->hscal[S_EST];
+        // the estimate of |M^-1 r| (one block: exact; several: the worst
+        // block's ratio to its target, times the first block's target)
+        trace(kTracePoll, j, NAN, ratio * est_tol[0]);
         if (W->hscal[S_BREAK] != 0.0) stop = true;
         if (!std::isfinite(ratio)) return finish(5);
         if (ratio <= 1.0) {
@@ -1033,6 +1063,7 @@ struct Solver {
       ++cycles;
       ED_T(approx_op(x, W->t));
       ED_T(resid_norms(rhs, W->t, W->r, W->w, nr, nz));
+      trace(kTraceCycleEnd, j, l2v(nr), l2v(nz));
       all_ok = true;
       for (int k = 0; k < g; ++k) {
         if (!std::isfinite(nr[k])) return finish(6);
@@ -1083,6 +1114,7 @@ static int gmres_solve(ed_plan* P, const double* b, double* x, int R, const doub
   GmresWS* W = nullptr;
   ED_T(ws_get(P, o.restart, R, &W));
   W->last_tiling = -1;
+  W->trace.clear();
   const int64_t n1 = P->n, n = W->n;
   const unsigned gb = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
   const unsigned gn = (unsigned)((n + 255) / 256), gn1 = (unsigned)((n1 + 255) / 256);
@@ -1146,9 +1178,11 @@ static int gmres_solve(ed_plan* P, const double* b, double* x, int R, const doub
           approx_resid = approx_resid && std::isfinite(rn[k]) &&
                          rn[k] > std::max(100.0 * tol_abs[k], 10.0 * o.inner_tol * bn[k]);
       }
+      S.refine_step = it;
       if (!approx_resid) {
         ED_T(S.accurate_op(W->x, W->t));
         ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
+        S.trace(kTraceRefine, 0, l2(rn), NAN);
       }
       info->refine_steps = it;
       bool finite = true, done = true;
@@ -1205,6 +1239,16 @@ static int gmres_solve(ed_plan* P, const double* b, double* x, int R, const doub
   ED_G(hipStreamWaitEvent(caller, ev, 0));
   ED_G(hipStreamSynchronize(W->stream));
   (void)hipEventDestroy(ev);
+  return ED_OK;
+}
+
+extern "C" int ed_gmres_trace(const ed_plan* P, double* rows, int32_t cap, int32_t* n_rows) {
+  if (!P || !n_rows) return fail(ED_ERR_ARG, "ed_gmres_trace: NULL argument");
+  const GmresWS* W = static_cast<const GmresWS*>(P->gmres_ws);
+  const int32_t nr = W ? (int32_t)(W->trace.size() / kTraceCols) : 0;
+  *n_rows = nr;
+  if (rows && cap > 0 && W)
+    std::memcpy(rows, W->trace.data(), sizeof(double) * kTraceCols * (size_t)std::min(cap, nr));
   return ED_OK;
 }
 

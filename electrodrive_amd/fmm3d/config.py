@@ -25,7 +25,8 @@ FMM_M2L_LMAX_FACTOR: int = 2
 BackendKind = Literal["cpu", "gpu", "auto"]
 PrecisionKind = Literal["single", "double", "mixed"]
 
-SUPPORTED_ORDERS = (4, 6, 8, 10, 12)
+# orders 14 and 16 run the VALU M2L (their tables do not fit the LDS)
+SUPPORTED_ORDERS = (4, 6, 8, 10, 12, 14, 16)
 M2L_PRECISIONS = {"valu": -1, "fp32": 0, "bf16": 1, "fp16": 2, "fp16x3": 3}
 
 

@@ -238,6 +238,27 @@ class FmmPlan:
                   "ed_gmres_block_resid")
         return r, bn, bool(pg.value)
 
+    TRACE_PHASES = ("initial", "arnoldi", "cycle_end", "refine")
+
+    def trace(self):
+        """Rows of the last native solve's per-poll trace (ed_gmres_trace) as
+        dicts with the SOLVER_TRACE columns (core/bem_solver.py:19-28) plus
+        refine_step: ts, cycle, iter, resid_true_l2 (NaN where not measured),
+        resid_precond_l2 (the solver's estimate at an Arnoldi poll),
+        arnoldi_dim and phase (initial / arnoldi / cycle_end / refine)."""
+        L = nat.lib()
+        nr = ctypes.c_int32(0)
+        nat.check(L.ed_gmres_trace(self.handle, None, 0, ctypes.byref(nr)), "ed_gmres_trace")
+        rows = np.zeros((max(nr.value, 1), 8))
+        nat.check(L.ed_gmres_trace(self.handle, rows.ctypes.data_as(ctypes.c_void_p), nr.value,
+                                   ctypes.byref(nr)), "ed_gmres_trace")
+        out = []
+        for r in rows[:nr.value]:
+            out.append(dict(ts=float(r[0]), refine_step=int(r[1]), cycle=int(r[2]), iter=int(r[3]),
+                            resid_true_l2=float(r[4]), resid_precond_l2=float(r[5]),
+                            arnoldi_dim=int(r[6]), phase=self.TRACE_PHASES[int(r[7])]))
+        return out
+
     def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
               tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,
               use_graph=True, check_every=4):

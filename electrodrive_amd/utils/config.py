@@ -47,6 +47,13 @@ class BEMConfig:
     plateau_max: Optional[int] = None
     plateau_rel_improvement: float = 0.05
     use_precond: bool = True
+    # failure recovery of bem_solve (core/bem.py:1604-1607): after a failed
+    # preconditioned solve, one retry without the preconditioner, then the
+    # damped Jacobi fall-back solve
+    allow_jacobi_fallback: bool = True
+    jacobi_maxiter: int = 256
+    jacobi_tol: float = 1e-5
+    jacobi_omega: float = 0.8
     # native FMM operator: expansion order and leaf size (None: the
     # autotuner's choice for the mesh at a 1e-5 relative error bound, or the
     # FP64 dense operator below fmm_min_panels panels), M2L precision

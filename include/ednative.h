@@ -41,7 +41,8 @@ int32_t ed_abi_version(void);
 /* ------------------------------------------------------------------------ */
 typedef struct ed_plan_opts {
   int32_t leaf_size;     /* max panels per leaf (reference default 64)        */
-  int32_t order;         /* expansion order p, 2..16 (reference default 8)   */
+  int32_t order;         /* expansion order p: 4, 6, ..., 16 (reference
+                            default 8); 14 and 16 run the VALU M2L          */
   int32_t ws;            /* well-separation radius in boxes: 1 or 2          */
   int32_t max_level;     /* deepest octree level, <= 20                      */
   int32_t m2l_precision; /* 0 = fp32 MFMA, 1 = bf16 MFMA, 2 = fp16 MFMA,
@@ -293,6 +294,17 @@ int ed_gmres_solve_multi(ed_plan* plan, const double* b, double* x, int32_t nrhs
  * the geometries (*per_geometry = 0); *n_blocks is their number. */
 int ed_gmres_block_resid(ed_plan* plan, double* resid, double* bnorm, int32_t cap,
                          int32_t* n_blocks, int32_t* per_geometry);
+
+/* Per-poll trace of the last ed_gmres_solve / ed_gmres_solve_multi on this
+ * plan, the native counterpart of the SOLVER_TRACE rows gmres_restart writes
+ * (core/bem_solver.py:19-28): 8 doubles a row = host time (s since the
+ * epoch), outer refinement step, restart cycle, Arnoldi steps so far, true
+ * residual norm (NaN if not measured at that point), preconditioned residual
+ * norm (at an Arnoldi poll: the solver's estimate), Krylov dimension, phase
+ * (0 initial residual of an inner solve, 1 Arnoldi poll, 2 cycle end,
+ * 3 refinement residual with the accurate operator).  *n_rows receives the
+ * row count; min(cap, *n_rows) rows are copied when rows is non-NULL. */
+int ed_gmres_trace(const ed_plan* plan, double* rows, int32_t cap, int32_t* n_rows);
 
 /* Sparse approximate inverse smoother for ed_gmres_solve (precond = 2; no
  * reference counterpart, whose setup_preconditioner offers Jacobi and mass

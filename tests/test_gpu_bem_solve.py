@@ -163,3 +163,29 @@ def test_bem_solve_plane_image_charge():
     print("plane V err", err, "Q_ind", q_ind)
     assert err < 3e-2
     assert -1.0 < q_ind < -0.85  # -q on an infinite plane; the patch misses the far tail
+
+
+def test_bem_solve_recovery_chain_on_failure(tmp_path):
+    """gmres_maxiter = 1 forces the failure from the host: the SAI solve
+    fails, the retry without preconditioner fails, and the damped Jacobi
+    fall-back (divergent on this first-kind operator: omega lambda_max of
+    D^-1 A is far above 2) fails too -> the reference's error dict with its
+    GMRES and Jacobi keys; with the fall-back disabled, the GMRES keys only."""
+    import json
+    from electrodrive_amd.core.bem import bem_solve
+    from electrodrive_amd.utils import BEMConfig, JsonlLogger
+    cfg = BEMConfig()
+    cfg.max_refine_passes = 1
+    cfg.initial_h = 0.3
+    cfg.gmres_maxiter = 1
+    cfg.jacobi_maxiter = 20
+    out = bem_solve(_sphere_spec(), cfg, JsonlLogger(tmp_path / "a"))
+    print(out.get("error"))
+    assert "error" in out and out["error"].startswith("BEM linear solve failed: GMRES(code=1")
+    assert out["gmres_code"] == 1 and "jacobi_resid" in out and out["jacobi_iters"] >= 1
+    text = "".join(p.read_text() for p in (tmp_path / "a").rglob("*") if p.is_file())
+    assert "retrying without preconditioner" in text and "attempting Jacobi fallback" in text
+    cfg.allow_jacobi_fallback = False
+    out = bem_solve(_sphere_spec(), cfg, JsonlLogger(tmp_path / "b"))
+    assert out["gmres_code"] == 1 and "jacobi_resid" not in out
+    _ = json

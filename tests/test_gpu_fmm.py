@@ -268,3 +268,26 @@ def test_matvec_repeatable_bit_for_bit(prec):
             b, e = off[g], off[g + 1]
             ref = odense.dense_matvec(meshes[g].centroids, meshes[g].areas, s0[b:e].cpu().numpy())
             assert _rel(first[b:e].cpu().numpy(), ref) < (2e-4 if prec == "fp16x3" else 2e-4)
+
+
+@pytest.mark.parametrize("order", [14, 16])
+def test_high_orders_accepted_and_accurate(order):
+    """Orders up to 16 (the reference's 2p <= 32, fmm3d/config.py:122-172):
+    FmmConfig accepts them, the plan runs them (VALU M2L: their tables do not
+    fit the LDS, whatever M2L form is asked for) and the matvec meets the FP32
+    floor against the FP64 dense oracle."""
+    from electrodrive_amd.fmm3d import FmmConfig
+    from electrodrive_amd.fmm3d.plan import FmmPlan
+    cfg = FmmConfig(expansion_order=order)
+    assert cfg.native_order == order
+    with pytest.raises(ValueError):
+        FmmConfig(expansion_order=17)
+    m = _mesh(3)
+    rng = np.random.default_rng(order)
+    sig = rng.normal(size=m.n_panels)
+    ref = odense.dense_matvec(m.centroids, m.areas, sig, None)
+    plan = FmmPlan(m.centroids, m.areas, leaf_size=32, order=order, ws=1, m2l_precision="fp16x3")
+    out = plan.matvec(torch.tensor(sig).cuda()).cpu().numpy()
+    err = _rel(out, ref)
+    print(f"p={order} rel err {err:.2e}")
+    assert err < 1e-6
