@@ -33,13 +33,21 @@ MFMA_FP32_PEAK_TFLOPS = 157.3
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 VALU_FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
-# WRITE_SIZE x 1.0, the factors a 256 MiB device copy reads) for the headline
-# workload (81,920 panels, order 10, leaf 256), keyed by (panels, order, leaf),
-# and the VALU-busy fraction of k_p2p from SQ counters; summary and commands
-# in profiles/pmc_traffic_icosphere81920_p10_leaf256.md
-PMC_SOURCE = "profiles/pmc_traffic_icosphere81920_p10_leaf256.md"
-TRAFFIC_BYTES = {("p2p", 81920, 10, 256): 2.189e7,
-                 ("m2l", 81920, 10, 256): 5.689e7}
+# WRITE_SIZE x 1.0, the factors a 256 MiB device copy reads), keyed by
+# (stage, panels, order, leaf, right-hand sides, M2L form), with the summary
+# holding the commands; and the VALU-busy fraction of k_p2p from SQ counters
+PMC_SPHERE = "profiles/pmc_traffic_icosphere81920_p10_leaf256.md"
+PMC_BATCH = "profiles/pmc_traffic_batch64_8rhs_p10_leaf512.md"
+PMC_TORUS = "profiles/pmc_traffic_torus300192_p10_leaf384_bf16.md"
+TRAFFIC_BYTES = {("p2p", 81920, 10, 256, 1, "fp16x3"): (2.189e7, PMC_SPHERE),
+                 ("m2l", 81920, 10, 256, 1, "fp16x3"): (3.598e7, PMC_SPHERE),
+                 ("m2l_reduce", 81920, 10, 256, 1, "fp16x3"): (2.823e7, PMC_SPHERE),
+                 ("p2p", 877882, 10, 512, 8, "fp16x3"): (1.230e9, PMC_BATCH),
+                 ("m2l", 877882, 10, 512, 8, "fp16x3"): (1.117e9, PMC_BATCH),
+                 ("m2l_reduce", 877882, 10, 512, 8, "fp16x3"): (7.569e8, PMC_BATCH),
+                 ("p2p", 300192, 10, 384, 1, "bf16"): (1.189e8, PMC_TORUS),
+                 ("m2l", 300192, 10, 384, 1, "bf16"): (8.601e7, PMC_TORUS),
+                 ("m2l_reduce", 300192, 10, 384, 1, "bf16"): (6.628e7, PMC_TORUS)}
 P2P_VALU_BUSY = {(81920, 10, 256): 0.69}
 
 
@@ -72,38 +80,46 @@ def parse():
     ap.add_argument("--n-geoms", type=int, default=64)
     ap.add_argument("--rhs", type=int, default=8)
     ap.add_argument("--torus-panels", type=int, default=300312)
+    ap.add_argument("--pmc-of", default="sphere", choices=["sphere", "batch", "torus"],
+                    help="workload of --workload pmc")
     return ap.parse_args()
 
 
-def stage_roofline(name, ms, st, n, order, m2l_peak=MFMA_FP32_PEAK_TFLOPS):
-    """Algorithmic work of one launch of a stage: M2L 2 (p+1)^4 flops per
-    translation pair; leaf bytes below; other stages 16 B per panel."""
+def stage_roofline(name, ms, st, n, order, m2l_peak=MFMA_FP32_PEAK_TFLOPS, R=1):
+    """Algorithmic work of one launch of a stage for R right-hand sides: M2L
+    2 (p+1)^4 flops per translation pair and RHS; leaf bytes below; other
+    stages 16 B per panel."""
     nc = (order + 1) ** 2
     s = ms * 1e-3
     if name == "m2l":
-        flops = 2.0 * nc * nc * st["n_m2l_pairs"]
+        flops = 2.0 * nc * nc * st["n_m2l_pairs"] * R
         return dict(bound="mfma", achieved=flops / s / 1e12, peak=m2l_peak,
                     unit="TFLOP/s", algorithmic=flops)
     if name == "p2p":
-        # target points + the source points of every P2P leaf pair (x, y, z, q)
-        # + the partial-sum write
-        byts = 16.0 * n + 16.0 * st["n_p2p_source_points"] + 4.0 * n
+        # target points + the source points of every P2P leaf pair (x, y, z,
+        # q; with R > 1 the points and R charges) + the partial-sum writes
+        src = 16.0 if R == 1 else 16.0 + 4.0 * R
+        byts = 16.0 * n + src * st["n_p2p_source_points"] + 4.0 * R * n
         return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                    algorithmic=byts, interactions_per_s=st["n_p2p_interactions"] / s,
-                    valu_tflops=20.0 * st["n_p2p_interactions"] / s / 1e12)
+                    algorithmic=byts, interactions_per_s=R * st["n_p2p_interactions"] / s,
+                    # 18 flops of distance and 1/r per source-target pair,
+                    # shared by the R right-hand sides, + one FMA per RHS
+                    valu_tflops=(18.0 + 2.0 * R) * st["n_p2p_interactions"] / s / 1e12)
     if name == "l2p_out":
         # local coefficients once per leaf + near CSR (weight, column, gathered
-        # sigma) + per panel: point, P2P sum, self term, row pointer, perm, out
-        byts = (4.0 * nc * st["n_leaves"] + 12.0 * st["n_near_pairs"] + 40.0 * n)
+        # sigma) + per panel: point, row pointer, perm and per RHS the P2P sum,
+        # self term and out
+        byts = (4.0 * nc * R * st["n_leaves"] + (8.0 + 4.0 * R) * st["n_near_pairs"]
+                + (28.0 + 12.0 * R) * n)
         return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     algorithmic=byts)
     if name == "m2l_reduce":
         # the GEMM row of every M2L pair and its index, the node's own part
         # written
-        byts = 4.0 * 96 * st["n_m2l_pairs"] + 8.0 * st["n_m2l_pairs"] + 4.0 * nc * st["n_nodes"]
+        byts = (4.0 * nc * R + 8.0) * st["n_m2l_pairs"] + 4.0 * nc * R * st["n_nodes"]
         return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     algorithmic=byts)
-    byts = 16.0 * n
+    byts = 16.0 * n * R
     return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                 algorithmic=byts)
 
@@ -201,7 +217,11 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
         torch.cuda.synchronize()
         ms_m = e2.elapsed_time(e3) / a.steps
         multi = dict(ms=ms_m, value=n_units / (ms_m * 1e-3), wall=time.perf_counter() - t1)
-    stages = op.plan.profile(sig[0], op.self_corr, reps=max(5, a.steps // 2))
+    # stage times of the launch the metric is quoted on (the multi-RHS matvec
+    # when there are several right-hand sides)
+    R = nrhs
+    stages = (op.plan.profile(sig[0], op.self_corr, reps=max(5, a.steps // 2)) if R == 1 else
+              op.plan.profile_multi(torch.stack(sig), op.self_corr, reps=max(5, a.steps // 2)))
     # P2P runs on a side stream beside the far-field chain, so the step is set
     # by the chain (gather .. l2p_out); the roofline is that of the chain's
     # largest kernel, the M2L GEMM (the upward stage is three launches)
@@ -212,16 +232,16 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
     prec = op.plan.m2l_precision
     peak = MFMA_BF16_PEAK_TFLOPS if prec in ("bf16", "fp16", "fp16x3") else MFMA_FP32_PEAK_TFLOPS
     mfma_per_product = 3.0 if prec == "fp16x3" else 1.0
-    rf = stage_roofline(dom, stages[dom], st, op.n, op.plan.order, peak)
+    rf = stage_roofline(dom, stages[dom], st, op.n, op.plan.order, peak, R)
     roofline = dict(kernel=dom, bound=rf["bound"], achieved=rf["achieved"], peak=rf["peak"],
                     unit=rf["unit"], frac=rf["achieved"] / rf["peak"], traffic=None,
-                    launch_ms=stages[dom], algorithmic_per_launch=rf["algorithmic"])
-    key = (op.n, op.plan.order, op.plan.leaf_size)
+                    launch_ms=stages[dom], algorithmic_per_launch=rf["algorithmic"], rhs=R)
+    key = (op.n, op.plan.order, op.plan.leaf_size, R, prec)
     if (dom,) + key in TRAFFIC_BYTES:
-        roofline["traffic"] = TRAFFIC_BYTES[(dom,) + key]
-        roofline["traffic_source"] = PMC_SOURCE
-    m2l = stage_roofline("m2l", stages["m2l"], st, op.n, op.plan.order, peak)
-    p2p = stage_roofline("p2p", stages["p2p"], st, op.n, op.plan.order)
+        roofline["traffic"], roofline["traffic_source"] = TRAFFIC_BYTES[(dom,) + key]
+    m2l = stage_roofline("m2l", stages["m2l"], st, op.n, op.plan.order, peak, R)
+    p2p = stage_roofline("p2p", stages["p2p"], st, op.n, op.plan.order, R=R)
+    red = stage_roofline("m2l_reduce", stages["m2l_reduce"], st, op.n, op.plan.order, R=R)
     extra = dict(m2l_mfma=dict(precision=prec, achieved_tflops=m2l["achieved"],
                                executed_mfma_tflops=m2l["achieved"] * mfma_per_product,
                                peak_tflops=peak,
@@ -231,10 +251,17 @@ def matvec_measure(op, sigma, a, n_units, nrhs=1):
                           interactions_per_s=p2p["interactions_per_s"],
                           valu_tflops=p2p["valu_tflops"],
                           frac_valu=p2p["valu_tflops"] / VALU_FP32_PEAK_TFLOPS,
-                          valu_busy_pmc=P2P_VALU_BUSY.get(key),
-                          launch_ms=stages["p2p"]))
+                          valu_busy_pmc=P2P_VALU_BUSY.get(key[:3]) if R == 1 else None,
+                          traffic=TRAFFIC_BYTES.get(("p2p",) + key, (None,))[0],
+                          launch_ms=stages["p2p"]),
+                 m2l_reduce=dict(achieved_gbs=red["achieved"], frac_hbm=red["achieved"] / HBM_PEAK_GBS,
+                                 algorithmic_per_launch=red["algorithmic"],
+                                 traffic=TRAFFIC_BYTES.get(("m2l_reduce",) + key, (None,))[0],
+                                 launch_ms=stages["m2l_reduce"]))
+    extra["m2l_mfma"]["traffic"] = TRAFFIC_BYTES.get(("m2l",) + key, (None,))[0]
     extra["critical_path"] = dict(chain_stages_ms=sum(stages[k] for k in chain),
-                                  p2p_side_stream_ms=stages["p2p"], ms_per_matvec=ms,
+                                  p2p_side_stream_ms=stages["p2p"],
+                                  ms_per_matvec=ms if multi is None else multi["ms"],
                                   note="profile-mode stage times include their event pairs")
     return dict(ms=ms, wall=wall, stages=stages, roofline=roofline, extra=extra, stats=st,
                 value=n_units / (ms * 1e-3), multi=multi)
@@ -673,25 +700,40 @@ def run_torus(a):
 
 
 def run_pmc(a):
-    """Counter run: --steps matvecs of the sphere workload (order/leaf as given)
-    and three copies of a 256 MiB device buffer, nothing else timed."""
+    """Counter run: --steps matvecs of one workload at the given order / leaf
+    (--pmc-of sphere: the 81,920-panel icosphere; batch: the 64-geometry
+    batch, one 8-RHS matvec per step; torus: the ~300k-panel torus), then
+    three copies of a 256 MiB device buffer (the calibration kernel of known
+    size); nothing else runs on the GPU in between."""
     import numpy as np
     import torch
     from electrodrive_amd.core import bem_mesh
     from electrodrive_amd.core.operator import BemOperator
-    mesh = bem_mesh.icosphere(a.subdiv)
-    op = BemOperator(mesh, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
-                     near=True, self_depth=3, near_depth=3)
-    sig = torch.as_tensor(np.random.default_rng(a.seed).normal(size=op.n), dtype=torch.float64,
-                          device="cuda")
-    out = torch.empty_like(sig)
-    (_, t_mv) = timed(lambda: [op.plan.matvec(sig, op.self_corr, out=out) for _ in range(a.steps)])
+    rng = np.random.default_rng(a.seed)
+    if a.pmc_of == "batch":
+        from electrodrive_amd.learn.train import FAMILIES
+        meshes = [batch_geometry(rng, FAMILIES[g % 4])[0] for g in range(a.n_geoms)]
+        op = BemOperator(meshes, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
+                         near=True, self_depth=3, near_depth=3)
+        S = torch.as_tensor(rng.normal(size=(a.rhs, op.n)), dtype=torch.float64, device="cuda")
+        out = torch.empty_like(S)
+        step = lambda: op.plan.matvec_multi(S, op.self_corr, out=out)  # noqa: E731
+    else:
+        mesh = (bem_mesh.torus_for_panels(1.0, 0.3, a.torus_panels) if a.pmc_of == "torus"
+                else bem_mesh.icosphere(a.subdiv))
+        op = BemOperator(mesh, order=a.order, ws=a.ws, leaf_size=a.leaf, m2l_precision=a.m2l,
+                         near=True, self_depth=3, near_depth=3)
+        sig = torch.as_tensor(rng.normal(size=op.n), dtype=torch.float64, device="cuda")
+        out = torch.empty_like(sig)
+        step = lambda: op.plan.matvec(sig, op.self_corr, out=out)  # noqa: E731
+    (_, t_mv) = timed(lambda: [step() for _ in range(a.steps)])
     src = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
     (_, t_cp) = timed(lambda: [dst.copy_(src) for _ in range(3)])
-    return {"workload": "pmc", "panels": op.n, "order": a.order, "leaf": a.leaf,
-            "matvecs": a.steps, "matvec_s": t_mv, "copy_bytes": 2 * src.numel() * 4,
-            "copies": 3, "copy_s": t_cp}
+    return {"workload": "pmc", "of": a.pmc_of, "panels": op.n, "order": a.order, "leaf": a.leaf,
+            "m2l": a.m2l, "stats": op.plan.stats(), "matvecs": a.steps, "matvec_s": t_mv,
+            "copy_bytes": 2 * src.numel() * 4, "copies": 3, "copy_s": t_cp}
+
 
 if __name__ == "__main__":
     sys.exit(main())

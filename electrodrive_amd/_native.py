@@ -60,6 +60,7 @@ _SIGS = {
     "ed_fmm_matvec_multi_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i32, c_vp]),
     "ed_plan_attach_near": (c_i32, [c_vp, c_vp, c_vp, c_i64]),
     "ed_fmm_profile": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i32, c_vp, c_i32, c_vp]),
+    "ed_fmm_profile_multi": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_i32, c_i32, c_vp, c_i32, c_vp]),
     "ed_dense_matvec_f64": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     "ed_dense_matvec_f32": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),
     "ed_dense_matvec_f64_noalloc": (c_i32, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp]),

@@ -1565,11 +1565,9 @@ int m2l_shape(int p, int prec, int* ncp, int* kp, int* cols, int* enabled) {
 // 4 after the M2L sums, 5 after the downward pass, 7 after P2P, 8 end):
 // stage_ms[0..7] = gather, upward (P2M + M2M), M2L, M2L sums, downward
 // (L2L), 0, P2P (inline here), l2p_out (L2P + near + self + output).
-extern "C" int ed_fmm_profile(ed_plan* P, const double* sigma, const double* selfi, double* out,
-                              int32_t reps, double* stage_ms, int32_t n_stages, ed_stream stream) {
+static int fmm_profile(ed_plan* P, const double* sigma, const double* selfi, double* out,
+                       int nrhs, int32_t reps, double* stage_ms, int32_t n_stages, hipStream_t st) {
   using namespace ed;
-  if (!P || !sigma || !out || !stage_ms || reps < 1) return fail(ED_ERR_ARG, "ed_fmm_profile: bad argument");
-  hipStream_t st = (hipStream_t)stream;
   constexpr int NE = 9;  // mark slots
   constexpr int from[8] = {0, 1, 2, 3, 4, -1, 5, 7};
   constexpr int to[8] = {1, 2, 3, 4, 5, -1, 7, 8};
@@ -1580,7 +1578,7 @@ extern "C" int ed_fmm_profile(ed_plan* P, const double* sigma, const double* sel
   int rc = ED_OK;
   P->prof_events = ev;
   for (int r = 0; r < reps && rc == ED_OK; ++r) {
-    rc = fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, st);
+    rc = fmm_launch(P, sigma, 1, 0, selfi, out, 1, 0, st, nrhs);
     if (rc != ED_OK) break;
     if (hipEventSynchronize(ev[NE - 1]) != hipSuccess) { rc = fail(ED_ERR_DEVICE, "ed_fmm_profile: sync"); break; }
     for (int k = 0; k < 8; ++k) {
@@ -1594,4 +1592,25 @@ extern "C" int ed_fmm_profile(ed_plan* P, const double* sigma, const double* sel
   for (int k = 0; k < NE; ++k) (void)hipEventDestroy(ev[k]);
   for (int k = 0; k < n_stages && k < 8; ++k) stage_ms[k] = acc[k] / reps;
   return rc;
+}
+
+extern "C" int ed_fmm_profile(ed_plan* P, const double* sigma, const double* selfi, double* out,
+                              int32_t reps, double* stage_ms, int32_t n_stages, ed_stream stream) {
+  using namespace ed;
+  if (!P || !sigma || !out || !stage_ms || reps < 1) return fail(ED_ERR_ARG, "ed_fmm_profile: bad argument");
+  return fmm_profile(P, sigma, selfi, out, 1, reps, stage_ms, n_stages, (hipStream_t)stream);
+}
+
+// The same for the multi-RHS matvec of nrhs (1..8) right-hand sides.
+extern "C" int ed_fmm_profile_multi(ed_plan* P, const double* sigma, const double* selfi,
+                                    double* out, int32_t nrhs, int32_t reps, double* stage_ms,
+                                    int32_t n_stages, ed_stream stream) {
+  using namespace ed;
+  if (!P || !sigma || !out || !stage_ms || reps < 1 || nrhs < 1 || nrhs > 8)
+    return fail(ED_ERR_ARG, "ed_fmm_profile_multi: bad argument");
+  if (nrhs > 1) {
+    const int rc = multi_reserve(P, nrhs);
+    if (rc != ED_OK) return rc;
+  }
+  return fmm_profile(P, sigma, selfi, out, nrhs, reps, stage_ms, n_stages, (hipStream_t)stream);
 }

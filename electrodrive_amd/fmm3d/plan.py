@@ -176,6 +176,24 @@ class FmmPlan:
                                            nat.stream_ptr(sigma.device)), "ed_fmm_profile")
         return {k: v for k, v in zip(self.STAGES, list(ms)) if k}
 
+    def profile_multi(self, sigma, self_integrals=None, reps=10):
+        """``profile`` of the multi-RHS matvec: sigma (R, n) FP64, R in 1..8."""
+        if (not sigma.is_cuda or sigma.dim() != 2 or sigma.shape[1] != self.n
+                or sigma.dtype != torch.float64):
+            raise ValueError("sigma must be a (R, n) float64 device tensor")
+        R = int(sigma.shape[0])
+        if not 1 <= R <= 8:
+            raise ValueError("1..8 right-hand sides per call")
+        sigma = sigma.contiguous()
+        out = torch.empty_like(sigma)
+        ms = (ctypes.c_double * len(self.STAGES))()
+        nat.check(nat.lib().ed_fmm_profile_multi(self.handle, nat.ptr(sigma),
+                                                 nat.ptr(self_integrals), nat.ptr(out), R,
+                                                 int(reps), ms, len(self.STAGES),
+                                                 nat.stream_ptr(sigma.device)),
+                  "ed_fmm_profile_multi")
+        return {k: v for k, v in zip(self.STAGES, list(ms)) if k}
+
     def set_coarse(self, Z, self_integrals=None):
         """Coarse space (n, k) float64 device tensor (k <= 32) for the
         solver's preconditioner; None removes it."""

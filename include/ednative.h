@@ -115,6 +115,11 @@ int ed_fmm_matvec_multi_f64(ed_plan* plan, const double* sigma, const double* se
 int ed_fmm_profile(ed_plan* plan, const double* sigma, const double* self_integrals,
                    double* out, int32_t reps, double* stage_ms, int32_t n_stages,
                    ed_stream stream);
+/* The same stage times for the multi-RHS matvec (ed_fmm_matvec_multi_f64) of
+ * nrhs (1..8) right-hand sides: sigma and out (nrhs, n) row-major FP64. */
+int ed_fmm_profile_multi(ed_plan* plan, const double* sigma, const double* self_integrals,
+                         double* out, int32_t nrhs, int32_t reps, double* stage_ms,
+                         int32_t n_stages, ed_stream stream);
 
 /* Attach the near-singular correction (core/bem.py:1838-1957): pairs (P,2)
  * int64 (i, j) and FP64 weights delta_ij, host arrays in original order.  They

@@ -237,11 +237,12 @@ def test_matvec_multi_equals_single(prec):
 
 @pytest.mark.parametrize("prec", ["fp16x3", "valu"])
 def test_matvec_repeatable_bit_for_bit(prec):
-    """The upward pass hands multipoles between blocks (the last child of a
-    parent to finish sums its siblings' products after a device fence): 30
-    matvecs of one vector, interleaved with other vectors, all give the same
-    bits, on one geometry (depth 4-5 trees) and on a batched plan, with one
-    right-hand side and with eight; the first also matches the oracle."""
+    """No atomics or run-order-dependent sums anywhere in the matvec (P2P on a
+    side stream beside the far-field chain, per-item partial sums added in a
+    fixed order): 30 matvecs of one vector, interleaved with other vectors,
+    all give the same bits, on one geometry (depth 4-5 trees) and on a
+    batched plan, with one right-hand side and with eight; the first also
+    matches the oracle."""
     from electrodrive_amd.core import bem_mesh
     from electrodrive_amd.fmm3d.plan import FmmPlan
     rng = np.random.default_rng(99)
@@ -291,3 +292,32 @@ def test_high_orders_accepted_and_accurate(order):
     err = _rel(out, ref)
     print(f"p={order} rel err {err:.2e}")
     assert err < 1e-6
+
+
+def test_stage_profile_one_and_multi_rhs():
+    """ed_fmm_profile / ed_fmm_profile_multi: every stage the launcher marks
+    gets a positive device time, their sum is of the order of one matvec, and
+    the profiled matvec (P2P inline) writes the same bits as the ordinary
+    one (P2P on the side stream)."""
+    from electrodrive_amd.fmm3d.plan import FmmPlan
+    m = _mesh(4)
+    plan = FmmPlan(m.centroids, m.areas, leaf_size=64, order=8, ws=1, m2l_precision="fp16x3")
+    rng = np.random.default_rng(5)
+    S = torch.as_tensor(rng.normal(size=(3, m.n_panels)), device="cuda")
+    one = plan.profile(S[0].contiguous(), reps=3)
+    assert set(one) == {k for k in plan.STAGES if k}
+    assert all(v > 0 for v in one.values()), one
+    multi = plan.profile_multi(S, reps=3)
+    assert set(multi) == set(one) and all(v > 0 for v in multi.values()), multi
+    with pytest.raises(ValueError):
+        plan.profile_multi(torch.zeros(9, m.n_panels, dtype=torch.float64, device="cuda"))
+    ref = plan.matvec_multi(S)
+    out = torch.empty_like(S)
+    import ctypes
+    from electrodrive_amd import _native as nat
+    ms = (ctypes.c_double * 8)()
+    nat.check(nat.lib().ed_fmm_profile_multi(plan.handle, nat.ptr(S), None, nat.ptr(out), 3, 1,
+                                             ms, 8, nat.stream_ptr(S.device)),
+              "ed_fmm_profile_multi")
+    torch.cuda.synchronize()
+    assert torch.equal(out, ref)
