@@ -20,6 +20,7 @@ C restatement of the dense reference matvec, OpenMP, timed on a bounded
 sample of rows on this host).
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -167,6 +168,15 @@ def cpu_baseline(mesh, sigma_np, self_corr_np, budget_s, repeats=3):
                        f"64-row warm-up ({', '.join(f'{x:.2f}' for x in times)} s); full matvec "
                        f"extrapolated {n / (rows / t):.1f} s",
                 host=host, repeats_s=times), n / (rows / t)
+
+
+def quiesce_gc():
+    """Collect, then move every live object out of the collector's
+    generations: otherwise a full collection of the interpreter's
+    (torch-sized) object graph lands inside a timed ~12 ms set-up in some runs
+    (+30 ms), depending only on allocation counts since start-up."""
+    gc.collect()
+    gc.freeze()
 
 
 def timed(fn):
@@ -371,6 +381,7 @@ def run_sphere(a):
     if not a.no_solve:
         op.solve(b, tol=1e-8, precond=a.precond)
         del op
+        quiesce_gc()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         op = build()
@@ -543,6 +554,7 @@ def run_batch(a):
     op = build()
     op.solve(rhs(op), tol=1e-8, precond=a.precond)
     del op
+    quiesce_gc()
     # all R right-hand sides in one solve (one Krylov space per (RHS,
     # geometry) block, one multi-RHS matvec per Arnoldi step)
     op, setup_s = timed(build)
@@ -654,6 +666,7 @@ def run_torus(a):
                              near=True, self_depth=3, near_depth=3)
             op.enable_sai()
             return op
+        quiesce_gc()
         op, setup_s = timed(build)
         b = op.rhs_point_charges(ch)
         (x, info), t = timed(lambda: op.solve(b, tol=1e-8, precond="sai"))

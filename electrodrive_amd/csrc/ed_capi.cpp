@@ -157,6 +157,8 @@ struct DeviceTables {
   float* f32 = nullptr;         // n_tab x ncp x kp (FP32 MFMA)
   uint16_t* h16 = nullptr;      // n_tab x hsz (16-bit and split-FP16 MFMA)
   float* m2m = nullptr;         // 8 x nc x nc, column-major
+  float* m2m7 = nullptr;        // octant 7's M2M alone, padded to a multiple of 4
+                                // floats (16-byte aligned for float4 staging)
   float* l2l = nullptr;         // 8 x nc x nc, column-major
   float* ascale = nullptr;      // nc
   std::vector<int32_t> slot_tab;  // offset slot -> table index, -1 if none
@@ -256,11 +258,17 @@ static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
   if (rc == ED_OK && !tm.empty()) rc = upload(&D.f32, tm);
   if (rc == ED_OK && !th.empty()) rc = upload(&D.h16, th);
   if (rc == ED_OK) rc = upload(&D.m2m, t8);
+  if (rc == ED_OK) {
+    std::vector<float> m7((((size_t)nc * nc + 3) / 4) * 4, 0.0f);
+    std::memcpy(m7.data(), t8.data() + (size_t)7 * nc * nc, sizeof(float) * nc * nc);
+    rc = upload(&D.m2m7, m7);
+  }
   if (rc == ED_OK) rc = upload(&D.l2l, l8);
   if (rc == ED_OK) rc = upload(&D.ascale, af);
   if (rc == ED_OK) rc = upload(&D.flip, reflection_flips(p, std::max(ncp, kp)));
   if (rc != ED_OK) {
-    for (void* q : {(void*)D.valu, (void*)D.f32, (void*)D.h16, (void*)D.m2m, (void*)D.l2l,
+    for (void* q : {(void*)D.valu, (void*)D.f32, (void*)D.h16, (void*)D.m2m, (void*)D.m2m7,
+                    (void*)D.l2l,
                     (void*)D.ascale, (void*)D.flip})
       if (q) (void)hipFree(q);
     return rc;
@@ -456,6 +464,7 @@ int plan_upload(ed_plan* P) {
   P->t_m2l_mfma = DT->f32;
   P->t_m2l_h = DT->h16;
   P->t_m2m = DT->m2m;
+  P->t_m2m7 = reinterpret_cast<const float4*>(DT->m2m7);
   P->t_l2l = DT->l2l;
   // T = diag(a) T_s diag(a): the GEMM kernels multiply each gathered
   // multipole coefficient by a (ascale) and the reduce multiplies each summed

@@ -51,6 +51,16 @@ struct RhsStride {
   int R = 1;
 };
 
+// Axis reflections.  A reflection R_s of the lattice axes (bit a of s: axis a
+// negated) maps a translation operator T(o) to T(R_s o) = D T(o) D, D =
+// diag(+-1), with coefficient k flipped when popcount(s & flip[k]) is odd
+// (the real solid-harmonic coefficients are even or odd under each axis
+// reflection): M2L offsets, M2M and L2L octants all use it.  A sign flip is
+// exact.
+__device__ __forceinline__ float refl(float v, int s, int fk) {
+  return __int_as_float(__float_as_int(v) ^ ((__popc((unsigned)(s & fk)) & 1) << 31));
+}
+
 // ---------------------------------------------------------------------------
 // Solid-harmonic streaming helpers (m-major recurrences; see ed_operators.cpp)
 // ---------------------------------------------------------------------------
@@ -128,9 +138,8 @@ template <typename TS>
 __global__ void k_gather(int n, const int32_t* __restrict__ perm, const TS* __restrict__ sigma,
                          int sigma_tree, const TS* __restrict__ selfi, float4* __restrict__ pts,
                          const float* __restrict__ area_t, float* __restrict__ sig_t,
-                         float* __restrict__ selfc_t, unsigned* __restrict__ mmax_bits) {
+                         float* __restrict__ selfc_t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kMaxSlots && mmax_bits) mmax_bits[i * kMaxSlotStride] = 0u;  // P2M / M2M fold in
   if (i >= n) return;
   const int o = perm[i];
   const float s = (float)sigma[sigma_tree ? i : o];
@@ -140,8 +149,9 @@ __global__ void k_gather(int n, const int32_t* __restrict__ perm, const TS* __re
 }
 
 // Largest |M_k * ascale_k| of the multipoles a wave wrote, folded into one
-// of kMaxSlots running maxima of the matvec (reset by k_gather; the 16-bit
-// M2L takes the largest) as the bit pattern of a non-negative float:
+// of kMaxSlots running maxima of the matvec (the 16-bit M2L takes the
+// largest; k_m2l_reduce resets them for the next matvec once the GEMM has
+// read them, and plans start with them zero) as the bit pattern of a non-negative float:
 // unsigned order is float order there, so the result does not depend on the
 // order of the atomics.  Slots sit a cache line apart, so the atomics of
 // different waves do not queue on one address.  NaN entries are skipped, as
@@ -200,13 +210,20 @@ __device__ __forceinline__ void op_apply(const float* __restrict__ T, const floa
 // ---------------------------------------------------------------------------
 // P2M: one 2-wave block per leaf at level >= 2 (lanes stream the points, the
 // two waves' sums meet in LDS, then a column reduction in a fixed order).
+// The charges come straight from the caller's sigma (q = (float)sigma[perm]
+// * area, the same float k_gather writes for P2P), so P2M need not wait for
+// k_gather, which runs on the side stream with P2P.  Only x, y, z of pts are
+// used (k_gather may be writing .w meanwhile).
 // ---------------------------------------------------------------------------
 constexpr int kP2MThreads = 128;
-template <int P, bool MULTI>
+template <int P, bool MULTI, typename TS>
 __global__ __launch_bounds__(kP2MThreads) void k_p2m(const int32_t* __restrict__ leaves,
                                                      const int2* __restrict__ rng,
                                                      const float4* __restrict__ ctr,
-                                                     const float4* __restrict__ pts, int node2,
+                                                     const float4* __restrict__ pts,
+                                                     const int32_t* __restrict__ perm,
+                                                     const TS* __restrict__ sigma, int sigma_tree,
+                                                     const float* __restrict__ area_t, int node2,
                                                      float* __restrict__ M,
                                                      const float* __restrict__ ascale,
                                                      unsigned* __restrict__ mmax_bits,
@@ -229,9 +246,10 @@ __global__ __launch_bounds__(kP2MThreads) void k_p2m(const int32_t* __restrict__
     float acc[NC];
 #pragma unroll
     for (int k = 0; k < NC; ++k) acc[k] = 0.0f;
+    const TS* sg = sigma + (MULTI ? (size_t)ry * rs.n : 0);
     for (int i = r.x + tid; i < r.y; i += W) {
       const float4 pq = pts[i];
-      const float q = MULTI ? rs.qT[(size_t)i * rs.R + ry] : pq.w;
+      const float q = (float)sg[sigma_tree ? i : perm[i]] * area_t[i];
       p2m_accum<P>((pq.x - c.x) * c.w, (pq.y - c.y) * c.w, (pq.z - c.z) * c.w, q, acc);
     }
     // the two waves' sums per lane: wave 1 writes, wave 0 adds its own
@@ -309,6 +327,113 @@ __global__ __launch_bounds__(256) void k_m2m(const int32_t* __restrict__ parents
 #pragma unroll
   for (int u = 0; u < KO; ++u)
     if (t + W * u < NC) part[h][t + W * u] = acc[u];
+  __syncthreads();
+  float mx = 0.0f;
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) {
+    const float v = part[0][k] + part[1][k];
+    M[(size_t)par * NC + k] = v;
+    if (mmax_bits) mx = fmaxf(mx, fabsf(v * ascale[k]));
+  }
+  if (mmax_bits) fold_absmax(mx, mmax_bits, blockIdx.x * 4 + (threadIdx.x >> 6));  // whole waves
+}
+
+// ---------------------------------------------------------------------------
+// M2M with the operator in LDS (orders whose (p+1)^2 x (p+1)^2 table fits,
+// p <= 12).  The eight octant operators are reflections of the one for
+// octant 7 (child on the + side of every axis): T(o) = D T(7) D with s = ~o &
+// 7, so a block stages that one table once and flips signs instead of
+// reading a child's table from L2 per child (8 x 58 KB per parent at p = 10,
+// which kept the L2-bound per-level M2M at ~10 us).
+// ---------------------------------------------------------------------------
+template <int P>
+struct OctLds {
+  static constexpr int NC = (P + 1) * (P + 1);
+  static constexpr int E4 = (NC * NC + 3) / 4;  // float4s of the padded table
+  static constexpr bool ON = (size_t)E4 * 16 + (size_t)NC * 40 <= 150 * 1024;
+};
+
+// Ts <- canonical octant table (column-major, T[j NC + k], padded to E4
+// float4s, 16-byte aligned in global memory and LDS) by nt = 256 threads,
+// 16 float4 loads in flight per thread.
+template <int E4>
+__device__ __forceinline__ void stage_oct_table(const float4* __restrict__ T7, float4* Ts, int t) {
+  constexpr int NT = 256, U = 16;
+  for (int e0 = 0; e0 < E4; e0 += U * NT) {
+    float4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + t + u * NT;
+      v[u] = T7[min(e, E4 - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int e = e0 + t + u * NT;
+      if (e < E4) Ts[e] = v[u];
+    }
+  }
+}
+
+// M2M, one launch per level, one 4-wave block per parent: the children's
+// multipoles (reflected to octant 7) sit in LDS as xs[j][8]; thread (h, k)
+// sums T(7)[j][k] xs[j][c] over its half of j for all eight children at
+// once (one table read, two broadcast float4 reads, eight FMAs per j), the
+// per-child results are reflected back and added in child order, then the
+// two halves in a fixed order.
+template <int P, bool MULTI>
+__global__ __launch_bounds__(256) void k_m2m_lds(const int32_t* __restrict__ parents,
+                                                 const int32_t* __restrict__ first_child,
+                                                 const int32_t* __restrict__ n_child,
+                                                 const int32_t* __restrict__ octant,
+                                                 const float4* __restrict__ T7,
+                                                 const uint8_t* __restrict__ flip,
+                                                 float* __restrict__ M,
+                                                 const float* __restrict__ ascale,
+                                                 unsigned* __restrict__ mmax_bits, RhsStride rs) {
+  chain_prio();
+  constexpr int NC = (P + 1) * (P + 1), W = 128, KO = (NC + W - 1) / W, JH = (NC + 1) / 2;
+  __shared__ __align__(16) float Ts[OctLds<P>::E4 * 4];
+  __shared__ __align__(16) float xs[NC][8];
+  __shared__ float part[2][NC];
+  __shared__ int sgn[8];
+  __shared__ int fl[NC];
+  if constexpr (MULTI) {
+    M += blockIdx.y * rs.node;
+    if (mmax_bits) mmax_bits += blockIdx.y * rs.slot;
+  }
+  const int par = parents[blockIdx.x];
+  const int c0 = first_child[par], nch = n_child[par];
+  if (threadIdx.x < 8) sgn[threadIdx.x] = threadIdx.x < nch ? (~octant[c0 + threadIdx.x]) & 7 : 0;
+  for (int k = threadIdx.x; k < NC; k += blockDim.x) fl[k] = flip[k];
+  stage_oct_table<OctLds<P>::E4>(T7, reinterpret_cast<float4*>(Ts), threadIdx.x);
+  __syncthreads();
+  for (int e = threadIdx.x; e < 8 * NC; e += blockDim.x) {
+    const int c = e / NC, j = e - c * NC;
+    xs[j][c] = c < nch ? refl(M[(size_t)(c0 + c) * NC + j], sgn[c], fl[j]) : 0.0f;
+  }
+  __syncthreads();
+  const int h = threadIdx.x >> 7, t = threadIdx.x & (W - 1);
+  const int jb = h * JH, je = h ? NC : JH;
+#pragma unroll
+  for (int u = 0; u < KO; ++u) {
+    const int k = min(t + W * u, NC - 1);
+    float acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = 0.0f;
+#pragma unroll 4
+    for (int j = jb; j < je; ++j) {
+      const float tv = Ts[j * NC + k];
+      const float4 a = *reinterpret_cast<const float4*>(&xs[j][0]);
+      const float4 b = *reinterpret_cast<const float4*>(&xs[j][4]);
+      acc[0] = fmaf(tv, a.x, acc[0]); acc[1] = fmaf(tv, a.y, acc[1]);
+      acc[2] = fmaf(tv, a.z, acc[2]); acc[3] = fmaf(tv, a.w, acc[3]);
+      acc[4] = fmaf(tv, b.x, acc[4]); acc[5] = fmaf(tv, b.y, acc[5]);
+      acc[6] = fmaf(tv, b.z, acc[6]); acc[7] = fmaf(tv, b.w, acc[7]);
+    }
+    float y = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; ++c) y += refl(acc[c], sgn[c], fl[k]);
+    if (t + W * u < NC) part[h][k] = y;
+  }
   __syncthreads();
   float mx = 0.0f;
   for (int k = threadIdx.x; k < NC; k += blockDim.x) {
@@ -406,14 +531,8 @@ struct M2LShapeH {
 // below the first right-hand side) and the scale is undone exactly on Y.
 constexpr int kM2LScaleExp = 14;
 
-// Tables exist for offsets with non-negative components only: a pair whose
-// offset is the reflection R_s o of a table's offset o (bit a of s: axis a
-// negated) uses D T(o) D, D = diag(+-1), with coefficient k flipped when
-// popcount(s & flip[k]) is odd.  Applied to the gathered multipole column
-// and to the output column; a sign flip is exact.
-__device__ __forceinline__ float refl(float v, int s, int fk) {
-  return __int_as_float(__float_as_int(v) ^ ((__popc((unsigned)(s & fk)) & 1) << 31));
-}
+// M2L tables exist for offsets with non-negative components only: a pair
+// whose offset is a reflection of a table's offset applies D T(o) D (refl).
 template <int P, bool MULTI>
 __global__ __launch_bounds__(M2LShape<P>::THREADS) void k_m2l_mfma(const int32_t* __restrict__ tile_tab,
                                                   const int32_t* __restrict__ tile_beg,
@@ -880,14 +999,21 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
                                                     const float4* __restrict__ ctr,
                                                     const float* __restrict__ binv,
                                                     const float* __restrict__ Y,
-                                                    float* __restrict__ Lown, RhsStride rs) {
+                                                    float* __restrict__ Lown,
+                                                    unsigned* __restrict__ mmax_bits,
+                                                    RhsStride rs) {
   chain_prio();
   using S = M2LShape<P>;
   constexpr int NC = S::NC, NCP = S::NCP, W = 128, KO = (NC + W - 1) / W, RB = 16;
   if constexpr (MULTI) {
     Y += blockIdx.y * rs.pair;
     Lown += blockIdx.y * rs.node;
+    if (mmax_bits) mmax_bits += blockIdx.y * rs.slot;
   }
+  // the GEMM (previous launch) has read the scale slots: clear them for the
+  // next matvec's P2M / M2M
+  if (mmax_bits && blockIdx.x == 0 && threadIdx.x < kMaxSlots)
+    mmax_bits[threadIdx.x * kMaxSlotStride] = 0u;
   const int t = node0 + blockIdx.x;
   const int b = tptr[t], e = tptr[t + 1];
   const float is = ctr[t].w;
@@ -1217,11 +1343,9 @@ __global__ void k_gather_multi(int n, int R, const int32_t* __restrict__ perm,
                                const TS* __restrict__ sigma, int sigma_tree,
                                const TS* __restrict__ selfi, const float* __restrict__ area_t,
                                float* __restrict__ sig_m, float* __restrict__ qT,
-                               float* __restrict__ selfc_t, unsigned* __restrict__ mmax_bits,
-                               int64_t slot_stride) {
+                               float* __restrict__ selfc_t) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
-  if (i < kMaxSlots && mmax_bits) mmax_bits[r * slot_stride + i * kMaxSlotStride] = 0u;
   if (i >= n) return;
   const int o = perm[i];
   const float s = (float)sigma[(size_t)r * n + (sigma_tree ? i : o)];
@@ -1373,56 +1497,74 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   // P2M and M2M fold into mmax as they write M
   const int prec = Pn->n_m2l_tiles > 0 ? Pn->m2l_precision : -1;
   unsigned* mmax = (prec >= 1 && prec <= 3) ? (multi ? Pn->multi.mmax : Pn->m2l_mmax) : nullptr;
-  if (multi) {
-    if (sig64)
-      hipLaunchKernelGGL(k_gather_multi<double>, dim3(gb, RY), dim3(256), 0, st, n, R, Pn->perm,
-                         (const double*)sigma, sig_tree, (const double*)selfi, Pn->area_t, sigb,
-                         Pn->multi.qT, Pn->selfc_t, mmax, rs.slot);
-    else
-      hipLaunchKernelGGL(k_gather_multi<float>, dim3(gb, RY), dim3(256), 0, st, n, R, Pn->perm,
-                         (const float*)sigma, sig_tree, (const float*)selfi, Pn->area_t, sigb,
-                         Pn->multi.qT, Pn->selfc_t, mmax, rs.slot);
-  } else if (sig64) {
-    hipLaunchKernelGGL(k_gather<double>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
-                       (const double*)sigma, sig_tree, (const double*)selfi, Pn->pts, Pn->area_t,
-                       Pn->sig_t, Pn->selfc_t, mmax);
-  } else {
-    hipLaunchKernelGGL(k_gather<float>, dim3(gb), dim3(256), 0, st, n, Pn->perm,
-                       (const float*)sigma, sig_tree, (const float*)selfi, Pn->pts, Pn->area_t,
-                       Pn->sig_t, Pn->selfc_t, mmax);
-  }
-  ED_LAUNCH_CHECK("gather");
-  ED_MARK(1);
-  // P2P depends only on the gathered points: off the profiling path it runs
-  // on the side stream beside the far-field passes.  (Forked after P2M
-  // instead, P2P became the longer branch: 0.125 vs 0.114 ms per matvec at
-  // 82k panels.)
+  // gather (tree-order sigma, P2P charges, self term) and P2P depend on
+  // nothing else of the matvec: off the profiling path they run on the side
+  // stream beside the far-field passes, whose P2M reads sigma itself.  (P2P
+  // forked after P2M instead became the longer branch: 0.125 vs 0.114 ms per
+  // matvec at 82k panels.)
   const bool fork = Pn->prof_events == nullptr && Pn->n_p2p_items > 0;
-  auto fork_p2p = [&]() -> int {
-    if (!fork) return ED_OK;
+  auto gather = [&](hipStream_t gs) -> int {
+    if (multi) {
+      if (sig64)
+        hipLaunchKernelGGL(k_gather_multi<double>, dim3(gb, RY), dim3(256), 0, gs, n, R, Pn->perm,
+                           (const double*)sigma, sig_tree, (const double*)selfi, Pn->area_t,
+                           sigb, Pn->multi.qT, Pn->selfc_t);
+      else
+        hipLaunchKernelGGL(k_gather_multi<float>, dim3(gb, RY), dim3(256), 0, gs, n, R, Pn->perm,
+                           (const float*)sigma, sig_tree, (const float*)selfi, Pn->area_t, sigb,
+                           Pn->multi.qT, Pn->selfc_t);
+    } else if (sig64) {
+      hipLaunchKernelGGL(k_gather<double>, dim3(gb), dim3(256), 0, gs, n, Pn->perm,
+                         (const double*)sigma, sig_tree, (const double*)selfi, Pn->pts,
+                         Pn->area_t, Pn->sig_t, Pn->selfc_t);
+    } else {
+      hipLaunchKernelGGL(k_gather<float>, dim3(gb), dim3(256), 0, gs, n, Pn->perm,
+                         (const float*)sigma, sig_tree, (const float*)selfi, Pn->pts,
+                         Pn->area_t, Pn->sig_t, Pn->selfc_t);
+    }
+    ED_LAUNCH_CHECK("gather");
+    return ED_OK;
+  };
+  if (fork) {
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_fork, st), "p2p fork");
     ED_HIP_LAUNCH(hipStreamWaitEvent(Pn->side, Pn->ev_fork, 0), "p2p fork");
+    if (int rc = gather(Pn->side); rc != ED_OK) return rc;
     p2p_launch(Pn, R, rs, Pn->side);
     ED_LAUNCH_CHECK("p2p");
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_join, Pn->side), "p2p join");
-    return ED_OK;
-  };
-  if (int rc = fork_p2p(); rc != ED_OK) return rc;
+  } else if (int rc = gather(st); rc != ED_OK) {
+    return rc;
+  }
+  ED_MARK(1);
   const int nleaf = (int)Pn->n_leaves;
   const auto& H = Pn->host;
   const int depth = H.depth;
   if (depth >= 2) {
     const int node2 = (int)H.level_begin[2];
-    hipLaunchKernelGGL((k_p2m<P, MULTI>), dim3(nleaf, RY), dim3(kP2MThreads), 0, st, Pn->leaves,
-                       Pn->node_rng, Pn->node_ctr, Pn->pts, node2, Mb, Pn->m2l_ascale, mmax, rs);
+    if (sig64)
+      hipLaunchKernelGGL((k_p2m<P, MULTI, double>), dim3(nleaf, RY), dim3(kP2MThreads), 0, st,
+                         Pn->leaves, Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->perm,
+                         (const double*)sigma, sig_tree, Pn->area_t, node2, Mb, Pn->m2l_ascale,
+                         mmax, rs);
+    else
+      hipLaunchKernelGGL((k_p2m<P, MULTI, float>), dim3(nleaf, RY), dim3(kP2MThreads), 0, st,
+                         Pn->leaves, Pn->node_rng, Pn->node_ctr, Pn->pts, Pn->perm,
+                         (const float*)sigma, sig_tree, Pn->area_t, node2, Mb, Pn->m2l_ascale,
+                         mmax, rs);
     ED_LAUNCH_CHECK("p2m");
     // upward pass: parents at level l (children at l+1), deepest first
     for (int l = depth - 1; l >= 2; --l) {
       const int64_t pb = Pn->parents_begin[l], pe = Pn->parents_begin[l + 1];
       if (pe > pb) {
-        hipLaunchKernelGGL((k_m2m<P, MULTI>), dim3((unsigned)(pe - pb), RY), dim3(256), 0, st,
-                           Pn->parents_list + pb, Pn->first_child, Pn->n_child, Pn->node_octant,
-                           Pn->t_m2m, Mb, Pn->m2l_ascale, mmax, rs);
+        if constexpr (OctLds<P>::ON)
+          hipLaunchKernelGGL((k_m2m_lds<P, MULTI>), dim3((unsigned)(pe - pb), RY), dim3(256), 0,
+                             st, Pn->parents_list + pb, Pn->first_child, Pn->n_child,
+                             Pn->node_octant, Pn->t_m2m7, Pn->m2l_flip, Mb,
+                             Pn->m2l_ascale, mmax, rs);
+        else
+          hipLaunchKernelGGL((k_m2m<P, MULTI>), dim3((unsigned)(pe - pb), RY), dim3(256), 0, st,
+                             Pn->parents_list + pb, Pn->first_child, Pn->n_child,
+                             Pn->node_octant, Pn->t_m2m, Mb, Pn->m2l_ascale, mmax, rs);
         ED_LAUNCH_CHECK("m2m");
       }
     }
@@ -1473,7 +1615,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
       if (!valu) {
         hipLaunchKernelGGL((k_m2l_reduce<P, MULTI>), dim3(nm, RY), dim3(128), 0, st, node0,
                            Pn->m2l_tgt_ptr, Pn->m2l_tgt_row, Pn->node_ctr, Pn->m2l_binv, Yp, Yb,
-                           rs);
+                           mmax, rs);
         ED_LAUNCH_CHECK("m2l_reduce");
       }
     }

@@ -86,6 +86,7 @@ struct ed_plan {
   const uint8_t* m2l_flip = nullptr;  // per coefficient: reflection sign bits (shared)
   unsigned* m2l_mmax = nullptr;    // 64 slots (128 B apart) of max |M * ascale| as float bits
   float* t_m2m = nullptr;       // 8 * nc * nc
+  const float4* t_m2m7 = nullptr;  // octant-7 M2M operator alone, padded to float4s
   float* t_l2l = nullptr;       // 8 * nc * nc
 
   // near-field correction CSR in tree order
