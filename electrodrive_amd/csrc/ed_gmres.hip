@@ -253,15 +253,6 @@ __global__ void k_finish_norms(const double* __restrict__ partial, int cnt, doub
   }
 }
 
-__global__ void k_copy_vj(int64_t n, const double* __restrict__ V, const int* __restrict__ dj,
-                          double* __restrict__ vin) {
-  const int j = *dj;
-  const double* vj = V + (size_t)j * n;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    vin[i] = vj[i];
-}
-
 // ---- segmented Arnoldi kernels --------------------------------------------
 // One block per tile (segment, lo, hi); per-tile partial sums are reduced per
 // segment over its tile range in a fixed order, so every result is
@@ -324,13 +315,17 @@ __global__ void k_finish_seg_norms(const double* __restrict__ partial, int nt, i
 // V0 = z / beta_g (0 for an inactive segment); g_g = [beta_g, 0...]; j = 0
 __global__ void k_start_seg(const int4* __restrict__ tiles, int G, int m,
                             const double* __restrict__ z, double* __restrict__ V,
-                            double* __restrict__ gv, const double* __restrict__ segv, int ldsv,
-                            int* dj) {
+                            double* __restrict__ vin, double* __restrict__ gv,
+                            const double* __restrict__ segv, int ldsv, int* dj) {
   const int4 tl = tiles[blockIdx.x];
   const double beta = segv[SV_NRM_Z * ldsv + tl.x];
   const bool act = segv[SV_ACTIVE * ldsv + tl.x] != 0.0 && beta > 0.0;
   const double inv = act ? 1.0 / beta : 0.0;
-  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) V[i] = z[i] * inv;
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) {
+    const double v = z[i] * inv;
+    V[i] = v;
+    vin[i] = v;  // the first Arnoldi step's operator input
+  }
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) *dj = 0;
     for (int e = threadIdx.x; e < G * (m + 1); e += blockDim.x) {
@@ -420,6 +415,7 @@ __global__ __launch_bounds__(kRedThreads) void k_update_seg(const int4* __restri
 // (core/bem_solver.py:590-616) and est_g = |g_g[j+1]|.  Scalars: S_EST =
 // max over active segments of est_g / est_tol_g, S_BREAK = 1 when an active
 // segment breaks down.  j++.
+constexpr int kGivLds = 4, kGivLen = 258;  // LDS path: segments, column length
 __global__ void k_givens_seg(int G, int m, const double* __restrict__ partial,
                              const int* __restrict__ seg_t0, const double* __restrict__ h1,
                              const double* __restrict__ h2, double* __restrict__ H,
@@ -444,6 +440,56 @@ __global__ void k_givens_seg(int G, int m, const double* __restrict__ partial,
   __syncthreads();
   double rmax = 0.0;
   int brk = 0;
+  if (G <= kGivLds && j + 2 <= kGivLen) {
+    // few segments (one geometry): stage the column and the previous
+    // rotations in LDS with all threads, so the serial rotation chain of
+    // each segment runs on LDS instead of dependent global loads
+    __shared__ double lcol[kGivLds][kGivLen], lc[kGivLds][kGivLen], ls[kGivLds][kGivLen];
+    for (int e = threadIdx.x; e < G * (j + 1); e += blockDim.x) {
+      const int sg = e / (j + 1), k = e - sg * (j + 1);
+      lcol[sg][k] = h1[(size_t)sg * (m + 2) + k] + h2[(size_t)sg * (m + 2) + k];
+      if (k < j) {
+        lc[sg][k] = cs[(size_t)sg * m + k];
+        ls[sg][k] = sn[(size_t)sg * m + k];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < G) {
+      const int sg = threadIdx.x;
+      double* col = lcol[sg];
+      double* g = gv + (size_t)sg * ld;
+      const double hn = sqrt(segv[SV_HNEXT * ldsv + sg]);
+      col[j + 1] = hn;
+      segv[SV_HNEXT * ldsv + sg] = hn;
+      const bool act = segv[SV_ACTIVE * ldsv + sg] != 0.0;
+      if (act && (!isfinite(hn) || hn <= 0.0)) brk = 1;
+      for (int k = 0; k < j; ++k) {
+        const double t = lc[sg][k] * col[k] + ls[sg][k] * col[k + 1];
+        col[k + 1] = -ls[sg][k] * col[k] + lc[sg][k] * col[k + 1];
+        col[k] = t;
+      }
+      const double den = sqrt(col[j] * col[j] + col[j + 1] * col[j + 1]);
+      double c = 1.0, s = 0.0;
+      if (den >= 1e-14 && isfinite(den)) { c = col[j] / den; s = col[j + 1] / den; }
+      cs[(size_t)sg * m + j] = c;
+      sn[(size_t)sg * m + j] = s;
+      col[j] = c * col[j] + s * col[j + 1];
+      col[j + 1] = 0.0;
+      const double gj = g[j];
+      g[j] = c * gj;
+      g[j + 1] = -s * gj;
+      if (act) {
+        const double et = segv[SV_EST_TOL * ldsv + sg];
+        const double ratio = fabs(g[j + 1]) / (et > 0.0 ? et : 1e-300);
+        rmax = isfinite(ratio) ? fmax(rmax, ratio) : INFINITY;
+      }
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < G * (j + 2); e += blockDim.x) {
+      const int sg = e / (j + 2), k = e - sg * (j + 2);
+      H[((size_t)sg * m + j) * ld + k] = lcol[sg][k];
+    }
+  } else
   for (int sg = threadIdx.x; sg < G; sg += blockDim.x) {
     const double v = segv[SV_HNEXT * ldsv + sg];
     const double* a1 = h1 + (size_t)sg * (m + 2);
@@ -495,17 +541,23 @@ __global__ void k_givens_seg(int G, int m, const double* __restrict__ partial,
   }
 }
 
-// V_j = w / hnext_g over the tile (j already incremented; 0 after a breakdown)
+// V_j = w / hnext_g over the tile (j already incremented; 0 after a
+// breakdown), also written to vin, the next step's operator input (a fixed
+// address, so a captured step needs no copy of V_j)
 __global__ void k_scale_next_seg(const int4* __restrict__ tiles, int64_t n,
                                  const double* __restrict__ w, double* __restrict__ V,
-                                 const int* __restrict__ dj, const double* __restrict__ segv,
-                                 int ldsv) {
+                                 double* __restrict__ vin, const int* __restrict__ dj,
+                                 const double* __restrict__ segv, int ldsv) {
   const int j = *dj;
   const int4 tl = tiles[blockIdx.x];
   const double hn = segv[SV_HNEXT * ldsv + tl.x];
   const double inv = (hn > 0.0 && isfinite(hn)) ? 1.0 / hn : 0.0;
   double* vj = V + (size_t)j * n;
-  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) vj[i] = w[i] * inv;
+  for (int i = tl.y + threadIdx.x; i < tl.z; i += blockDim.x) {
+    const double v = w[i] * inv;
+    vj[i] = v;
+    vin[i] = v;
+  }
 }
 
 // y_g = R_g^{-1} g_g (back substitution, m_use = *dj), one thread per segment,
@@ -938,7 +990,6 @@ struct Solver {
   }
   int arnoldi_step() {
     const int m = W->m, nt = NT(), g = G(), ld = W->Gmax;
-    hipLaunchKernelGGL(k_copy_vj, dim3(gb), dim3(256), 0, W->stream, n, W->V, W->dj, W->vin);
     if (use_sai()) {
       ED_T(approx_op(W->vin, W->u2));
       sai_precond(W->u2, W->w);
@@ -960,7 +1011,7 @@ struct Solver {
     hipLaunchKernelGGL(k_givens_seg, dim3(1), dim3(256), 0, W->stream, g, m, W->partial, seg_t0(),
                        W->h, W->h2, W->H, W->g, W->cs, W->sn, W->segv, ld, W->scal, W->dj);
     hipLaunchKernelGGL(k_scale_next_seg, dim3(nt), dim3(256), 0, W->stream, tiles(), n, W->w,
-                       W->V, W->dj, W->segv, ld);
+                       W->V, W->vin, W->dj, W->segv, ld);
     ED_KCHECK("arnoldi");
     return ED_OK;
   }
@@ -1009,7 +1060,7 @@ struct Solver {
       ED_G(hipMemcpyAsync(W->segv + SV_EST_TOL * ld, W->hsegv + SV_EST_TOL * ld,
                           sizeof(double) * 2 * ld, hipMemcpyHostToDevice, W->stream));
       hipLaunchKernelGGL(k_start_seg, dim3(nt), dim3(256), 0, W->stream, tiles(), g, m, W->w, W->V,
-                         W->g, W->segv, ld, W->dj);
+                         W->vin, W->g, W->segv, ld, W->dj);
       ED_G(hipMemsetAsync(W->scal + S_BREAK, 0, sizeof(double), W->stream));
       ED_KCHECK("start");
       int j = 0;
