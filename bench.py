@@ -432,6 +432,14 @@ def run_sphere(a):
                 iteration_ratio_plain_over_learned=ip["iters"] / max(1, il["iters"]),
                 iteration_ratio_without_over_with_network=if_["iters"] / max(1, il["iters"]),
                 weights="electrodrive_amd/learn/weights/precond_mlp.npz")
+            pays = il["iters"] < if_["iters"] and tl <= ts
+            solve["learned_vs_plain"]["network_pays_here"] = pays
+            solve["learned_vs_plain"]["note"] = (
+                "the network's coarse space beats the fixed one in iterations and SAI alone in "
+                "wall time" if pays else
+                "the iteration cut over plain Jacobi GMRES comes from the SAI smoother and the "
+                "coarse space; the network (trained on 2k-8k-panel geometries) adds no "
+                "iterations saved over the fixed coarse space and costs wall time at this size")
         sigma = x
     else:
         sigma = torch.as_tensor(rng.normal(size=n), dtype=torch.float64, device=dev)
