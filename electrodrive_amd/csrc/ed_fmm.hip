@@ -669,63 +669,83 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
     Y += blockIdx.y * rs.pair;
     mmax_bits += blockIdx.y * rs.slot;
   }
-  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, COLS = S::COLS;
+  constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
-  __shared__ __attribute__((aligned(16))) T16 bm[COLS * KS];
+  __shared__ float as_l[KP];  // ascale x the power-of-two input scale, 0 past NC
   __shared__ uint8_t fl[NCP];
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
-  for (int k = threadIdx.x; k < NCP; k += 256) fl[k] = flip[k];
-  // power-of-two input scale from the largest |M * ascale| of this matvec
-  const float gmax = slots_absmax(mmax_bits);
-  const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
-  const float inv_sc = 1.0f / sc;
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
-  // table -> LDS, 8 elements (16 bytes) per load; rows are KP (a multiple of
-  // 16) elements long, so a chunk never crosses a row
-  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * NCP * KP);
-  constexpr int N8 = NCP * KP / 8;
-#pragma unroll 4
-  for (int t = threadIdx.x; t < N8; t += 256) {
-    const int e = 8 * t, i = e / KP, k = e - i * KP;
-    *reinterpret_cast<uint4*>(ta + i * KS + k) = T8[t];
-  }
-  // gathered, scaled multipoles: consecutive threads read consecutive
-  // coefficients of one source row
-  // (unconditional clamped loads, masked after: no serialising branches)
-  __syncthreads();  // fl
-#pragma unroll 8
-  for (int t = threadIdx.x; t < COLS * KP; t += 256) {
-    const int c = t / KP, k = t - c * KP;
-    const int cc = min(c, cnt - 1), kk = min(k, NC - 1);
-    const float v = refl(M[(size_t)pair_src[beg + cc] * NC + kk] * ascale[kk] * sc,
-                         pair_sgn[beg + cc], fl[kk]);
-    bm[c * KS + k] = (T16)((c < cnt && k < NC) ? v : 0.0f);
-  }
-  __syncthreads();
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int col0 = wave * 32;
-  if (col0 >= cnt) return;
   const int li = lane & 31, lk = lane >> 5;
+  const int col = col0 + li;
+  const bool cv = col < cnt;
+  // Every global load before the first use (as in k_m2l_mfma16x3): the
+  // pair's source and sign, the scale slots, ascale / flip, the table (-> LDS,
+  // 16 bytes per load, all in flight at p <= 10) and this lane's multipole
+  // column, which goes into the B fragment straight from registers.
+  const int sg = pair_sgn[beg + min(col, cnt - 1)];
+  const int src = pair_src[beg + min(col, cnt - 1)];
+  const unsigned slot_bits = mmax_bits[(threadIdx.x & (kMaxSlots - 1)) * kMaxSlotStride];
+  static_assert(KP <= 256 && NCP <= 256, "one ascale / flip entry per thread");
+  const float as_v = (int)threadIdx.x < NC ? ascale[threadIdx.x] : 0.0f;
+  const uint8_t fl_v = (int)threadIdx.x < NCP ? flip[threadIdx.x] : (uint8_t)0;
+  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * NCP * KP);
+  constexpr int N8 = NCP * KP / 8, TU = 16;
+  constexpr bool kWhole = N8 % (TU * 256) == 0;
+  float mv[NSV][8];  // k = 16 sv + 8 lk + j
+  for (int t0 = 0; t0 < N8; t0 += TU * 256) {
+    uint4 tv[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) tv[u] = T8[min(t0 + (int)threadIdx.x + 256 * u, N8 - 1)];
+    if (t0 == 0 && col0 < cnt) {
+      const float* mrow = M + (size_t)src * NC;
+#pragma unroll
+      for (int sv = 0; sv < NSV; ++sv)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, NC - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      const int t = t0 + (int)threadIdx.x + 256 * u;
+      if (kWhole || t < N8) {
+        const int e = 8 * t, i = e / KP, k = e - i * KP;
+        *reinterpret_cast<uint4*>(ta + i * KS + k) = tv[u];
+      }
+    }
+  }
+  // power-of-two input scale from the largest |M * ascale| of this matvec
+  float gmax = __uint_as_float(slot_bits);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, o, 64));
+  const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
+  const float inv_sc = 1.0f / sc;
+  if ((int)threadIdx.x < KP) as_l[threadIdx.x] = as_v * sc;
+  if ((int)threadIdx.x < NCP) fl[threadIdx.x] = fl_v;
+  __syncthreads();
+  if (col0 >= cnt) return;  // no barrier below
   constexpr int NRB = NCP / 32;
   f32x16 acc[NRB];
 #pragma unroll
   for (int rb = 0; rb < NRB; ++rb)
 #pragma unroll
     for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
-  const T16* brow = bm + (col0 + li) * KS + 8 * lk;
-#pragma unroll 2
-  for (int sv = 0; sv < KP / 16; ++sv) {
-    const V b = *reinterpret_cast<const V*>(brow + 16 * sv);
+#pragma unroll
+  for (int sv = 0; sv < NSV; ++sv) {
+    V bv;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int k = 16 * sv + 8 * lk + j;
+      bv[j] = (T16)(cv ? refl(mv[sv][j] * as_l[k], sg, fl[k]) : 0.0f);
+    }
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb) {
       const V a = *reinterpret_cast<const V*>(ta + (rb * 32 + li) * KS + 16 * sv + 8 * lk);
-      acc[rb] = Mfma16<T16>::run(a, b, acc[rb]);
+      acc[rb] = Mfma16<T16>::run(a, bv, acc[rb]);
     }
   }
-  const int col = col0 + li;
-  if (col < cnt) {
-    const int sg = pair_sgn[beg + col];
+  if (cv) {
     float* yrow = Y + (size_t)(beg + col) * NCP;
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb)
@@ -851,75 +871,56 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
   const int li = lane & 31, lk = lane >> 5;
   const int col = col0 + li;
   const bool cv = col < cnt;
-  // this lane's multipole values: k = 16 sv + 8 lk + j (clamped addresses)
-  float mv[NSV][8];
+  // Every global load of the block is issued before the first use, the
+  // table first: its two halves -> LDS, 8 elements (16 bytes) per load, at
+  // most 16 in flight per thread (all of them at p <= 10); rows are KP (a
+  // multiple of 16) elements long, so a chunk never crosses a row.  (In
+  // batches of 4 behind the scale lookup this took five dependent round
+  // trips.)
   const int sg = pair_sgn[beg + min(col, cnt - 1)];
-  if (col0 < cnt) {
-    const float* mrow = M + (size_t)pair_src[beg + min(col, cnt - 1)] * NC;
+  const int src = pair_src[beg + min(col, cnt - 1)];
+  const unsigned slot_bits = mmax_bits[(threadIdx.x & (kMaxSlots - 1)) * kMaxSlotStride];
+  static_assert(KP <= 256 && NCP <= 256, "one ascale / flip entry per thread");
+  const float as_v = (int)threadIdx.x < NC ? ascale[threadIdx.x] : 0.0f;
+  const uint8_t fl_v = (int)threadIdx.x < NCP ? flip[threadIdx.x] : (uint8_t)0;
+  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
+  constexpr int N8 = NCP * KP / 8, TU = 16;
+  constexpr bool kWhole = (2 * N8) % (TU * 256) == 0;  // no partial batch (p = 10)
+  float mv[NSV][8];  // this lane's multipole values: k = 16 sv + 8 lk + j
+  for (int t0 = 0; t0 < 2 * N8; t0 += TU * 256) {
+    uint4 tv[TU];
 #pragma unroll
-    for (int sv = 0; sv < NSV; ++sv)
+    for (int u = 0; u < TU; ++u) tv[u] = T8[min(t0 + (int)threadIdx.x + 256 * u, 2 * N8 - 1)];
+    if (t0 == 0 && col0 < cnt) {  // the multipole gathers behind the first batch
+      const float* mrow = M + (size_t)src * NC;
 #pragma unroll
-      for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, NC - 1)];
+      for (int sv = 0; sv < NSV; ++sv)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) mv[sv][j] = mrow[min(16 * sv + 8 * lk + j, NC - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      const int t = t0 + (int)threadIdx.x + 256 * u;
+      if (kWhole || t < 2 * N8) {
+        const int hh = t >= N8 ? 1 : 0;
+        const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
+        *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = tv[u];
+      }
+    }
   }
   // power-of-two input scale from the largest |M * ascale| of this matvec
-  const float gmax = slots_absmax(mmax_bits);
+  // (the slots, loaded above, reduced across the wave)
+  float gmax = __uint_as_float(slot_bits);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, o, 64));
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
   const float inv_sc = 1.0f / sc;
-  // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
-  // multiple of 16) elements long, so a chunk never crosses a row
-  const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
-  constexpr int N8 = NCP * KP / 8;
-#pragma unroll 4
-  for (int t = threadIdx.x; t < 2 * N8; t += 256) {
-    const int hh = t >= N8 ? 1 : 0;
-    const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
-    *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
-  }
-  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc : 0.0f;
-  for (int k = threadIdx.x; k < NCP; k += 256) fl[k] = flip[k];
+  if ((int)threadIdx.x < KP) as_l[threadIdx.x] = as_v * sc;
+  if ((int)threadIdx.x < NCP) fl[threadIdx.x] = fl_v;
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
-  constexpr int NRB = NCP / 32;
-  f32x16 acc[NRB];
-#pragma unroll
-  for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
-#pragma unroll
-  for (int sv = 0; sv < NSV; ++sv) {
-    f16x8 bh, bl;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = 16 * sv + 8 * lk + j;
-      const float v = cv ? refl(mv[sv][j] * as_l[k], sg, fl[k]) : 0.0f;
-      const _Float16 h = (_Float16)v;
-      bh[j] = h;
-      bl[j] = (_Float16)(v - (float)h);
-    }
-#pragma unroll
-    for (int rb = 0; rb < NRB; ++rb) {
-      const int at = (rb * 32 + li) * KS + 16 * sv + 8 * lk;
-      const f16x8 ah = *reinterpret_cast<const f16x8*>(ta + at);
-      const f16x8 al = *reinterpret_cast<const f16x8*>(ta + NCP * KS + at);
-      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh, acc[rb], 0, 0, 0);
-      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl, acc[rb], 0, 0, 0);
-      acc[rb] = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh, acc[rb], 0, 0, 0);
-    }
-  }
-  if (cv) {
-    float* yrow = Y + (size_t)(beg + col) * NCP;
-#pragma unroll
-    for (int rb = 0; rb < NRB; ++rb)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = rb * 32 + q * 8 + lk * 4;
-        float4 v = make_float4(refl(acc[rb][4 * q + 0] * inv_sc, sg, fl[row]),
-                               refl(acc[rb][4 * q + 1] * inv_sc, sg, fl[row + 1]),
-                               refl(acc[rb][4 * q + 2] * inv_sc, sg, fl[row + 2]),
-                               refl(acc[rb][4 * q + 3] * inv_sc, sg, fl[row + 3]));
-        *reinterpret_cast<float4*>(yrow + row) = v;
-      }
-  }
+  m2l16x3_column<P, false>(ta, as_l, fl, sg, mv, 1.0f, inv_sc, cv, li, lk,
+                           Y + (size_t)(beg + col) * NCP);
 }
 
 // Several right-hand sides: one block per tile runs the tile for every RHS
@@ -953,27 +954,40 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
   const int li = lane & 31, lk = lane >> 5;
   const int col = col0 + li;
   const bool cv = col < cnt;
-  // this lane's multipole values of RHS r: k = 16 sv + 8 lk + j (clamped)
-  float mv[NSV][8];
-  int src = 0;
+  // every global load before the first use, as in k_m2l_mfma16x3
   const int sg = pair_sgn[beg + min(col, cnt - 1)];
-  if (col0 < cnt) {
-    src = pair_src[beg + min(col, cnt - 1)];
-    m2l16x3_load<P>(M + (size_t)src * NC, lk, mv);
-  }
-  const float sc0 = m2l_scale(mmax_bits);
-  // both halves -> LDS, 8 elements (16 bytes) per load; rows are KP (a
-  // multiple of 16) elements long, so a chunk never crosses a row
+  const int src = pair_src[beg + min(col, cnt - 1)];
+  const unsigned slot_bits = mmax_bits[(threadIdx.x & (kMaxSlots - 1)) * kMaxSlotStride];
+  static_assert(KP <= 256 && NCP <= 256, "one ascale / flip entry per thread");
+  const float as_v = (int)threadIdx.x < NC ? ascale[threadIdx.x] : 0.0f;
+  const uint8_t fl_v = (int)threadIdx.x < NCP ? flip[threadIdx.x] : (uint8_t)0;
+  // both table halves -> LDS, 8 elements (16 bytes) per load, 16 in flight
+  // per thread; rows are KP (a multiple of 16) elements long
   const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
-  constexpr int N8 = NCP * KP / 8;
-#pragma unroll 4
-  for (int t = threadIdx.x; t < 2 * N8; t += 256) {
-    const int hh = t >= N8 ? 1 : 0;
-    const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
-    *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = T8[t];
+  constexpr int N8 = NCP * KP / 8, TU = 16;
+  constexpr bool kWhole = (2 * N8) % (TU * 256) == 0;
+  float mv[NSV][8];  // this lane's multipole values of RHS r: k = 16 sv + 8 lk + j
+  for (int t0 = 0; t0 < 2 * N8; t0 += TU * 256) {
+    uint4 tv[TU];
+#pragma unroll
+    for (int u = 0; u < TU; ++u) tv[u] = T8[min(t0 + (int)threadIdx.x + 256 * u, 2 * N8 - 1)];
+    if (t0 == 0 && col0 < cnt) m2l16x3_load<P>(M + (size_t)src * NC, lk, mv);
+#pragma unroll
+    for (int u = 0; u < TU; ++u) {
+      const int t = t0 + (int)threadIdx.x + 256 * u;
+      if (kWhole || t < 2 * N8) {
+        const int hh = t >= N8 ? 1 : 0;
+        const int e = 8 * (t - hh * N8), i = e / KP, k = e - i * KP;
+        *reinterpret_cast<uint4*>(ta + hh * NCP * KS + i * KS + k) = tv[u];
+      }
+    }
   }
-  for (int k = threadIdx.x; k < KP; k += 256) as_l[k] = k < NC ? ascale[k] * sc0 : 0.0f;
-  for (int k = threadIdx.x; k < NCP; k += 256) fl[k] = flip[k];
+  float gmax = __uint_as_float(slot_bits);
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, o, 64));
+  const float sc0 = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
+  if ((int)threadIdx.x < KP) as_l[threadIdx.x] = as_v * sc0;
+  if ((int)threadIdx.x < NCP) fl[threadIdx.x] = fl_v;
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
   m2l16x3_column<P, false>(ta, as_l, fl, sg, mv, 1.0f, 1.0f / sc0, cv, li, lk,
