@@ -35,6 +35,11 @@ from .kernels import K_E
 # most solves), deeper asks stall the inner GMRES at the operator's rounding.
 INNER_TOL = {"fp32": 1e-5, "fp16x3": 1e-5, "bf16": 1e-2, "fp16": 5e-4}
 
+# target-panel pair count from which potential_at(method="auto") takes the
+# FMM with a one-shot tree: at 1M targets x 82k panels (8e10 pairs) the FP64
+# direct kernel takes 47 ms and the FMM 113 ms, most of it the host tree
+FMM_TARGET_PAIRS = 1 << 38
+
 
 class FmmOperator:
     """Callable A(sigma) backed by a native plan; exposes the native solver to
@@ -273,19 +278,44 @@ class BemOperator:
         return out
 
     def potential_at(self, points, sigma, charges=(), near=True, near_depth=None,
-                     distance_factor=1.5, quad_order=2):
-        """V_total = V_free + V_induced at arbitrary points, the induced part by
-        the native target kernel plus (``near``) the near-singular correction
-        of core/bem.py:940-1047."""
+                     distance_factor=1.5, quad_order=2, method="auto"):
+        """V_total = V_free + V_induced at arbitrary points (core/bem.py:
+        1441-1522), the induced part by the native target kernel (``method=
+        "direct"``: FP64, O(M N)) or by the FMM (``"fmm"``: the panels and the
+        targets in one lattice tree, targets as charge-free points, O(M + N),
+        FP32 with the plan's expansion order), plus (``near``) the
+        near-singular correction of core/bem.py:940-1047.  ``"auto"`` takes
+        the FMM from M N >= 2^38 on; for many densities at the same targets
+        ``fmm_targets(points)`` keeps the tree."""
         from .bem_kernel import bem_potential_targets
         P = torch.as_tensor(points, dtype=torch.float64).to(self.device).reshape(-1, 3)
         s = sigma.to(self.device, torch.float64)
-        V = bem_potential_targets(P, self.C, self.A, s)
+        if method == "auto":
+            method = "fmm" if P.shape[0] * self.n >= FMM_TARGET_PAIRS else "direct"
+        if method == "fmm":
+            V = self._fmm_potential_targets(P, s)
+        elif method == "direct":
+            V = bem_potential_targets(P, self.C, self.A, s)
+        else:
+            raise ValueError(f"method must be 'auto', 'direct' or 'fmm', got {method!r}")
         if near:
             V = bq.near_quadrature_potentials(V, P, self.C, self.A, s, self.PV,
                                               distance_factor=distance_factor,
                                               quad_order=quad_order, max_depth=near_depth)
         return V + self.free_space(P, charges)
+
+    def fmm_targets(self, points):
+        """An FMM evaluator of the induced potential at fixed target points
+        for any number of densities (``FmmTargets.potential(sigma)``): the
+        panels and the targets in one lattice tree, built once."""
+        return FmmTargets(self, points)
+
+    def _fmm_potential_targets(self, P, s):
+        ev = FmmTargets(self, P)
+        try:
+            return ev.potential(s)
+        finally:
+            ev.close()
 
     def field_at(self, points, sigma, charges=()):
         """E_total = E_free + E_induced at arbitrary points (core/bem.py:1497-1520)."""
@@ -296,6 +326,46 @@ class BemOperator:
 
     def total_charge(self, sigma):
         return float((sigma.to(torch.float64) * self.A).sum())
+
+
+class FmmTargets:
+    """sum_j K_E A_j sigma_j / |x - c_j| at fixed targets by FMM matvecs over
+    the panels and the targets together: targets are zero-area (charge-free)
+    points of one lattice tree, so they receive the far field by L2P from
+    their leaves and the near field by P2P like panels while contributing
+    nothing.  Built once per target set (host tree O((N + M) log(N + M))),
+    then O(N + M) per density; FP32 with the operator's order and leaf size
+    and split-FP16 M2L (relative error ~2e-6 against the FP64 direct sum)."""
+
+    def __init__(self, op, points):
+        P = torch.as_tensor(points, dtype=torch.float64).reshape(-1, 3)
+        if not bool(torch.isfinite(P).all()):
+            raise ValueError("target points must be finite")
+        self.op, self.m, self.plan = op, int(P.shape[0]), None
+        if self.m == 0:
+            return
+        pts = np.concatenate([op.C.cpu().numpy(), P.cpu().numpy()])
+        area = np.concatenate([op.A.cpu().numpy(), np.zeros(self.m)])
+        self.plan = FmmPlan(pts, area, leaf_size=op.plan.leaf_size, order=op.plan.order,
+                            ws=op.plan.ws, m2l_precision="fp16x3")
+
+    def potential(self, sigma):
+        op = self.op
+        if self.m == 0:
+            return torch.zeros(0, dtype=torch.float64, device=op.device)
+        if self.plan is None:
+            raise RuntimeError("FmmTargets is closed")
+        s = sigma.to(op.device, torch.float64).reshape(-1)
+        if s.shape[0] != op.n:
+            raise ValueError(f"sigma must have {op.n} entries, got {s.shape[0]}")
+        sig = torch.cat([s, torch.zeros(self.m, dtype=torch.float64, device=op.device)])
+        return self.plan.matvec(sig)[op.n:].clone()
+
+    def close(self):
+        if self.plan is not None:
+            torch.cuda.current_stream(self.op.device).synchronize()  # before the plan goes
+            self.plan.close()
+            self.plan = None
 
 
 _ = np
