@@ -170,6 +170,50 @@ def cpu_baseline(mesh, sigma_np, self_corr_np, budget_s, repeats=3):
                 host=host, repeats_s=times), n / (rows / t)
 
 
+def cpu_reference_fmm(m2l_sample=48, subdiv=4, order=8, leaf=64, theta=0.5):
+    """Second CPU-baseline leg: the reference's own FMM algorithm (oracle's
+    numpy restatement: adaptive tree and dual-tree lists, every translation
+    matrix built per pair by least squares at every matvec) at a small size,
+    icosphere s = 4 (5,120 panels), the headline's p = 8 / leaf 64 / theta =
+    0.5.  Bounded sample: every stage runs in full except M2L, of which the
+    first ``m2l_sample`` pairs are built and applied and the rest
+    extrapolated at their mean time; the dense leg at the same size is timed
+    beside it (the faster of the two is the baseline at that size)."""
+    import numpy as np
+    from electrodrive_amd.core import bem_mesh
+    from oracle import cdense, ref_fmm  # CPU baseline only
+    try:
+        from threadpoolctl import threadpool_info
+        blas = max([d.get("num_threads", 1) for d in threadpool_info()] or [1])
+    except Exception:
+        blas = None
+    mesh = bem_mesh.icosphere(subdiv)
+    n = mesh.n_panels
+    sig = np.random.default_rng(0).normal(size=n)
+    st = {}
+    t0 = time.perf_counter()
+    ref_fmm.fmm_matvec(mesh.centroids, mesh.areas, sig, p=order, leaf_size=leaf, theta=theta,
+                       stats=st, m2l_sample=m2l_sample)
+    wall = time.perf_counter() - t0
+    stages = st["stage_s"]
+    t_full = wall - stages["m2l"] + st["m2l_s_per_pair"] * st["n_m2l_pairs"]
+    t1 = time.perf_counter()
+    cdense.dense_rows(mesh.centroids, mesh.areas, sig, np.zeros(n), 0, n)
+    t_dense = time.perf_counter() - t1
+    return dict(value=n / t_full, unit="panels/s", cores=blas, kind="port",
+                panels=n, order=order, leaf=leaf, theta=theta,
+                matvec_s_extrapolated=t_full, m2l_pairs=st["n_m2l_pairs"],
+                m2l_s_per_pair=st["m2l_s_per_pair"], stage_s=stages,
+                dense_leg_same_size=dict(value=n / t_dense, unit="panels/s",
+                                         cores=cdense.max_threads(), matvec_s=t_dense),
+                sample=f"restated reference FMM (oracle/ref_fmm.py, numpy/LAPACK) on {n} panels: "
+                       f"tree, lists, P2M, M2M, L2L, L2P, P2P in full, {m2l_sample} of "
+                       f"{st['n_m2l_pairs']} per-pair least-squares M2L builds timed "
+                       f"({st['m2l_s_per_pair'] * 1e3:.0f} ms each) and the rest extrapolated; "
+                       f"the dense leg at that size takes {t_dense:.3f} s, so it is the "
+                       f"baseline there")
+
+
 def quiesce_gc():
     """Collect, then move every live object out of the collector's
     generations: otherwise a full collection of the interpreter's
@@ -453,6 +497,7 @@ def run_sphere(a):
     if not a.no_cpu:
         cpu, cpu_full = cpu_baseline(mesh, sigma.cpu().numpy(), op.self_corr.cpu().numpy(),
                                      a.cpu_seconds)
+        cpu["reference_fmm_leg"] = cpu_reference_fmm()
         if solve is not None:
             # the reference method: Jacobi GMRES, one dense matvec per iteration
             lvp = solve.get("learned_vs_plain")

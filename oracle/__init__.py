@@ -27,6 +27,10 @@ to the Electrodrive repository root) in plain numpy (FP64) or C.  Modules:
 - ``ref_tree``    the reference's adaptive tight-bbox octree and dual-tree MAC
                   interaction lists (fmm3d/tree.py:801-960,
                   fmm3d/interaction_lists.py:273-528)
+- ``ref_fmm``     the reference's CPU FMM matvec with per-pair least-squares
+                  translation builders (fmm3d/bem_fmm.py:460-560,
+                  fmm3d/multipole_operators.py:369-981), timed as the second
+                  CPU-baseline leg
 - ``mlp``         FP64 forward/backward of the learned-preconditioner MLP and
                   its projection loss (no reference counterpart)
 - ``sai``         sparse approximate inverse smoother (no reference counterpart)
