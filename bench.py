@@ -362,6 +362,30 @@ def matvec_accuracy(mesh, op, a, rng):
     return out
 
 
+def refine_record(op, b, info, solve_s, dense_s, precond, compare=True):
+    """Refinement residuals of a solve: dense FP64 residuals and updated ones
+    (r <- r - A_fmm d), the updated residual's bound, the dense residual's
+    time and its share of the solve; with ``compare`` the same solve with
+    every refinement residual dense beside it."""
+    import torch
+    bn = float(torch.linalg.norm(b))
+    rec = dict(dense_residuals=info["n_accurate"], updated_residuals=info["n_updated"],
+               update_operator_rel_err=info["update_err"],
+               updated_resid_bound_rel=info["resid_bound"] / bn,
+               dense_residual_s=dense_s,
+               dense_residual_share=min(1.0, info["n_accurate"] * dense_s / max(solve_s, 1e-12)))
+    if compare:
+        op.solve(b, tol=1e-8, precond=precond, residual_update=False)  # warm
+        (x0, i0), t0 = timed(lambda: op.solve(b, tol=1e-8, precond=precond,
+                                              residual_update=False))
+        rr0 = float(torch.linalg.norm(b - op.accurate(x0))) / bn
+        rec["all_dense_refinement"] = dict(
+            solve_s=t0, dense_residuals=i0["n_accurate"], refine_steps=i0["refine_steps"],
+            iters=i0["iters"], code=i0["code"], rel_resid_fp64_dense=rr0,
+            dense_residual_share=min(1.0, i0["n_accurate"] * dense_s / max(t0, 1e-12)))
+    return rec
+
+
 def base_line(a, value, ms, dtype, config):
     return {
         "metric": "time-to-1e-8 residual + FMM matvec panels/s; learned-precond vs plain GMRES",
@@ -440,13 +464,14 @@ def run_sphere(a):
         from oracle import analytic  # checker only
         sig_k = analytic.kelvin_sigma(mesh.centroids, 1.0, r0, 1.0)
         err_k = float(np.linalg.norm(x.cpu().numpy() - sig_k) / np.linalg.norm(sig_k))
-        r = b - op.accurate(x)
+        r, t_acc = timed(lambda: b - op.accurate(x))
         rel = float(torch.linalg.norm(r) / torch.linalg.norm(b))
         solve = dict(time_to_solution_s=setup_s + first_solve_s, setup_s=setup_s,
                      first_solve_s=first_solve_s, repeat_solve_s=solve_s, precond=a.precond,
                      gmres_iters=info["iters"], refine_steps=info["refine_steps"],
                      code=info["code"], rel_resid_fp64_dense=rel, err_vs_kelvin=err_k,
-                     total_charge=op.total_charge(x), expected_charge=-1.0 / rho)
+                     total_charge=op.total_charge(x), expected_charge=-1.0 / rho,
+                     **refine_record(op, b, info, solve_s, t_acc, a.precond))
         if not a.no_learned:
             # learned-preconditioned vs plain GMRES at the same 1e-8 accuracy
             from electrodrive_amd.learn import precond as lp_mod
@@ -726,7 +751,8 @@ def run_torus(a):
         if dense is None:
             dense = op.accurate(sig)
         fa = op.approx(sig)
-        rr = float(torch.linalg.norm(b - op.accurate(x)) / torch.linalg.norm(b))
+        rv, t_acc = timed(lambda: b - op.accurate(x))
+        rr = float(torch.linalg.norm(rv) / torch.linalg.norm(b))
         if ref_x is None:
             ref_x = x
         mm = matvec_measure(op, sig, a, n)
@@ -738,7 +764,9 @@ def run_torus(a):
                          matvec_rel_err_vs_fp64_dense=float(torch.linalg.norm(fa - dense)
                                                             / torch.linalg.norm(dense)),
                          ms_per_matvec=mm["ms"], panels_per_s=mm["value"], stage_ms=mm["stages"],
-                         m2l=mm["extra"]["m2l_mfma"])
+                         m2l=mm["extra"]["m2l_mfma"],
+                         **refine_record(op, b, info, t, t_acc, "sai",
+                                         compare=prec in ("bf16", "fp16")))
         if prec == "bf16":
             head = mm
         del op

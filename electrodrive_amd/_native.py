@@ -35,13 +35,14 @@ class PlanOpts(ctypes.Structure):
 class GmresOpts(ctypes.Structure):
     _fields_ = [("restart", c_i32), ("maxiter", c_i32), ("tol", c_dbl), ("precond", c_i32),
                 ("refine", c_i32), ("max_refine", c_i32), ("inner_tol", c_dbl),
-                ("use_graph", c_i32), ("check_every", c_i32)]
+                ("use_graph", c_i32), ("check_every", c_i32), ("residual_update", c_i32)]
 
 
 class GmresInfo(ctypes.Structure):
     _fields_ = [("code", c_i32), ("iters", c_i32), ("cycles", c_i32), ("refine_steps", c_i32),
                 ("first_iter_below_tol", c_i32), ("resid", c_dbl), ("resid0", c_dbl),
-                ("rel_resid", c_dbl), ("tol_abs", c_dbl)]
+                ("rel_resid", c_dbl), ("tol_abs", c_dbl), ("n_accurate", c_i32),
+                ("n_updated", c_i32), ("resid_bound", c_dbl), ("update_err", c_dbl)]
 
 
 _SIGS = {

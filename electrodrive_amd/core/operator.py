@@ -185,13 +185,20 @@ class BemOperator:
         self._sai = True
 
     def solve(self, b, x0=None, *, tol=1e-8, restart=50, maxiter=2000, refine=True,
-              inner_tol=None, max_refine=None, precond=True, use_graph=True, check_every=4):
+              inner_tol=None, max_refine=None, precond=True, use_graph=True, check_every=4,
+              residual_update=True):
         """FP64 GMRES on the FMM operator with FP64 iterative refinement against
         the accurate operator (``refine=True``); returns (sigma, info).
         ``precond``: True / "jacobi" (core/bem.py:2141-2158), "sai" (sparse
         approximate inverse, built on first use) or False / "none".
         ``b`` of shape (R, n), R <= 8, solves R right-hand sides together (one
-        bem_solve per charge set in the reference); sigma has b's shape."""
+        bem_solve per charge set in the reference); sigma has b's shape.
+        ``residual_update``: once a dense residual is small enough, later
+        refinement residuals are updated with the split-FP16 FMM product of
+        each correction (r <- r - A_fmm d) instead of recomputed with the
+        O(N^2) dense operator; the solver bounds the distance of the updated
+        residual from the true one (info ``resid_bound``) and counts both
+        kinds (``n_accurate``, ``n_updated``)."""
         if x0 is None:
             x0 = (b / self.diag).contiguous()  # core/bem.py:2060
         # The inner solves run on the FMM operator, whose rounding (16-bit M2L:
@@ -210,7 +217,7 @@ class BemOperator:
         multi = b.dim() == 2
         kw = dict(restart=restart, maxiter=maxiter, precond=precond, refine=refine,
                   max_refine=max_refine, inner_tol=inner_tol, use_graph=use_graph,
-                  check_every=check_every)
+                  check_every=check_every, residual_update=residual_update)
         if G == 1 and not multi:
             return self.plan.solve(b, x0, self.diag if use_diag else None, self.self_corr,
                                    tol=tol, **kw)

@@ -463,6 +463,16 @@ int plan_upload(ed_plan* P) {
   P->t_m2l = DT->valu;
   P->t_m2l_mfma = DT->f32;
   P->t_m2l_h = DT->h16;
+  if (enabled && (prec == 1 || prec == 2)) {
+    // split-FP16 tables for the solver's residual updates: same shape (the
+    // 16-bit forms share ncp, kp and the tile schedule) and, built from the
+    // same offsets, the same table ids
+    const DeviceTables* D3 = nullptr;
+    ED_TRY(device_tables(P->device, p, 3, P->ws, ncp, kp, &D3));
+    if (D3->n_tab != ntab || D3->slot_tab != DT->slot_tab)
+      return fail(ED_ERR_INTERNAL, "plan: split-FP16 table set differs from the 16-bit one");
+    P->t_m2l_h3 = D3->h16;
+  }
   P->t_m2m = DT->m2m;
   P->t_m2m7 = reinterpret_cast<const float4*>(DT->m2m7);
   P->t_l2l = DT->l2l;

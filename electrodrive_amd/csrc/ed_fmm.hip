@@ -1509,7 +1509,10 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   // M2L form: -1 VALU, 0 FP32 MFMA, 1 / 2 / 3 BF16 / FP16 / split FP16 MFMA;
   // the 16-bit forms scale their inputs by the largest |M * ascale|, which
   // P2M and M2M fold into mmax as they write M
-  const int prec = Pn->n_m2l_tiles > 0 ? Pn->m2l_precision : -1;
+  // (m2l_prec_override 3 on a 16-bit plan: split FP16 through its t_m2l_h3)
+  const bool split_override = Pn->m2l_prec_override == 3 && Pn->t_m2l_h3 != nullptr;
+  const int prec = Pn->n_m2l_tiles > 0 ? (split_override ? 3 : Pn->m2l_precision) : -1;
+  const uint16_t* th = split_override ? Pn->t_m2l_h3 : Pn->t_m2l_h;
   unsigned* mmax = (prec >= 1 && prec <= 3) ? (multi ? Pn->multi.mmax : Pn->m2l_mmax) : nullptr;
   // gather (tree-order sigma, P2P charges, self term) and P2P depend on
   // nothing else of the matvec: off the profiling path they run on the side
@@ -1600,12 +1603,12 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
           hipLaunchKernelGGL((k_m2l_mfma16x3_multi<P>), dim3((unsigned)Pn->n_m2l_tiles),
                              dim3(256), 0, st, Pn->m2l_tile_tab, Pn->m2l_tile_beg,
                              Pn->m2l_tile_cnt, Pn->m2l_pair_src, Pn->m2l_pair_sgn, Pn->m2l_flip,
-                             (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp, rs);
+                             (const _Float16*)th, Pn->m2l_ascale, Mb, mmax, Yp, rs);
         else
           hipLaunchKernelGGL((k_m2l_mfma16x3<P>), dim3((unsigned)Pn->n_m2l_tiles), dim3(256), 0,
                              st, Pn->m2l_tile_tab, Pn->m2l_tile_beg, Pn->m2l_tile_cnt,
                              Pn->m2l_pair_src, Pn->m2l_pair_sgn, Pn->m2l_flip,
-                             (const _Float16*)Pn->t_m2l_h, Pn->m2l_ascale, Mb, mmax, Yp);
+                             (const _Float16*)th, Pn->m2l_ascale, Mb, mmax, Yp);
       }
       else if (prec == 1)
         hipLaunchKernelGGL((k_m2l_mfma16<P, __bf16, MULTI>), tiles, dim3(256), 0, st, Pn->m2l_tile_tab,

@@ -821,7 +821,14 @@ int dense_launch_f64(const double* C, const double* A, const double* sigma, cons
 // (0 initial residual, 1 Arnoldi poll, 2 cycle end, 3 refinement residual
 // with the accurate operator).
 constexpr int kTraceCols = 8;
-enum TracePhase { kTraceInitial = 0, kTracePoll = 1, kTraceCycleEnd = 2, kTraceRefine = 3 };
+enum TracePhase {
+  kTraceInitial = 0,
+  kTracePoll = 1,
+  kTraceCycleEnd = 2,
+  kTraceRefine = 3,
+  kTraceUpdate = 4,
+  kTraceApprox = 5
+};
 
 struct Solver {
   ed_plan* P;
@@ -852,6 +859,20 @@ struct Solver {
   // the FMM operator on all R right-hand sides (one multi-RHS matvec)
   int approx_op(const double* in_tree, double* out_tree) {
     return fmm_launch(P, in_tree, 1, 1, selfi_orig, out_tree, 1, 1, W->stream, R);
+  }
+  // the operator of the residual updates: the FMM with its M2L in split FP16
+  // (FP32-level accuracy; on a 16-bit plan through its split tables), the
+  // plan's own form otherwise
+  int update_op(const double* in_tree, double* out_tree) {
+    const bool sixteen = P->m2l_precision == 1 || P->m2l_precision == 2;
+    if (sixteen && P->t_m2l_h3) P->m2l_prec_override = 3;
+    const int rc = approx_op(in_tree, out_tree);
+    P->m2l_prec_override = -1;
+    return rc;
+  }
+  bool update_op_ok() const {
+    const bool sixteen = P->m2l_precision == 1 || P->m2l_precision == 2;
+    return !sixteen || P->n_m2l_tiles == 0 || P->t_m2l_h3 != nullptr;
   }
   int accurate_op(const double* in_tree, double* out_tree) {
     if (R > 1) return accurate_op_multi(in_tree, out_tree);
@@ -1143,6 +1164,7 @@ extern "C" void ed_gmres_opts_default(ed_gmres_opts* o) {
   o->inner_tol = 1e-4;
   o->use_graph = 1;
   o->check_every = 4;
+  o->residual_update = 1;
 }
 
 // R right-hand sides stacked (R x n, original order); every (right-hand
@@ -1207,46 +1229,103 @@ static int gmres_solve(ed_plan* P, const double* b, double* x, int R, const doub
     info->tol_abs = o.tol * (info->resid0 > 1e-300 ? info->resid0 : 1.0);
   } else {
     // outer FP64 iterative refinement with the accurate operator; segment g
-    // is done at |r_g| <= tol |b_g|
-    std::vector<double> rn, zn, tol_abs(G);
+    // is done at |r_g| <= tol |b_g|.
+    // Residual updates (o.residual_update): once an accurate residual r is
+    // small, the next correction d is small too (|A d| ~ |r|), so
+    // r' = r - A_u d with an operator A_u of relative error e is within
+    // ~e |r| of the true b - A x'.  e is measured on the iterate when the
+    // switch is considered (|A x - A_u x| / |A x|, A_u = the FMM with a
+    // split-FP16 M2L), the switch is taken when kUpdSafety e |r| <= tol / 4,
+    // and every update adds kUpdSafety e |A_u d| to a per-segment bound; a
+    // segment is done at |r_upd| + bound <= tol |b|.
+    constexpr double kUpdSafety = 10.0;
+    std::vector<double> rn, zn, tol_abs(G), eps(G, 0.0), bound(G, 0.0), an, dn, dummy2;
     for (int k = 0; k < G; ++k) tol_abs[k] = o.tol * (bn[k] > 1e-300 ? bn[k] : 1.0);
     info->tol_abs = o.tol * (bn_all > 1e-300 ? bn_all : 1.0);
+    const bool upd_on = o.residual_update != 0 && S.update_op_ok();
+    bool updating = false;  // rn holds an updated residual
     code = 1;
+    // Residuals b - A_u x with the update operator (error ~eps |b|, eps taken
+    // as kUpdPrior before it is measured) drive the refinement, not as a
+    // convergence test, while they exceed 10 kUpdPrior |b| and still fall by
+    // half a step; the accurate residual comes below that (or when the
+    // decrease stalls).
+    constexpr double kUpdPrior = 1e-5;
+    std::vector<double> rn_prev(G, INFINITY);
     for (int it = 0; it <= o.max_refine; ++it) {
       // The first correction only has to bring the residual down by the inner
       // tolerance, so its right-hand side may come from the approximate
       // operator; a residual that already looks converged is re-measured with
       // the accurate one, and every later step (and the final test) uses it.
+      // With residual updates, steps whose residual is far above the update
+      // operator's error take b - A_u x, the one accurate residual comes when
+      // it is small enough to switch to updates, and an initial guess whose
+      // residual exceeds |b| is dropped for x = 0 (r = b exactly).
       bool approx_resid = false;
-      if (it == 0) {
-        ED_T(S.approx_op(W->x, W->t));
-        ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
-        info->resid0 = l2(rn);
-        // every segment far from converged relative to the approximate
-        // operator's accuracy (inner_tol is set at or above that floor)
-        approx_resid = true;
-        for (int k = 0; k < G; ++k)
-          approx_resid = approx_resid && std::isfinite(rn[k]) &&
-                         rn[k] > std::max(100.0 * tol_abs[k], 10.0 * o.inner_tol * bn[k]);
-      }
       S.refine_step = it;
-      if (!approx_resid) {
+      if (it == 0 || (upd_on && !updating)) {
+        if (upd_on) ED_T(S.update_op(W->x, W->t));
+        else ED_T(S.approx_op(W->x, W->t));
+        ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
+        if (it == 0) info->resid0 = l2(rn);
+        if (it == 0 && upd_on && !(l2(rn) < bn_all)) {
+          ED_G(hipMemsetAsync(W->x, 0, sizeof(double) * n, W->stream));
+          ED_G(hipMemcpyAsync(W->r, W->b, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                              W->stream));
+          rn = bn;
+        }
+        approx_resid = true;
+        for (int k = 0; k < G; ++k) {
+          const double floor_k =
+              upd_on ? std::max({10.0 * kUpdPrior * bn[k], 100.0 * tol_abs[k],
+                                 0.25 * tol_abs[k] / (kUpdSafety * kUpdPrior)})
+                     : std::max(100.0 * tol_abs[k], 10.0 * o.inner_tol * bn[k]);
+          approx_resid = approx_resid && std::isfinite(rn[k]) && rn[k] > floor_k &&
+                         rn[k] < 0.5 * rn_prev[k];
+        }
+        if (approx_resid) {
+          rn_prev = rn;
+          S.trace(kTraceApprox, 0, l2(rn), NAN);
+        }
+        // keep A_u x for the error measurement below (d is free here)
+        if (upd_on && !approx_resid)
+          ED_G(hipMemcpyAsync(W->d, W->t, sizeof(double) * n, hipMemcpyDeviceToDevice, W->stream));
+      }
+      if (!approx_resid && !updating) {
         ED_T(S.accurate_op(W->x, W->t));
         ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
         S.trace(kTraceRefine, 0, l2(rn), NAN);
+        ++info->n_accurate;
+        bool conv = true;
+        for (int k = 0; k < G; ++k) conv = conv && rn[k] <= tol_abs[k];
+        if (upd_on && !conv && it < o.max_refine) {
+          // error of the update operator on this iterate (A_u x is in d)
+          ED_T(S.resid_norms(W->t, W->d, nullptr, nullptr, dn, dummy2));
+          ED_T(S.resid_norms(W->t, nullptr, nullptr, nullptr, an, dummy2));
+          bool sw = true;
+          for (int k = 0; k < G; ++k) {
+            eps[k] = an[k] > 1e-300 ? dn[k] / an[k] : 0.0;
+            info->update_err = std::max(info->update_err, eps[k]);
+            sw = sw && std::isfinite(eps[k]) &&
+                 (rn[k] <= tol_abs[k] || kUpdSafety * eps[k] * rn[k] <= 0.25 * tol_abs[k]);
+          }
+          updating = sw;
+        }
       }
       info->refine_steps = it;
       bool finite = true, done = true;
       for (int k = 0; k < G; ++k) {
         finite = finite && std::isfinite(rn[k]);
-        done = done && rn[k] <= tol_abs[k];
+        done = done && rn[k] + bound[k] <= tol_abs[k];
       }
       if (!finite) { code = 6; break; }
       if (!approx_resid && done) { code = 0; break; }
       if (it == o.max_refine || S.total_iters >= o.maxiter) {
-        if (approx_resid) {  // report an accurate residual
+        if (approx_resid || updating) {  // report an accurate residual
           ED_T(S.accurate_op(W->x, W->t));
           ED_T(S.resid_norms(W->b, W->t, W->r, nullptr, rn, zn));
+          ++info->n_accurate;
+          std::fill(bound.begin(), bound.end(), 0.0);
         }
         code = 1;
         break;
@@ -1260,7 +1339,7 @@ static int gmres_solve(ed_plan* P, const double* b, double* x, int R, const doub
       // converged segments are left alone (tolerance 1)
       std::vector<double> itol(G);
       for (int k = 0; k < G; ++k)
-        itol[k] = rn[k] <= tol_abs[k]
+        itol[k] = rn[k] + bound[k] <= tol_abs[k]
                       ? 1.0
                       : std::min(0.5, std::max(o.inner_tol, 0.25 * tol_abs[k] / rn[k]));
       // an inner solve that stalls on the approximate operator's rounding
@@ -1271,7 +1350,18 @@ static int gmres_solve(ed_plan* P, const double* b, double* x, int R, const doub
       if (icode == 3 || icode == 5 || icode == 6 || icode == 7) { code = icode; break; }
       hipLaunchKernelGGL(k_axpy, dim3(gn), dim3(256), 0, W->stream, n, 1.0, W->d, W->x);
       ED_KCHECK("refine update");
+      if (updating) {
+        // r <- r - A_u d (the old residual is in rhs)
+        ED_T(S.update_op(W->d, W->t));
+        ED_T(S.resid_norms(W->rhs, W->t, W->r, nullptr, rn, zn));
+        ED_T(S.resid_norms(W->t, nullptr, nullptr, nullptr, an, dummy2));
+        for (int k = 0; k < G; ++k) bound[k] += kUpdSafety * eps[k] * an[k];
+        ++info->n_updated;
+        S.refine_step = it + 1;
+        S.trace(kTraceUpdate, 0, l2(rn), l2(bound));
+      }
     }
+    info->resid_bound = l2(bound);
     info->resid = l2(rn);
     W->last_rn = rn;
   }

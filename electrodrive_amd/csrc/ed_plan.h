@@ -81,6 +81,12 @@ struct ed_plan {
   float* t_m2l = nullptr;       // n_off * nc * nc, fp32 (VALU path)
   float* t_m2l_mfma = nullptr;  // n_off * ncp * kp, row-major padded, scaled (MFMA A)
   uint16_t* t_m2l_h = nullptr;  // n_off * ncp * kp16, bf16/fp16 (scaled)
+  // split-FP16 tables of a 16-bit plan (the same tiles and table ids): the
+  // solver's residual updates run the M2L in split FP16 through them
+  const uint16_t* t_m2l_h3 = nullptr;
+  // M2L form of the next matvecs when >= 0 (3: split FP16 with t_m2l_h3 on a
+  // 16-bit plan); -1: the plan's own form
+  int m2l_prec_override = -1;
   float* m2l_ascale = nullptr;  // per input coefficient: sqrt((n+m)!(n-m)!)
   float* m2l_binv = nullptr;    // per output coefficient: sqrt((n+m)!(n-m)!)
   const uint8_t* m2l_flip = nullptr;  // per coefficient: reflection sign bits (shared)

@@ -256,14 +256,19 @@ class FmmPlan:
                   "ed_gmres_block_resid")
         return r, bn, bool(pg.value)
 
-    TRACE_PHASES = ("initial", "arnoldi", "cycle_end", "refine")
+    TRACE_PHASES = ("initial", "arnoldi", "cycle_end", "refine", "update", "approx")
 
     def trace(self):
         """Rows of the last native solve's per-poll trace (ed_gmres_trace) as
         dicts with the SOLVER_TRACE columns (core/bem_solver.py:19-28) plus
         refine_step: ts, cycle, iter, resid_true_l2 (NaN where not measured),
         resid_precond_l2 (the solver's estimate at an Arnoldi poll),
-        arnoldi_dim and phase (initial / arnoldi / cycle_end / refine)."""
+        arnoldi_dim and phase (initial / arnoldi / cycle_end / refine: a
+        refinement residual with the dense FP64 operator / update: a residual
+        updated with the FMM product of the last correction, resid_precond_l2
+        then holding the bound on its distance from the true residual /
+        approx: a refinement residual b - A_fmm x far above the FMM's
+        error)."""
         L = nat.lib()
         nr = ctypes.c_int32(0)
         nat.check(L.ed_gmres_trace(self.handle, None, 0, ctypes.byref(nr)), "ed_gmres_trace")
@@ -279,7 +284,7 @@ class FmmPlan:
 
     def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
               tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,
-              use_graph=True, check_every=4):
+              use_graph=True, check_every=4, residual_update=True):
         """Native FP64 GMRES (+ iterative refinement); returns (x, info dict).
         ``b`` of shape (n,) or (R, n) with R <= 8: the R right-hand sides are
         solved together (one multi-RHS FMM matvec per Arnoldi step), each
@@ -309,6 +314,7 @@ class FmmPlan:
             raise ValueError(f"unknown preconditioner {precond!r} (none, jacobi, sai)")
         o.precond, o.refine, o.max_refine = pc, int(bool(refine)), int(max_refine)
         o.inner_tol, o.use_graph, o.check_every = float(inner_tol), int(bool(use_graph)), int(check_every)
+        o.residual_update = int(bool(residual_update))
         info = nat.GmresInfo()
         if b.dim() == 2:
             nat.check(L.ed_gmres_solve_multi(self.handle, nat.ptr(b), nat.ptr(x), int(b.shape[0]),

@@ -251,6 +251,15 @@ typedef struct ed_gmres_opts {
   double inner_tol;       /* relative tolerance of each inner solve          */
   int32_t use_graph;      /* capture one Arnoldi step in a hipGraph          */
   int32_t check_every;    /* host poll interval in Arnoldi steps             */
+  int32_t residual_update; /* 1: refinement residuals b - A_fmm x with the  */
+                          /*    split-FP16 FMM while they are far above its  */
+                          /*    error, one dense FP64 residual once small    */
+                          /*    enough that the FMM product of the next      */
+                          /*    correction is accurate to well below tol,    */
+                          /*    then updates r <- r - A_fmm d (error bound   */
+                          /*    tracked); an initial guess with |r| >= |b|   */
+                          /*    is replaced by 0.  0: every refinement       */
+                          /*    residual after the first is a dense one      */
 } ed_gmres_opts;
 
 typedef struct ed_gmres_info {
@@ -263,6 +272,10 @@ typedef struct ed_gmres_info {
   double resid0;          /* initial true residual norm                      */
   double rel_resid;       /* resid / ||b||_2                                 */
   double tol_abs;
+  int32_t n_accurate;     /* residuals taken with the dense FP64 operator    */
+  int32_t n_updated;      /* residuals updated with the FMM product          */
+  double resid_bound;     /* bound on |updated - true residual| (0 if none)  */
+  double update_err;      /* measured relative error of the update operator  */
 } ed_gmres_info;
 
 void ed_gmres_opts_default(ed_gmres_opts* opts);
@@ -307,7 +320,10 @@ int ed_gmres_block_resid(ed_plan* plan, double* resid, double* bnorm, int32_t ca
  * residual norm (NaN if not measured at that point), preconditioned residual
  * norm (at an Arnoldi poll: the solver's estimate), Krylov dimension, phase
  * (0 initial residual of an inner solve, 1 Arnoldi poll, 2 cycle end,
- * 3 refinement residual with the accurate operator).  *n_rows receives the
+ * 3 refinement residual with the accurate operator, 4 residual update
+ * r <- r - A_fmm d: residual column = the updated residual, preconditioned
+ * column = the bound on its distance from the true residual, 5 refinement
+ * residual b - A_fmm x taken while it is far above the FMM's error).  *n_rows receives the
  * row count; min(cap, *n_rows) rows are copied when rows is non-NULL. */
 int ed_gmres_trace(const ed_plan* plan, double* rows, int32_t cap, int32_t* n_rows);
 

@@ -13,7 +13,9 @@ row.
   from numpy.random.default_rng(0)) in the production configuration with the
   SAI smoother: code 0, ||b - A_acc x|| / ||b|| <= 1e-8 against the FP64
   dense + FP64 near operator, and sigma within 5e-3 of the Kelvin image
-  density (discretisation error of the flat-panel mesh).
+  density (discretisation error of the flat-panel mesh); with residual
+  updates, one dense residual and the solver's updated residual within its
+  bound and 1e-9 |b| (a tenth of the tolerance) of the true one.
 """
 import numpy as np
 import pytest
@@ -68,10 +70,24 @@ def test_seed0_solve_production_config():
                      self_depth=3, near_depth=3)
     op.enable_sai()
     b = op.rhs_point_charges([(1.0, tuple(r0))])
-    x, info = op.solve(b, tol=1e-8, precond="sai")
-    rr = float(torch.linalg.norm(b - op.accurate(x)) / torch.linalg.norm(b))
     sk = analytic.kelvin_sigma(m.centroids, 1.0, r0, 1.0)
-    ek = float(np.linalg.norm(x.cpu().numpy() - sk) / np.linalg.norm(sk))
-    print(f"seed 0 solve: code {info['code']} iters {info['iters']} refine "
-          f"{info['refine_steps']} FP64 resid {rr:.2e} Kelvin err {ek:.2e}")
-    assert info["code"] == 0 and rr <= 1e-8 and ek <= 5e-3
+    bn = float(torch.linalg.norm(b))
+    for upd in (True, False):
+        x, info = op.solve(b, tol=1e-8, precond="sai", residual_update=upd)
+        rt = float(torch.linalg.norm(b - op.accurate(x)))
+        rr = rt / bn
+        ek = float(np.linalg.norm(x.cpu().numpy() - sk) / np.linalg.norm(sk))
+        print(f"seed 0 solve (residual updates {upd}): code {info['code']} iters "
+              f"{info['iters']} refine {info['refine_steps']} dense residuals "
+              f"{info['n_accurate']} updates {info['n_updated']} (update operator err "
+              f"{info['update_err']:.2e}, bound {info['resid_bound'] / bn:.2e}) FP64 resid "
+              f"{rr:.2e} (solver's {info['rel_resid']:.2e}) Kelvin err {ek:.2e}")
+        assert info["code"] == 0 and rr <= 1e-8 and ek <= 5e-3
+        if upd:
+            # one dense residual; the updated residual the solver stopped on is
+            # within its own bound and a tenth of the tolerance of the true one
+            assert info["n_accurate"] == 1 and info["n_updated"] >= 1
+            assert abs(info["resid"] - rt) <= info["resid_bound"]
+            assert abs(info["resid"] - rt) <= 1e-9 * bn
+        else:
+            assert info["n_updated"] == 0 and info["resid_bound"] == 0.0

@@ -14,7 +14,11 @@ Bounds:
 - the solution within 2e-6 (relative L2) of the split-FP16 (FP32-accurate)
   M2L solve, refined to the same residual;
 - the 16-bit FMM matvec against the oracle's FP64 dense matvec
-  (oracle/csrc/dense_ref.c): BF16 <= 5e-3, FP16 <= 1e-3 relative L2.
+  (oracle/csrc/dense_ref.c): BF16 <= 5e-3, FP16 <= 1e-3 relative L2;
+- residual updates (later refinement residuals r <- r - A_fmm d with the
+  split-FP16 FMM): fewer dense residuals than the all-dense refinement, which
+  also reaches 1e-8, and the solver's final updated residual within its
+  bound and 1e-9 |b| (a tenth of the tolerance) of the true one.
 """
 import numpy as np
 import pytest
@@ -64,7 +68,9 @@ def test_mixed_precision_solve_refines_to_1e8(torus_case, prec, mv_bound):
     op.enable_sai()
     b = op.rhs_point_charges(torus_case["charges"])
     x, info = op.solve(b, tol=1e-8, precond="sai")
-    rr = float(torch.linalg.norm(b - op.accurate(x)) / torch.linalg.norm(b))
+    bn = float(torch.linalg.norm(b))
+    rt = float(torch.linalg.norm(b - op.accurate(x)))
+    rr = rt / bn
     xr = torus_case["x_ref"]
     dx = float(torch.linalg.norm(x - xr) / torch.linalg.norm(xr))
     # the 16-bit FMM matvec alone (no near correction) vs the oracle
@@ -73,11 +79,25 @@ def test_mixed_precision_solve_refines_to_1e8(torus_case, prec, mv_bound):
     mv = plan.matvec(sig, op.self_corr).cpu().numpy()
     ref = torus_case["ref_mv"]
     mv_err = float(np.linalg.norm(mv - ref) / np.linalg.norm(ref))
+    # the same solve with every refinement residual dense
+    x0, info0 = op.solve(b, tol=1e-8, precond="sai", residual_update=False)
+    rr0 = float(torch.linalg.norm(b - op.accurate(x0))) / bn
     print(f"{prec}: code {info['code']} iters {info['iters']} refine {info['refine_steps']} "
-          f"inner tol {INNER_TOL[prec]:g}; FP64 resid {rr:.2e}; vs split-FP16 solve {dx:.2e}; "
-          f"matvec vs oracle {mv_err:.2e}")
-    assert info["code"] == 0
-    assert rr <= 1e-8
+          f"inner tol {INNER_TOL[prec]:g}; dense residuals {info['n_accurate']} (without "
+          f"updates {info0['n_accurate']}), updates {info['n_updated']}, update operator err "
+          f"{info['update_err']:.2e}; FP64 resid {rr:.2e} (solver's {info['rel_resid']:.2e}, "
+          f"bound {info['resid_bound'] / bn:.2e}; without updates {rr0:.2e}); vs split-FP16 "
+          f"solve {dx:.2e}; matvec vs oracle {mv_err:.2e}")
+    assert info["code"] == 0 and info0["code"] == 0
+    assert rr <= 1e-8 and rr0 <= 1e-8
     assert 1 <= info["refine_steps"] <= 16
     assert dx <= 2e-6
     assert mv_err <= mv_bound
+    # residual updates: fewer dense residuals, and the updated residual the
+    # solver stopped on within its bound and a tenth of the tolerance of the
+    # true one
+    assert info["n_updated"] >= 1 and info["n_accurate"] < info0["n_accurate"]
+    assert abs(info["resid"] - rt) <= info["resid_bound"]
+    assert abs(info["resid"] - rt) <= 1e-9 * bn
+    # the updates run the M2L in split FP16: FP32-level error on the iterate
+    assert info["update_err"] <= 2e-5

@@ -145,12 +145,20 @@ def test_native_solve_writes_solver_trace(tmp_path):
     assert int(polls[0]["iter"]) == 4 and polls[0]["resid_true_l2"] == "nan"
     assert float(rows[-1]["resid_true_l2"]) <= info["tol_abs"] * 1.0001
     assert abs(float(rows[0]["resid_true_l2"]) - info["resid_true0"]) <= 1e-9 * info["resid_true0"]
-    # refined solve: the accurate operator's residuals appear as refine rows
-    x, info = op.solve(b, tol=1e-8)
-    tr = op.plan.trace()
-    ref = [r for r in tr if r["phase"] == "refine"]
-    assert ref and ref[-1]["resid_true_l2"] <= 1e-8 * float(torch.linalg.norm(b)) * 1.0001
-    assert ref[-1]["refine_step"] == info["refine_steps"]
+    # refined solve: the accurate operator's residuals appear as refine rows,
+    # residual updates as update rows (bound in the precond column); the last
+    # of them met the tolerance at the last refinement step
+    for upd in (False, True):
+        x, info = op.solve(b, tol=1e-8, residual_update=upd)
+        tr = op.plan.trace()
+        ref = [r for r in tr if r["phase"] == "refine"]
+        outer = [r for r in tr if r["phase"] in ("refine", "update")]
+        assert len(ref) == info["n_accurate"] >= 1
+        assert len(outer) - len(ref) == info["n_updated"]
+        last = outer[-1]
+        bound = last["resid_precond_l2"] if last["phase"] == "update" else 0.0
+        assert last["resid_true_l2"] + bound <= 1e-8 * float(torch.linalg.norm(b)) * 1.0001
+        assert last["refine_step"] == info["refine_steps"]
 
 
 def test_sai_rows_match_oracle():
