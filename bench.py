@@ -98,14 +98,20 @@ def stage_roofline(name, ms, st, n, order, m2l_peak=MFMA_FP32_PEAK_TFLOPS, R=1):
                     unit="TFLOP/s", algorithmic=flops)
     if name == "p2p":
         # target points + the source points of every P2P leaf pair (x, y, z,
-        # q; with R > 1 the points and R charges) + the partial-sum writes
+        # q; with R > 1 the points and R charges) + the partial-sum writes;
+        # one RHS evaluates r_ij and its rsqrt once per symmetric pair
         src = 16.0 if R == 1 else 16.0 + 4.0 * R
-        byts = 16.0 * n + src * st["n_p2p_source_points"] + 4.0 * R * n
+        sym = st.get("n_p2p_symmetric_interactions", 0) if R == 1 else 0
+        parts = st.get("n_p2p_partials", n) if R == 1 else R * n
+        evals = st["n_p2p_interactions"] - sym // 2
+        byts = 16.0 * n + src * st["n_p2p_source_points"] + 4.0 * parts
         return dict(bound="hbm", achieved=byts / s / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
                     algorithmic=byts, interactions_per_s=R * st["n_p2p_interactions"] / s,
-                    # 18 flops of distance and 1/r per source-target pair,
-                    # shared by the R right-hand sides, + one FMA per RHS
-                    valu_tflops=(18.0 + 2.0 * R) * st["n_p2p_interactions"] / s / 1e12)
+                    pair_evaluations_per_s=evals / s,
+                    # 18 flops of distance and 1/r per evaluated pair, shared
+                    # by the R right-hand sides and both directions of a
+                    # symmetric pair, + one FMA per interaction and RHS
+                    valu_tflops=(18.0 * evals + 2.0 * R * st["n_p2p_interactions"]) / s / 1e12)
     if name == "l2p_out":
         # local coefficients once per leaf + near CSR (weight, column, gathered
         # sigma) + per panel: point, row pointer, perm and per RHS the P2P sum,

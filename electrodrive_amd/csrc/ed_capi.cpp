@@ -71,7 +71,7 @@ static int upload(T** dst, const std::vector<T>& src) {
 void plan_free(ed_plan* P) {
   void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->perm, P->node_ctr, P->node_rng,
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
-                  P->leaves, P->pt_node, P->p2p_items, P->p2p_seg, P->p2p_part, P->p2p_ref, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
+                  P->leaves, P->pt_node, P->p2p_items, P->p2p_seg, P->p2p_ref, P->p2s_items, P->p2s_seg, P->p2s_sym, P->p2s_tseg, P->p2s_part, P->p2s_leaf, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
                   P->m2l_pair_src, P->m2l_pair_sgn, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
                   P->m2l_tile_cnt, P->m2l_tgt_ptr, P->m2l_tgt_row, P->ypair,
                   P->near_ptr, P->near_col, P->near_w, P->cent64_t, P->area64_t, P->near_w64,
@@ -349,7 +349,200 @@ static int p2p_schedule(ed_plan* P, const std::vector<int32_t>& leaves) {
   if (!items.empty())
     ED_HIP(hipMemcpy(P->p2p_items, items.data(), sizeof(ed::P2PItem) * items.size(),
                      hipMemcpyHostToDevice));
-  ED_HIP(hipMalloc((void**)&P->p2p_part, sizeof(float) * 64 * std::max<size_t>(items.size(), 1)));
+  // (partial rows of this schedule live in the multi-RHS buffers)
+  return ED_OK;
+}
+
+// One-right-hand-side P2P schedule (see ed_plan.h).  Each unordered pair of
+// distinct U-list leaves goes to the larger leaf S (lower node id on a tie)
+// as a source leaf; S's partner leaves are its targets, packed 64 panels to
+// a chunk in pair order, and every (chunk, piece of S) is one symmetric item.
+// A leaf's own interactions (and any pair listed in one direction only) run
+// one-directional items like p2p_schedule's over the leaf's stream.  Rows:
+// every piece of S gives each of its target leaves one row, every chunk of
+// S's targets gives S one row, every piece of a leaf's one-directional
+// stream one row (written chunk by chunk).
+static int p2p_schedule_sym(ed_plan* P, const std::vector<int32_t>& leaves) {
+  const auto& H = P->host;
+  const int64_t M = (int64_t)H.level.size();
+  const size_t NL = leaves.size();
+  std::vector<int32_t> ord((size_t)M, -1);
+  for (size_t a = 0; a < NL; ++a) ord[(size_t)leaves[a]] = (int32_t)a;
+  auto nb = [&](size_t a) { return (int32_t)H.begin[leaves[a]]; };
+  auto npt = [&](size_t a) { return (int32_t)(H.end[leaves[a]] - H.begin[leaves[a]]); };
+  // directed U pairs (target, source) in leaf ordinals, sorted
+  std::vector<std::pair<int32_t, int32_t>> dir;
+  dir.reserve((size_t)H.u_src.size());
+  for (size_t a = 0; a < NL; ++a) {
+    const int32_t li = leaves[a];
+    for (int64_t q = H.u_ptr[li]; q < H.u_ptr[li + 1]; ++q)
+      if (H.u_src[q] != li) dir.emplace_back((int32_t)a, ord[(size_t)H.u_src[q]]);
+  }
+  std::sort(dir.begin(), dir.end());
+  std::vector<int32_t> dptr(NL + 1, 0);  // dir entries of target a: [dptr[a], dptr[a + 1])
+  for (const auto& d : dir) ++dptr[(size_t)d.first + 1];
+  for (size_t a = 0; a < NL; ++a) dptr[a + 1] += dptr[a];
+  auto listed = [&](int32_t t, int32_t s_) {  // s_ in the U list of t
+    return std::binary_search(dir.begin() + dptr[(size_t)t], dir.begin() + dptr[(size_t)t + 1],
+                              std::make_pair(t, s_));
+  };
+  // partners of each source leaf (CSR) and the one-directional extra
+  // sources; pairs listed in one direction only go to the target's stream
+  std::vector<int32_t> pcnt(NL + 1, 0), xcnt(NL + 1, 0);
+  std::vector<uint8_t> kind(dir.size(), 0);  // 0 skip, 1 symmetric (A source), 2 (B source), 3 one-way
+  for (size_t e = 0; e < dir.size(); ++e) {
+    const int32_t A = dir[e].first, B = dir[e].second;
+    if (npt((size_t)A) == 0 || npt((size_t)B) == 0) continue;
+    if (!listed(B, A)) {
+      kind[e] = 3;
+      ++xcnt[(size_t)A + 1];
+    } else if (A < B) {
+      const bool a_src = npt((size_t)A) >= npt((size_t)B);
+      kind[e] = a_src ? 1 : 2;
+      ++pcnt[(size_t)(a_src ? A : B) + 1];
+    }
+  }
+  for (size_t a = 0; a < NL; ++a) { pcnt[a + 1] += pcnt[a]; xcnt[a + 1] += xcnt[a]; }
+  std::vector<int32_t> partner((size_t)pcnt[NL]), extra((size_t)xcnt[NL]);
+  {
+    std::vector<int32_t> pf(pcnt.begin(), pcnt.end() - 1), xf(xcnt.begin(), xcnt.end() - 1);
+    for (size_t e = 0; e < dir.size(); ++e) {
+      const int32_t A = dir[e].first, B = dir[e].second;
+      if (kind[e] == 1) partner[(size_t)pf[(size_t)A]++] = B;
+      else if (kind[e] == 2) partner[(size_t)pf[(size_t)B]++] = A;
+      else if (kind[e] == 3) extra[(size_t)xf[(size_t)A]++] = B;
+    }
+  }
+  // pass 1: rows of every leaf and item counts
+  constexpr int32_t PS = ed::kP2PSymSources;
+  const int steps = ed::kP2PItemSteps;
+  std::vector<int32_t> rows(NL, 0), kT0((size_t)pcnt[NL]), kS0(NL, 0), nchunk(NL, 0),
+      npc(NL, 0), k1d(NL, 0), np1d(NL, 0);
+  std::vector<int64_t> budget1d(NL, 0), total1d(NL, 0);
+  int64_t n_sym = 0, n_one = 0, sym_int = 0;
+  for (size_t S = 0; S < NL; ++S) {
+    if (pcnt[S + 1] == pcnt[S]) continue;
+    npc[S] = (npt(S) + PS - 1) / PS;
+    int64_t tot = 0;
+    for (int32_t e = pcnt[S]; e < pcnt[S + 1]; ++e) {
+      const size_t T = (size_t)partner[(size_t)e];
+      kT0[(size_t)e] = rows[T];
+      rows[T] += npc[S];
+      tot += npt(T);
+    }
+    nchunk[S] = (int32_t)((tot + 63) / 64);
+    kS0[S] = rows[S];
+    rows[S] += nchunk[S];
+    n_sym += (int64_t)nchunk[S] * npc[S];
+    sym_int += 2 * tot * npt(S);
+  }
+  for (size_t a = 0; a < NL; ++a) {
+    const int32_t n_a = npt(a);
+    if (n_a == 0) continue;
+    int64_t total = n_a;
+    for (int32_t e = xcnt[a]; e < xcnt[a + 1]; ++e) total += npt((size_t)extra[(size_t)e]);
+    // pieces of one source count for every chunk (kP2PItemSteps per slice of
+    // the leaf's first, fullest chunk), so piece q of every chunk writes row q
+    const int32_t S0 = std::min<int32_t>(64 / std::min<int32_t>(64, n_a), 8);
+    budget1d[a] = (int64_t)steps * S0;
+    total1d[a] = total;
+    np1d[a] = (int32_t)((total + budget1d[a] - 1) / budget1d[a]);
+    k1d[a] = rows[a];
+    rows[a] += np1d[a];
+    n_one += (int64_t)((n_a + 63) / 64) * np1d[a];
+  }
+  std::vector<int64_t> base(NL + 1, 0);
+  for (size_t a = 0; a < NL; ++a) base[a + 1] = base[a] + (int64_t)rows[a] * npt(a);
+  if (base[NL] > INT32_MAX || n_sym > INT32_MAX)
+    return fail(ED_ERR_ARG, "too many P2P partial slots");
+  // pass 2: symmetric items; per chunk its target segments (one per target
+  // leaf in the chunk), shared by the chunk's pieces
+  std::vector<ed::P2PSymItem> sym((size_t)n_sym);
+  std::vector<ed::P2PSymSeg> tseg;
+  tseg.reserve((size_t)n_sym * 2);
+  size_t it = 0;
+  for (size_t S = 0; S < NL; ++S) {
+    if (nchunk[S] == 0) continue;
+    const int32_t sb = nb(S), nS = npt(S);
+    int32_t e = pcnt[S], off = 0;  // next target: partner e, panel offset off
+    for (int32_t c = 0; c < nchunk[S]; ++c) {
+      const int32_t g0 = (int32_t)tseg.size();
+      int32_t nl = 0;
+      while (nl < 64 && e < pcnt[S + 1]) {
+        const size_t T = (size_t)partner[(size_t)e];
+        const int32_t nT = npt(T);
+        const int32_t take = std::min(64 - nl, nT - off);
+        const int64_t slot0 = base[T] + (int64_t)kT0[(size_t)e] * nT + off;
+        tseg.push_back(ed::P2PSymSeg{nl, nb(T) + off, (int32_t)slot0, nT});
+        nl += take;
+        off += take;
+        if (off == nT) { ++e; off = 0; }
+      }
+      const int32_t ng = (int32_t)tseg.size() - g0;
+      const int64_t srow = base[S] + (int64_t)(kS0[S] + c) * nS;
+      for (int32_t q = 0; q < npc[S]; ++q, ++it) {
+        const int32_t s0 = sb + q * PS, cnt = std::min(PS, sb + nS - s0);
+        sym[it] = ed::P2PSymItem{g0, ng, q, nl, s0, cnt, (int32_t)(srow + (s0 - sb)), 0};
+      }
+    }
+  }
+  // one-directional items: chunks of <= 64 targets over the leaf's stream
+  // (itself, then its one-way sources)
+  std::vector<int2> seg;
+  std::vector<ed::P2PItem> items;
+  items.reserve((size_t)n_one);
+  for (size_t a = 0; a < NL; ++a) {
+    const int32_t b = nb(a), e = b + npt(a);
+    if (e <= b) continue;
+    for (int32_t t0 = b; t0 < e; t0 += 64) {
+      const int32_t ntc = std::min<int32_t>(64, e - t0);
+      const int32_t Sl = std::min<int32_t>(64 / ntc, 8);
+      const int32_t kseg0 = (int32_t)seg.size();
+      seg.push_back(make_int2(b, e));
+      for (int32_t x = xcnt[a]; x < xcnt[a + 1]; ++x) {
+        const size_t B = (size_t)extra[(size_t)x];
+        seg.push_back(make_int2(nb(B), nb(B) + npt(B)));
+      }
+      const int32_t kseg1 = (int32_t)seg.size();
+      int32_t k = kseg0, so = 0;
+      int64_t left = total1d[a];
+      for (int32_t q = 0; q < np1d[a]; ++q) {
+        const int32_t cnt = (int32_t)std::min<int64_t>(budget1d[a], left);
+        const int64_t slot = base[a] + (int64_t)(k1d[a] + q) * (e - b) + (t0 - b);
+        items.push_back(ed::P2PItem{t0, ntc, k, so, cnt, (int32_t)slot, Sl, 0});
+        int32_t c = cnt;
+        while (c > 0 && k < kseg1) {
+          const int32_t len = seg[k].y - seg[k].x - so;
+          const int32_t take = std::min(len, c);
+          so += take;
+          c -= take;
+          if (so == seg[k].y - seg[k].x) { ++k; so = 0; }
+        }
+        left -= cnt;
+      }
+    }
+  }
+  std::vector<int2> leafrow((size_t)std::max<int64_t>(M, 1), make_int2(0, 0));
+  for (size_t a = 0; a < NL; ++a) leafrow[(size_t)leaves[a]] = make_int2((int32_t)base[a], rows[a]);
+  P->n_p2s_items = (int64_t)items.size();
+  P->n_p2s_sym = (int64_t)sym.size();
+  P->n_p2s_slots = base[NL];
+  P->n_p2s_sym_interactions = sym_int;
+  ED_TRY(upload(&P->p2s_seg, seg));
+  ED_TRY(upload(&P->p2s_leaf, leafrow));
+  ED_HIP(hipMalloc(&P->p2s_tseg, sizeof(ed::P2PSymSeg) * std::max<size_t>(tseg.size(), 1)));
+  if (!tseg.empty())
+    ED_HIP(hipMemcpy(P->p2s_tseg, tseg.data(), sizeof(ed::P2PSymSeg) * tseg.size(),
+                     hipMemcpyHostToDevice));
+  ED_HIP(hipMalloc(&P->p2s_items, sizeof(ed::P2PItem) * std::max<size_t>(items.size(), 1)));
+  if (!items.empty())
+    ED_HIP(hipMemcpy(P->p2s_items, items.data(), sizeof(ed::P2PItem) * items.size(),
+                     hipMemcpyHostToDevice));
+  ED_HIP(hipMalloc(&P->p2s_sym, sizeof(ed::P2PSymItem) * std::max<size_t>(sym.size(), 1)));
+  if (!sym.empty())
+    ED_HIP(hipMemcpy(P->p2s_sym, sym.data(), sizeof(ed::P2PSymItem) * sym.size(),
+                     hipMemcpyHostToDevice));
+  ED_HIP(hipMalloc((void**)&P->p2s_part, sizeof(float) * (size_t)std::max<int64_t>(base[NL], 1)));
   return ED_OK;
 }
 
@@ -415,6 +608,7 @@ int plan_upload(ed_plan* P) {
     ED_TRY(upload(&P->pt_node, ptn));
   }
   ED_TRY(p2p_schedule(P, leaves));
+  ED_TRY(p2p_schedule_sym(P, leaves));
   ED_TRY(side_stream(P->device, &P->side));
   ED_HIP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
   ED_HIP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
@@ -763,10 +957,12 @@ int ed_plan_stats(const ed_plan* P, int64_t* out, int32_t n_out) {
       any = P->host.v_ptr[t + 1] > P->host.v_ptr[t];
     levels_with_m2l += any;
   }
-  const int64_t vals[13] = {P->n, P->n_nodes, P->n_leaves, P->host.depth, P->n_u,
+  const int64_t vals[16] = {P->n, P->n_nodes, P->n_leaves, P->host.depth, P->n_u,
                             P->n_p2p_interactions, P->n_v, P->n_off, P->p, P->ws,
-                            P->n_near, levels_with_m2l, P->n_p2p_src};
-  for (int i = 0; i < n_out && i < 13; ++i) out[i] = vals[i];
+                            P->n_near, levels_with_m2l, P->n_p2p_src,
+                            P->n_p2s_sym_interactions, P->n_p2s_sym + P->n_p2s_items,
+                            P->n_p2s_slots};
+  for (int i = 0; i < n_out && i < 16; ++i) out[i] = vals[i];
   return ED_OK;
 }
 

@@ -12,8 +12,11 @@
 //               into each node's own part of its local expansion
 //   down        one block per node: the L2L recursion over its own chain of
 //               ancestors (all levels in one launch)
-//   p2p         one wave per work item (<= 64 targets x a piece of their
-//               leaf's U-list source stream), partial sums per item
+//   p2p         one wave per work item, partial sums per item: for one
+//               right-hand side every unordered pair of distinct U-list
+//               leaves once for both directions (symmetric items) plus a
+//               leaf with itself; for several, <= 64 targets x a piece of
+//               their leaf's U-list source stream
 //   l2p_out     one thread per panel: L2P + P2P partials + near CSR + self
 //               term, K_E scaling, scatter to the output order
 // Several right-hand sides (up to 8) run through the same launches with the
@@ -95,40 +98,6 @@ __device__ __forceinline__ void p2m_accum(float x, float y, float z, float q, fl
       }
     }
   }
-}
-
-template <int P>
-__device__ __forceinline__ float l2p_eval(float x, float y, float z, const float* __restrict__ Lc) {
-  const float r2 = x * x + y * y + z * z;
-  float pmr = 1.0f, pmi = 0.0f;
-  float acc0 = 0.0f, acc1 = 0.0f;
-#pragma unroll
-  for (int m = 0; m <= P; ++m) {
-    if (m > 0) {
-      const float f = -0.5f / (float)m;
-      const float nr = f * (x * pmr - y * pmi);
-      const float ni = f * (x * pmi + y * pmr);
-      pmr = nr;
-      pmi = ni;
-    }
-    float ar = 0.0f, ai = 0.0f, br = pmr, bi = pmi;
-#pragma unroll
-    for (int n = m; n <= P; ++n) {
-      if (m == 0) {
-        acc0 += Lc[n * n] * br;
-      } else {
-        acc1 += Lc[n * n + 2 * m - 1] * br - Lc[n * n + 2 * m] * bi;
-      }
-      if (n < P) {
-        const float inv = 1.0f / (float)((n + 1) * (n + 1) - m * m);
-        const float c = (float)(2 * n + 1) * z;
-        const float nr = (c * br - r2 * ar) * inv;
-        const float ni = (c * bi - r2 * ai) * inv;
-        ar = br; ai = bi; br = nr; bi = ni;
-      }
-    }
-  }
-  return acc0 + 2.0f * acc1;
 }
 
 // ---------------------------------------------------------------------------
@@ -1201,16 +1170,10 @@ __device__ __forceinline__ float p2p_sliced(const float4* __restrict__ pts, int 
   return acc;
 }
 
-__global__ __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_p2p(const ed::P2PItem* __restrict__ items,
-                                                       int n_items,
-                                                       const int2* __restrict__ seg,
-                                                       const float4* __restrict__ pts,
-                                                       float* __restrict__ part) {
-  // the wave index is uniform: readfirstlane lets the item and every source
-  // address derived from it live in SGPRs (scalar loads)
-  const int item = blockIdx.x * kP2PWaves + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if (item >= n_items) return;  // whole waves only
-  const ed::P2PItem it = items[item];
+// One one-directional item.
+__device__ __forceinline__ void p2p_one_item(const ed::P2PItem& it, const int2* __restrict__ seg,
+                                             const float4* __restrict__ pts,
+                                             float* __restrict__ part) {
   const int lane = threadIdx.x & 63;
   const int ntc = it.ntc, S = it.slices;
   const int tl = lane % ntc, sl = lane / ntc;
@@ -1240,6 +1203,111 @@ __global__ __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(
     acc = v;
   }
   if (lane < ntc) part[it.out + lane] = acc;
+}
+
+// Column sums of a wave's 64 x 16 block: v[u] in lane l is row l, column u;
+// lane l returns the sum over all 64 rows of column (l >> 2) & 15, added in a
+// fixed order.  Six pairing steps, each between lanes that differ in one
+// deciding bit of the column index: v_permlane32_swap (lane bit 5 picks
+// columns u or u + 8), v_permlane16_swap (bit 4: + 4), DPP row_mirror (bit 3:
+// + 2) and row_half_mirror (bit 2: + 1) each halve the columns a lane holds
+// while adding its partner's copy; the last two (DPP quad_perm) add the four
+// lanes that hold the same column.  35 VALU instructions for 1,024 entries.
+__device__ __forceinline__ float p2p_colsum16(const float (&v)[16], int lane) {
+  float w[8];
+#pragma unroll
+  for (int u = 0; u < 8; ++u) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(v[u]),
+                                                    __float_as_uint(v[u + 8]), false, false);
+    w[u] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+  float x[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(w[u]),
+                                                    __float_as_uint(w[u + 4]), false, false);
+    x[u] = __uint_as_float(r[0]) + __uint_as_float(r[1]);
+  }
+  const bool b3 = (lane & 8) != 0, b2 = (lane & 4) != 0;
+  constexpr int kRowMirror = 0x140, kRowHalfMirror = 0x141, kQuadXor1 = 0xB1, kQuadXor2 = 0x4E;
+  float y[2];
+#pragma unroll
+  for (int u = 0; u < 2; ++u) {
+    const float keep = b3 ? x[u + 2] : x[u], send = b3 ? x[u] : x[u + 2];
+    y[u] = keep + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(send), kRowMirror,
+                                                          0xF, 0xF, false));
+  }
+  const float keep = b2 ? y[1] : y[0], send = b2 ? y[0] : y[1];
+  float z = keep + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(send), kRowHalfMirror,
+                                                           0xF, 0xF, false));
+  z += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(z), kQuadXor1, 0xF, 0xF, false));
+  z += __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(z), kQuadXor2, 0xF, 0xF, false));
+  return z;
+}
+
+// One symmetric item: lane l holds the target its chunk's segments give it
+// (lanes past nl carry charge 0 and write nothing); sources stream in groups
+// of 16 at wave-uniform addresses.  Per pair one r^2 and one rsqrt k_ij: the
+// target side adds q_j k_ij per lane, the source side sums q_i k_ij over the
+// lanes (p2p_colsum16 per group).  Targets and sources are different
+// leaves, so there is no j == i exclusion.
+__device__ __forceinline__ void p2p_sym_item(const ed::P2PSymItem& it,
+                                             const ed::P2PSymSeg* __restrict__ tseg,
+                                             const float4* __restrict__ pts,
+                                             float* __restrict__ part) {
+  const int lane = threadIdx.x & 63;
+  int panel = 0, slot = -1;
+  for (int k = 0; k < it.nseg; ++k) {  // segments in lane order: the last one at or below l
+    const ed::P2PSymSeg g = tseg[it.seg0 + k];
+    if (lane >= g.lane0) {
+      panel = g.panel0 + (lane - g.lane0);
+      slot = g.slot0 + (lane - g.lane0) + it.q * g.stride;
+    }
+  }
+  if (lane >= it.nl) {
+    panel = tseg[it.seg0].panel0;
+    slot = -1;
+  }
+  const float4 xi = pts[panel];
+  const float qi = slot >= 0 ? xi.w : 0.0f;
+  const int s0 = it.s0, s1 = it.s0 + it.cnt;
+  float acc = 0.0f;
+  for (int j0 = s0; j0 < s1; j0 += 16) {
+    float4 sp[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) sp[u] = pts[min(j0 + u, s1 - 1)];
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const float qs = j0 + u < s1 ? sp[u].w : 0.0f;  // clamped repeats add nothing
+      const float k = __builtin_amdgcn_rsqf(p2p_r2(xi, sp[u]));
+      acc = fmaf(qs, k, acc);
+      v[u] = qi * k;
+    }
+    const float csum = p2p_colsum16(v, lane);
+    const int u = (lane >> 2) & 15;
+    if ((lane & 3) == 0 && j0 + u < s1) part[it.outS + (j0 - s0) + u] = csum;
+  }
+  if (slot >= 0) part[slot] = acc;
+}
+
+// One-right-hand-side P2P: blocks [0, nb_sym) run the symmetric items (the
+// heavier ones first), the rest the one-directional items.
+__global__ __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_p2p_sym(
+    const ed::P2PSymItem* __restrict__ sym, const ed::P2PSymSeg* __restrict__ tseg, int n_sym,
+    int nb_sym,
+    const ed::P2PItem* __restrict__ items, int n_items, const int2* __restrict__ seg,
+    const float4* __restrict__ pts, float* __restrict__ part) {
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if ((int)blockIdx.x < nb_sym) {
+    const int item = blockIdx.x * kP2PWaves + wave;
+    if (item >= n_sym) return;  // whole waves only
+    p2p_sym_item(sym[item], tseg, pts, part);
+  } else {
+    const int item = ((int)blockIdx.x - nb_sym) * kP2PWaves + wave;
+    if (item >= n_items) return;
+    p2p_one_item(items[item], seg, pts, part);
+  }
 }
 
 // P2P for R right-hand sides: the same items and schedule; r^2 and its
@@ -1368,20 +1436,68 @@ __global__ void k_gather_multi(int n, int R, const int32_t* __restrict__ perm,
   if (r == 0) selfc_t[i] = selfi ? (float)selfi[o] : 0.0f;
 }
 
+// Part Q (of four) of the L2P sum: the orders m with l2p_owner(m) == Q (a
+// snake over m, so that the four parts hold about equal numbers of terms:
+// at p = 10 {0,7,8}, {1,6,9}, {2,5,10}, {3,4}); every part runs the diagonal
+// recurrence R_m^m itself.
+constexpr int l2p_owner(int m) { return (m % 8) < 4 ? m % 8 : 7 - m % 8; }
+
+template <int P, int Q>
+__device__ __forceinline__ float l2p_part(float x, float y, float z, const float* __restrict__ Lc) {
+  const float r2 = x * x + y * y + z * z;
+  float pmr = 1.0f, pmi = 0.0f;
+  float acc0 = 0.0f, acc1 = 0.0f;
+#pragma unroll
+  for (int m = 0; m <= P; ++m) {
+    if (m > 0) {
+      const float f = -0.5f / (float)m;
+      const float nr = f * (x * pmr - y * pmi);
+      const float ni = f * (x * pmi + y * pmr);
+      pmr = nr;
+      pmi = ni;
+    }
+    if (l2p_owner(m) == Q) {  // folded per m by the full unroll
+      float ar = 0.0f, ai = 0.0f, br = pmr, bi = pmi;
+#pragma unroll
+      for (int n = m; n <= P; ++n) {
+        if (m == 0) {
+          acc0 += Lc[n * n] * br;
+        } else {
+          acc1 += Lc[n * n + 2 * m - 1] * br - Lc[n * n + 2 * m] * bi;
+        }
+        if (n < P) {
+          const float inv = 1.0f / (float)((n + 1) * (n + 1) - m * m);
+          const float c = (float)(2 * n + 1) * z;
+          const float nr = (c * br - r2 * ar) * inv;
+          const float ni = (c * bi - r2 * ai) * inv;
+          ar = br; ai = bi; br = nr; bi = ni;
+        }
+      }
+    }
+  }
+  return acc0 + 2.0f * acc1;
+}
+
 // Per panel (tree order): far field by L2P from its leaf's local expansion,
 // the P2P sum, the near-field correction (CSR row), the self term; output
 // in tree or original order.  Non-finite results become 0
-// (fmm3d/bem_fmm.py:741-749).
+// (fmm3d/bem_fmm.py:741-749).  A block of four waves takes 64 panels: wave q
+// evaluates part q of the L2P (l2p_part), the near-correction entries and
+// partial rows q, q + 4, ... of each panel (a quarter of the dependent loads
+// per lane and four times the waves of one thread per panel), and wave 0
+// adds the four parts from LDS in order (0 + 1) + (2 + 3).
 template <int P, typename TO, bool MULTI>
 __global__ __launch_bounds__(256) void k_l2p_out(
     int n, const int32_t* __restrict__ pt_node, const float4* __restrict__ ctr,
     const float4* __restrict__ pts, const float* __restrict__ L,
     const int2* __restrict__ p2p_ref, const float* __restrict__ p2p_part,
+    const int2* __restrict__ node_rng, const int2* __restrict__ p2s_leaf,
     const int32_t* __restrict__ near_ptr, const int32_t* __restrict__ near_col,
     const float* __restrict__ near_w, const float* __restrict__ sig_t,
     const float* __restrict__ selfc_t, const int32_t* __restrict__ perm, int out_tree, float ke,
     TO* __restrict__ out, RhsStride rs) {
   constexpr int NC = (P + 1) * (P + 1);
+  __shared__ float part4[4][64];
   const int ry = MULTI ? blockIdx.y : 0;
   if constexpr (MULTI) {
     L += ry * rs.node;
@@ -1389,24 +1505,26 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     sig_t += ry * rs.n;
     out += ry * rs.n;
   }
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int i0 = blockIdx.x * 64 + lane;
+  const int i = min(i0, n - 1);  // lanes past n compute a copy and write nothing
   const int node = pt_node[i];
   const float4 xi = pts[i];
   const int nb = near_ptr[i], ne = near_ptr[i + 1];
   float corr = 0.0f;
-  // groups of four with clamped indices and zero weights past the row end:
-  // every load of a group is issued before the first wait (the row's column
-  // and weight loads, then the four dependent sigma gathers)
-  for (int q = nb; q < ne; q += 4) {
+  // this wave's entries nb + q, nb + q + 4, ... in groups of four with
+  // clamped indices and zero weights past the row end: every load of a group
+  // is issued before the first wait
+  for (int e0 = nb + q; e0 < ne; e0 += 16) {
     int c[4];
     float wq[4], sv[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      const int e = min(q + u, ne - 1);
+      const int e = min(e0 + 4 * u, ne - 1);
       c[u] = near_col[e];
       const float wv = near_w[e];
-      wq[u] = q + u < ne ? wv : 0.0f;
+      wq[u] = e0 + 4 * u < ne ? wv : 0.0f;
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) sv[u] = sig_t[c[u]];
@@ -1414,21 +1532,43 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     for (int u = 0; u < 4; ++u) corr += wq[u] * sv[u];
   }
   const float4 c = ctr[node];
-  const float far = l2p_eval<P>((xi.x - c.x) * c.w, (xi.y - c.y) * c.w, (xi.z - c.z) * c.w,
-                                L + (size_t)node * NC);
-  const int2 pr = p2p_ref[i];
-  float near = 0.0f;
-  int k = 0;
-  for (; k + 4 <= pr.y; k += 4) {  // four loads in flight, added in order
-    float a[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = p2p_part[pr.x + 64 * (k + u)];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) near += a[u];
+  const float x = (xi.x - c.x) * c.w, y = (xi.y - c.y) * c.w, z = (xi.z - c.z) * c.w;
+  const float* Lc = L + (size_t)node * NC;
+  float far;
+  switch (q) {
+    case 0: far = l2p_part<P, 0>(x, y, z, Lc); break;
+    case 1: far = l2p_part<P, 1>(x, y, z, Lc); break;
+    case 2: far = l2p_part<P, 2>(x, y, z, Lc); break;
+    default: far = l2p_part<P, 3>(x, y, z, Lc); break;
   }
-  for (; k < pr.y; ++k) near += p2p_part[pr.x + 64 * k];
+  float near = 0.0f;
+  if constexpr (MULTI) {  // the one-directional schedule: rows of 64 per item
+    const int2 pr = p2p_ref[i];
+    for (int k = q; k < pr.y; k += 16) {  // four loads in flight, added in order
+      float a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = k + 4 * u < pr.y ? p2p_part[pr.x + 64 * (k + 4 * u)] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) near += a[u];
+    }
+  } else {  // the symmetric schedule: K rows of n_L slots per leaf, in order
+    const int2 lr = p2s_leaf[node], nr = node_rng[node];
+    const int nl = nr.y - nr.x;
+    const float* pp = p2p_part + lr.x + (i - nr.x);
+    for (int k = q; k < lr.y; k += 32) {  // eight loads in flight, added in order
+      float a[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) a[u] = k + 4 * u < lr.y ? pp[(size_t)(k + 4 * u) * nl] : 0.0f;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) near += a[u];
+    }
+  }
+  part4[q][lane] = ke * (far + near) + corr;
+  __syncthreads();
+  if (q != 0 || i0 >= n) return;
   const float qi = MULTI ? rs.qT[(size_t)i * rs.R + ry] : xi.w;
-  float v = ke * (far + near) + selfc_t[i] * qi + corr;
+  float v = ((part4[0][lane] + part4[1][lane]) + (part4[2][lane] + part4[3][lane])) +
+            selfc_t[i] * qi;
   if (!isfinite(v)) v = 0.0f;
   out[out_tree ? i : perm[i]] = (TO)v;
 }
@@ -1458,10 +1598,16 @@ __global__ __launch_bounds__(256) void k_l2p_out(
 
 static void p2p_launch(ed_plan* Pn, int R, const RhsStride& rs, hipStream_t s) {
   const unsigned nb = (unsigned)((Pn->n_p2p_items + kP2PWaves - 1) / kP2PWaves);
-  if (R == 1) {
-    hipLaunchKernelGGL(k_p2p, dim3(nb), dim3(64 * kP2PWaves), 0, s,
-                       (const ed::P2PItem*)Pn->p2p_items, (int)Pn->n_p2p_items, Pn->p2p_seg,
-                       Pn->pts, Pn->p2p_part);
+  if (R == 1) {  // the symmetric schedule
+    const int nb_sym = (int)((Pn->n_p2s_sym + kP2PWaves - 1) / kP2PWaves);
+    const int nb_one = (int)((Pn->n_p2s_items + kP2PWaves - 1) / kP2PWaves);
+    if (nb_sym + nb_one > 0)
+      hipLaunchKernelGGL(k_p2p_sym, dim3((unsigned)(nb_sym + nb_one)), dim3(64 * kP2PWaves), 0, s,
+                         (const ed::P2PSymItem*)Pn->p2s_sym, (const ed::P2PSymSeg*)Pn->p2s_tseg,
+                         (int)Pn->n_p2s_sym,
+                         nb_sym,
+                         (const ed::P2PItem*)Pn->p2s_items, (int)Pn->n_p2s_items, Pn->p2s_seg,
+                         Pn->pts, Pn->p2s_part);
     return;
   }
   const auto* items = (const ed::P2PItem*)Pn->p2p_items;
@@ -1502,7 +1648,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   float* Yb = multi ? Pn->multi.ym2m : Pn->ym2m;   // own parts of the local expansions
   float* Yp = multi ? Pn->multi.ypair : Pn->ypair;
   float* sigb = multi ? Pn->multi.sig : Pn->sig_t;
-  float* partb = multi ? Pn->multi.part : Pn->p2p_part;
+  float* partb = multi ? Pn->multi.part : Pn->p2s_part;
   const unsigned RY = (unsigned)R;
   ED_MARK(0);
   const int gb = (n + 255) / 256;
@@ -1655,14 +1801,17 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     ED_LAUNCH_CHECK("p2p");
   }
   const float ke = (float)kKE;
+  const unsigned gb64 = (unsigned)((n + 63) / 64);  // k_l2p_out: 64 panels a block
   ED_MARK(7);
   if (out64)
-    hipLaunchKernelGGL((k_l2p_out<P, double, MULTI>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
-                       Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->near_ptr, Pn->near_col,
+    hipLaunchKernelGGL((k_l2p_out<P, double, MULTI>), dim3(gb64, RY), dim3(256), 0, st, n, Pn->pt_node,
+                       Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->node_rng,
+                       Pn->p2s_leaf, Pn->near_ptr, Pn->near_col,
                        Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (double*)out, rs);
   else
-    hipLaunchKernelGGL((k_l2p_out<P, float, MULTI>), dim3(gb, RY), dim3(256), 0, st, n, Pn->pt_node,
-                       Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->near_ptr, Pn->near_col,
+    hipLaunchKernelGGL((k_l2p_out<P, float, MULTI>), dim3(gb64, RY), dim3(256), 0, st, n, Pn->pt_node,
+                       Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->node_rng,
+                       Pn->p2s_leaf, Pn->near_ptr, Pn->near_col,
                        Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (float*)out, rs);
   ED_LAUNCH_CHECK("l2p_out");
   ED_MARK(8);

@@ -39,14 +39,37 @@ struct ed_plan {
 
   // leaves (node ids) and their U lists (CSR over leaf ordinal)
   int32_t* pt_node = nullptr;   // n, leaf node of each tree-ordered panel
-  // P2P work items (ed::P2PItem), their source segments (per leaf: the leaf
-  // itself, then its other U-list leaves), one row of 64 partial sums per
-  // item, and per tree-ordered panel (first partial, number of partials)
+  // Multi-right-hand-side P2P work items (ed::P2PItem), their source
+  // segments (per leaf: the leaf itself, then its other U-list leaves) and
+  // per tree-ordered panel (first partial row, number of rows); the rows of
+  // 64 partial sums per item and RHS sit in multi.part
   void* p2p_items = nullptr;
   int2* p2p_seg = nullptr;
-  float* p2p_part = nullptr;
   int2* p2p_ref = nullptr;
   int64_t n_p2p_items = 0;
+  // One-right-hand-side P2P: every unordered pair of distinct U-list leaves
+  // is evaluated once for both directions (ed::P2PSymItem: r_ij and its
+  // rsqrt once, the target side summed per lane, the source side by a
+  // cross-lane column sum).  The smaller leaf of a pair is the target side,
+  // and the targets of all the smaller neighbours of a source leaf are packed
+  // 64 to a chunk (a chunk is a few lane ranges, ed::P2PSymSeg, each the
+  // panels of one target leaf).  A leaf with itself runs one-directional
+  // items (ed::P2PItem over
+  // p2s_seg).  Partial sums sit in rows per leaf: leaf L (panels b .. b +
+  // n_L - 1) owns K_L rows of n_L slots from p2s_leaf[L].x, row k slot i - b
+  // for panel i; every row is written whole, by the items of one (pair,
+  // source piece), one chunk or one piece of the own stream, and k_l2p_out
+  // adds a panel's rows in order.  (The multi-right-hand-side matvec keeps
+  // the one-directional schedule above: with R charges per source the column
+  // sums would cost more than the shared rsqrt saves.)
+  void* p2s_items = nullptr;
+  int2* p2s_seg = nullptr;
+  void* p2s_sym = nullptr;
+  void* p2s_tseg = nullptr;  // ed::P2PSymSeg: target segments of the chunks
+  int64_t n_p2s_items = 0, n_p2s_sym = 0;
+  float* p2s_part = nullptr;
+  int2* p2s_leaf = nullptr;  // per node: (first slot, rows K); leaves only
+  int64_t n_p2s_slots = 0, n_p2s_sym_interactions = 0;
   int32_t* leaves = nullptr;
   int32_t* u_ptr = nullptr;
   int32_t* u_src = nullptr;  // node ids
@@ -158,6 +181,19 @@ struct P2PItem {
   int32_t tgt0, ntc, seg, off, cnt, out, slices, pad;
 };
 constexpr int kP2PItemSteps = 256;
+// Symmetric P2P item: the nl <= 64 targets of chunk segments seg0 ..
+// seg0 + nseg - 1 against the sources s0 .. s0 + cnt - 1 (one leaf, piece q
+// of it) in both directions; the sources' sums go to part[outS + j - s0].
+struct P2PSymItem {
+  int32_t seg0, nseg, q, nl, s0, cnt, outS, pad;
+};
+// Lanes lane0 .. (the next segment's lane0) - 1 of a chunk hold the panels
+// panel0 + (l - lane0) of one target leaf, whose sums for source piece q go
+// to part[slot0 + (l - lane0) + q * stride] (stride: the leaf's size).
+struct P2PSymSeg {
+  int32_t lane0, panel0, slot0, stride;
+};
+constexpr int kP2PSymSources = 256;  // sources per symmetric item
 // Work item of the split dense operator: up to 512 target rows and one
 // contiguous source range of their geometry.
 struct DenseItem {

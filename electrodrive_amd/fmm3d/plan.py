@@ -9,7 +9,8 @@ from .config import M2L_PRECISIONS
 
 STAT_NAMES = ("n_points", "n_nodes", "n_leaves", "depth", "n_p2p_leaf_pairs",
               "n_p2p_interactions", "n_m2l_pairs", "n_m2l_offsets", "order", "ws",
-              "n_near_pairs", "n_levels_with_m2l", "n_p2p_source_points")
+              "n_near_pairs", "n_levels_with_m2l", "n_p2p_source_points",
+              "n_p2p_symmetric_interactions", "n_p2p_items", "n_p2p_partials")
 
 
 class FmmPlan:

@@ -76,7 +76,10 @@ int ed_plan_geometries(const ed_plan* plan, int64_t* n_geoms_out, int64_t* offse
  * 0 n_points 1 n_nodes 2 n_leaves 3 depth 4 n_p2p_leaf_pairs
  * 5 n_p2p_interactions 6 n_m2l_pairs 7 n_m2l_offsets 8 order 9 ws
  * 10 n_near_pairs 11 n_levels_with_m2l 12 n_p2p_source_points (sum over P2P
- * leaf pairs of the source leaf size) */
+ * leaf pairs of the source leaf size) 13 n_p2p_symmetric_interactions
+ * (ordered interactions the one-RHS P2P evaluates in symmetric pairs)
+ * 14 n_p2p_items (one-RHS P2P work items) 15 n_p2p_partials (one-RHS P2P
+ * partial sums per matvec) */
 int ed_plan_stats(const ed_plan* plan, int64_t* out, int32_t n_out);
 
 /* Tree-order permutation (index_map, fmm3d/tree.py:428-485): perm[i] is the
