@@ -879,7 +879,11 @@ static int plan_create_impl(const double* centroids, const double* areas, int64_
   if (rc == ED_OK) rc = plan_upload(P);
   if (rc == ED_OK) {
     const HostTree& H = P->host;
-    std::vector<float4> pts(n);
+    // pts[n]: a zero-charge point far outside every geometry, the source that
+    // the P2P's last group of a source range reads past the range's end
+    // (it adds 0; k_gather writes only pts[0 .. n-1])
+    std::vector<float4> pts(n + 1);
+    pts[n] = make_float4(1e18f, 1e18f, 1e18f, 0.0f);
     std::vector<float> at(n);
     std::vector<int32_t> perm(n);
     std::vector<double> c64(3 * n), a64(n);

@@ -1247,13 +1247,13 @@ __device__ __forceinline__ float p2p_colsum16(const float (&v)[16], int lane) {
 
 // One symmetric item: lane l holds the target its chunk's segments give it
 // (lanes past nl carry charge 0 and write nothing); sources stream in groups
-// of 16 at wave-uniform addresses.  Per pair one r^2 and one rsqrt k_ij: the
+// of 16 at wave-uniform addresses (pts[n_pts] is the zero-charge sentinel).  Per pair one r^2 and one rsqrt k_ij: the
 // target side adds q_j k_ij per lane, the source side sums q_i k_ij over the
 // lanes (p2p_colsum16 per group).  Targets and sources are different
 // leaves, so there is no j == i exclusion.
 __device__ __forceinline__ void p2p_sym_item(const ed::P2PSymItem& it,
                                              const ed::P2PSymSeg* __restrict__ tseg,
-                                             const float4* __restrict__ pts,
+                                             const float4* __restrict__ pts, int n_pts,
                                              float* __restrict__ part) {
   const int lane = threadIdx.x & 63;
   int panel = 0, slot = -1;
@@ -1272,37 +1272,45 @@ __device__ __forceinline__ void p2p_sym_item(const ed::P2PSymItem& it,
   const float qi = slot >= 0 ? xi.w : 0.0f;
   const int s0 = it.s0, s1 = it.s0 + it.cnt;
   float acc = 0.0f;
-  for (int j0 = s0; j0 < s1; j0 += 16) {
+  // one group of 16 sources: full groups read 16 consecutive points (no
+  // masks: contiguous scalar loads), the last group reads the zero-charge
+  // sentinel pts[n] past s1 (its column sums are not stored)
+  auto group = [&](int j0, bool tail) {
     float4 sp[16];
 #pragma unroll
-    for (int u = 0; u < 16; ++u) sp[u] = pts[min(j0 + u, s1 - 1)];
+    for (int u = 0; u < 16; ++u) sp[u] = pts[tail && j0 + u >= s1 ? n_pts : j0 + u];
     float v[16];
 #pragma unroll
     for (int u = 0; u < 16; ++u) {
-      const float qs = j0 + u < s1 ? sp[u].w : 0.0f;  // clamped repeats add nothing
       const float k = __builtin_amdgcn_rsqf(p2p_r2(xi, sp[u]));
-      acc = fmaf(qs, k, acc);
+      acc = fmaf(sp[u].w, k, acc);
       v[u] = qi * k;
     }
     const float csum = p2p_colsum16(v, lane);
     const int u = (lane >> 2) & 15;
     if ((lane & 3) == 0 && j0 + u < s1) part[it.outS + (j0 - s0) + u] = csum;
-  }
+  };
+  int j0 = s0;
+  for (; j0 + 16 <= s1; j0 += 16) group(j0, false);
+  if (j0 < s1) group(j0, true);
   if (slot >= 0) part[slot] = acc;
 }
 
 // One-right-hand-side P2P: blocks [0, nb_sym) run the symmetric items (the
 // heavier ones first), the rest the one-directional items.
-__global__ __launch_bounds__(64 * kP2PWaves) __attribute__((amdgpu_waves_per_eu(8))) void k_p2p_sym(
+// (No waves-per-EU cap: at 8 waves the compiler holds the SGPR budget so
+// low that the 64 source SGPRs of a group spill to VGPR lanes; uncapped it
+// needs 52 VGPRs, which still gives 8 waves per SIMD.)
+__global__ __launch_bounds__(64 * kP2PWaves) void k_p2p_sym(
     const ed::P2PSymItem* __restrict__ sym, const ed::P2PSymSeg* __restrict__ tseg, int n_sym,
     int nb_sym,
     const ed::P2PItem* __restrict__ items, int n_items, const int2* __restrict__ seg,
-    const float4* __restrict__ pts, float* __restrict__ part) {
+    const float4* __restrict__ pts, int n_pts, float* __restrict__ part) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   if ((int)blockIdx.x < nb_sym) {
     const int item = blockIdx.x * kP2PWaves + wave;
     if (item >= n_sym) return;  // whole waves only
-    p2p_sym_item(sym[item], tseg, pts, part);
+    p2p_sym_item(sym[item], tseg, pts, n_pts, part);
   } else {
     const int item = ((int)blockIdx.x - nb_sym) * kP2PWaves + wave;
     if (item >= n_items) return;
@@ -1607,7 +1615,7 @@ static void p2p_launch(ed_plan* Pn, int R, const RhsStride& rs, hipStream_t s) {
                          (int)Pn->n_p2s_sym,
                          nb_sym,
                          (const ed::P2PItem*)Pn->p2s_items, (int)Pn->n_p2s_items, Pn->p2s_seg,
-                         Pn->pts, Pn->p2s_part);
+                         Pn->pts, (int)Pn->n, Pn->p2s_part);
     return;
   }
   const auto* items = (const ed::P2PItem*)Pn->p2p_items;

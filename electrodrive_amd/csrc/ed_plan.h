@@ -18,7 +18,8 @@ struct ed_plan {
   int64_t n = 0, n_nodes = 0, n_leaves = 0;
   ed::HostTree host;
 
-  // points in tree order: x, y, z relative to the root centre (FP32), w = q
+  // points in tree order: x, y, z relative to the root centre (FP32), w = q;
+  // pts[n] is a zero-charge sentinel far away (the P2P's padding source)
   float4* pts = nullptr;
   float* area_t = nullptr;   // areas, tree order
   float* sig_t = nullptr;    // sigma, tree order (written per matvec)
