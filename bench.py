@@ -36,20 +36,20 @@ VALU_FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE x 2.0 +
 # WRITE_SIZE x 1.0, the factors a 256 MiB device copy reads), keyed by
 # (stage, panels, order, leaf, right-hand sides, M2L form), with the summary
-# holding the commands; and the VALU-busy fraction of k_p2p from SQ counters
+# holding the commands; and the VALU-busy fraction of k_p2p_sym from SQ counters
 PMC_SPHERE = "profiles/pmc_traffic_icosphere81920_p10_leaf256.md"
 PMC_BATCH = "profiles/pmc_traffic_batch64_8rhs_p10_leaf512.md"
 PMC_TORUS = "profiles/pmc_traffic_torus300192_p10_leaf384_bf16.md"
-TRAFFIC_BYTES = {("p2p", 81920, 10, 256, 1, "fp16x3"): (2.189e7, PMC_SPHERE),
-                 ("m2l", 81920, 10, 256, 1, "fp16x3"): (3.598e7, PMC_SPHERE),
+TRAFFIC_BYTES = {("p2p", 81920, 10, 256, 1, "fp16x3"): (2.127e7, PMC_SPHERE),
+                 ("m2l", 81920, 10, 256, 1, "fp16x3"): (3.597e7, PMC_SPHERE),
                  ("m2l_reduce", 81920, 10, 256, 1, "fp16x3"): (2.823e7, PMC_SPHERE),
                  ("p2p", 877882, 10, 512, 8, "fp16x3"): (1.230e9, PMC_BATCH),
                  ("m2l", 877882, 10, 512, 8, "fp16x3"): (1.117e9, PMC_BATCH),
                  ("m2l_reduce", 877882, 10, 512, 8, "fp16x3"): (7.569e8, PMC_BATCH),
-                 ("p2p", 300192, 10, 384, 1, "bf16"): (1.189e8, PMC_TORUS),
-                 ("m2l", 300192, 10, 384, 1, "bf16"): (8.601e7, PMC_TORUS),
+                 ("p2p", 300192, 10, 384, 1, "bf16"): (1.111e8, PMC_TORUS),
+                 ("m2l", 300192, 10, 384, 1, "bf16"): (8.611e7, PMC_TORUS),
                  ("m2l_reduce", 300192, 10, 384, 1, "bf16"): (6.628e7, PMC_TORUS)}
-P2P_VALU_BUSY = {(81920, 10, 256): 0.69}
+P2P_VALU_BUSY = {(81920, 10, 256): 0.68}
 
 
 def parse():
