@@ -610,6 +610,7 @@ int plan_upload(ed_plan* P) {
   ED_TRY(p2p_schedule(P, leaves));
   ED_TRY(p2p_schedule_sym(P, leaves));
   ED_TRY(side_stream(P->device, &P->side));
+  ED_HIP(hipDeviceGetAttribute(&P->n_cu, hipDeviceAttributeMultiprocessorCount, P->device));
   ED_HIP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
   ED_HIP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
   ED_TRY(upload(&P->u_ptr, uptr));

@@ -1105,6 +1105,7 @@ __global__ __launch_bounds__(128) void k_m2l_down(int node0, const int32_t* __re
 // ---------------------------------------------------------------------------
 constexpr int kP2PUnroll = 8;
 constexpr int kP2PWaves = 4;  // independent items per block
+constexpr int kP2PBlocksPerCU = 4;  // grid cap of the one-RHS P2P (see k_p2p_sym)
 
 // r^2 + 1e-24 in place of max(r^2, 1e-24): equal for every r^2 above 3e-17
 // (the addend is below half an ulp there), and both give 1e12 for r = 0.
@@ -1296,8 +1297,14 @@ __device__ __forceinline__ void p2p_sym_item(const ed::P2PSymItem& it,
   if (slot >= 0) part[slot] = acc;
 }
 
-// One-right-hand-side P2P: blocks [0, nb_sym) run the symmetric items (the
-// heavier ones first), the rest the one-directional items.
+// One-right-hand-side P2P: item blocks [0, nb_sym) run the symmetric items
+// (the heavier ones first), the rest the one-directional items.  The grid is
+// capped at kP2PBlocksPerCU blocks per compute unit, each walking the item
+// blocks with a grid stride: P2P runs beside the far-field chain, and with
+// every SIMD's register file full of P2P waves the chain's kernels waited
+// for P2P waves to retire (P2M 8.3 us alone, 17 us beside the uncapped
+// P2P); at four blocks per CU the 82k matvec went 99.3 -> 91.0 us (three:
+// 95.6, five: 91.9, six: 99.6).
 // (No waves-per-EU cap: at 8 waves the compiler holds the SGPR budget so
 // low that the 64 source SGPRs of a group spill to VGPR lanes; uncapped it
 // needs 52 VGPRs, which still gives 8 waves per SIMD.)
@@ -1307,14 +1314,16 @@ __global__ __launch_bounds__(64 * kP2PWaves) void k_p2p_sym(
     const ed::P2PItem* __restrict__ items, int n_items, const int2* __restrict__ seg,
     const float4* __restrict__ pts, int n_pts, float* __restrict__ part) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  if ((int)blockIdx.x < nb_sym) {
-    const int item = blockIdx.x * kP2PWaves + wave;
-    if (item >= n_sym) return;  // whole waves only
-    p2p_sym_item(sym[item], tseg, pts, n_pts, part);
-  } else {
-    const int item = ((int)blockIdx.x - nb_sym) * kP2PWaves + wave;
-    if (item >= n_items) return;
-    p2p_one_item(items[item], seg, pts, part);
+  const int nb_all = nb_sym + (n_items + kP2PWaves - 1) / kP2PWaves;
+  // a capped grid walks the item blocks (each wave on its own; no barriers)
+  for (int blk = blockIdx.x; blk < nb_all; blk += gridDim.x) {
+    if (blk < nb_sym) {
+      const int item = blk * kP2PWaves + wave;
+      if (item < n_sym) p2p_sym_item(sym[item], tseg, pts, n_pts, part);
+    } else {
+      const int item = (blk - nb_sym) * kP2PWaves + wave;
+      if (item < n_items) p2p_one_item(items[item], seg, pts, part);
+    }
   }
 }
 
@@ -1609,8 +1618,9 @@ static void p2p_launch(ed_plan* Pn, int R, const RhsStride& rs, hipStream_t s) {
   if (R == 1) {  // the symmetric schedule
     const int nb_sym = (int)((Pn->n_p2s_sym + kP2PWaves - 1) / kP2PWaves);
     const int nb_one = (int)((Pn->n_p2s_items + kP2PWaves - 1) / kP2PWaves);
-    if (nb_sym + nb_one > 0)
-      hipLaunchKernelGGL(k_p2p_sym, dim3((unsigned)(nb_sym + nb_one)), dim3(64 * kP2PWaves), 0, s,
+    const int grid = std::min(nb_sym + nb_one, kP2PBlocksPerCU * std::max(Pn->n_cu, 1));
+    if (grid > 0)
+      hipLaunchKernelGGL(k_p2p_sym, dim3((unsigned)grid), dim3(64 * kP2PWaves), 0, s,
                          (const ed::P2PSymItem*)Pn->p2s_sym, (const ed::P2PSymSeg*)Pn->p2s_tseg,
                          (int)Pn->n_p2s_sym,
                          nb_sym,

@@ -153,6 +153,7 @@ struct ed_plan {
   // the stream is shared by all plans on the device and never destroyed
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int n_cu = 256;  // compute units of the plan's device (caps the one-RHS P2P grid)
 
   // GMRES workspace (allocated lazily by the solver, owned by the plan)
   void* gmres_ws = nullptr;
