@@ -69,7 +69,7 @@ static int upload(T** dst, const std::vector<T>& src) {
   } while (0)
 
 void plan_free(ed_plan* P) {
-  void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->perm, P->node_ctr, P->node_rng,
+  void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->near_t, P->perm, P->node_ctr, P->node_rng,
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
                   P->leaves, P->pt_node, P->p2p_items, P->p2p_seg, P->p2p_ref, P->p2s_items, P->p2s_seg, P->p2s_sym, P->p2s_tseg, P->p2s_part, P->p2s_leaf, P->ym2m, P->u_ptr, P->u_src, P->parents_list, P->v_ptr, P->v_src, P->v_tab,
                   P->m2l_pair_src, P->m2l_pair_sgn, P->m2l_pair_tgt, P->m2l_tile_tab, P->m2l_tile_beg,
@@ -613,6 +613,7 @@ int plan_upload(ed_plan* P) {
   ED_HIP(hipDeviceGetAttribute(&P->n_cu, hipDeviceAttributeMultiprocessorCount, P->device));
   ED_HIP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
   ED_HIP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+
   ED_TRY(upload(&P->u_ptr, uptr));
   ED_TRY(upload(&P->u_src, usrc));
 
@@ -910,7 +911,9 @@ static int plan_create_impl(const double* centroids, const double* areas, int64_
     if (rc == ED_OK) {
       hipError_t e1 = hipMalloc((void**)&P->sig_t, sizeof(float) * n);
       hipError_t e2 = hipMalloc((void**)&P->selfc_t, sizeof(float) * n);
-      if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(ED_ERR_ALLOC, "ed_plan_create: alloc");
+      hipError_t e3 = hipMalloc((void**)&P->near_t, sizeof(float) * n);
+      if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+        rc = fail(ED_ERR_ALLOC, "ed_plan_create: alloc");
     }
   }
   if (rc != ED_OK) {

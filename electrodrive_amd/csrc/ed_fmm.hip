@@ -35,9 +35,12 @@ constexpr int kLeafBlock = 64;
 // slots of the running multipole maximum (see fold_absmax)
 constexpr int kMaxSlots = 64, kMaxSlotStride = 32;  // 32 x 4 B = 128 B
 
-// The far-field passes raise their waves' issue priority: P2P runs beside
-// them on a side stream, and its VALU-bound waves would otherwise take most
-// issue cycles of every SIMD the two share, stretching the dependent chain.
+// The far-field passes up to the M2L GEMM raise their waves' issue priority:
+// P2P runs beside them on a side stream, and its VALU-bound waves would
+// otherwise take most issue cycles of every SIMD the two share, stretching
+// the dependent chain.  The per-node reduce and the downward pass keep the
+// default priority: by then the chain runs ahead of the P2P, which then sets
+// the matvec (at 82k, 88.6 us with them at the default, 90.3 us raised).
 __device__ __forceinline__ void chain_prio() { __builtin_amdgcn_s_setprio(3); }
 
 // Several right-hand sides in one matvec: blockIdx.y is the RHS of every
@@ -115,6 +118,33 @@ __global__ void k_gather(int n, const int32_t* __restrict__ perm, const TS* __re
   sig_t[i] = s;
   pts[i].w = s * area_t[i];
   selfc_t[i] = selfi ? (float)selfi[o] : 0.0f;
+}
+
+// Near-field correction of each panel for one RHS (tree order):
+// near_t[i] = sum of w_e sig_t[col_e] over row i of the near CSR.  Four lanes
+// a panel: lane r takes entries b + r, b + r + 4, ..., two of them in flight,
+// and the four partial sums are added as (r0 + r1) + (r2 + r3).  It runs
+// beside the far-field chain, so the output kernel only reads one value.
+__global__ __launch_bounds__(256) void k_near(int n, const int32_t* __restrict__ near_ptr,
+                                              const int32_t* __restrict__ near_col,
+                                              const float* __restrict__ near_w,
+                                              const float* __restrict__ sig_t,
+                                              float* __restrict__ near_t) {
+  const int t = blockIdx.x * 256 + (int)threadIdx.x;
+  const int i = min(t >> 2, n - 1), r = t & 3;
+  const int b = near_ptr[i], e = near_ptr[i + 1];
+  float acc = 0.0f;
+  for (int k = b + r; k < e; k += 8) {
+    const int k2 = min(k + 4, e - 1);
+    const int c1 = near_col[k], c2 = near_col[k2];
+    const float w1 = near_w[k], w2 = k + 4 < e ? near_w[k2] : 0.0f;
+    const float s1 = sig_t[c1], s2 = sig_t[c2];
+    acc += w1 * s1;
+    acc += w2 * s2;
+  }
+  acc += __shfl_xor(acc, 1, 64);
+  acc += __shfl_xor(acc, 2, 64);
+  if (r == 0 && (t >> 2) < n) near_t[i] = acc;
 }
 
 // Largest |M_k * ascale_k| of the multipoles a wave wrote, folded into one
@@ -500,6 +530,47 @@ struct M2LShapeH {
 // below the first right-hand side) and the scale is undone exactly on Y.
 constexpr int kM2LScaleExp = 14;
 
+// Reflection-folded coefficients of a 16-bit M2L tile, in LDS: row s (one per
+// axis reflection s = 0..7) of sa holds refl(ascale[k] * sc, s, flip[k]) (0
+// past NC), so one multiply scales and reflects a gathered multipole
+// coefficient; row s of so holds refl(1, s, flip[i]), the sign of result row
+// i.  Both products are exact sign flips of the unreflected ones.  Rows are
+// padded by 4 floats so that lanes of different reflections start on
+// different LDS banks.  load() fetches each thread's share before the scale
+// is known; store() writes the rows once it is.
+template <int P>
+struct M2LRefl {
+  using S = M2LShapeH<P>;
+  static constexpr int SA = S::KP + 4, SO = S::NCP + 4;
+  static constexpr int NA = (8 * S::KP + 255) / 256, NO = (8 * S::NCP + 255) / 256;
+  float av[NA];
+  int fa[NA], fo[NO];
+  __device__ __forceinline__ void load(const float* __restrict__ ascale,
+                                       const uint8_t* __restrict__ flip) {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int idx = (int)threadIdx.x + 256 * u, k = idx % S::KP;
+      av[u] = (idx < 8 * S::KP && k < S::NC) ? ascale[k] : 0.0f;
+      fa[u] = flip[min(k, S::NC - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < NO; ++u) fo[u] = flip[((int)threadIdx.x + 256 * u) % S::NCP];
+  }
+  __device__ __forceinline__ void store(float sc, float* __restrict__ sa,
+                                        float* __restrict__ so) const {
+#pragma unroll
+    for (int u = 0; u < NA; ++u) {
+      const int idx = (int)threadIdx.x + 256 * u, s = idx / S::KP;
+      if (idx < 8 * S::KP) sa[s * SA + idx - s * S::KP] = refl(av[u] * sc, s, fa[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < NO; ++u) {
+      const int idx = (int)threadIdx.x + 256 * u, s = idx / S::NCP;
+      if (idx < 8 * S::NCP) so[s * SO + idx - s * S::NCP] = refl(1.0f, s, fo[u]);
+    }
+  }
+};
+
 // M2L tables exist for offsets with non-negative components only: a pair
 // whose offset is a reflection of a table's offset applies D T(o) D (refl).
 template <int P, bool MULTI>
@@ -640,8 +711,9 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   }
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) T16 ta[NCP * KS];
-  __shared__ float as_l[KP];  // ascale x the power-of-two input scale, 0 past NC
-  __shared__ uint8_t fl[NCP];
+  using R = M2LRefl<P>;
+  __shared__ __attribute__((aligned(16))) float sa[8 * R::SA];  // reflection-folded ascale x scale
+  __shared__ __attribute__((aligned(16))) float so[8 * R::SO];  // result-row signs
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
@@ -657,9 +729,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   const int sg = pair_sgn[beg + min(col, cnt - 1)];
   const int src = pair_src[beg + min(col, cnt - 1)];
   const unsigned slot_bits = mmax_bits[(threadIdx.x & (kMaxSlots - 1)) * kMaxSlotStride];
-  static_assert(KP <= 256 && NCP <= 256, "one ascale / flip entry per thread");
-  const float as_v = (int)threadIdx.x < NC ? ascale[threadIdx.x] : 0.0f;
-  const uint8_t fl_v = (int)threadIdx.x < NCP ? flip[threadIdx.x] : (uint8_t)0;
+  R rf;
+  rf.load(ascale, flip);
   const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * NCP * KP);
   constexpr int N8 = NCP * KP / 8, TU = 16;
   constexpr bool kWhole = N8 % (TU * 256) == 0;
@@ -690,8 +761,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
   for (int o = 32; o > 0; o >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, o, 64));
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
   const float inv_sc = 1.0f / sc;
-  if ((int)threadIdx.x < KP) as_l[threadIdx.x] = as_v * sc;
-  if ((int)threadIdx.x < NCP) fl[threadIdx.x] = fl_v;
+  rf.store(sc, sa, so);
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
   constexpr int NRB = NCP / 32;
@@ -702,12 +772,14 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
     for (int r = 0; r < 16; ++r) acc[rb][r] = 0.0f;
 #pragma unroll
   for (int sv = 0; sv < NSV; ++sv) {
+    // a column past cnt computes from the clamped pair's multipoles and is
+    // not stored (the columns of the product are independent)
+    const float4 s0 = *reinterpret_cast<const float4*>(sa + sg * R::SA + 16 * sv + 8 * lk);
+    const float4 s1 = *reinterpret_cast<const float4*>(sa + sg * R::SA + 16 * sv + 8 * lk + 4);
+    const float sk[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
     V bv;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      const int k = 16 * sv + 8 * lk + j;
-      bv[j] = (T16)(cv ? refl(mv[sv][j] * as_l[k], sg, fl[k]) : 0.0f);
-    }
+    for (int j = 0; j < 8; ++j) bv[j] = (T16)(mv[sv][j] * sk[j]);
 #pragma unroll
     for (int rb = 0; rb < NRB; ++rb) {
       const V a = *reinterpret_cast<const V*>(ta + (rb * 32 + li) * KS + 16 * sv + 8 * lk);
@@ -721,10 +793,9 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16(const int32_t* __restrict__ 
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int row = rb * 32 + q * 8 + lk * 4;
-        float4 v = make_float4(refl(acc[rb][4 * q + 0] * inv_sc, sg, fl[row]),
-                               refl(acc[rb][4 * q + 1] * inv_sc, sg, fl[row + 1]),
-                               refl(acc[rb][4 * q + 2] * inv_sc, sg, fl[row + 2]),
-                               refl(acc[rb][4 * q + 3] * inv_sc, sg, fl[row + 3]));
+        const float4 sr = *reinterpret_cast<const float4*>(so + sg * R::SO + row);
+        float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc * sr.x, acc[rb][4 * q + 1] * inv_sc * sr.y,
+                               acc[rb][4 * q + 2] * inv_sc * sr.z, acc[rb][4 * q + 3] * inv_sc * sr.w);
         *reinterpret_cast<float4*>(yrow + row) = v;
       }
   }
@@ -761,13 +832,15 @@ __device__ __forceinline__ float m2l_scale(const unsigned* __restrict__ slots) {
 }
 
 // One wave's 32 columns x all rows of the split-FP16 product for one RHS,
-// stored to its Y rows.  as_l holds ascale x sc0; for another RHS with scale
-// sc the inputs take the power of two rel = sc / sc0 on top (RESCALE), which
-// rounds exactly like ascale x sc.
+// stored to its Y rows (lanes with cv false compute from the clamped pair's
+// multipoles and store nothing: the columns are independent).  sa_l / so_l
+// are the lane's reflection rows of M2LRefl; sa_l holds ascale x sc0, and for
+// another RHS with scale sc the inputs take the power of two rel = sc / sc0
+// on top (RESCALE), which rounds exactly like ascale x sc.
 template <int P, bool RESCALE>
 __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
-                                               const float* __restrict__ as_l,
-                                               const uint8_t* __restrict__ fl, int sg,
+                                               const float* __restrict__ sa_l,
+                                               const float* __restrict__ so_l,
                                                const float (&mv)[M2LShapeH<P>::KP / 16][8],
                                                float rel, float inv_sc, bool cv, int li, int lk,
                                                float* __restrict__ yrow) {
@@ -780,11 +853,13 @@ __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
     for (int q = 0; q < 16; ++q) acc[rb][q] = 0.0f;
 #pragma unroll
   for (int sv = 0; sv < NSV; ++sv) {
+    const float4 s0 = *reinterpret_cast<const float4*>(sa_l + 16 * sv + 8 * lk);
+    const float4 s1 = *reinterpret_cast<const float4*>(sa_l + 16 * sv + 8 * lk + 4);
+    const float sk[8] = {s0.x, s0.y, s0.z, s0.w, s1.x, s1.y, s1.z, s1.w};
     f16x8 bh, bl;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-      const int k = 16 * sv + 8 * lk + j;
-      float v = cv ? refl(mv[sv][j] * as_l[k], sg, fl[k]) : 0.0f;
+      float v = mv[sv][j] * sk[j];
       if (RESCALE) v *= rel;
       const _Float16 h = (_Float16)v;
       bh[j] = h;
@@ -806,10 +881,9 @@ __device__ __forceinline__ void m2l16x3_column(const _Float16* __restrict__ ta,
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int row = rb * 32 + q * 8 + lk * 4;
-      float4 v = make_float4(refl(acc[rb][4 * q + 0] * inv_sc, sg, fl[row]),
-                             refl(acc[rb][4 * q + 1] * inv_sc, sg, fl[row + 1]),
-                             refl(acc[rb][4 * q + 2] * inv_sc, sg, fl[row + 2]),
-                             refl(acc[rb][4 * q + 3] * inv_sc, sg, fl[row + 3]));
+      const float4 sr = *reinterpret_cast<const float4*>(so_l + row);
+      float4 v = make_float4(acc[rb][4 * q + 0] * inv_sc * sr.x, acc[rb][4 * q + 1] * inv_sc * sr.y,
+                             acc[rb][4 * q + 2] * inv_sc * sr.z, acc[rb][4 * q + 3] * inv_sc * sr.w);
       *reinterpret_cast<float4*>(yrow + row) = v;
     }
 }
@@ -830,8 +904,9 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
   using S = M2LShapeH<P>;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
-  __shared__ float as_l[KP];  // ascale x the power-of-two input scale, 0 past NC
-  __shared__ uint8_t fl[NCP];
+  using R = M2LRefl<P>;
+  __shared__ __attribute__((aligned(16))) float sa[8 * R::SA];  // reflection-folded ascale x scale
+  __shared__ __attribute__((aligned(16))) float so[8 * R::SO];  // result-row signs
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
@@ -849,9 +924,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
   const int sg = pair_sgn[beg + min(col, cnt - 1)];
   const int src = pair_src[beg + min(col, cnt - 1)];
   const unsigned slot_bits = mmax_bits[(threadIdx.x & (kMaxSlots - 1)) * kMaxSlotStride];
-  static_assert(KP <= 256 && NCP <= 256, "one ascale / flip entry per thread");
-  const float as_v = (int)threadIdx.x < NC ? ascale[threadIdx.x] : 0.0f;
-  const uint8_t fl_v = (int)threadIdx.x < NCP ? flip[threadIdx.x] : (uint8_t)0;
+  R rf;
+  rf.load(ascale, flip);
   const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
   constexpr int N8 = NCP * KP / 8, TU = 16;
   constexpr bool kWhole = (2 * N8) % (TU * 256) == 0;  // no partial batch (p = 10)
@@ -884,11 +958,10 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3(const int32_t* __restrict_
   for (int o = 32; o > 0; o >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, o, 64));
   const float sc = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
   const float inv_sc = 1.0f / sc;
-  if ((int)threadIdx.x < KP) as_l[threadIdx.x] = as_v * sc;
-  if ((int)threadIdx.x < NCP) fl[threadIdx.x] = fl_v;
+  rf.store(sc, sa, so);
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
-  m2l16x3_column<P, false>(ta, as_l, fl, sg, mv, 1.0f, inv_sc, cv, li, lk,
+  m2l16x3_column<P, false>(ta, sa + sg * R::SA, so + sg * R::SO, mv, 1.0f, inv_sc, cv, li, lk,
                            Y + (size_t)(beg + col) * NCP);
 }
 
@@ -913,8 +986,9 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
   using S = M2LShapeH<P>;
   constexpr int NC = S::NC, NCP = S::NCP, KP = S::KP, KS = S::KS, NSV = KP / 16;
   __shared__ __attribute__((aligned(16))) _Float16 ta[2 * NCP * KS];  // hi, then lo
-  __shared__ float as_l[KP];  // ascale x the first RHS's scale, 0 past NC
-  __shared__ uint8_t fl[NCP];
+  using R = M2LRefl<P>;
+  __shared__ __attribute__((aligned(16))) float sa[8 * R::SA];  // reflection-folded ascale x sc0
+  __shared__ __attribute__((aligned(16))) float so[8 * R::SO];  // result-row signs
   if (tile_cnt[blockIdx.x] == 0) return;  // padding tile of the XCD-aware order
   const int tile = blockIdx.x;
   const int tab = tile_tab[tile], beg = tile_beg[tile], cnt = tile_cnt[tile];
@@ -927,9 +1001,8 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
   const int sg = pair_sgn[beg + min(col, cnt - 1)];
   const int src = pair_src[beg + min(col, cnt - 1)];
   const unsigned slot_bits = mmax_bits[(threadIdx.x & (kMaxSlots - 1)) * kMaxSlotStride];
-  static_assert(KP <= 256 && NCP <= 256, "one ascale / flip entry per thread");
-  const float as_v = (int)threadIdx.x < NC ? ascale[threadIdx.x] : 0.0f;
-  const uint8_t fl_v = (int)threadIdx.x < NCP ? flip[threadIdx.x] : (uint8_t)0;
+  R rf;
+  rf.load(ascale, flip);
   // both table halves -> LDS, 8 elements (16 bytes) per load, 16 in flight
   // per thread; rows are KP (a multiple of 16) elements long
   const uint4* T8 = reinterpret_cast<const uint4*>(Tm + (size_t)tab * 2 * NCP * KP);
@@ -955,11 +1028,10 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) gmax = fmaxf(gmax, __shfl_xor(gmax, o, 64));
   const float sc0 = gmax > 0.0f && isfinite(gmax) ? exp2f((float)kM2LScaleExp - ceilf(log2f(gmax))) : 1.0f;
-  if ((int)threadIdx.x < KP) as_l[threadIdx.x] = as_v * sc0;
-  if ((int)threadIdx.x < NCP) fl[threadIdx.x] = fl_v;
+  rf.store(sc0, sa, so);
   __syncthreads();
   if (col0 >= cnt) return;  // no barrier below
-  m2l16x3_column<P, false>(ta, as_l, fl, sg, mv, 1.0f, 1.0f / sc0, cv, li, lk,
+  m2l16x3_column<P, false>(ta, sa + sg * R::SA, so + sg * R::SO, mv, 1.0f, 1.0f / sc0, cv, li, lk,
                            Y + (size_t)(beg + col) * NCP);
 #pragma unroll 1
   for (int r = 1; r < rs.R; ++r) {
@@ -968,7 +1040,7 @@ __global__ __launch_bounds__(256) void k_m2l_mfma16x3_multi(const int32_t* __res
     asm volatile("" ::: "memory");
     m2l16x3_load<P>(M + r * rs.node + (size_t)src * NC, lk, mv);
     const float sc = m2l_scale(mmax_bits + r * rs.slot);
-    m2l16x3_column<P, true>(ta, as_l, fl, sg, mv, sc / sc0, 1.0f / sc, cv, li, lk,
+    m2l16x3_column<P, true>(ta, sa + sg * R::SA, so + sg * R::SO, mv, sc / sc0, 1.0f / sc, cv, li, lk,
                             Y + r * rs.pair + (size_t)(beg + col) * NCP);
   }
 }
@@ -985,7 +1057,7 @@ __global__ __launch_bounds__(128) void k_m2l_reduce(int node0, const int32_t* __
                                                     float* __restrict__ Lown,
                                                     unsigned* __restrict__ mmax_bits,
                                                     RhsStride rs) {
-  chain_prio();
+  if constexpr (MULTI) chain_prio();  // one RHS: default priority (see chain_prio)
   using S = M2LShape<P>;
   constexpr int NC = S::NC, NCP = S::NCP, W = 128, KO = (NC + W - 1) / W, RB = 16;
   if constexpr (MULTI) {
@@ -1037,7 +1109,7 @@ __global__ __launch_bounds__(128) void k_m2l_down(int node0, const int32_t* __re
                                                   const float* __restrict__ Lown,
                                                   const float* __restrict__ T,
                                                   float* __restrict__ L, RhsStride rs) {
-  chain_prio();
+  if constexpr (MULTI) chain_prio();  // one RHS: default priority (see chain_prio)
   constexpr int NC = (P + 1) * (P + 1), W = 128, KO = (NC + W - 1) / W;
   __shared__ float xs[2][NC];
   __shared__ int chain[kMaxChain], octs[kMaxChain];
@@ -1105,7 +1177,7 @@ __global__ __launch_bounds__(128) void k_m2l_down(int node0, const int32_t* __re
 // ---------------------------------------------------------------------------
 constexpr int kP2PUnroll = 8;
 constexpr int kP2PWaves = 4;  // independent items per block
-constexpr int kP2PBlocksPerCU = 4;  // grid cap of the one-RHS P2P (see k_p2p_sym)
+constexpr int kP2PBlocksPerCU = 5;  // grid cap of the one-RHS P2P (see k_p2p_sym)
 
 // r^2 + 1e-24 in place of max(r^2, 1e-24): equal for every r^2 above 3e-17
 // (the addend is below half an ulp there), and both give 1e12 for r = 0.
@@ -1303,8 +1375,9 @@ __device__ __forceinline__ void p2p_sym_item(const ed::P2PSymItem& it,
 // blocks with a grid stride: P2P runs beside the far-field chain, and with
 // every SIMD's register file full of P2P waves the chain's kernels waited
 // for P2P waves to retire (P2M 8.3 us alone, 17 us beside the uncapped
-// P2P); at four blocks per CU the 82k matvec went 99.3 -> 91.0 us (three:
-// 95.6, five: 91.9, six: 99.6).
+// P2P).  82k matvec: 99.3 us uncapped; with the reduce and downward pass at
+// the default priority and the near-field correction after the chain,
+// 87.5 us at five blocks per CU (three: 92.6, four: 88.6, six: 97.5).
 // (No waves-per-EU cap: at 8 waves the compiler holds the SGPR budget so
 // low that the 64 source SGPRs of a group spill to VGPR lanes; uncapped it
 // needs 52 VGPRs, which still gives 8 waves per SIMD.)
@@ -1511,8 +1584,9 @@ __global__ __launch_bounds__(256) void k_l2p_out(
     const int2* __restrict__ node_rng, const int2* __restrict__ p2s_leaf,
     const int32_t* __restrict__ near_ptr, const int32_t* __restrict__ near_col,
     const float* __restrict__ near_w, const float* __restrict__ sig_t,
-    const float* __restrict__ selfc_t, const int32_t* __restrict__ perm, int out_tree, float ke,
-    TO* __restrict__ out, RhsStride rs) {
+    const float* __restrict__ near_t, const float* __restrict__ selfc_t,
+    const int32_t* __restrict__ perm, int out_tree, float ke, TO* __restrict__ out,
+    RhsStride rs) {
   constexpr int NC = (P + 1) * (P + 1);
   __shared__ float part4[4][64];
   const int ry = MULTI ? blockIdx.y : 0;
@@ -1528,11 +1602,12 @@ __global__ __launch_bounds__(256) void k_l2p_out(
   const int i = min(i0, n - 1);  // lanes past n compute a copy and write nothing
   const int node = pt_node[i];
   const float4 xi = pts[i];
-  const int nb = near_ptr[i], ne = near_ptr[i + 1];
   float corr = 0.0f;
-  // this wave's entries nb + q, nb + q + 4, ... in groups of four with
-  // clamped indices and zero weights past the row end: every load of a group
-  // is issued before the first wait
+  // several RHS: this wave's near-field entries nb + q, nb + q + 4, ... in
+  // groups of four with clamped indices and zero weights past the row end
+  // (every load of a group issued before the first wait); one RHS: k_near's
+  // sum, added once below
+  const int nb = MULTI ? near_ptr[i] : 0, ne = MULTI ? near_ptr[i + 1] : 0;
   for (int e0 = nb + q; e0 < ne; e0 += 16) {
     int c[4];
     float wq[4], sv[4];
@@ -1586,6 +1661,7 @@ __global__ __launch_bounds__(256) void k_l2p_out(
   const float qi = MULTI ? rs.qT[(size_t)i * rs.R + ry] : xi.w;
   float v = ((part4[0][lane] + part4[1][lane]) + (part4[2][lane] + part4[3][lane])) +
             selfc_t[i] * qi;
+  if constexpr (!MULTI) v += near_t[i];
   if (!isfinite(v)) v = 0.0f;
   out[out_tree ? i : perm[i]] = (TO)v;
 }
@@ -1706,6 +1782,14 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     ED_LAUNCH_CHECK("gather");
     return ED_OK;
   };
+  auto near = [&](hipStream_t ns) -> int {
+    if (!multi) {
+      hipLaunchKernelGGL(k_near, dim3((unsigned)((4 * (int64_t)n + 255) / 256)), dim3(256), 0, ns,
+                         n, Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->near_t);
+      ED_LAUNCH_CHECK("near");
+    }
+    return ED_OK;
+  };
   if (fork) {
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_fork, st), "p2p fork");
     ED_HIP_LAUNCH(hipStreamWaitEvent(Pn->side, Pn->ev_fork, 0), "p2p fork");
@@ -1713,8 +1797,9 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     p2p_launch(Pn, R, rs, Pn->side);
     ED_LAUNCH_CHECK("p2p");
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_join, Pn->side), "p2p join");
-  } else if (int rc = gather(st); rc != ED_OK) {
-    return rc;
+  } else {
+    if (int rc = gather(st); rc != ED_OK) return rc;
+    if (int rc = near(st); rc != ED_OK) return rc;
   }
   ED_MARK(1);
   const int nleaf = (int)Pn->n_leaves;
@@ -1813,6 +1898,9 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     ED_MARK(5);
   }
   if (fork) {
+    // the near-field correction after the far-field chain, while the P2P
+    // (the longer branch) finishes on the side stream
+    if (int rc = near(st); rc != ED_OK) return rc;
     ED_HIP_LAUNCH(hipStreamWaitEvent(st, Pn->ev_join, 0), "p2p join");
   } else if (Pn->n_p2p_items > 0) {
     p2p_launch(Pn, R, rs, st);
@@ -1825,12 +1913,14 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     hipLaunchKernelGGL((k_l2p_out<P, double, MULTI>), dim3(gb64, RY), dim3(256), 0, st, n, Pn->pt_node,
                        Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->node_rng,
                        Pn->p2s_leaf, Pn->near_ptr, Pn->near_col,
-                       Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (double*)out, rs);
+                       Pn->near_w, sigb, Pn->near_t, Pn->selfc_t, Pn->perm, out_tree, ke,
+                       (double*)out, rs);
   else
     hipLaunchKernelGGL((k_l2p_out<P, float, MULTI>), dim3(gb64, RY), dim3(256), 0, st, n, Pn->pt_node,
                        Pn->node_ctr, Pn->pts, Lb, Pn->p2p_ref, partb, Pn->node_rng,
                        Pn->p2s_leaf, Pn->near_ptr, Pn->near_col,
-                       Pn->near_w, sigb, Pn->selfc_t, Pn->perm, out_tree, ke, (float*)out, rs);
+                       Pn->near_w, sigb, Pn->near_t, Pn->selfc_t, Pn->perm, out_tree, ke,
+                       (float*)out, rs);
   ED_LAUNCH_CHECK("l2p_out");
   ED_MARK(8);
   (void)NC;

@@ -24,6 +24,7 @@ struct ed_plan {
   float* area_t = nullptr;   // areas, tree order
   float* sig_t = nullptr;    // sigma, tree order (written per matvec)
   float* selfc_t = nullptr;  // per-area self term, tree order (per matvec)
+  float* near_t = nullptr;   // near-field correction, tree order (per one-RHS matvec)
   int32_t* perm = nullptr;   // tree position -> original index
 
   // nodes
