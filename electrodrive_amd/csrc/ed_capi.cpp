@@ -653,6 +653,18 @@ int plan_upload(ed_plan* P) {
   }
   P->n_off = (int)std::count(used.begin(), used.end(), (char)1);  // distinct offsets in use
   P->n_v = (int64_t)vsrc.size();
+  {
+    // The P2P grid cap leaves the far-field chain's kernels room beside the
+    // P2P; it pays only while the chain is the longer branch.  Branch lengths
+    // from rates measured on MI355X at p = 10: the chain about 1.27 ns per
+    // M2L pair (82k sphere: 66 us for 52,160 pairs; 300k torus: 173 us for
+    // 136,112), taken to scale with the coefficient count, and the symmetric
+    // P2P about 3.8e6 evaluations per us (46 us for 157 M, 199 us for 843 M).
+    const double nc = (double)((P->p + 1) * (P->p + 1));
+    const double chain_us = 1.27e-3 * (nc / 121.0) * (double)P->n_v;
+    const double p2p_us = (double)P->n_p2s_sym_interactions / 3.8e6;
+    P->p2p_blocks_per_cu = p2p_us < chain_us ? ed::kP2PBlocksPerCU : 0;
+  }
   ED_TRY(upload(&P->v_ptr, vptr));
   ED_TRY(upload(&P->v_src, vsrc));
   ED_TRY(upload(&P->v_tab, vtab));

@@ -1177,7 +1177,6 @@ __global__ __launch_bounds__(128) void k_m2l_down(int node0, const int32_t* __re
 // ---------------------------------------------------------------------------
 constexpr int kP2PUnroll = 8;
 constexpr int kP2PWaves = 4;  // independent items per block
-constexpr int kP2PBlocksPerCU = 5;  // grid cap of the one-RHS P2P (see k_p2p_sym)
 
 // r^2 + 1e-24 in place of max(r^2, 1e-24): equal for every r^2 above 3e-17
 // (the addend is below half an ulp there), and both give 1e12 for r = 0.
@@ -1370,14 +1369,15 @@ __device__ __forceinline__ void p2p_sym_item(const ed::P2PSymItem& it,
 }
 
 // One-right-hand-side P2P: item blocks [0, nb_sym) run the symmetric items
-// (the heavier ones first), the rest the one-directional items.  The grid is
-// capped at kP2PBlocksPerCU blocks per compute unit, each walking the item
-// blocks with a grid stride: P2P runs beside the far-field chain, and with
-// every SIMD's register file full of P2P waves the chain's kernels waited
-// for P2P waves to retire (P2M 8.3 us alone, 17 us beside the uncapped
-// P2P).  82k matvec: 99.3 us uncapped; with the reduce and downward pass at
-// the default priority and the near-field correction after the chain,
-// 87.5 us at five blocks per CU (three: 92.6, four: 88.6, six: 97.5).
+// (the heavier ones first), the rest the one-directional items, each block
+// walking them with a grid stride.  When the far-field chain is the longer
+// branch the grid is capped at kP2PBlocksPerCU blocks per compute unit (the
+// plan decides, see ed_plan.h): with every SIMD's register file full of P2P
+// waves the chain's kernels waited for P2P waves to retire (P2M 8.3 us
+// alone, 17 us beside the uncapped P2P).  82k sphere matvec: 94.6 us
+// uncapped, 87.0 us at five blocks per CU (three: 92.6, four: 88.6, six:
+// 97.5, eight: 98.7); the 300k torus, where the P2P is the longer branch,
+// takes 244 us uncapped and 292 us capped at five.
 // (No waves-per-EU cap: at 8 waves the compiler holds the SGPR budget so
 // low that the 64 source SGPRs of a group spill to VGPR lanes; uncapped it
 // needs 52 VGPRs, which still gives 8 waves per SIMD.)
@@ -1694,7 +1694,8 @@ static void p2p_launch(ed_plan* Pn, int R, const RhsStride& rs, hipStream_t s) {
   if (R == 1) {  // the symmetric schedule
     const int nb_sym = (int)((Pn->n_p2s_sym + kP2PWaves - 1) / kP2PWaves);
     const int nb_one = (int)((Pn->n_p2s_items + kP2PWaves - 1) / kP2PWaves);
-    const int grid = std::min(nb_sym + nb_one, kP2PBlocksPerCU * std::max(Pn->n_cu, 1));
+    const int cap = Pn->p2p_blocks_per_cu * std::max(Pn->n_cu, 1);
+    const int grid = cap > 0 ? std::min(nb_sym + nb_one, cap) : nb_sym + nb_one;
     if (grid > 0)
       hipLaunchKernelGGL(k_p2p_sym, dim3((unsigned)grid), dim3(64 * kP2PWaves), 0, s,
                          (const ed::P2PSymItem*)Pn->p2s_sym, (const ed::P2PSymSeg*)Pn->p2s_tseg,

@@ -155,6 +155,9 @@ struct ed_plan {
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int n_cu = 256;  // compute units of the plan's device (caps the one-RHS P2P grid)
+  // one-RHS P2P grid cap in blocks per CU (0: one block per item block); set
+  // at plan creation from the estimated lengths of the two matvec branches
+  int p2p_blocks_per_cu = 0;
 
   // GMRES workspace (allocated lazily by the solver, owned by the plan)
   void* gmres_ws = nullptr;
@@ -196,7 +199,10 @@ struct P2PSymItem {
 struct P2PSymSeg {
   int32_t lane0, panel0, slot0, stride;
 };
-constexpr int kP2PSymSources = 256;  // sources per symmetric item
+constexpr int kP2PSymSources = 256;
+// Grid cap of the one-RHS P2P in blocks per compute unit, used when the
+// far-field chain is the longer of the two branches of a matvec.
+constexpr int kP2PBlocksPerCU = 5;  // sources per symmetric item
 // Work item of the split dense operator: up to 512 target rows and one
 // contiguous source range of their geometry.
 struct DenseItem {
