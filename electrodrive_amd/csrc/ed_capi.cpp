@@ -53,20 +53,113 @@ static double f16_to_f64(uint16_t u) {
   return (double)(float)h;
 }
 
-template <typename T>
-static int upload(T** dst, const std::vector<T>& src) {
-  *dst = nullptr;
-  const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
-  ED_HIP(hipMalloc((void**)dst, bytes));
-  if (!src.empty()) ED_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
-  return ED_OK;
-}
-
 #define ED_TRY(x)              \
   do {                         \
     int rc_ = (x);             \
     if (rc_ != ED_OK) return rc_; \
   } while (0)
+
+// Plan creation sends its host arrays through pinned staging memory with
+// asynchronous copies on a non-blocking upload stream and synchronises once
+// at the end (stage_finish), instead of one synchronous pageable copy per
+// array: the host goes on building the schedules while the copies run.  The
+// pinned chunks are kept for the process (one set, guarded by a mutex held
+// from a plan's first staged copy to its synchronisation).
+struct PinnedStage {
+  std::mutex mu;
+  std::vector<std::pair<char*, size_t>> chunks;
+  std::map<int, hipStream_t> streams;
+};
+static PinnedStage& pinned_stage() {
+  static PinnedStage* S = new PinnedStage();  // never destroyed: outlives every plan
+  return *S;
+}
+struct UploadStage {
+  std::unique_lock<std::mutex> lock;
+  hipStream_t st = nullptr;
+  size_t chunk = 0, off = 0;
+  bool used = false;
+};
+static thread_local UploadStage* t_stage = nullptr;
+
+// takes the staging memory and the device's upload stream on first use
+static int stage_begin(UploadStage& U) {
+  PinnedStage& S = pinned_stage();
+  if (!U.used) {
+    U.lock = std::unique_lock<std::mutex>(S.mu);
+    int dev = 0;
+    ED_HIP(hipGetDevice(&dev));
+    auto it = S.streams.find(dev);
+    if (it == S.streams.end()) {
+      hipStream_t st = nullptr;
+      ED_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+      it = S.streams.emplace(dev, st).first;
+    }
+    U.st = it->second;
+    U.chunk = 0;
+    U.off = 0;
+    U.used = true;
+  }
+  return ED_OK;
+}
+
+static int h2d(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return ED_OK;
+  if (!t_stage) {
+    ED_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return ED_OK;
+  }
+  UploadStage& U = *t_stage;
+  PinnedStage& S = pinned_stage();
+  ED_TRY(stage_begin(U));
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  while (U.chunk < S.chunks.size() && U.off + need > S.chunks[U.chunk].second) {
+    ++U.chunk;
+    U.off = 0;
+  }
+  if (U.chunk == S.chunks.size()) {
+    const size_t sz = std::max<size_t>(need, (size_t)16 << 20);
+    char* b = nullptr;
+    ED_HIP(hipHostMalloc((void**)&b, sz, hipHostMallocDefault));
+    S.chunks.emplace_back(b, sz);
+  }
+  char* stage = S.chunks[U.chunk].first + U.off;
+  std::memcpy(stage, src, bytes);
+  U.off += need;
+  ED_HIP(hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, U.st));
+  return ED_OK;
+}
+
+// zeros to device memory (on the upload stream while staging)
+static int d_zero(void* dst, size_t bytes) {
+  if (t_stage) {
+    ED_TRY(stage_begin(*t_stage));
+    ED_HIP(hipMemsetAsync(dst, 0, bytes, t_stage->st));
+    return ED_OK;
+  }
+  ED_HIP(hipMemset(dst, 0, bytes));
+  return ED_OK;
+}
+
+static int stage_finish(UploadStage& U) {
+  int rc = ED_OK;
+  if (U.used) {
+    const hipError_t e = hipStreamSynchronize(U.st);
+    if (e != hipSuccess) rc = fail(ED_ERR_DEVICE, std::string("plan upload: ") + hipGetErrorString(e));
+    U.used = false;
+    U.lock.unlock();
+  }
+  return rc;
+}
+
+template <typename T>
+static int upload(T** dst, const std::vector<T>& src) {
+  *dst = nullptr;
+  const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+  ED_HIP(hipMalloc((void**)dst, bytes));
+  return h2d(*dst, src.data(), src.size() * sizeof(T));
+}
+
 
 void plan_free(ed_plan* P) {
   void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->near_t, P->perm, P->node_ctr, P->node_rng,
@@ -192,6 +285,12 @@ static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, int>, DeviceTables> cache;
   std::lock_guard<std::mutex> lock(mu);
+  // shared tables outlive the plan being created: copied at once, not staged
+  struct NoStage {
+    UploadStage* saved = t_stage;
+    NoStage() { t_stage = nullptr; }
+    ~NoStage() { t_stage = saved; }
+  } no_stage;
   const auto key = std::make_tuple(device, p, kind, ws);
   auto it = cache.find(key);
   if (it != cache.end()) {
@@ -346,9 +445,7 @@ static int p2p_schedule(ed_plan* P, const std::vector<int32_t>& leaves) {
   ED_TRY(upload(&P->p2p_seg, seg));
   ED_TRY(upload(&P->p2p_ref, ref));
   ED_HIP(hipMalloc(&P->p2p_items, sizeof(ed::P2PItem) * std::max<size_t>(items.size(), 1)));
-  if (!items.empty())
-    ED_HIP(hipMemcpy(P->p2p_items, items.data(), sizeof(ed::P2PItem) * items.size(),
-                     hipMemcpyHostToDevice));
+  ED_TRY(h2d(P->p2p_items, items.data(), sizeof(ed::P2PItem) * items.size()));
   // (partial rows of this schedule live in the multi-RHS buffers)
   return ED_OK;
 }
@@ -531,17 +628,11 @@ static int p2p_schedule_sym(ed_plan* P, const std::vector<int32_t>& leaves) {
   ED_TRY(upload(&P->p2s_seg, seg));
   ED_TRY(upload(&P->p2s_leaf, leafrow));
   ED_HIP(hipMalloc(&P->p2s_tseg, sizeof(ed::P2PSymSeg) * std::max<size_t>(tseg.size(), 1)));
-  if (!tseg.empty())
-    ED_HIP(hipMemcpy(P->p2s_tseg, tseg.data(), sizeof(ed::P2PSymSeg) * tseg.size(),
-                     hipMemcpyHostToDevice));
+  ED_TRY(h2d(P->p2s_tseg, tseg.data(), sizeof(ed::P2PSymSeg) * tseg.size()));
   ED_HIP(hipMalloc(&P->p2s_items, sizeof(ed::P2PItem) * std::max<size_t>(items.size(), 1)));
-  if (!items.empty())
-    ED_HIP(hipMemcpy(P->p2s_items, items.data(), sizeof(ed::P2PItem) * items.size(),
-                     hipMemcpyHostToDevice));
+  ED_TRY(h2d(P->p2s_items, items.data(), sizeof(ed::P2PItem) * items.size()));
   ED_HIP(hipMalloc(&P->p2s_sym, sizeof(ed::P2PSymItem) * std::max<size_t>(sym.size(), 1)));
-  if (!sym.empty())
-    ED_HIP(hipMemcpy(P->p2s_sym, sym.data(), sizeof(ed::P2PSymItem) * sym.size(),
-                     hipMemcpyHostToDevice));
+  ED_TRY(h2d(P->p2s_sym, sym.data(), sizeof(ed::P2PSymItem) * sym.size()));
   ED_HIP(hipMalloc((void**)&P->p2s_part, sizeof(float) * (size_t)std::max<int64_t>(base[NL], 1)));
   return ED_OK;
 }
@@ -579,8 +670,8 @@ int plan_upload(ed_plan* P) {
   ED_HIP(hipMalloc((void**)&P->M, sizeof(float) * (size_t)M * nc));
   ED_HIP(hipMalloc((void**)&P->L, sizeof(float) * (size_t)M * nc));
   ED_HIP(hipMalloc((void**)&P->ym2m, sizeof(float) * (size_t)M * nc));
-  ED_HIP(hipMemset(P->M, 0, sizeof(float) * (size_t)M * nc));
-  ED_HIP(hipMemset(P->L, 0, sizeof(float) * (size_t)M * nc));
+  ED_TRY(d_zero(P->M, sizeof(float) * (size_t)M * nc));
+  ED_TRY(d_zero(P->L, sizeof(float) * (size_t)M * nc));
 
   // leaves and U lists
   std::vector<int32_t> leaves, uptr{0}, usrc;
@@ -890,6 +981,8 @@ static int plan_create_impl(const double* centroids, const double* areas, int64_
                            o.leaf_size, o.max_level, o.ws, parts[(size_t)g]);
     if (rc == ED_OK) rc = merge_host_trees(parts, P->host);
   }
+  UploadStage stage;
+  t_stage = &stage;
   if (rc == ED_OK) rc = plan_upload(P);
   if (rc == ED_OK) {
     const HostTree& H = P->host;
@@ -927,6 +1020,11 @@ static int plan_create_impl(const double* centroids, const double* areas, int64_
       if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
         rc = fail(ED_ERR_ALLOC, "ed_plan_create: alloc");
     }
+  }
+  t_stage = nullptr;
+  {
+    const int rc2 = stage_finish(stage);  // always: releases the staging memory
+    if (rc == ED_OK) rc = rc2;
   }
   if (rc != ED_OK) {
     plan_free(P);

@@ -83,11 +83,20 @@ void morton_keys_host(const double* pts, int64_t n, int level,
   }
   const double eps = 2.220446049250313e-16;
   for (int a = 0; a < 3; ++a) ext[a] = std::max(maxs[a] - mins[a], eps);
-  for (int64_t i = 0; i < n; ++i) {
-    int64_t ijk[3];
-    quantize(pts + 3 * i, mins, ext, level, ijk);
-    keys[i] = (int64_t)interleave3(ijk[0], ijk[1], ijk[2], level);
-  }
+  // each key depends on its point alone: contiguous ranges on worker threads
+  auto run = [&](int64_t i0, int64_t i1) {
+    for (int64_t i = i0; i < i1; ++i) {
+      int64_t ijk[3];
+      quantize(pts + 3 * i, mins, ext, level, ijk);
+      keys[i] = (int64_t)interleave3(ijk[0], ijk[1], ijk[2], level);
+    }
+  };
+  const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+  const int nthr = (int)std::max<int64_t>(1, std::min<int64_t>(std::min(hw, 8), n / 8192));
+  std::vector<std::thread> pool;
+  for (int k = 1; k < nthr; ++k) pool.emplace_back(run, n * k / nthr, n * (k + 1) / nthr);
+  run(0, n / nthr);
+  for (auto& th : pool) th.join();
 }
 
 int64_t HostTree::n_leaves() const {
@@ -129,20 +138,22 @@ int build_host_tree(const double* C, int64_t n, int leaf_size, int max_level,
   const double bh[3] = {half, half, half};
   std::vector<int64_t> keys(n);
   morton_keys_host(C, n, kKeyLevel, t.center, bh, keys.data());
-  // stable LSD radix sort of (key, index) on 16-bit digits: key order, ties
+  // stable LSD radix sort of (key, index) on 11-bit digits (the 2,048
+  // counters stay in L1), skipping a digit every key shares: key order, ties
   // in index order
   {
     std::vector<uint64_t> ka(n), kb(n);
     std::vector<int64_t> ia(n), ib(n);
     for (int64_t i = 0; i < n; ++i) { ka[i] = (uint64_t)keys[i]; ia[i] = i; }
-    std::vector<int64_t> cnt(65537);
-    for (int pass = 0; pass < 4; ++pass) {
-      const int sh = 16 * pass;
+    constexpr int kBits = 11, kBuckets = 1 << kBits;
+    std::vector<int64_t> cnt(kBuckets + 1);
+    for (int sh = 0; sh < 3 * kKeyLevel; sh += kBits) {
       std::fill(cnt.begin(), cnt.end(), 0);
-      for (int64_t i = 0; i < n; ++i) ++cnt[((ka[i] >> sh) & 0xffff) + 1];
-      for (int d = 0; d < 65536; ++d) cnt[d + 1] += cnt[d];
+      for (int64_t i = 0; i < n; ++i) ++cnt[((ka[i] >> sh) & (kBuckets - 1)) + 1];
+      if (cnt[((ka[0] >> sh) & (kBuckets - 1)) + 1] == n) continue;
+      for (int d = 0; d < kBuckets; ++d) cnt[d + 1] += cnt[d];
       for (int64_t i = 0; i < n; ++i) {
-        const int64_t dst = cnt[(ka[i] >> sh) & 0xffff]++;
+        const int64_t dst = cnt[(ka[i] >> sh) & (kBuckets - 1)]++;
         kb[dst] = ka[i];
         ib[dst] = ia[i];
       }
