@@ -9,6 +9,8 @@
 #include <vector>
 #include <tuple>
 #include <mutex>
+#include <thread>
+#include <atomic>
 
 #include "ed_plan.h"
 
@@ -196,14 +198,19 @@ struct OperatorTables {
                                // the split form, hi then lo (2 x ncp x kp FP16)
 };
 
+// The tables of one offset are computed outside the cache lock, so several
+// threads can fill the cache at once; a table two threads computed together
+// is kept once (both results are the same).
 static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, int oz, int ncp,
                                                int kp, const std::vector<double>& asc) {
   static std::mutex mu;
   static std::map<std::tuple<int, int, int, int, int>, OperatorTables> cache;
-  std::lock_guard<std::mutex> lock(mu);
   const auto key = std::make_tuple(p, prec, ox, oy, oz);
-  auto it = cache.find(key);
-  if (it != cache.end()) return it->second;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+  }
   const int nc = (p + 1) * (p + 1);
   std::vector<double> Td((size_t)nc * nc);
   m2l_matrix(p, ox, oy, oz, Td.data());
@@ -230,6 +237,7 @@ static const OperatorTables& cached_m2l_tables(int p, int prec, int ox, int oy, 
         }
       }
   }
+  std::lock_guard<std::mutex> lock(mu);
   return cache.emplace(key, std::move(ot)).first->second;
 }
 
@@ -329,6 +337,27 @@ static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
   std::vector<float> tv(kind < 0 ? (size_t)D.n_tab * nc * nc : 0);
   std::vector<float> tm(kind == 0 ? (size_t)D.n_tab * ncp * kp : 0);
   std::vector<uint16_t> th(kind > 0 ? (size_t)D.n_tab * hsz : 0);
+  {
+    // every table from its canonical offset, computed on worker threads
+    std::vector<size_t> todo;
+    for (size_t slot = 0; slot < D.slot_tab.size(); ++slot)
+      if (D.slot_tab[slot] >= 0 && D.slot_sgn[slot] == 0) todo.push_back(slot);
+    std::atomic<size_t> next{0};
+    auto work = [&]() {
+      for (size_t k = next++; k < todo.size(); k = next++) {
+        const size_t slot = todo[k];
+        const int ox = (int)(slot % W) - R, oy = (int)((slot / W) % W) - R,
+                  oz = (int)(slot / (W * W)) - R;
+        (void)cached_m2l_tables(p, kind, ox, oy, oz, ncp, kp, asc);
+      }
+    };
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nthr = (int)std::min<size_t>((size_t)std::min(hw, 8), std::max<size_t>(todo.size(), 1));
+    std::vector<std::thread> pool;
+    for (int k = 1; k < nthr; ++k) pool.emplace_back(work);
+    work();
+    for (auto& th_ : pool) th_.join();
+  }
   for (size_t slot = 0; slot < D.slot_tab.size(); ++slot) {
     const int tab = D.slot_tab[slot];
     if (tab < 0 || D.slot_sgn[slot] != 0) continue;  // each table from its canonical offset
@@ -340,15 +369,22 @@ static int device_tables(int device, int p, int kind, int ws, int ncp, int kp,
     else
       std::memcpy(th.data() + (size_t)tab * hsz, ot.h16.data(), sizeof(uint16_t) * hsz);
   }
-  std::vector<double> Td((size_t)nc * nc);
+  // the eight M2M and eight L2L matrices, one octant and kind per task
   std::vector<float> t8((size_t)8 * nc * nc), l8((size_t)8 * nc * nc);
-  for (int o = 0; o < 8; ++o) {
-    m2m_matrix(p, o, Td.data());
-    for (int k = 0; k < nc; ++k)
-      for (int j = 0; j < nc; ++j) t8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
-    l2l_matrix(p, o, Td.data());
-    for (int k = 0; k < nc; ++k)
-      for (int j = 0; j < nc; ++j) l8[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
+  {
+    auto task = [&](int q) {
+      std::vector<double> Td((size_t)nc * nc);
+      const int o = q & 7;
+      std::vector<float>& dst = q < 8 ? t8 : l8;
+      if (q < 8) m2m_matrix(p, o, Td.data());
+      else l2l_matrix(p, o, Td.data());
+      for (int k = 0; k < nc; ++k)
+        for (int j = 0; j < nc; ++j) dst[(size_t)o * nc * nc + (size_t)j * nc + k] = (float)Td[(size_t)k * nc + j];
+    };
+    std::vector<std::thread> pool;
+    for (int q = 1; q < 16; ++q) pool.emplace_back(task, q);
+    task(0);
+    for (auto& th_ : pool) th_.join();
   }
   std::vector<float> af(nc);
   for (int k = 0; k < nc; ++k) af[k] = (float)asc[k];
