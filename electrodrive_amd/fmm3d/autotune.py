@@ -12,7 +12,15 @@ is the relative L2 difference to the FP64 dense operator on the same panels
 Candidates: expansion order in ``SUPPORTED_ORDERS`` at or above 6 and leaf
 sizes 32 .. 512, with ``ws`` and the M2L precision of ``base_cfg``.  The fastest
 candidate whose error meets ``target_rel_error`` wins; when none does, the most
-accurate one.  Pass ``centroids`` / ``areas`` to tune on the actual geometry
+accurate one.  Not every candidate is measured: the error is set by the order
+(the leaf size moves it by a few percent), so an order whose first candidate
+misses the target twice over is dropped, the search stops at an order whose
+first candidate is twice as slow as the best one that met the target, and
+for the other orders the leaf size walks from 256 towards the faster
+neighbour while the matvec gets faster (its time is unimodal in the leaf
+size).  On the 82k sphere, the 300k torus and the 64-geometry batch about 10
+of the 24 candidates are measured, and the small-leaf plans, the slowest to
+build and to run, are skipped.  Pass ``centroids`` / ``areas`` to tune on the actual geometry
 (otherwise a unit sphere of ``problem_size`` panels is used).
 
 The cache file (``$ELECTRODRIVE_AMD_AUTOTUNE_CACHE`` or
@@ -210,18 +218,41 @@ def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: boo
         sigma = torch.cat([sigma[None], torch.as_tensor(rng.normal(size=(nrhs - 1, n)),
                                                         dtype=torch.float64, device=dev)])
 
-    cands: List[FmmConfig] = []
-    for p in (o for o in SUPPORTED_ORDERS if o >= 6):
-        for leaf in LEAF_SIZES:
-            cands.append(dataclasses.replace(base_cfg, expansion_order=p, leaf_size=leaf))
-    cands = cands[:max_trials]
     trials = []
-    for cfg in cands:
+    budget = [max_trials]
+
+    def run(p, leaf):
+        if budget[0] <= 0:
+            return None
+        budget[0] -= 1
+        cfg = dataclasses.replace(base_cfg, expansion_order=p, leaf_size=leaf)
         try:
             m = _measure(cfg, C, A, sigma, ref, reps, dev, geom_offsets)
         except RuntimeError:
-            continue
+            return None
         trials.append((cfg, m))
+        return m
+
+    leaves = list(LEAF_SIZES)
+    start = leaves.index(256) if 256 in leaves else len(leaves) // 2
+    for p in (o for o in SUPPORTED_ORDERS if o >= 6):
+        first = run(p, leaves[start])
+        ok_ms = [t[1]["ms"] for t in trials if t[1]["rel_err"] <= target_rel_error]
+        if first is not None and ok_ms and first["ms"] > 2.0 * min(ok_ms):
+            break  # a higher order only costs more once a lower one meets the target
+        if first is None or first["rel_err"] > 2.0 * target_rel_error:
+            continue
+        best = first["ms"]
+        for step in (1, -1):  # towards larger leaves first, else smaller
+            i, moved = start + step, False
+            while 0 <= i < len(leaves):
+                m = run(p, leaves[i])
+                if m is None or m["ms"] >= best:
+                    break
+                best, moved = m["ms"], True
+                i += step
+            if moved:
+                break
     if not trials:
         return TuningResult(config=base_cfg, metrics={"mode": 1.0, "problem_size": float(n)})
     ok = [t for t in trials if t[1]["rel_err"] <= target_rel_error]
