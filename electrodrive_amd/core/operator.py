@@ -185,7 +185,7 @@ class BemOperator:
         self._sai = True
 
     def solve(self, b, x0=None, *, tol=1e-8, restart=50, maxiter=2000, refine=True,
-              inner_tol=None, max_refine=None, precond=True, use_graph=True, check_every=4,
+              inner_tol=None, max_refine=None, precond=True, use_graph=False, check_every=4,
               residual_update=True):
         """FP64 GMRES on the FMM operator with FP64 iterative refinement against
         the accurate operator (``refine=True``); returns (sigma, info).

@@ -1162,7 +1162,7 @@ extern "C" void ed_gmres_opts_default(ed_gmres_opts* o) {
   o->refine = 1;
   o->max_refine = 8;
   o->inner_tol = 1e-4;
-  o->use_graph = 1;
+  o->use_graph = 0;  // direct launches: measured faster than graph replay (5 k - 300 k panels)
   o->check_every = 4;
   o->residual_update = 1;
 }

@@ -285,7 +285,7 @@ class FmmPlan:
 
     def solve(self, b, x0=None, diag=None, self_integrals=None, *, restart=50, maxiter=2000,
               tol=1e-8, precond=True, refine=True, max_refine=8, inner_tol=1e-4,
-              use_graph=True, check_every=4, residual_update=True):
+              use_graph=False, check_every=4, residual_update=True):
         """Native FP64 GMRES (+ iterative refinement); returns (x, info dict).
         ``b`` of shape (n,) or (R, n) with R <= 8: the R right-hand sides are
         solved together (one multi-RHS FMM matvec per Arnoldi step), each

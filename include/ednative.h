@@ -252,7 +252,8 @@ typedef struct ed_gmres_opts {
                           /*    dense FP64 operator as the residual operator */
   int32_t max_refine;     /* outer refinement steps                          */
   double inner_tol;       /* relative tolerance of each inner solve          */
-  int32_t use_graph;      /* capture one Arnoldi step in a hipGraph          */
+  int32_t use_graph;      /* capture one Arnoldi step in a hipGraph (default */
+                          /*    0: direct launches measured 3-8 % faster)  */
   int32_t check_every;    /* host poll interval in Arnoldi steps             */
   int32_t residual_update; /* 1: refinement residuals b - A_fmm x with the  */
                           /*    split-FP16 FMM while they are far above its  */

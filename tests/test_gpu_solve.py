@@ -183,6 +183,25 @@ def test_sai_rows_match_oracle():
         assert err < 1e-9
 
 
+def test_graph_replay_matches_direct_launches():
+    """The Arnoldi step replayed from a captured hipGraph (use_graph=True)
+    gives the solve of direct launches (the default) bit for bit, with the
+    Jacobi and the SAI smoother, twice in a row (the second call replays the
+    graph captured by the first)."""
+    from electrodrive_amd.core import bem_mesh
+    from electrodrive_amd.core.operator import BemOperator
+    m = bem_mesh.icosphere(4)
+    op = BemOperator(m, order=10, ws=1, leaf_size=64)
+    b = op.rhs_point_charges([(1.0, (0.3, -0.5, 1.4))])
+    for precond in (True, "sai"):
+        xd, idd = op.solve(b, tol=1e-8, precond=precond)
+        for _ in range(2):
+            xg, ig = op.solve(b, tol=1e-8, precond=precond, use_graph=True)
+            print(precond, "iters direct", idd["iters"], "graph", ig["iters"])
+            assert ig["code"] == 0 and ig["iters"] == idd["iters"]
+            assert torch.equal(xg, xd)
+
+
 def test_sai_solve_same_solution_fewer_iterations():
     """Jacobi and SAI smoothers (with and without a fixed coarse space) reach
     the same FP64-refined solution; the SAI takes far fewer iterations."""
