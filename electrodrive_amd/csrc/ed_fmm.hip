@@ -121,14 +121,19 @@ __global__ void k_gather(int n, const int32_t* __restrict__ perm, const TS* __re
 }
 
 // Near-field correction of each panel for one RHS (tree order):
-// near_t[i] = sum of w_e sig_t[col_e] over row i of the near CSR.  Four lanes
-// a panel: lane r takes entries b + r, b + r + 4, ..., two of them in flight,
-// and the four partial sums are added as (r0 + r1) + (r2 + r3).  It runs
-// beside the far-field chain, so the output kernel only reads one value.
+// near_t[i] = sum of w_e sigma_t[col_e] over row i of the near CSR.  Four
+// lanes a panel: lane r takes entries b + r, b + r + 4, ..., two of them in
+// flight, and the four partial sums are added as (r0 + r1) + (r2 + r3).  It
+// runs on the caller's stream after the far-field chain while the P2P
+// finishes on the side stream, so it reads sigma through the permutation
+// itself (the same float the gather writes) instead of waiting for the
+// gather's output.
+template <typename TS>
 __global__ __launch_bounds__(256) void k_near(int n, const int32_t* __restrict__ near_ptr,
                                               const int32_t* __restrict__ near_col,
                                               const float* __restrict__ near_w,
-                                              const float* __restrict__ sig_t,
+                                              const int32_t* __restrict__ perm,
+                                              const TS* __restrict__ sigma, int sigma_tree,
                                               float* __restrict__ near_t) {
   const int t = blockIdx.x * 256 + (int)threadIdx.x;
   const int i = min(t >> 2, n - 1), r = t & 3;
@@ -138,7 +143,8 @@ __global__ __launch_bounds__(256) void k_near(int n, const int32_t* __restrict__
     const int k2 = min(k + 4, e - 1);
     const int c1 = near_col[k], c2 = near_col[k2];
     const float w1 = near_w[k], w2 = k + 4 < e ? near_w[k2] : 0.0f;
-    const float s1 = sig_t[c1], s2 = sig_t[c2];
+    const float s1 = (float)sigma[sigma_tree ? c1 : perm[c1]];
+    const float s2 = (float)sigma[sigma_tree ? c2 : perm[c2]];
     acc += w1 * s1;
     acc += w2 * s2;
   }
@@ -1785,8 +1791,13 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   };
   auto near = [&](hipStream_t ns) -> int {
     if (!multi) {
-      hipLaunchKernelGGL(k_near, dim3((unsigned)((4 * (int64_t)n + 255) / 256)), dim3(256), 0, ns,
-                         n, Pn->near_ptr, Pn->near_col, Pn->near_w, Pn->sig_t, Pn->near_t);
+      const dim3 gn((unsigned)((4 * (int64_t)n + 255) / 256));
+      if (sig64)
+        hipLaunchKernelGGL(k_near<double>, gn, dim3(256), 0, ns, n, Pn->near_ptr, Pn->near_col,
+                           Pn->near_w, Pn->perm, (const double*)sigma, sig_tree, Pn->near_t);
+      else
+        hipLaunchKernelGGL(k_near<float>, gn, dim3(256), 0, ns, n, Pn->near_ptr, Pn->near_col,
+                           Pn->near_w, Pn->perm, (const float*)sigma, sig_tree, Pn->near_t);
       ED_LAUNCH_CHECK("near");
     }
     return ED_OK;
