@@ -163,6 +163,8 @@ static int upload(T** dst, const std::vector<T>& src) {
 }
 
 
+static void sync_release(ed_plan* P);
+
 void plan_free(ed_plan* P) {
   void* ptrs[] = {P->pts, P->area_t, P->sig_t, P->selfc_t, P->near_t, P->perm, P->node_ctr, P->node_rng,
                   P->node_parent, P->node_octant, P->first_child, P->n_child, P->M, P->L,
@@ -184,6 +186,7 @@ void plan_free(ed_plan* P) {
   }
   if (P->ev_fork) (void)hipEventDestroy(P->ev_fork);
   if (P->ev_join) (void)hipEventDestroy(P->ev_join);
+  sync_release(P);
   sai_free(P);
   gmres_free(P);
 }
@@ -427,6 +430,54 @@ static int side_stream(int device, hipStream_t* out) {
   }
   *out = it->second;
   return ED_OK;
+}
+
+// Join signals of the matvec (hipMallocSignalMemory, 8 bytes each), kept per
+// device for the life of the process: a destroyed plan hands its signal, with
+// its last sequence number, to the next plan on the device, so no signal
+// memory is freed and a signal's value never goes down (all its writes come
+// from the device's one side stream, in order).
+struct SyncSignal {
+  uint64_t* join = nullptr;
+  uint64_t seq = 0;
+};
+static std::mutex g_sync_mu;
+static std::map<int, std::vector<SyncSignal>> g_sync_pool;
+
+static int sync_acquire(ed_plan* P) {
+  {
+    std::lock_guard<std::mutex> lock(g_sync_mu);
+    auto& v = g_sync_pool[P->device];
+    if (!v.empty()) {
+      P->sync_join = v.back().join;
+      P->sync_seq = v.back().seq;
+      v.pop_back();
+      return ED_OK;
+    }
+  }
+  int can = 0;
+  if (hipDeviceGetAttribute(&can, hipDeviceAttributeCanUseStreamWaitValue, P->device) !=
+          hipSuccess || !can) {
+    (void)hipGetLastError();
+    return ED_OK;  // the join uses events
+  }
+  uint64_t* j = nullptr;
+  if (hipExtMallocWithFlags((void**)&j, 8, hipMallocSignalMemory) != hipSuccess) {
+    (void)hipGetLastError();
+    return ED_OK;
+  }
+  ED_HIP(hipStreamWriteValue64(P->side, j, 0, 0));
+  ED_HIP(hipStreamSynchronize(P->side));
+  P->sync_join = j;
+  P->sync_seq = 0;
+  return ED_OK;
+}
+
+static void sync_release(ed_plan* P) {
+  if (!P->sync_join) return;
+  std::lock_guard<std::mutex> lock(g_sync_mu);
+  g_sync_pool[P->device].push_back(SyncSignal{P->sync_join, P->sync_seq});
+  P->sync_join = nullptr;
 }
 
 // P2P schedule: the interactions of every 64-target chunk of a leaf are cut
@@ -740,6 +791,7 @@ int plan_upload(ed_plan* P) {
   ED_HIP(hipDeviceGetAttribute(&P->n_cu, hipDeviceAttributeMultiprocessorCount, P->device));
   ED_HIP(hipEventCreateWithFlags(&P->ev_fork, hipEventDisableTiming));
   ED_HIP(hipEventCreateWithFlags(&P->ev_join, hipEventDisableTiming));
+  ED_TRY(sync_acquire(P));
 
   ED_TRY(upload(&P->u_ptr, uptr));
   ED_TRY(upload(&P->u_src, usrc));

@@ -1802,13 +1802,32 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     }
     return ED_OK;
   };
+  // the join: a signal (stream memory write and wait) when the P2P is the
+  // longer branch (the plan leaves its grid uncapped), else an event, and an
+  // event under graph capture, where a replay could not advance the sequence
+  // number the wait compares against.  300k-panel torus, P2P longer: 245.5
+  // -> 241.3 us per matvec with the signal; 82k sphere, far-field chain
+  // longer: 87.1 -> 88.5 us, so it keeps the event there.
+  bool use_sig = false;
+  uint64_t seq = 0;
   if (fork) {
+    if (Pn->sync_join && Pn->p2p_blocks_per_cu == 0) {
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      ED_HIP_LAUNCH(hipStreamIsCapturing(st, &cs), "p2p fork");
+      use_sig = cs == hipStreamCaptureStatusNone;
+    }
     ED_HIP_LAUNCH(hipEventRecord(Pn->ev_fork, st), "p2p fork");
     ED_HIP_LAUNCH(hipStreamWaitEvent(Pn->side, Pn->ev_fork, 0), "p2p fork");
     if (int rc = gather(Pn->side); rc != ED_OK) return rc;
     p2p_launch(Pn, R, rs, Pn->side);
     ED_LAUNCH_CHECK("p2p");
-    ED_HIP_LAUNCH(hipEventRecord(Pn->ev_join, Pn->side), "p2p join");
+    if (use_sig) {
+      seq = Pn->sync_seq + 1;
+      ED_HIP_LAUNCH(hipStreamWriteValue64(Pn->side, Pn->sync_join, seq, 0), "p2p join");
+      Pn->sync_seq = seq;
+    } else {
+      ED_HIP_LAUNCH(hipEventRecord(Pn->ev_join, Pn->side), "p2p join");
+    }
   } else {
     if (int rc = gather(st); rc != ED_OK) return rc;
     if (int rc = near(st); rc != ED_OK) return rc;
@@ -1913,7 +1932,11 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
     // the near-field correction after the far-field chain, while the P2P
     // (the longer branch) finishes on the side stream
     if (int rc = near(st); rc != ED_OK) return rc;
-    ED_HIP_LAUNCH(hipStreamWaitEvent(st, Pn->ev_join, 0), "p2p join");
+    if (use_sig)
+      ED_HIP_LAUNCH(hipStreamWaitValue64(st, Pn->sync_join, seq, hipStreamWaitValueGte, ~0ull),
+                    "p2p join");
+    else
+      ED_HIP_LAUNCH(hipStreamWaitEvent(st, Pn->ev_join, 0), "p2p join");
   } else if (Pn->n_p2p_items > 0) {
     p2p_launch(Pn, R, rs, st);
     ED_LAUNCH_CHECK("p2p");

@@ -149,11 +149,23 @@ struct ed_plan {
 
   // stage events while ed_fmm_profile runs (8 markers), else NULL
   hipEvent_t* prof_events = nullptr;
-  // P2P runs on a side stream beside the far-field passes (fork/join by
-  // events on the caller's stream, so graph capture records both branches);
+  // P2P runs on a side stream beside the far-field passes (fork by an event
+  // on the caller's stream, join by an event under graph capture, so that
+  // capture records both branches, else by the signal below);
   // the stream is shared by all plans on the device and never destroyed
   hipStream_t side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // where the P2P is the longer branch (p2p_blocks_per_cu == 0) and outside
+  // graph capture, the join is a stream memory operation on a signal instead
+  // of an event: the side stream writes sync_join = k (the matvec's sequence
+  // number) after the P2P and the caller's stream waits for >= k.  The write
+  // and the wait cost about 5 us on the host against 8-15 us for an event
+  // record plus cross-stream wait.  (The fork stays an event: a side stream
+  // waiting on a signal value that the caller's stream writes later resumed
+  // tens of microseconds late.)  Every write comes from the device's one side
+  // stream, in order, so the value only grows.  NULL: events only.
+  uint64_t* sync_join = nullptr;
+  uint64_t sync_seq = 0;
   int n_cu = 256;  // compute units of the plan's device (caps the one-RHS P2P grid)
   // one-RHS P2P grid cap in blocks per CU (0: one block per item block); set
   // at plan creation from the estimated lengths of the two matvec branches

@@ -143,8 +143,13 @@ def _measure(cfg, C, A, sigma, ref, reps, dev, geom_offsets=None):
     out = torch.empty_like(sigma)
     mv = (lambda: plan.matvec_multi(sigma, out=out)) if sigma.dim() == 2 else \
         (lambda: plan.matvec(sigma, out=out))
-    for _ in range(2):
+    # warm-up, then a run long enough that the start-up gap (the host ahead
+    # of an idle device by one matvec) stays small: with 5 timed matvecs the
+    # 82k sphere read 0.102 ms at leaf 256 and 384 alike, against 0.087 and
+    # 0.100 ms in a long run
+    for _ in range(3):
         mv()
+    torch.cuda.synchronize(dev)
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
@@ -167,7 +172,7 @@ def _measure(cfg, C, A, sigma, ref, reps, dev, geom_offsets=None):
 def autotune_for_problem(base_cfg: FmmConfig, problem_size: int, *, dry_run: bool = True,
                          target_rel_error: float = 1e-2, max_trials: int = 24, device=None,
                          use_cache: bool = True, centroids=None, areas=None,
-                         reps: int = 5, geom_offsets=None, nrhs: int = 1) -> TuningResult:
+                         reps: int = 20, geom_offsets=None, nrhs: int = 1) -> TuningResult:
     """Picks (order, leaf) with the fastest matvec whose relative L2 error
     against the FP64 dense operator is at most ``target_rel_error``.  With
     ``geom_offsets`` the plans are batched over those geometries and the error

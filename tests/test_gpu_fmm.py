@@ -324,6 +324,64 @@ def test_matvec_repeatable_bit_for_bit(prec):
             assert _rel(first[b:e].cpu().numpy(), ref) < (2e-4 if prec == "fp16x3" else 2e-4)
 
 
+@pytest.mark.parametrize("leaf,order", [(64, 8), (256, 6)])
+def test_matvec_fork_join_back_to_back_and_across_streams(leaf, order):
+    """The P2P branch forks from the caller's stream by an event and joins it
+    again by an event where the far-field chain is the longer branch (leaf
+    64, p = 8 on 20k panels) and through a signal where the P2P is (leaf 256,
+    p = 6: a stream memory write on the side stream, a wait for the matvec's
+    sequence number on the caller's stream).  Matvecs
+    enqueued back to back with no host synchronisation, matvecs that
+    alternate between two streams (each ordered after the previous one by the
+    caller), and matvecs of a plan created after another was destroyed (it
+    takes over the destroyed plan's signal and sequence number) all give the
+    bits of matvecs run one at a time with a device synchronisation between
+    them."""
+    import gc
+    from electrodrive_amd.fmm3d.plan import FmmPlan
+    m = _mesh(5)
+    rng = np.random.default_rng(7)
+    S = [torch.as_tensor(rng.normal(size=m.n_panels), device="cuda") for _ in range(6)]
+
+    def plan():
+        return FmmPlan(m.centroids, m.areas, leaf_size=leaf, order=order, ws=1,
+                       m2l_precision="fp16x3")
+
+    p = plan()
+    ref = []
+    for s in S:
+        ref.append(p.matvec(s).clone())
+        torch.cuda.synchronize()
+    outs = [torch.empty_like(s) for s in S]
+    for rep in range(4):  # back to back on torch's current stream
+        for s, o in zip(S, outs):
+            p.matvec(s, out=o)
+    torch.cuda.synchronize()
+    for o, r in zip(outs, ref):
+        assert torch.equal(o, r)
+    streams = (torch.cuda.Stream(), torch.cuda.Stream())
+    outs = [torch.empty_like(s) for s in S]
+    prev = torch.cuda.current_stream()
+    for rep in range(3):
+        for i, (s, o) in enumerate(zip(S, outs)):
+            st = streams[(i + rep) % 2]
+            st.wait_stream(prev)
+            with torch.cuda.stream(st):
+                p.matvec(s, out=o)
+            prev = st
+    torch.cuda.current_stream().wait_stream(prev)
+    torch.cuda.synchronize()
+    for o, r in zip(outs, ref):
+        assert torch.equal(o, r)
+    del p
+    gc.collect()
+    q = plan()
+    outs = [q.matvec(s) for s in S]
+    torch.cuda.synchronize()
+    for o, r in zip(outs, ref):
+        assert torch.equal(o, r)
+
+
 @pytest.mark.parametrize("order", [14, 16])
 def test_high_orders_accepted_and_accurate(order):
     """Orders up to 16 (the reference's 2p <= 32, fmm3d/config.py:122-172):
