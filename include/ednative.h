@@ -92,7 +92,11 @@ int ed_plan_permutation(const ed_plan* plan, int64_t* perm_host);
  *         + sum_j delta_ij sigma_j                      (if near attached).
  * sigma / self_integrals / out are device arrays in original panel order;
  * non-finite outputs are clamped to 0 (bem_fmm.py:741-749).  The f64 variant
- * takes and returns FP64 with FP32 arithmetic inside.  No allocation. */
+ * takes and returns FP64 with FP32 arithmetic inside.  No allocation.
+ * The P2P runs on a per-device side stream that forks from `stream` and
+ * joins it again before the output; the plan's buffers are shared by its
+ * calls, so calls on one plan from several streams must be ordered by the
+ * caller (as for one stream). */
 int ed_fmm_matvec_f32(ed_plan* plan, const float* sigma, const float* self_integrals,
                       float* out, ed_stream stream);
 int ed_fmm_matvec_f64(ed_plan* plan, const double* sigma, const double* self_integrals,
