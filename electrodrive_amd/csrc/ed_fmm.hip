@@ -1807,7 +1807,7 @@ static int fmm_run(ed_plan* Pn, const void* sigma, int sig64, int sig_tree, cons
   // event under graph capture, where a replay could not advance the sequence
   // number the wait compares against.  300k-panel torus, P2P longer: 245.5
   // -> 241.3 us per matvec with the signal; 82k sphere, far-field chain
-  // longer: 87.1 -> 88.5 us, so it keeps the event there.
+  // longer: 87.8 -> 88.5 us, so it keeps the event there.
   bool use_sig = false;
   uint64_t seq = 0;
   if (fork) {
